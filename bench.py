@@ -1,0 +1,201 @@
+"""Benchmark of the volume-filter hot path on one MI355X.
+
+Workload (one "step"): the circle-fitting forward of runCircleFittingMethod
+(tube-segmentation.cpp:212-939) on a seeded 512^3 thin-vessel MRA-like volume
+(120 tubes, radius 0.5-3, Rician noise 0.08): small-scale branch (blur 0.5 ->
+vector field -> TDF r = 0.5..3.0) and large-scale branch (blur 0.5 -> vector
+field -> GVF Euler, mu = 0.05, 400 iterations, field stored in FP16 with
+float arithmetic -> TDF r = 2.5..4.0, step 0.5) -> combine.
+
+Headline value: GVF throughput in Gvoxel-iter/s (voxels x iterations / GVF
+time) of the FP16-stored field.  Also reported: forward ms per step, the same
+GVF in plain FP32 storage (speed-up), the GVF kernel's achieved HBM bandwidth
+against the 8 TB/s peak, and the oracle's FP32 GVF on the host CPU.
+
+All inputs are resident in device memory before the timed region; timings use
+HIP events recorded on the stream the kernels run on.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tube-segmentation-framework_amd"))
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM = 8.0e12  # bytes/s, MI355X HBM3E
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=400)
+    ap.add_argument("--mu", type=float, default=0.05)
+    ap.add_argument("--storage", choices=["f16", "bf16", "f32"], default="f16")
+    ap.add_argument("--fuse", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-f32-compare", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the volume-filter path runs on a single device "
+              "(multi-GPU z-slab sharding is a design in DESIGN.md, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import tsf
+    from tsf import synth
+
+    if tsf.device_count() < 1:
+        raise RuntimeError("no GPU visible")
+    S = {"f32": tsf.STORAGE_F32, "f16": tsf.STORAGE_F16, "bf16": tsf.STORAGE_BF16}[args.storage]
+    scale = 2.0 ** 14 if S == tsf.STORAGE_F16 else 1.0
+    n = args.size
+    shape = (n, n, n)
+    N = n ** 3
+
+    t0 = time.time()
+    vol, _ = synth.config5(shape=shape)
+    log(f"synthetic {shape} volume in {time.time() - t0:.1f}s")
+
+    stream = tsf.Stream()
+    d_vol = tsf.DeviceArray.from_host(vol, stream=stream)
+    d_blur = tsf.DeviceArray(shape, np.float32)
+    F_small = tsf.Field(shape, S, scale)
+    F = tsf.Field(shape, S, scale)
+    V = tsf.Field(shape, S, scale)
+    ws = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, S),), np.uint8)
+    t_s, r_s = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    t_l, r_l = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    ev = [tsf.Event() for _ in range(4)]
+
+    def step():
+        ev[0].record(stream)
+        # small-radius branch (radius-min < 2.5)
+        tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
+        tsf.create_vector_field(d_blur, F_small, 0.2, 1, stream)
+        tsf.tdf_circle(F_small, 0.5, 3.0, 0.5, t_s, r_s, stream)
+        # large-radius branch
+        tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
+        tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
+        ev[1].record(stream)
+        tsf.gvf_euler(F, V, args.mu, args.iters, args.fuse, ws, stream)
+        ev[2].record(stream)
+        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, stream)
+        tsf.combine(t_s, r_s, t_l, r_l, stream)
+        ev[3].record(stream)
+
+    for i in range(args.warmup):
+        step()
+        stream.synchronize()
+        log(f"warmup {i} done")
+    fwd, gvf = [], []
+    stream.synchronize()
+    wall0 = time.time()
+    for i in range(args.steps):
+        step()
+        stream.synchronize()
+        fwd.append(ev[0].elapsed_ms(ev[3]))
+        gvf.append(ev[1].elapsed_ms(ev[2]))
+        log(f"step {i}: forward {fwd[-1]:.1f} ms, GVF {gvf[-1]:.1f} ms")
+    stream.synchronize()
+    wall = time.time() - wall0
+
+    gvf_ms = float(np.median(gvf))
+    fwd_ms = float(np.median(fwd))
+    gvox = N * args.iters / (gvf_ms * 1e-3) / 1e9
+    launches = max(1, -(-args.iters // args.fuse))
+    bytes_per_voxel = 9 * (4 if S == tsf.STORAGE_F32 else 2)  # read v, f; write v (3 components each)
+    launch_ms = gvf_ms / launches
+    achieved = bytes_per_voxel * N / (launch_ms * 1e-3)
+
+    # same GVF with float storage, for the speed-up of the reduced-precision form
+    f32 = None
+    if not args.no_f32_compare and S != tsf.STORAGE_F32:
+        F32 = tsf.Field(shape)
+        V32 = tsf.Field(shape)
+        ws32 = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, 0),), np.uint8)
+        tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
+        tsf.create_vector_field(d_blur, F32, 0.2, 1, stream)
+        tsf.gvf_euler(F32, V32, args.mu, args.iters, args.fuse, ws32, stream)  # warm
+        stream.synchronize()
+        ts = []
+        for _ in range(max(1, args.steps)):
+            ev[1].record(stream)
+            tsf.gvf_euler(F32, V32, args.mu, args.iters, args.fuse, ws32, stream)
+            ev[2].record(stream)
+            stream.synchronize()
+            ts.append(ev[1].elapsed_ms(ev[2]))
+        f32_ms = float(np.median(ts))
+        f32 = {"gvf_ms": round(f32_ms, 3), "gvoxel_iter_per_s": round(N * args.iters / (f32_ms * 1e-3) / 1e9, 2),
+               "speedup_reduced_vs_f32": round(f32_ms / gvf_ms, 3),
+               "hbm_frac": round(36 * N / (f32_ms / launches * 1e-3) / PEAK_HBM, 4)}
+        del F32, V32, ws32
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(vol)
+
+    out = {
+        "metric": "GVF Gvoxel-iter/s (FP16-stored field, 512^3, mu 0.05, 400 iterations)",
+        "value": round(gvox, 3),
+        "unit": "Gvoxel-iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fwd_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f16": "f16-storage/f32-arith", "bf16": "bf16-storage/f32-arith", "f32": "f32"}[args.storage],
+        "data": "synthetic (seeded 512^3 thin-vessel MRA-like tree, Rician noise)",
+        "config": {"workload": f"circle-fitting forward at {n}^3: blur, vector field, GVF Euler "
+                               f"{args.iters} it mu {args.mu} ({args.storage} storage), TDF small 0.5-3 + large "
+                               f"2.5-4, combine", "voxels": N, "gvf_iterations": args.iters,
+                   "fuse": args.fuse, "parallelism": "single-gpu"},
+        "gvf_ms": round(gvf_ms, 3),
+        "forward_ms": round(fwd_ms, 3),
+        "wall_s_timed": round(wall, 3),
+        "f32_gvf": f32,
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
+                     "kernel": "k_gvf_step", "bytes_per_voxel_per_launch": bytes_per_voxel,
+                     "launch_ms": round(launch_ms, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(vol):
+    """Oracle FP32 GVF (OpenMP, all host threads) on a bounded sample: the
+    central 256^3 block of the same volume after the same blur and vector
+    field, timed over enough iterations for ~10 s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    c = min(256, vol.shape[0])
+    o = (vol.shape[0] - c) // 2
+    sub = np.ascontiguousarray(vol[o:o + c, o:o + c, o:o + c])
+    F = oracle.vector_field(oracle.blur(sub, 0.5, "f32"), 0.2, 1, "f32")
+    t0 = time.time()
+    oracle.gvf_euler(F, 0.05, 2, "f32")
+    t1 = time.time() - t0
+    iters = max(2, int(10.0 / max(t1 / 2, 1e-3)))
+    t0 = time.time()
+    oracle.gvf_euler(F, 0.05, iters, "f32")
+    dt = time.time() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(sub.size * iters / dt / 1e9, 4), "unit": "Gvoxel-iter/s", "cores": threads,
+            "kind": "port", "sample": f"oracle FP32 GVF Euler, {c}^3 centre block, {iters} iterations, {dt:.1f}s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,231 @@
+/*
+ * tsf.h -- C ABI of the MI355X (gfx950) volume-filter library (libtsf_hip.so).
+ *
+ * The library is the device side of the Tube Segmentation Framework's
+ * volume-filter path: gradient vector field creation, gradient vector flow
+ * (Euler iterations and the FMG multigrid solver), Hessian eigenanalysis,
+ * circle-fitting TDF, and histogram-pyramid compaction of centreline
+ * candidates.  Each entry point below names the reference function or kernel
+ * it replaces (paths relative to the reference source tree).
+ *
+ * Conventions
+ *  - Volumes are dense X*Y*Z arrays in device memory, linear index
+ *    x + X*(y + Y*z) (the reference's LPOS macro, kernels_no_3d_write.cl:6).
+ *  - A vector field is a tsf_field: four separate planes x, y, z, w (SoA).
+ *    Plane w holds the per-voxel magnitude written by the GVF finish step
+ *    (|v|, or 1 where |v| == 0) and is read by the normalised Hessian.  The
+ *    reference stores the same four channels interleaved as one RGBA texel.
+ *  - All calls are asynchronous on the given stream and return a tsf_status;
+ *    nothing throws across the ABI.  tsf_last_error() returns a message for
+ *    the last failure on the calling thread.
+ *  - Entry points that need temporary device memory take (workspace,
+ *    workspace_bytes).  Pass NULL/0 to let the library allocate it (the call
+ *    then synchronises the stream before freeing); query the size with the
+ *    matching *_workspace_size function to supply it yourself.
+ */
+#ifndef TSF_H
+#define TSF_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tsf_status {
+    TSF_OK = 0,
+    TSF_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad enum, bad parameter value */
+    TSF_ERR_DIMENSIONS = 2,       /* volume too small for the operator */
+    TSF_ERR_HIP = 3,              /* a HIP runtime call or kernel launch failed */
+    TSF_ERR_OUT_OF_MEMORY = 4,    /* workspace allocation failed or too small */
+    TSF_ERR_CAPACITY = 5,         /* output list capacity exceeded (count still reported) */
+    TSF_ERR_RANGE = 6             /* result outside the range the caller accepts */
+} tsf_status;
+
+typedef enum tsf_storage {
+    TSF_STORAGE_F32 = 0,  /* float planes */
+    TSF_STORAGE_F16 = 1,  /* IEEE half planes holding value * scale */
+    TSF_STORAGE_BF16 = 2  /* bfloat16 planes holding value * scale */
+} tsf_storage;
+
+typedef struct tsf_dims {
+    int32_t x, y, z;
+} tsf_dims;
+
+typedef struct tsf_field {
+    void* x;         /* plane of component x (X*Y*Z elements of `storage`) */
+    void* y;
+    void* z;
+    void* w;         /* magnitude plane; may be NULL where an entry point says so */
+    int32_t storage; /* tsf_storage */
+    float scale;     /* stored = value * scale; 1 for F32; a power of two */
+} tsf_field;
+
+/* ------------------------------------------------------------ housekeeping */
+const char* tsf_last_error(void);
+const char* tsf_version(void);
+/* Device memory and stream helpers for hosts without their own allocator. */
+tsf_status tsf_malloc(void** ptr, size_t bytes);
+tsf_status tsf_free(void* ptr);
+tsf_status tsf_memcpy_htod(void* dst, const void* src, size_t bytes, hipStream_t stream);
+tsf_status tsf_memcpy_dtoh(void* dst, const void* src, size_t bytes, hipStream_t stream);
+tsf_status tsf_memcpy_dtod(void* dst, const void* src, size_t bytes, hipStream_t stream);
+tsf_status tsf_memset(void* dst, int value, size_t bytes, hipStream_t stream);
+tsf_status tsf_stream_create(hipStream_t* stream);
+tsf_status tsf_stream_destroy(hipStream_t stream);
+tsf_status tsf_stream_synchronize(hipStream_t stream);
+/* Event timing on a stream: ms between two recorded events. */
+tsf_status tsf_event_create(hipEvent_t* ev);
+tsf_status tsf_event_destroy(hipEvent_t ev);
+tsf_status tsf_event_record(hipEvent_t ev, hipStream_t stream);
+tsf_status tsf_event_elapsed_ms(hipEvent_t start, hipEvent_t stop, float* ms);
+tsf_status tsf_device_count(int* count);
+
+/* ------------------------------------------------------------ volume prep */
+
+/* Normalised Gaussian mask of createBlurMask (tube-segmentation.cpp:188-211):
+ * maskSize m = clamp(ceil(sigma/0.5), 1, 5), (2m+1)^3 weights
+ * exp(-(a^2+b^2+c^2)/(2 sigma^2)) / sum, index a + b*M + c*M*M.  Host only;
+ * `mask` must hold 1331 floats. */
+tsf_status tsf_create_blur_mask(float sigma, float* mask, int32_t* mask_size);
+
+/* blurVolumeWithGaussian (kernels_no_3d_write.cl:1442-1465) with the mask of
+ * tsf_create_blur_mask(sigma): clamp-to-edge (2m+1)^3 correlation, summed in
+ * the reference's loop order.  Replaces the two blur call sites of
+ * runCircleFittingMethod (tube-segmentation.cpp:247-291, 536-582). */
+tsf_status tsf_blur_gaussian(const float* in, float* out, tsf_dims dims, float sigma, hipStream_t stream);
+
+/* toFloat (kernels_no_3d_write.cl:1414-1440): clamp to [minimum, maximum],
+ * then (v - minimum) / (maximum - minimum).  in_type: 0 = uint8, 1 = int8,
+ * 2 = uint16, 3 = int16, 4 = float (the ElementTypes readDatasetAndTransfer
+ * accepts, tube-segmentation.cpp:1490-1536). */
+tsf_status tsf_to_float(const void* in, int32_t in_type, float* out, tsf_dims dims, float minimum,
+                        float maximum, hipStream_t stream);
+
+/* createVectorField (kernels_no_3d_write.cl:1470-1498): F = sign * central
+ * difference of I (clamp-to-edge); F /= Fmax if |F| < Fmax else F /= |F|;
+ * w = 1.  `out` must have all four planes.  sign = -1 for mode "black"
+ * (tube-segmentation.cpp:220). */
+tsf_status tsf_create_vector_field(const float* volume, const tsf_field* out, tsf_dims dims, float fmax,
+                                   int32_t sign, hipStream_t stream);
+
+/* ------------------------------------------------------------ GVF */
+
+/* runFastGVF (gradientVectorFlow.cpp:1007-1110) with GVF3DIteration
+ * (kernels_no_3d_write.cl:1746-1787) and GVF3DFinish (1798-1807):
+ * v0 = f; `iterations` Jacobi steps
+ *   v(w) <- v(q) + mu*lap(v)(q) - (v(q) - f(q))*|f(q)|^2,  q = mirror(w)
+ * with the reference's boundary rule (0 -> 2, N-1 -> N-3); then
+ * w = |v| (or 1 where |v| == 0).  f needs planes x, y, z; out needs all four.
+ * f and out may use different storage types.  `fuse` iterations are executed
+ * per launch (1 = one launch per iteration; the result does not depend on it
+ * for F32 storage; for F16/BF16 storage intermediate iterates of a fused
+ * launch stay in float).  Note: the reference returns iterate
+ * 2*floor(iterations/2) (its finish kernel reads the first ping-pong buffer);
+ * this entry point returns iterate `iterations`, the C++ runGVF adapter
+ * reproduces the reference's rounding.  Dimensions must be >= 4. */
+tsf_status tsf_gvf_euler(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
+                         int32_t fuse, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_gvf_euler_workspace_size(tsf_dims dims, int32_t storage);
+
+/* runFMGGVF (gradientVectorFlow.cpp:729-1005): per component, `iterations`
+ * defect-correction steps r = fmgResidual(f); e = fullMultigrid(r); f += e
+ * with red-black Gauss-Seidel smoothing (GVFgaussSeidel/GVFgaussSeidel2,
+ * kernels_no_3d_write.cl:2071-2150), 2x2x2 restriction, injection
+ * prolongation, v0 V-cycles of v1 pre- and v2 post-sweeps over
+ * levels+1 grids (reference: v0=1, v1=2, v2=2, levels = l_max = 2); then
+ * w = |v| (no zero guard, MGGVFFinish 2202-2216).  F32 storage only.
+ * `residual_norms` (optional, host array of 3*iterations doubles) receives
+ * the L2 norm of each component's residual before every correction, reduced
+ * in double on the device. */
+tsf_status tsf_gvf_fmg(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
+                       int32_t levels, int32_t v0, int32_t v1, int32_t v2, double* residual_norms,
+                       void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_gvf_fmg_workspace_size(tsf_dims dims, int32_t levels);
+
+/* ------------------------------------------------------------ Hessian / eigen */
+
+/* Per-voxel Jacobian "Hessian" of the field (gradient / gradientNormalized,
+ * kernels_no_3d_write.cl:90-200; matrix layout 1522-1526) followed by the
+ * EISPACK tred2/tql2 eigensolver (1824-2068): eigenvalues sorted by ascending
+ * |lambda|.  normalize != 0 divides every sample by its voxel's w plane.
+ * lambda: 3 planes (float, X*Y*Z each, contiguous).  vectors (optional, may
+ * be NULL): 9 planes, plane 3*j + k = component k of eigenvector j (e1 = the
+ * tube direction).  Replaces the device-side use of eigen_decomposition and
+ * the CPU doEigen batch (eigenanalysisOfHessian.cpp:343-389). */
+tsf_status tsf_hessian_eigen(const tsf_field* field, tsf_dims dims, int32_t normalize, float* lambda,
+                             float* vectors, hipStream_t stream);
+
+/* doEigen / getTubeDirection (eigenanalysisOfHessian.cpp:343-389,
+ * eigenanalysisOfHessian.hpp:7-9) for a batch of points: Jacobian of the
+ * field where every sample is divided by sqrt(Fx^2+Fy^2+Fz^2) of its voxel,
+ * then tred2/tql2.  positions: device int32[3*count] (x,y,z), each at least
+ * one voxel from every border.  out: device float[12*count] per point
+ * (lambda x,y,z, e1 xyz, e2 xyz, e3 xyz). */
+tsf_status tsf_do_eigen_points(const float* fx, const float* fy, const float* fz, tsf_dims dims,
+                               const int32_t* positions, int64_t count, float* out, hipStream_t stream);
+
+/* ------------------------------------------------------------ TDF */
+
+/* circleFittingTDF (kernels_no_3d_write.cl:1503-1572) via runCircleFittingTDF
+ * (tubeDetectionFilters.cpp:57-73): Hessian (normalised when rmax >= 4) ->
+ * eigenvectors e2, e3 -> for r = rmin; r <= rmax; r += rstep (float
+ * accumulation) the mean over 32 directions V_j = cos(j) e3 + sin(j) e2
+ * (j in integer radians) of dot(-F_lin(p + r V_j), V_j), F_lin = trilinear
+ * sampling with the OpenCL linear-filter convention (-0.5 shift,
+ * clamp-to-edge); keep the first maximum, stop at the first r that does not
+ * increase it.  tdf = max(0, best), radius = its r (0 if none).  The field
+ * needs plane w when rmax >= 4. */
+tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep, float* tdf,
+                          float* radius, hipStream_t stream);
+
+/* combine (kernels_no_3d_write.cl:1087-1098): where tdf_large < tdf_small,
+ * take the small-scale TDF and radius.  In place on the large buffers. */
+tsf_status tsf_combine(const float* tdf_small, const float* radius_small, float* tdf_large, float* radius_large,
+                       int64_t count, hipStream_t stream);
+
+/* ------------------------------------------------------------ candidates */
+
+/* findCandidateCenterpoints (kernels_no_3d_write.cl:1006-1017):
+ * mask = tdf < threshold ? 0 : 1. */
+tsf_status tsf_candidates(const float* tdf, uint8_t* mask, int64_t count, float threshold, hipStream_t stream);
+
+/* Histogram-pyramid compaction (oul::HistogramPyramid3DBuffer create /
+ * getSum / createPositionBuffer as used at parallelCenterlineExtraction.cpp:
+ * 727-754 and 839-892; kernels constructHPLevel*, scanHPLevel*,
+ * traverseHP3DBuffer, createPositions3DBuffer, kernels_no_3d_write.cl:
+ * 246-698).  Writes the (x,y,z) of every nonzero mask voxel into `positions`
+ * (device int32[3*capacity]) in the pyramid's traversal order: ascending key
+ * sum_b (x_b + 2 z_b + 4 y_b) 8^b.  *count (host) receives the number of
+ * set voxels (the pyramid's sum); the call synchronises the stream to read
+ * it.  Returns TSF_ERR_CAPACITY (with *count set and the first `capacity`
+ * entries written) when the list does not fit.  positions == NULL with
+ * capacity == 0 only counts (the pyramid's getSum). */
+tsf_status tsf_hp_compact(const uint8_t* mask, tsf_dims dims, int32_t* positions, int64_t capacity,
+                          int64_t* count, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_hp_compact_workspace_size(tsf_dims dims);
+
+/* findCandidateCenterpoints2 (kernels_no_3d_write.cl:1019-1084; the image
+ * path kernels.cl:1309-1390 differs only in the cap of maxD): for each listed
+ * candidate, maxD = clamp(round(radius), 1, maxd_cap); e1 from the normalised
+ * Hessian; the candidate is rejected if a voxel n = p + r, 0 < |r| < maxD,
+ * within 0.5 rad of the plane normal to e1 has w(n) < w(p).  Writes 1 (kept)
+ * or 0 (rejected) into `out` at each candidate; other voxels are untouched.
+ * maxd_cap: 5 (buffer path) or 8 (image path). */
+tsf_status tsf_candidate_nms(const float* tdf, const float* radius, const tsf_field* field, tsf_dims dims,
+                             const int32_t* positions, int64_t count, int32_t maxd_cap, uint8_t* out,
+                             hipStream_t stream);
+
+/* dd (kernels_no_3d_write.cl:970-1004): for every cube x cube x cube cell,
+ * mark in `out` the candidate (mask == 1) with the strictly greatest TDF
+ * (first in x-outer, z-inner order; TDF must be > 0).  `out` is not cleared. */
+tsf_status tsf_cube_max(const float* tdf, const uint8_t* candidates, tsf_dims dims, int32_t cube, uint8_t* out,
+                        hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSF_H */

@@ -1,0 +1,211 @@
+"""ctypes front end of the CPU oracle (oracle/tsf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the cpu_baseline leg of bench.py, as the checker.  The product library never
+imports it.
+
+Volumes are numpy arrays of shape (Z, Y, X) (C order), so the flat index is
+x + X*(y + Y*z) as in the reference's LPOS macro.  Vector fields are arrays of
+shape (4, Z, Y, X): planes x, y, z and w.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libtsf_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dt(prec):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def _fn(name, prec=None):
+    return getattr(lib(), f"orc_{name}_{prec}" if prec else f"orc_{name}")
+
+
+def _dims(vol):
+    Z, Y, X = vol.shape[-3:]
+    return int(X), int(Y), int(Z)
+
+
+def blur_mask(sigma):
+    mask = np.zeros(11 ** 3, np.float32)
+    m = ctypes.c_int(0)
+    _fn("blur_mask")(ctypes.c_float(sigma), _p(mask), ctypes.byref(m))
+    M = 2 * m.value + 1
+    return m.value, mask[: M ** 3].copy()
+
+
+def circle_tables():
+    c = np.zeros(32, np.float32)
+    s = np.zeros(32, np.float32)
+    _fn("circle_tables")(_p(c), _p(s))
+    return c, s
+
+
+def blur(vol, sigma, prec="f32"):
+    vol = np.ascontiguousarray(vol, _dt(prec))
+    m, mask = blur_mask(sigma)
+    out = np.empty_like(vol)
+    X, Y, Z = _dims(vol)
+    _fn("blur", prec)(_p(vol), _p(out), X, Y, Z, m, _p(mask))
+    return out
+
+
+def to_float(data, minimum, maximum, prec="f32"):
+    d = np.ascontiguousarray(data, np.float64)
+    out = np.empty(d.shape, _dt(prec))
+    _fn("to_float", prec)(_p(d), _p(out), ctypes.c_longlong(d.size), ctypes.c_float(minimum),
+                          ctypes.c_float(maximum))
+    return out
+
+
+def vector_field(vol, fmax, sign=1, prec="f32"):
+    vol = np.ascontiguousarray(vol, _dt(prec))
+    X, Y, Z = _dims(vol)
+    F = np.empty((4,) + vol.shape, _dt(prec))
+    _fn("vector_field", prec)(_p(vol), _p(F), X, Y, Z, ctypes.c_float(fmax), int(sign))
+    return F
+
+
+def gvf_euler(F, mu, iters, prec="f32"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    V = np.empty((4,) + F.shape[1:], _dt(prec))
+    _fn("gvf_euler", prec)(_p(F), _p(V), X, Y, Z, ctypes.c_float(mu), int(iters))
+    return V
+
+
+def gvf_step(f3, v3, mu, prec="f32"):
+    f3 = np.ascontiguousarray(f3[:3], _dt(prec))
+    v3 = np.ascontiguousarray(v3[:3], _dt(prec))
+    X, Y, Z = _dims(f3)
+    o = np.empty_like(v3)
+    _fn("gvf_step", prec)(_p(f3), _p(v3), _p(o), X, Y, Z, ctypes.c_float(mu))
+    return o
+
+
+def gvf_fmg(F, mu, iters, v0=1, v1=2, v2=2, lmax=2, prec="f32"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    V = np.empty((4,) + F.shape[1:], _dt(prec))
+    _fn("gvf_fmg", prec)(_p(F), _p(V), X, Y, Z, ctypes.c_float(mu), int(iters), v0, v1, v2, lmax)
+    return V
+
+
+def eigen3(A, prec="f32"):
+    A = np.ascontiguousarray(A, _dt(prec)).reshape(9)
+    V = np.empty(9, _dt(prec))
+    d = np.empty(3, _dt(prec))
+    _fn("eigen3", prec)(_p(A), _p(V), _p(d))
+    return d, V.reshape(3, 3)
+
+
+def hessian_eigen(F, normalize, vectors=True, prec="f32"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    lam = np.empty((3,) + F.shape[1:], _dt(prec))
+    vec = np.empty((9,) + F.shape[1:], _dt(prec)) if vectors else None
+    _fn("hessian_eigen", prec)(_p(F), X, Y, Z, int(normalize), _p(lam),
+                               _p(vec) if vectors else None)
+    return lam, vec
+
+
+def tdf_circle(F, rmin, rmax, rstep, prec="f32"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    tdf = np.empty(F.shape[1:], _dt(prec))
+    rad = np.empty(F.shape[1:], _dt(prec))
+    _fn("tdf_circle", prec)(_p(F), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax),
+                            ctypes.c_float(rstep), _p(tdf), _p(rad))
+    return tdf, rad
+
+
+def combine(ts, rs, tl, rl, prec="f32"):
+    ts, rs = np.ascontiguousarray(ts, _dt(prec)), np.ascontiguousarray(rs, _dt(prec))
+    tl, rl = np.array(tl, _dt(prec)), np.array(rl, _dt(prec))
+    _fn("combine", prec)(_p(ts), _p(rs), _p(tl), _p(rl), ctypes.c_longlong(ts.size))
+    return tl, rl
+
+
+def candidates(tdf, thr, prec="f32"):
+    tdf = np.ascontiguousarray(tdf, _dt(prec))
+    m = np.empty(tdf.shape, np.uint8)
+    _fn("candidates", prec)(_p(tdf), _p(m), ctypes.c_longlong(tdf.size), ctypes.c_float(thr))
+    return m
+
+
+def hp_compact(mask, method="sort"):
+    mask = np.ascontiguousarray(mask, np.uint8)
+    X, Y, Z = _dims(mask)
+    cap = int(np.count_nonzero(mask))
+    pos = np.zeros((max(cap, 1), 3), np.int32)
+    n = _fn(f"hp_compact_{method}")(_p(mask), X, Y, Z, _p(pos), ctypes.c_longlong(cap))
+    return pos[:n]
+
+
+def hp_key(x, y, z):
+    f = _fn("hp_key")
+    f.restype = ctypes.c_ulonglong
+    return f(int(x), int(y), int(z))
+
+
+def candidate_nms(tdf, radius, F, positions, maxd_cap, prec="f32"):
+    tdf = np.ascontiguousarray(tdf, _dt(prec))
+    radius = np.ascontiguousarray(radius, _dt(prec))
+    F = np.ascontiguousarray(F, _dt(prec))
+    pos = np.ascontiguousarray(positions, np.int32).reshape(-1, 3)
+    X, Y, Z = _dims(F)
+    out = np.zeros(tdf.shape, np.uint8)
+    _fn("candidate_nms", prec)(_p(tdf), _p(radius), _p(F), X, Y, Z, _p(pos),
+                               ctypes.c_longlong(pos.shape[0]), int(maxd_cap), _p(out))
+    return out
+
+
+def cube_max(tdf, cand, cube, prec="f32"):
+    tdf = np.ascontiguousarray(tdf, _dt(prec))
+    cand = np.ascontiguousarray(cand, np.uint8)
+    X, Y, Z = _dims(tdf)
+    out = np.zeros(tdf.shape, np.uint8)
+    _fn("cube_max", prec)(_p(tdf), _p(cand), X, Y, Z, int(cube), _p(out))
+    return out
+
+
+def do_eigen(Fx, Fy, Fz, pos, prec="f32"):
+    Fx, Fy, Fz = (np.ascontiguousarray(a, _dt(prec)) for a in (Fx, Fy, Fz))
+    X, Y, Z = _dims(Fx)
+    outs = [np.empty(3, _dt(prec)) for _ in range(4)]
+    _fn("do_eigen", prec)(_p(Fx), _p(Fy), _p(Fz), X, Y, Z, int(pos[0]), int(pos[1]), int(pos[2]),
+                          *[_p(o) for o in outs])
+    return tuple(outs)
+
+
+def trilinear(F, u, v, w, prec="f64"):
+    """Sample planes 0..2 of F at unnormalised (u, v, w) with the linear-filter rule."""
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    out = np.empty(3, _dt(prec))
+    ct = ctypes.c_float if prec == "f32" else ctypes.c_double
+    _fn("trilinear", prec)(_p(F), X, Y, Z, ct(u), ct(v), ct(w), _p(out))
+    return out

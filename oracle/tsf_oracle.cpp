@@ -1,0 +1,986 @@
+// CPU oracle for the volume-filter hot path of the Tube Segmentation Framework.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
+// compared against; it is never linked into, loaded by, or called from the
+// product path (tube-segmentation-framework_amd/).  Only tests/, the smoke
+// check in __graft_entry__.py and the cpu_baseline leg of bench.py use it.
+//
+// Every function restates, in plain scalar C++, the arithmetic of one function
+// of the reference (file:line citations are relative to the reference tree).
+// Each function is instantiated twice:
+//   *_f32  -- float arithmetic in the reference's operation order, compiled
+//             with -ffp-contract=off (no FMA) so it is a deterministic IEEE
+//             restatement of the reference's FP32 buffer path;
+//   *_f64  -- the same algorithm in double, used as the error reference for
+//             the reduced-precision storage forms.
+// Parameters that are part of the algorithm's definition (radius sequence,
+// the 32 circle directions, mu, Fmax, blur mask) stay float in both modes.
+//
+// Where the reference's behaviour is undefined or can hang, the oracle fixes a
+// rule and the HIP path follows the same rule:
+//   * tql2: the small-subdiagonal search stops at index n-1 (the reference
+//     relies on e[n-1] == 0; with NaN input it would read past the array),
+//     and the QL iteration is capped at TSF_QL_MAX_ITER sweeps (the reference
+//     has no cap, kernels_no_3d_write.cl:1973).
+//   * cube max (dd): only in-range voxels are scanned (the reference's last
+//     partial cells read clamped coordinates, kernels_no_3d_write.cl:983-1003;
+//     strict '>' makes the clamped duplicates lose, so the result is equal).
+//   * GS smoother: native_divide is replaced by an IEEE division.
+//
+// Pinning status of each function is listed in the header of tests/test_oracle.py.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+#define TSF_QL_MAX_ITER 100
+
+namespace {
+
+typedef long long i64;
+
+struct Dims {
+    int X, Y, Z;
+    i64 n() const { return (i64)X * Y * Z; }
+    i64 at(int x, int y, int z) const { return (i64)x + (i64)X * ((i64)y + (i64)Y * z); }
+};
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Nearest read with CLK_ADDRESS_CLAMP_TO_EDGE (kernels_no_3d_write.cl:1).
+template <typename T>
+static inline T rd_edge(const T* p, const Dims& d, int x, int y, int z) {
+    return p[d.at(clampi(x, 0, d.X - 1), clampi(y, 0, d.Y - 1), clampi(z, 0, d.Z - 1))];
+}
+// Nearest read with CLK_ADDRESS_CLAMP: zero outside (kernels_no_3d_write.cl:4).
+template <typename T>
+static inline T rd_zero(const T* p, const Dims& d, int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= d.X || y >= d.Y || z >= d.Z) return T(0);
+    return p[d.at(x, y, z)];
+}
+// Mirror remap used by every GVF / FMG kernel: 0 -> 2, >= N-1 -> N-3
+// (kernels_no_3d_write.cl:1759-1760, 2086-2087).
+static inline int remap(int p, int N) {
+    if (p == 0) return 2;
+    if (p >= N - 1) return N - 3;
+    return p;
+}
+
+// ---------------------------------------------------------------- blur (A1)
+// createBlurMask, tube-segmentation.cpp:188-211 (float arithmetic, expf).
+static void blur_mask(float sigma, float* mask, int* maskSize) {
+    int m = (int)std::ceil(sigma / 0.5f);
+    if (m < 1) m = 1;
+    if (m > 5) m = 5;
+    const int M = 2 * m + 1;
+    float sum = 0.0f;
+    for (int a = -m; a <= m; a++)
+        for (int b = -m; b <= m; b++)
+            for (int c = -m; c <= m; c++) {
+                float w = std::exp(-((float)(a * a + b * b + c * c) / (2 * sigma * sigma)));
+                sum += w;
+                mask[(a + m) + (b + m) * M + (c + m) * M * M] = w;
+            }
+    for (int i = 0; i < M * M * M; i++) mask[i] = mask[i] / sum;
+    *maskSize = m;
+}
+
+// blurVolumeWithGaussian, kernels_no_3d_write.cl:1442-1465.  Loop a (x) outer,
+// b (y), c (z) inner; clamp-to-edge reads.
+template <typename T>
+static void blur(const T* in, T* out, Dims d, int m, const float* mask) {
+    const int M = 2 * m + 1;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T sum = 0;
+                for (int a = -m; a <= m; a++)
+                    for (int b = -m; b <= m; b++)
+                        for (int c = -m; c <= m; c++)
+                            sum += (T)mask[(a + m) + (b + m) * M + (c + m) * M * M] *
+                                   rd_edge(in, d, x + a, y + b, z + c);
+                out[d.at(x, y, z)] = sum;
+            }
+}
+
+// ---------------------------------------------------------------- toFloat (A2)
+// toFloat, kernels_no_3d_write.cl:1414-1440.
+template <typename T>
+static void to_float(const double* in, T* out, i64 n, float mn, float mx) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < n; i++) {
+        T v = (T)(float)in[i];
+        v = v > (T)mx ? (T)mx : v;
+        v = v < (T)mn ? (T)mn : v;
+        out[i] = (v - (T)mn) / ((T)mx - (T)mn);
+    }
+}
+
+// ---------------------------------------------------------------- vector field (A3)
+// createVectorField + gradient, kernels_no_3d_write.cl:1470-1498 and 146-200.
+// field is SoA: plane 0..2 = F.xyz, plane 3 = w (= 1).
+template <typename T>
+static void vector_field(const T* I, T* field, Dims d, float Fmax, int sign) {
+    const i64 N = d.n();
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T gx = (T)0.5 * (rd_edge(I, d, x + 1, y, z) - rd_edge(I, d, x - 1, y, z));
+                T gy = (T)0.5 * (rd_edge(I, d, x, y + 1, z) - rd_edge(I, d, x, y - 1, z));
+                T gz = (T)0.5 * (rd_edge(I, d, x, y, z + 1) - rd_edge(I, d, x, y, z - 1));
+                gx = (T)sign * gx; gy = (T)sign * gy; gz = (T)sign * gz;
+                T l = std::sqrt(gx * gx + gy * gy + gz * gz);
+                T s = l < (T)Fmax ? (T)Fmax : l;
+                i64 i = d.at(x, y, z);
+                field[i] = gx / s;
+                field[N + i] = gy / s;
+                field[2 * N + i] = gz / s;
+                field[3 * N + i] = (T)1;
+            }
+}
+
+// ---------------------------------------------------------------- GVF Euler (A4)
+// One Jacobi step of GVF3DIteration, kernels_no_3d_write.cl:1746-1787.
+// f and v are 3-plane SoA.  The data-term magnitude |f(q)|^2 is recomputed.
+template <typename T>
+static void gvf_step(const T* f, const T* v, T* out, Dims d, T mu) {
+    const i64 N = d.n();
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                const int qx = remap(x, d.X), qy = remap(y, d.Y), qz = remap(z, d.Z);
+                const i64 q = d.at(qx, qy, qz);
+                const T ix = f[q], iy = f[N + q], iz = f[2 * N + q];
+                const T m2 = ix * ix + iy * iy + iz * iz;
+                const i64 w = d.at(x, y, z);
+                const i64 sx = 1, sy = d.X, sz = (i64)d.X * d.Y;
+                for (int c = 0; c < 3; c++) {
+                    const T* p = v + c * N;
+                    const T vc = p[q];
+                    T lap = (T)-6 * vc;
+                    lap = lap + p[q + sx];
+                    lap = lap + p[q - sx];
+                    lap = lap + p[q + sy];
+                    lap = lap + p[q - sy];
+                    lap = lap + p[q + sz];
+                    lap = lap + p[q - sz];
+                    const T fc = c == 0 ? ix : (c == 1 ? iy : iz);
+                    out[c * N + w] = vc + (mu * lap - (vc - fc) * m2);
+                }
+            }
+}
+
+// GVF3DFinish, kernels_no_3d_write.cl:1798-1807: w = |v| if > 0 else 1.
+template <typename T>
+static void gvf_finish(T* v, Dims d) {
+    const i64 N = d.n();
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < N; i++) {
+        T l = std::sqrt(v[i] * v[i] + v[N + i] * v[N + i] + v[2 * N + i] * v[2 * N + i]);
+        v[3 * N + i] = l > (T)0 ? l : (T)1;
+    }
+}
+
+// runFastGVF, gradientVectorFlow.cpp:1007-1110: v0 = f, `iters` Jacobi steps.
+template <typename T>
+static void gvf_euler(const T* f4, T* v4, Dims d, float mu, int iters) {
+    const i64 N = d.n();
+    std::vector<T> a(f4, f4 + 3 * N), b(3 * N);
+    for (int it = 0; it < iters; it++) {
+        gvf_step(f4, a.data(), b.data(), d, (T)mu);
+        a.swap(b);
+    }
+    std::memcpy(v4, a.data(), sizeof(T) * 3 * N);
+    gvf_finish(v4, d);
+}
+
+// ---------------------------------------------------------------- FMG GVF (A7)
+// Multigrid solver of gradientVectorFlow.cpp:5-441, 631-1005 with the kernels
+// of kernels_no_3d_write.cl:2071-2354 (FP32 path: no SNORM16 quantisation).
+template <typename T>
+struct FMG {
+    T mu;
+    int v0, v1, v2, lmax;
+
+    // calculateNewSize, gradientVectorFlow.cpp:371-395: cubic, next power of
+    // two of the largest dimension, halved.
+    static Dims coarser(Dims d) {
+        int mx = std::max(d.X, std::max(d.Y, d.Z));
+        int p = 1;
+        while (p < mx) p *= 2;
+        int s = std::max(1, p / 2);
+        return Dims{s, s, s};
+    }
+
+    // GVFgaussSeidel + GVFgaussSeidel2 (red then black), cl:2071-2150.
+    void smooth(std::vector<T>& v, const std::vector<T>& r, const std::vector<T>& s, int iters,
+                Dims d, T h) {
+        const i64 N = d.n();
+        std::vector<T> v2(N, (T)0), out(N);
+        const i64 sx = 1, sy = d.X, sz = (i64)d.X * d.Y;
+        for (int it = 0; it < iters; it++) {
+            // red
+#pragma omp parallel for schedule(static)
+            for (int z = 0; z < d.Z; z++)
+                for (int y = 0; y < d.Y; y++)
+                    for (int x = 0; x < d.X; x++) {
+                        int px = remap(x, d.X), py = remap(y, d.Y), pz = remap(z, d.Z);
+                        if ((px + py + pz) % 2 != 0) continue;
+                        i64 q = d.at(px, py, pz);
+                        T nb = v[q + sx];
+                        nb = nb + v[q - sx];
+                        nb = nb + v[q + sy];
+                        nb = nb + v[q - sy];
+                        nb = nb + v[q + sz];
+                        nb = nb + v[q - sz];
+                        T num = (T)2 * mu * nb - (T)2 * h * h * r[q];
+                        T den = (T)12 * mu + h * h * s[q];
+                        v2[d.at(x, y, z)] = num / den;
+                    }
+            // black (red cells copy the value at the remapped position)
+#pragma omp parallel for schedule(static)
+            for (int z = 0; z < d.Z; z++)
+                for (int y = 0; y < d.Y; y++)
+                    for (int x = 0; x < d.X; x++) {
+                        int px = remap(x, d.X), py = remap(y, d.Y), pz = remap(z, d.Z);
+                        i64 q = d.at(px, py, pz);
+                        T val;
+                        if ((px + py + pz) % 2 == 0) {
+                            val = v2[q];
+                        } else {
+                            T nb = v2[q + sx];
+                            nb = nb + v2[q - sx];
+                            nb = nb + v2[q + sy];
+                            nb = nb + v2[q - sy];
+                            nb = nb + v2[q + sz];
+                            nb = nb + v2[q - sz];
+                            T num = (T)2 * mu * nb - (T)2 * h * h * r[q];
+                            T den = (T)12 * mu + h * h * s[q];
+                            val = num / den;
+                        }
+                        out[d.at(x, y, z)] = val;
+                    }
+            v.swap(out);
+        }
+    }
+
+    // residual, cl:2268-2301.
+    std::vector<T> residual(const std::vector<T>& r, const std::vector<T>& v, const std::vector<T>& s,
+                            Dims d, T h) {
+        std::vector<T> p(d.n());
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < d.Z; z++)
+            for (int y = 0; y < d.Y; y++)
+                for (int x = 0; x < d.X; x++) {
+                    int px = remap(x, d.X), py = remap(y, d.Y), pz = remap(z, d.Z);
+                    T c = rd_zero(v.data(), d, px, py, pz);
+                    T nb = rd_zero(v.data(), d, px + 1, py, pz);
+                    nb = nb + rd_zero(v.data(), d, px - 1, py, pz);
+                    nb = nb + rd_zero(v.data(), d, px, py + 1, pz);
+                    nb = nb + rd_zero(v.data(), d, px, py - 1, pz);
+                    nb = nb + rd_zero(v.data(), d, px, py, pz + 1);
+                    nb = nb + rd_zero(v.data(), d, px, py, pz - 1);
+                    T lap = (mu * (nb - (T)6 * c)) / (h * h);
+                    p[d.at(x, y, z)] = rd_zero(r.data(), d, px, py, pz) -
+                                       (lap - rd_zero(s.data(), d, px, py, pz) * c);
+                }
+        return p;
+    }
+
+    // restrictVolume, cl:2218-2246: 2x2x2 mean, zero outside the fine grid.
+    static std::vector<T> restrict_(const std::vector<T>& v, Dims fine, Dims coarse) {
+        std::vector<T> o(coarse.n());
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < coarse.Z; z++)
+            for (int y = 0; y < coarse.Y; y++)
+                for (int x = 0; x < coarse.X; x++) {
+                    int X = 2 * x, Y = 2 * y, Zz = 2 * z;
+                    const T* p = v.data();
+                    T s = rd_zero(p, fine, X, Y, Zz);
+                    s = s + rd_zero(p, fine, X + 1, Y, Zz);
+                    s = s + rd_zero(p, fine, X, Y + 1, Zz);
+                    s = s + rd_zero(p, fine, X, Y, Zz + 1);
+                    s = s + rd_zero(p, fine, X + 1, Y + 1, Zz);
+                    s = s + rd_zero(p, fine, X, Y + 1, Zz + 1);
+                    s = s + rd_zero(p, fine, X + 1, Y + 1, Zz + 1);
+                    s = s + rd_zero(p, fine, X + 1, Y, Zz + 1);
+                    o[coarse.at(x, y, z)] = (T)0.125 * s;
+                }
+        return o;
+    }
+
+    // prolongate (add = true) and prolongate2 (add = false), cl:2248-2266:
+    // injection from floor(p/2), zero outside the coarse grid.
+    static std::vector<T> prolong(const std::vector<T>* fineAdd, const std::vector<T>& c, Dims fine,
+                                  Dims coarse) {
+        std::vector<T> o(fine.n());
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < fine.Z; z++)
+            for (int y = 0; y < fine.Y; y++)
+                for (int x = 0; x < fine.X; x++) {
+                    T cv = rd_zero(c.data(), coarse, x / 2, y / 2, z / 2);
+                    i64 i = fine.at(x, y, z);
+                    o[i] = fineAdd ? (*fineAdd)[i] + cv : cv;
+                }
+        return o;
+    }
+
+    // multigridVcycle, gradientVectorFlow.cpp:397-441.
+    void vcycle(const std::vector<T>& r, std::vector<T>& v, const std::vector<T>& s, int l, Dims d, T h) {
+        smooth(v, r, s, v1, d, h);
+        if (l < lmax) {
+            Dims c = coarser(d);
+            std::vector<T> p = residual(r, v, s, d, h);
+            std::vector<T> r1 = restrict_(p, d, c);
+            std::vector<T> s1 = restrict_(s, d, c);
+            std::vector<T> vc(c.n(), (T)0);
+            vcycle(r1, vc, s1, l + 1, c, h * 2);
+            v = prolong(&v, vc, d, c);
+        }
+        smooth(v, r, s, v2, d, h);
+    }
+
+    // fullMultigrid, gradientVectorFlow.cpp:694-727.
+    std::vector<T> fmg(const std::vector<T>& r, const std::vector<T>& s, int l, Dims d, T h) {
+        std::vector<T> v;
+        if (l < lmax) {
+            Dims c = coarser(d);
+            std::vector<T> r1 = restrict_(r, d, c);
+            std::vector<T> s1 = restrict_(s, d, c);
+            std::vector<T> vc = fmg(r1, s1, l + 1, c, h * 2);
+            v = prolong(nullptr, vc, d, c);
+        } else {
+            v.assign(d.n(), (T)0);
+        }
+        for (int i = 0; i < v0; i++) vcycle(r, v, s, l, d, h);
+        return v;
+    }
+
+    // fmgResidual, cl:2303-2348 (h = 1 at the finest level).
+    static std::vector<T> fmg_residual(const T* F, const std::vector<T>& f, Dims d, T mu, int comp) {
+        const i64 N = d.n();
+        std::vector<T> r(N);
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < d.Z; z++)
+            for (int y = 0; y < d.Y; y++)
+                for (int x = 0; x < d.X; x++) {
+                    int px = remap(x, d.X), py = remap(y, d.Y), pz = remap(z, d.Z);
+                    i64 q = d.at(px, py, pz);
+                    T fx = F[q], fy = F[N + q], fz = F[2 * N + q];
+                    T v0c = comp == 0 ? fx : (comp == 1 ? fy : fz);
+                    T sq = fx * fx + fy * fy + fz * fz;
+                    T c = rd_zero(f.data(), d, px, py, pz);
+                    T nb = rd_zero(f.data(), d, px + 1, py, pz);
+                    nb = nb + rd_zero(f.data(), d, px - 1, py, pz);
+                    nb = nb + rd_zero(f.data(), d, px, py + 1, pz);
+                    nb = nb + rd_zero(f.data(), d, px, py - 1, pz);
+                    nb = nb + rd_zero(f.data(), d, px, py, pz + 1);
+                    nb = nb + rd_zero(f.data(), d, px, py, pz - 1);
+                    T residue = (mu * (nb - (T)6 * c)) / ((T)1 * (T)1);
+                    r[d.at(x, y, z)] = -sq * v0c - (residue - sq * c);
+                }
+        return r;
+    }
+};
+
+// runFMGGVF, gradientVectorFlow.cpp:729-1005.
+template <typename T>
+static void gvf_fmg(const T* F, T* out, Dims d, float mu, int iters, int v0, int v1, int v2, int lmax) {
+    const i64 N = d.n();
+    FMG<T> M{(T)mu, v0, v1, v2, lmax};
+    std::vector<T> s(N);
+    for (i64 i = 0; i < N; i++) s[i] = F[i] * F[i] + F[N + i] * F[N + i] + F[2 * N + i] * F[2 * N + i];
+    for (int c = 0; c < 3; c++) {
+        std::vector<T> f(N, (T)0);
+        for (int it = 0; it < iters; it++) {
+            std::vector<T> r = FMG<T>::fmg_residual(F, f, d, (T)mu, c);
+            std::vector<T> e = M.fmg(r, s, 0, d, (T)1);
+            for (i64 i = 0; i < N; i++) f[i] = f[i] + e[i];
+        }
+        std::memcpy(out + c * N, f.data(), sizeof(T) * N);
+    }
+    // MGGVFFinish, cl:2202-2216: w = |v| with no zero guard.
+    for (i64 i = 0; i < N; i++)
+        out[3 * N + i] = std::sqrt(out[i] * out[i] + out[N + i] * out[N + i] + out[2 * N + i] * out[2 * N + i]);
+}
+
+// ---------------------------------------------------------------- eigen (A9)
+// Householder tridiagonalisation followed by the implicit QL method with
+// eigenvector accumulation, as in the EISPACK routines tred2 and tql2 (Bowdler,
+// Martin, Reinsch, Wilkinson; Handbook for Automatic Computation II) in their
+// public-domain JAMA form, n = 3.  The reference's copy is at
+// kernels_no_3d_write.cl:1824-2068 (CPU copy eigenanalysisOfHessian.cpp:10-260).
+// The operation order below is the one those routines define, so that a float
+// instantiation rounds exactly like the reference's float code.
+template <typename T>
+static void householder3(T V[3][3], T d[3], T e[3]) {
+    const int n = 3;
+    for (int j = 0; j < n; j++) d[j] = V[n - 1][j];
+    for (int i = n - 1; i > 0; i--) {
+        T scale = 0, h = 0;
+        for (int k = 0; k < i; k++) scale = scale + std::fabs(d[k]);
+        if (scale == (T)0) {
+            e[i] = d[i - 1];
+            for (int j = 0; j < i; j++) {
+                d[j] = V[i - 1][j];
+                V[i][j] = 0;
+                V[j][i] = 0;
+            }
+        } else {
+            for (int k = 0; k < i; k++) {
+                d[k] /= scale;
+                h += d[k] * d[k];
+            }
+            T f = d[i - 1];
+            T g = std::sqrt(h);
+            if (f > 0) g = -g;
+            e[i] = scale * g;
+            h = h - f * g;
+            d[i - 1] = f - g;
+            for (int j = 0; j < i; j++) e[j] = 0;
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                V[j][i] = f;
+                g = e[j] + V[j][j] * f;
+                for (int k = j + 1; k <= i - 1; k++) {
+                    g += V[k][j] * d[k];
+                    e[k] += V[k][j] * f;
+                }
+                e[j] = g;
+            }
+            f = 0;
+            for (int j = 0; j < i; j++) {
+                e[j] /= h;
+                f += e[j] * d[j];
+            }
+            T hh = f / (h + h);
+            for (int j = 0; j < i; j++) e[j] -= hh * d[j];
+            for (int j = 0; j < i; j++) {
+                f = d[j];
+                g = e[j];
+                for (int k = j; k <= i - 1; k++) V[k][j] -= (f * e[k] + g * d[k]);
+                d[j] = V[i - 1][j];
+                V[i][j] = 0;
+            }
+        }
+        d[i] = h;
+    }
+    for (int i = 0; i < n - 1; i++) {
+        V[n - 1][i] = V[i][i];
+        V[i][i] = 1;
+        T h = d[i + 1];
+        if (h != (T)0) {
+            for (int k = 0; k <= i; k++) d[k] = V[k][i + 1] / h;
+            for (int j = 0; j <= i; j++) {
+                T g = 0;
+                for (int k = 0; k <= i; k++) g += V[k][i + 1] * V[k][j];
+                for (int k = 0; k <= i; k++) V[k][j] -= g * d[k];
+            }
+        }
+        for (int k = 0; k <= i; k++) V[k][i + 1] = 0;
+    }
+    for (int j = 0; j < n; j++) {
+        d[j] = V[n - 1][j];
+        V[n - 1][j] = 0;
+    }
+    V[n - 1][n - 1] = 1;
+    e[0] = 0;
+}
+
+template <typename T>
+static inline T hyp(T a, T b) { return std::sqrt(a * a + b * b); }
+
+template <typename T>
+static void ql_implicit3(T V[3][3], T d[3], T e[3]) {
+    const int n = 3;
+    for (int i = 1; i < n; i++) e[i - 1] = e[i];
+    e[n - 1] = 0;
+    T f = 0, tst1 = 0;
+    const T eps = (T)std::ldexp(1.0, -52);
+    for (int l = 0; l < n; l++) {
+        tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+        int m = l;
+        while (m < n - 1 && !(std::fabs(e[m]) <= eps * tst1)) m++;
+        if (m > l) {
+            int iter = 0;
+            do {
+                iter++;
+                T g = d[l];
+                T p = (d[l + 1] - g) / ((T)2 * e[l]);
+                T r = hyp(p, (T)1);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                T dl1 = d[l + 1];
+                T h = g - d[l];
+                for (int i = l + 2; i < n; i++) d[i] -= h;
+                f = f + h;
+                p = d[m];
+                T c = 1, c2 = c, c3 = c;
+                T el1 = e[l + 1];
+                T s = 0, s2 = 0;
+                for (int i = m - 1; i >= l; i--) {
+                    c3 = c2;
+                    c2 = c;
+                    s2 = s;
+                    g = c * e[i];
+                    h = c * p;
+                    r = hyp(p, e[i]);
+                    e[i + 1] = s * r;
+                    s = e[i] / r;
+                    c = p / r;
+                    p = c * d[i] - s * g;
+                    d[i + 1] = h + s * (c * g + s * d[i]);
+                    for (int k = 0; k < n; k++) {
+                        h = V[k][i + 1];
+                        V[k][i + 1] = s * V[k][i] + c * h;
+                        V[k][i] = c * V[k][i] - s * h;
+                    }
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
+            } while (std::fabs(e[l]) > eps * tst1 && iter < TSF_QL_MAX_ITER);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0;
+    }
+    // Selection sort by ascending |lambda| with column swaps.
+    for (int i = 0; i < n - 1; i++) {
+        int k = i;
+        T p = d[i];
+        for (int j = i + 1; j < n; j++)
+            if (std::fabs(d[j]) < std::fabs(p)) {
+                k = j;
+                p = d[j];
+            }
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int j = 0; j < n; j++) {
+                p = V[j][i];
+                V[j][i] = V[j][k];
+                V[j][k] = p;
+            }
+        }
+    }
+}
+
+template <typename T>
+static void eigen3(const T A[3][3], T V[3][3], T d[3]) {
+    T e[3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) V[i][j] = A[i][j];
+    householder3(V, d, e);
+    ql_implicit3(V, d, e);
+}
+
+// ---------------------------------------------------------------- Hessian (A8)
+// gradient / gradientNormalized on the vector field (kernels_no_3d_write.cl:90-200)
+// and the matrix layout of circleFittingTDF (1522-1526).  With normalize, each
+// sample is divided by the w plane of the sampled voxel.  Clamp-to-edge reads.
+template <typename T>
+static void hessian_at(const T* F, Dims d, int x, int y, int z, bool normalize, T H[3][3]) {
+    const i64 N = d.n();
+    auto s = [&](int c, int xx, int yy, int zz) -> T {
+        T v = rd_edge(F + c * N, d, xx, yy, zz);
+        if (normalize) v = v / rd_edge(F + 3 * N, d, xx, yy, zz);
+        return v;
+    };
+    T fxx = (T)0.5 * (s(0, x + 1, y, z) - s(0, x - 1, y, z));
+    T fyx = (T)0.5 * (s(1, x + 1, y, z) - s(1, x - 1, y, z));
+    T fyy = (T)0.5 * (s(1, x, y + 1, z) - s(1, x, y - 1, z));
+    T fzx = (T)0.5 * (s(2, x + 1, y, z) - s(2, x - 1, y, z));
+    T fzy = (T)0.5 * (s(2, x, y + 1, z) - s(2, x, y - 1, z));
+    T fzz = (T)0.5 * (s(2, x, y, z + 1) - s(2, x, y, z - 1));
+    H[0][0] = fxx; H[0][1] = fyx; H[0][2] = fzx;
+    H[1][0] = fyx; H[1][1] = fyy; H[1][2] = fzy;
+    H[2][0] = fzx; H[2][1] = fzy; H[2][2] = fzz;
+}
+
+template <typename T>
+static void hessian_eigen(const T* F, Dims d, int normalize, T* lam, T* vec) {
+    const i64 N = d.n();
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T H[3][3], V[3][3], ev[3];
+                hessian_at(F, d, x, y, z, normalize != 0, H);
+                eigen3(H, V, ev);
+                i64 i = d.at(x, y, z);
+                for (int k = 0; k < 3; k++) lam[k * N + i] = ev[k];
+                if (vec)  // plane 3*j + k = component k of eigenvector j
+                    for (int j = 0; j < 3; j++)
+                        for (int k = 0; k < 3; k++) vec[(3 * j + k) * N + i] = V[k][j];
+            }
+}
+
+// ---------------------------------------------------------------- TDF (A11)
+// The 32 directions are taken at integer radians j = 0..31 with the tables
+// rounded to 6 significant digits ("%g"), as in kernels_no_3d_write.cl:1500-1501.
+static float g_cos[32], g_sin[32];
+static bool g_tab = false;
+static void make_tables() {
+    if (g_tab) return;
+    char buf[64];
+    for (int j = 0; j < 32; j++) {
+        snprintf(buf, sizeof buf, "%g", std::cos((double)j));
+        g_cos[j] = strtof(buf, nullptr);
+        snprintf(buf, sizeof buf, "%g", std::sin((double)j));
+        g_sin[j] = strtof(buf, nullptr);
+    }
+    g_tab = true;
+}
+
+// OpenCL CLK_FILTER_LINEAR with unnormalised coordinates and clamp-to-edge
+// (OpenCL 1.2 spec 8.2): u' = u - 0.5, i0 = floor(u'), a = u' - i0, indices
+// clamped to [0, N-1]; the eight terms are summed in the spec's order.
+template <typename T>
+static inline void trilinear3(const T* F, Dims d, T u, T v, T w, T out[3]) {
+    const i64 N = d.n();
+    T uu = u - (T)0.5, vv = v - (T)0.5, ww = w - (T)0.5;
+    T fu = std::floor(uu), fv = std::floor(vv), fw = std::floor(ww);
+    T a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1); j0 = clampi(j0, 0, d.Y - 1); k0 = clampi(k0, 0, d.Z - 1);
+    T wt[8] = {((T)1 - a) * ((T)1 - b) * ((T)1 - c), a * ((T)1 - b) * ((T)1 - c),
+               ((T)1 - a) * b * ((T)1 - c),          a * b * ((T)1 - c),
+               ((T)1 - a) * ((T)1 - b) * c,          a * ((T)1 - b) * c,
+               ((T)1 - a) * b * c,                    a * b * c};
+    i64 idx[8] = {d.at(i0, j0, k0), d.at(i1, j0, k0), d.at(i0, j1, k0), d.at(i1, j1, k0),
+                  d.at(i0, j0, k1), d.at(i1, j0, k1), d.at(i0, j1, k1), d.at(i1, j1, k1)};
+    for (int comp = 0; comp < 3; comp++) {
+        const T* p = F + comp * N;
+        T s = wt[0] * p[idx[0]];
+        for (int k = 1; k < 8; k++) s = s + wt[k] * p[idx[k]];
+        out[comp] = s;
+    }
+}
+
+// circleFittingTDF, kernels_no_3d_write.cl:1503-1572.
+template <typename T>
+static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, T* tdf, T* rad) {
+    make_tables();
+    const bool normalize = !(rMax < 4);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T H[3][3], V[3][3], ev[3];
+                hessian_at(F, d, x, y, z, normalize, H);
+                eigen3(H, V, ev);
+                const T e2[3] = {V[0][1], V[1][1], V[2][1]};
+                const T e3[3] = {V[0][2], V[1][2], V[2][2]};
+                T maxSum = 0, maxRadius = 0;
+                for (float radius = rMin; radius <= rMax; radius += rStep) {
+                    T sum = 0;
+                    for (int j = 0; j < 32; j++) {
+                        T va[3];
+                        for (int k = 0; k < 3; k++) va[k] = (T)g_cos[j] * e3[k] + (T)g_sin[j] * e2[k];
+                        T s[3];
+                        trilinear3(F, d, (T)x + (T)radius * va[0], (T)y + (T)radius * va[1],
+                                   (T)z + (T)radius * va[2], s);
+                        T dp = (-s[0]) * va[0] + (-s[1]) * va[1] + (-s[2]) * va[2];
+                        sum += dp;
+                    }
+                    sum /= (T)32;
+                    if (sum > maxSum) {
+                        maxSum = sum;
+                        maxRadius = (T)radius;
+                    } else {
+                        break;
+                    }
+                }
+                i64 i = d.at(x, y, z);
+                tdf[i] = maxSum;
+                rad[i] = maxRadius;
+            }
+}
+
+// ---------------------------------------------------------------- combine / candidates (A12, A13)
+// combine, kernels_no_3d_write.cl:1087-1098.
+template <typename T>
+static void combine(const T* ts, const T* rs, T* tl, T* rl, i64 n) {
+    for (i64 i = 0; i < n; i++)
+        if (tl[i] < ts[i]) {
+            tl[i] = ts[i];
+            rl[i] = rs[i];
+        }
+}
+// findCandidateCenterpoints, kernels_no_3d_write.cl:1006-1017.
+template <typename T>
+static void candidates(const T* tdf, uint8_t* mask, i64 n, float thr) {
+    for (i64 i = 0; i < n; i++) mask[i] = tdf[i] < (T)thr ? 0 : 1;
+}
+
+// ---------------------------------------------------------------- NMS (A16)
+// findCandidateCenterpoints2, kernels_no_3d_write.cl:1019-1084 (maxD cap 5 on
+// the buffer path, 8 on the image path, kernels.cl:1334).
+template <typename T>
+static void candidate_nms(const T* tdf, const T* radius, const T* F, Dims d, const int32_t* pos, i64 count,
+                          int maxDcap, uint8_t* out) {
+    (void)tdf;
+    const i64 N = d.n();
+    const T* W = F + 3 * N;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (i64 t = 0; t < count; t++) {
+        const int x = pos[3 * t], y = pos[3 * t + 1], z = pos[3 * t + 2];
+        const float rr = (float)rd_edge(radius, d, x, y, z);
+        const int maxD = (int)std::max(std::min(std::round(rr), (float)maxDcap), 1.0f);
+        T H[3][3], V[3][3], ev[3];
+        hessian_at(F, d, x, y, z, true, H);
+        eigen3(H, V, ev);
+        const T e1[3] = {V[0][0], V[1][0], V[2][0]};
+        const T wp = rd_edge(W, d, x, y, z);
+        bool invalid = false;
+        for (int a = -maxD; a <= maxD; a++)
+            for (int b = -maxD; b <= maxD; b++)
+                for (int c = -maxD; c <= maxD; c++) {
+                    if (a == 0 && b == 0 && c == 0) continue;
+                    T r[3] = {(T)a, (T)b, (T)c};
+                    T dp = e1[0] * r[0] + e1[1] * r[1] + e1[2] * r[2];
+                    T rp[3] = {r[0] - e1[0] * dp, r[1] - e1[1] * dp, r[2] - e1[2] * dp};
+                    T lr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                    T lp = std::sqrt(rp[0] * rp[0] + rp[1] * rp[1] + rp[2] * rp[2]);
+                    T cs = (r[0] / lr) * (rp[0] / lp) + (r[1] / lr) * (rp[1] / lp) + (r[2] / lr) * (rp[2] / lp);
+                    T theta = std::acos(cs);
+                    if (theta < (T)0.5f && lr < (T)maxD) {
+                        if (rd_edge(W, d, x + a, y + b, z + c) < wp) invalid = true;
+                    }
+                }
+        out[d.at(x, y, z)] = invalid ? 0 : 1;
+    }
+}
+
+// ---------------------------------------------------------------- cube max (A17)
+// dd, kernels_no_3d_write.cl:970-1004 (in-range voxels only, see header).
+template <typename T>
+static void cube_max(const T* tdf, const uint8_t* cand, Dims d, int cube, uint8_t* out) {
+    const int cx = (d.X + cube - 1) / cube, cy = (d.Y + cube - 1) / cube, cz = (d.Z + cube - 1) / cube;
+    for (int gz = 0; gz < cz; gz++)
+        for (int gy = 0; gy < cy; gy++)
+            for (int gx = 0; gx < cx; gx++) {
+                T best = 0;
+                i64 bi = -1;
+                for (int a = 0; a < cube; a++)
+                    for (int b = 0; b < cube; b++)
+                        for (int c = 0; c < cube; c++) {
+                            int x = gx * cube + a, y = gy * cube + b, z = gz * cube + c;
+                            if (x >= d.X || y >= d.Y || z >= d.Z) continue;
+                            i64 i = d.at(x, y, z);
+                            if (cand[i] == 1 && tdf[i] > best) {
+                                best = tdf[i];
+                                bi = i;
+                            }
+                        }
+                if (bi >= 0) out[bi] = 1;
+            }
+}
+
+// ---------------------------------------------------------------- HP compaction (A14, A15)
+// Histogram-pyramid order: at every level the eight children are visited in
+// the order (0,0,0),(1,0,0),(0,0,1),(1,0,1),(0,1,0),(1,1,0),(0,1,1),(1,1,1)
+// (cubeOffsets, kernels_no_3d_write.cl:78-87), i.e. child k = x + 2z + 4y.
+static const int kOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                               {0, 1, 0}, {1, 1, 0}, {0, 1, 1}, {1, 1, 1}};
+
+static inline uint64_t hp_key(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t k = 0;
+    for (int b = 0; b < 21; b++) {
+        uint64_t dgt = ((x >> b) & 1u) + 2u * ((z >> b) & 1u) + 4u * ((y >> b) & 1u);
+        k |= dgt << (3 * b);
+    }
+    return k;
+}
+
+// Method 1: sort the set voxels by the pyramid key.
+static i64 hp_compact_sort(const uint8_t* mask, Dims d, int32_t* pos, i64 cap) {
+    std::vector<std::pair<uint64_t, i64>> v;
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++)
+                if (mask[d.at(x, y, z)]) v.push_back({hp_key(x, y, z), d.at(x, y, z)});
+    std::sort(v.begin(), v.end());
+    i64 n = (i64)v.size();
+    for (i64 t = 0; t < n && t < cap; t++) {
+        i64 i = v[t].second;
+        pos[3 * t] = (int32_t)(i % d.X);
+        pos[3 * t + 1] = (int32_t)((i / d.X) % d.Y);
+        pos[3 * t + 2] = (int32_t)(i / ((i64)d.X * d.Y));
+    }
+    return n;
+}
+
+// Method 2: build the pyramid literally (constructHPLevel*, cl:246-347: each
+// cell is the sum of its 8 children, zero outside) and descend it for every
+// target (scanHPLevel* / traverseHP3DBuffer, cl:393-616; target >= sum -> 0).
+static i64 hp_compact_pyramid(const uint8_t* mask, Dims d, int32_t* pos, i64 cap) {
+    int size = 8;
+    while (size < d.X || size < d.Y || size < d.Z) size *= 2;
+    int L = 0;
+    while ((1 << L) < size) L++;
+    // level l has size >> l cells per axis; level 0 is the mask padded with zeros
+    std::vector<std::vector<i64>> lev(L + 1);
+    lev[0].assign((i64)size * size * size, 0);
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++)
+                lev[0][(i64)x + (i64)size * (y + (i64)size * z)] = mask[d.at(x, y, z)] ? 1 : 0;
+    for (int l = 1; l <= L; l++) {
+        int s = size >> l, sp = size >> (l - 1);
+        lev[l].assign((i64)s * s * s, 0);
+        for (int z = 0; z < s; z++)
+            for (int y = 0; y < s; y++)
+                for (int x = 0; x < s; x++) {
+                    i64 acc = 0;
+                    for (int k = 0; k < 8; k++)
+                        acc += lev[l - 1][(i64)(2 * x + kOff[k][0]) +
+                                          (i64)sp * ((2 * y + kOff[k][1]) + (i64)sp * (2 * z + kOff[k][2]))];
+                    lev[l][(i64)x + (i64)s * (y + (i64)s * z)] = acc;
+                }
+    }
+    const i64 sum = lev[L][0];
+    for (i64 t0 = 0; t0 < sum && t0 < cap; t0++) {
+        i64 target = t0;
+        int px = 0, py = 0, pz = 0;
+        i64 base = 0;
+        for (int l = L - 1; l >= 0; l--) {
+            int s = size >> l;
+            i64 nb[8];
+            for (int k = 0; k < 8; k++)
+                nb[k] = lev[l][(i64)(px + kOff[k][0]) + (i64)s * ((py + kOff[k][1]) + (i64)s * (pz + kOff[k][2]))];
+            i64 acc = base;
+            int sel = 0;
+            for (int k = 0; k < 7; k++) {
+                acc += nb[k];
+                if (acc <= target) sel++;
+            }
+            i64 add = 0;
+            for (int k = 0; k < sel; k++) add += nb[k];
+            base += add;
+            px = 2 * (px + kOff[sel][0]);
+            py = 2 * (py + kOff[sel][1]);
+            pz = 2 * (pz + kOff[sel][2]);
+        }
+        pos[3 * t0] = px / 2;
+        pos[3 * t0 + 1] = py / 2;
+        pos[3 * t0 + 2] = pz / 2;
+    }
+    return sum;
+}
+
+// ---------------------------------------------------------------- doEigen (A10)
+// eigenanalysisOfHessian.cpp:264-389: every sample is divided by the magnitude
+// sqrt(Fx^2+Fy^2+Fz^2) computed on the host from the sampled voxel; there is no
+// clamping (callers keep pos one voxel away from the border).
+template <typename T>
+static void do_eigen(const T* Fx, const T* Fy, const T* Fz, Dims d, int x, int y, int z, T* lam, T* e1, T* e2,
+                     T* e3) {
+    auto nv = [&](const T* c, int xx, int yy, int zz) -> T {
+        i64 i = d.at(xx, yy, zz);
+        return c[i] / std::sqrt(Fx[i] * Fx[i] + Fy[i] * Fy[i] + Fz[i] * Fz[i]);
+    };
+    T fxx = (T)0.5 * (nv(Fx, x + 1, y, z) - nv(Fx, x - 1, y, z));
+    T fyx = (T)0.5 * (nv(Fy, x + 1, y, z) - nv(Fy, x - 1, y, z));
+    T fyy = (T)0.5 * (nv(Fy, x, y + 1, z) - nv(Fy, x, y - 1, z));
+    T fzx = (T)0.5 * (nv(Fz, x + 1, y, z) - nv(Fz, x - 1, y, z));
+    T fzy = (T)0.5 * (nv(Fz, x, y + 1, z) - nv(Fz, x, y - 1, z));
+    T fzz = (T)0.5 * (nv(Fz, x, y, z + 1) - nv(Fz, x, y, z - 1));
+    T H[3][3] = {{fxx, fyx, fzx}, {fyx, fyy, fzy}, {fzx, fzy, fzz}};
+    T V[3][3], ev[3];
+    eigen3(H, V, ev);
+    for (int k = 0; k < 3; k++) {
+        lam[k] = ev[k];
+        e1[k] = V[k][0];
+        e2[k] = V[k][1];
+        e3[k] = V[k][2];
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C exports
+#define DIMS Dims{X, Y, Z}
+#define EXPORT_T(T, S)                                                                                       \
+    extern "C" void orc_blur_##S(const T* in, T* out, int X, int Y, int Z, int m, const float* mask) {      \
+        blur<T>(in, out, DIMS, m, mask);                                                                     \
+    }                                                                                                        \
+    extern "C" void orc_to_float_##S(const double* in, T* out, long long n, float mn, float mx) {            \
+        to_float<T>(in, out, n, mn, mx);                                                                     \
+    }                                                                                                        \
+    extern "C" void orc_vector_field_##S(const T* I, T* F, int X, int Y, int Z, float Fmax, int sign) {      \
+        vector_field<T>(I, F, DIMS, Fmax, sign);                                                             \
+    }                                                                                                        \
+    extern "C" void orc_gvf_euler_##S(const T* f4, T* v4, int X, int Y, int Z, float mu, int iters) {       \
+        gvf_euler<T>(f4, v4, DIMS, mu, iters);                                                               \
+    }                                                                                                        \
+    extern "C" void orc_gvf_step_##S(const T* f3, const T* v3, T* o3, int X, int Y, int Z, float mu) {      \
+        gvf_step<T>(f3, v3, o3, DIMS, (T)mu);                                                                \
+    }                                                                                                        \
+    extern "C" void orc_gvf_fmg_##S(const T* F, T* out, int X, int Y, int Z, float mu, int iters, int v0,   \
+                                    int v1, int v2, int lmax) {                                              \
+        gvf_fmg<T>(F, out, DIMS, mu, iters, v0, v1, v2, lmax);                                               \
+    }                                                                                                        \
+    extern "C" void orc_eigen3_##S(const T* A, T* V, T* d) {                                                 \
+        T a[3][3], v[3][3];                                                                                  \
+        for (int i = 0; i < 9; i++) a[i / 3][i % 3] = A[i];                                                  \
+        eigen3<T>(a, v, d);                                                                                  \
+        for (int i = 0; i < 9; i++) V[i] = v[i / 3][i % 3];                                                  \
+    }                                                                                                        \
+    extern "C" void orc_hessian_eigen_##S(const T* F, int X, int Y, int Z, int normalize, T* lam, T* vec) { \
+        hessian_eigen<T>(F, DIMS, normalize, lam, vec);                                                      \
+    }                                                                                                        \
+    extern "C" void orc_tdf_circle_##S(const T* F, int X, int Y, int Z, float rmin, float rmax, float rstep, \
+                                       T* tdf, T* rad) {                                                     \
+        tdf_circle<T>(F, DIMS, rmin, rmax, rstep, tdf, rad);                                                 \
+    }                                                                                                        \
+    extern "C" void orc_combine_##S(const T* ts, const T* rs, T* tl, T* rl, long long n) {                  \
+        combine<T>(ts, rs, tl, rl, n);                                                                       \
+    }                                                                                                        \
+    extern "C" void orc_candidates_##S(const T* tdf, uint8_t* mask, long long n, float thr) {               \
+        candidates<T>(tdf, mask, n, thr);                                                                    \
+    }                                                                                                        \
+    extern "C" void orc_candidate_nms_##S(const T* tdf, const T* rad, const T* F, int X, int Y, int Z,      \
+                                          const int32_t* pos, long long count, int cap, uint8_t* out) {      \
+        candidate_nms<T>(tdf, rad, F, DIMS, pos, count, cap, out);                                           \
+    }                                                                                                        \
+    extern "C" void orc_cube_max_##S(const T* tdf, const uint8_t* cand, int X, int Y, int Z, int cube,      \
+                                     uint8_t* out) {                                                         \
+        cube_max<T>(tdf, cand, DIMS, cube, out);                                                             \
+    }                                                                                                        \
+    extern "C" void orc_do_eigen_##S(const T* Fx, const T* Fy, const T* Fz, int X, int Y, int Z, int x,     \
+                                     int y, int z, T* lam, T* e1, T* e2, T* e3) {                            \
+        do_eigen<T>(Fx, Fy, Fz, DIMS, x, y, z, lam, e1, e2, e3);                                             \
+    }                                                                                                        \
+    extern "C" void orc_trilinear_##S(const T* F, int X, int Y, int Z, T u, T v, T w, T* out) {             \
+        trilinear3<T>(F, DIMS, u, v, w, out);                                                                \
+    }
+
+EXPORT_T(float, f32)
+EXPORT_T(double, f64)
+
+extern "C" void orc_blur_mask(float sigma, float* mask, int* maskSize) { blur_mask(sigma, mask, maskSize); }
+extern "C" void orc_circle_tables(float* c, float* s) {
+    make_tables();
+    for (int j = 0; j < 32; j++) {
+        c[j] = g_cos[j];
+        s[j] = g_sin[j];
+    }
+}
+extern "C" long long orc_hp_compact_sort(const uint8_t* mask, int X, int Y, int Z, int32_t* pos, long long cap) {
+    return hp_compact_sort(mask, DIMS, pos, cap);
+}
+extern "C" long long orc_hp_compact_pyramid(const uint8_t* mask, int X, int Y, int Z, int32_t* pos,
+                                            long long cap) {
+    return hp_compact_pyramid(mask, DIMS, pos, cap);
+}
+extern "C" unsigned long long orc_hp_key(unsigned x, unsigned y, unsigned z) { return hp_key(x, y, z); }

@@ -1,0 +1,467 @@
+"""Pins the CPU oracle (oracle/tsf_oracle.cpp) before it is trusted as the checker.
+
+The reference holds no function-level golden vectors (its tests are
+system-level quality bars on tests/data/synthetic/dataset_1), so every oracle
+function is pinned here by what the mathematics fixes -- closed forms,
+invariants, eigen-decompositions from numpy, a direct sparse solve, and two
+independent implementations of the pyramid order -- chosen so that a
+plausible mistake (dropped term, wrong sign, wrong index, swapped operand)
+fails at least one check.
+
+Pinning status per oracle function:
+  blur_mask / blur ........ sum = 1, sigma->size table, constant and linear
+                            invariance, separable scipy equivalent
+  to_float ................ closed form
+  vector_field ............ linear ramps (both Fmax branches), edge clamp, sign
+  gvf_step / gvf_euler .... cosine eigenmode of the mirror-BC Laplacian,
+                            data-term closed form, constant fixed point, mu = 0
+  gvf_fmg ................. converges to the direct sparse solution of the
+                            reference's discrete system; agrees with the Euler
+                            steady state
+  eigen3 .................. numpy eigh spectra (sorted by |lambda|),
+                            orthonormality, diagonal/permuted cases
+  hessian_eigen ........... linear fields (exact Jacobian), normalisation by w
+  trilinear ............... exact on linear fields incl. the -0.5 shift, clamp
+  tdf_circle .............. uniform-field closed form (both signs), linear
+                            inward field: TDF = k*rmax*mean(c^2+s^2), radius
+                            rmax; table entries equal the reference's literals
+  combine / candidates .... closed form
+  hp_compact .............. sort-by-key == literal pyramid descent, key digits
+  candidate_nms ........... constructed cross-section / axial neighbours
+  cube_max ................ independent numpy reduction
+  do_eigen ................ equals hessian_eigen(normalize) with w = |F|
+"""
+import numpy as np
+import pytest
+import scipy.ndimage as ndi
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+# ------------------------------------------------------------------ A1 blur
+@pytest.mark.parametrize("sigma,m", [(0.25, 1), (0.5, 1), (0.75, 2), (1.0, 2), (1.5, 3), (2.0, 4), (2.6, 5),
+                                     (10.0, 5)])
+def test_blur_mask_size_and_sum(orc, sigma, m):
+    mm, mask = orc.blur_mask(sigma)
+    assert mm == m
+    assert mask.size == (2 * m + 1) ** 3
+    assert abs(float(mask.astype(np.float64).sum()) - 1.0) < 1e-5  # float accumulation of the sum
+    M = 2 * m + 1
+    k = mask.reshape(M, M, M)  # [c][b][a]
+    assert np.allclose(k, k[::-1, :, :]) and np.allclose(k, k[:, ::-1, :]) and np.allclose(k, k[:, :, ::-1])
+    # separable: w(a,b,c) proportional to g(a) g(b) g(c)
+    g = np.exp(-np.arange(-m, m + 1) ** 2 / (2 * sigma * sigma))
+    sep = np.einsum("i,j,k->ijk", g, g, g)
+    assert np.allclose(k, sep / sep.sum(), rtol=1e-5, atol=1e-9)
+
+
+def test_blur_constant_and_linear(orc):
+    # the float mask sums to 1 only up to float rounding; invariance is w.r.t. that sum
+    wsum = float(orc.blur_mask(1.0)[1].astype(np.float64).sum())
+    vol = np.full((12, 11, 10), 3.25, np.float64)
+    assert np.allclose(orc.blur(vol, 1.0, "f64"), 3.25 * wsum, rtol=1e-12)
+    z, y, x = np.meshgrid(np.arange(12), np.arange(11), np.arange(10), indexing="ij")
+    lin = 0.3 * x - 0.2 * y + 0.1 * z + 1.0
+    out = orc.blur(lin, 1.0, "f64")  # m = 2: exact away from a 2-voxel border
+    assert np.allclose(out[2:-2, 2:-2, 2:-2], wsum * lin[2:-2, 2:-2, 2:-2], atol=1e-12)
+
+
+def test_blur_matches_separable_scipy(orc):
+    v = rng(1).standard_normal((9, 10, 11))
+    sigma = 0.9
+    m, mask = orc.blur_mask(sigma)
+    g = np.exp(-np.arange(-m, m + 1) ** 2 / (2 * np.float32(sigma) ** 2))
+    g = g / g.sum()
+    ref = v
+    for ax in range(3):
+        ref = ndi.correlate1d(ref, g, axis=ax, mode="nearest")
+    out = orc.blur(v, sigma, "f64")
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-6)
+
+
+# ------------------------------------------------------------------ A2 toFloat
+def test_to_float_closed_form(orc):
+    d = np.array([-5, 0, 10, 50, 75, 100, 255], np.float64)
+    out = orc.to_float(d, 0.0, 75.0, "f64")
+    exp = np.clip(d, 0, 75) / 75.0
+    assert np.allclose(out, exp, rtol=1e-7)
+
+
+# ------------------------------------------------------------------ A3 vector field
+@pytest.mark.parametrize("a,fmax,sign", [(0.05, 0.2, 1), (0.5, 0.2, 1), (0.05, 0.2, -1), (0.3, 0.1, -1)])
+def test_vector_field_linear_ramp(orc, a, fmax, sign):
+    Z, Y, X = 6, 7, 8
+    x = np.arange(X, dtype=np.float64)
+    vol = np.broadcast_to(a * x, (Z, Y, X)).copy()
+    F = orc.vector_field(vol, fmax, sign, "f64")
+    g = sign * a
+    exp_x = g / fmax if abs(g) < fmax else np.sign(g)
+    assert np.allclose(F[0][:, :, 1:-1], exp_x, rtol=1e-6)
+    # clamp-to-edge: at x = 0 the difference is 0.5*(I1 - I0) = a/2
+    ge = sign * 0.5 * a
+    exp_e = ge / fmax if abs(ge) < fmax else np.sign(ge)
+    assert np.allclose(F[0][:, :, 0], exp_e, rtol=1e-6)
+    assert np.allclose(F[1], 0) and np.allclose(F[2], 0) and np.all(F[3] == 1)
+
+
+def test_vector_field_direction_components(orc):
+    Z, Y, X = 6, 6, 6
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    vol = 3.0 * x + 4.0 * y + 12.0 * z  # |grad| = 13 > Fmax: unit vector
+    F = orc.vector_field(vol.astype(np.float64), 0.2, 1, "f64")
+    c = (slice(1, -1),) * 3
+    assert np.allclose(F[0][c], 3 / 13) and np.allclose(F[1][c], 4 / 13) and np.allclose(F[2][c], 12 / 13)
+
+
+# ------------------------------------------------------------------ A4 GVF
+def _cos_mode(shape, m):
+    Z, Y, X = shape
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    v = (np.cos(np.pi * m[0] * (x - 1) / (X - 3)) * np.cos(np.pi * m[1] * (y - 1) / (Y - 3)) *
+         np.cos(np.pi * m[2] * (z - 1) / (Z - 3)))
+    lam = 1 + 2 * float(np.float32(0.1)) * sum(np.cos(np.pi * mk / (N - 3)) - 1 for mk, N in zip(m, (X, Y, Z)))
+    return v, lam
+
+
+def test_gvf_step_cosine_eigenmode(orc):
+    shape = (10, 9, 12)
+    v, lam = _cos_mode(shape, (2, 1, 3))
+    f = np.zeros((3,) + shape)
+    V = np.stack([v, 0.5 * v, -v])
+    out = orc.gvf_step(f, V, 0.1, "f64")
+    assert np.allclose(out, lam * V, rtol=0, atol=1e-13)
+    # several steps: lam^n
+    W = V
+    for _ in range(5):
+        W = orc.gvf_step(f, W, 0.1, "f64")
+    assert np.allclose(W, lam ** 5 * V, atol=1e-12)
+
+
+def test_gvf_step_data_term(orc):
+    shape = (6, 7, 8)
+    f = rng(2).uniform(-1, 1, (3,) + shape)
+    out = orc.gvf_step(f, np.zeros_like(f), 0.1, "f64")
+    # v = 0: v' = f(q) |f(q)|^2 at q = mirror(w)
+    def remap(N):
+        i = np.arange(N)
+        i = np.where(i == 0, 2, i)
+        return np.where(i >= N - 1, N - 3, i)
+    qz, qy, qx = np.ix_(remap(shape[0]), remap(shape[1]), remap(shape[2]))
+    fq = f[:, qz, qy, qx]
+    exp = fq * (fq ** 2).sum(0)
+    assert np.allclose(out, exp, atol=1e-14)
+
+
+def test_gvf_fixed_points(orc):
+    shape = (8, 8, 8)
+    c = np.array([0.3, -0.2, 0.5])
+    F = np.concatenate([np.broadcast_to(c[:, None, None, None], (3,) + shape), np.ones((1,) + shape)])
+    V = orc.gvf_euler(F, 0.1, 7, "f64")
+    assert np.allclose(V[:3], F[:3], atol=1e-14)
+    assert np.allclose(V[3], np.linalg.norm(c))
+    # mu = 0 with v0 = f: v stays f
+    F2 = np.concatenate([rng(3).uniform(-1, 1, (3,) + shape), np.ones((1,) + shape)])
+    V2 = orc.gvf_euler(F2, 0.0, 9, "f64")
+    q = (slice(1, -1),) * 3
+    assert np.allclose(V2[:3][(slice(None),) + q], F2[:3][(slice(None),) + q], atol=1e-14)
+
+
+def test_gvf_finish_zero_guard(orc):
+    F = np.zeros((4, 6, 6, 6))
+    V = orc.gvf_euler(F, 0.1, 2, "f64")
+    assert np.all(V[3] == 1.0)
+
+
+# ------------------------------------------------------------------ A7 FMG
+def _direct_solution(F, mu, comp):
+    """Solve mu*lap(v) - s v = -s F_c on interior voxels with v(0) = v(2),
+    v(N-1) = v(N-3) (the fixed point of the reference's defect correction)."""
+    _, Z, Y, X = F.shape
+    n = X * Y * Z
+    s = (F[:3] ** 2).sum(0).ravel()
+    idx = np.arange(n).reshape(Z, Y, X)
+
+    def rm(p, N):
+        return 2 if p == 0 else (N - 3 if p >= N - 1 else p)
+
+    rows, cols, vals, b = [], [], [], np.zeros(n)
+    for z in range(Z):
+        for y in range(Y):
+            for x in range(X):
+                i = idx[z, y, x]
+                qx, qy, qz = rm(x, X), rm(y, Y), rm(z, Z)
+                if (qx, qy, qz) != (x, y, z):  # boundary: v(w) - v(q) = 0
+                    rows += [i, i]
+                    cols += [i, idx[qz, qy, qx]]
+                    vals += [1.0, -1.0]
+                    continue
+                rows.append(i); cols.append(i); vals.append(-6 * mu - s[i])
+                for dz, dy, dx in ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)):
+                    rows.append(i); cols.append(idx[z + dz, y + dy, x + dx]); vals.append(mu)
+                b[i] = -s[i] * F[comp].ravel()[i]
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    return spla.spsolve(A.tocsc(), b).reshape(Z, Y, X)
+
+
+def test_fmg_converges_to_direct_solution(orc):
+    shape = (16, 16, 16)
+    F = np.zeros((4,) + shape)
+    F[:3] = rng(4).uniform(-1, 1, (3,) + shape) * (rng(5).uniform(0, 1, shape) > 0.7)
+    F[3] = 1
+    mu = float(np.float32(0.1))  # the oracle takes mu as float
+    # the defect correction converges linearly (about 0.77 per step with the
+    # reference's smoother), so run it long enough to reach 1e-8
+    V = orc.gvf_fmg(F, mu, 100, prec="f64")
+    for c in range(3):
+        ref = _direct_solution(F, mu, c)
+        err = np.abs(V[c] - ref).max() / np.abs(ref).max()
+        assert err < 1e-8, (c, err)
+    assert np.allclose(V[3], np.sqrt((V[:3] ** 2).sum(0)))
+
+
+def test_fmg_equals_euler_steady_state(orc):
+    shape = (12, 12, 12)
+    F = np.zeros((4,) + shape)
+    F[:3] = rng(6).uniform(-1, 1, (3,) + shape) / np.sqrt(3)  # |f| <= 1 keeps the explicit scheme stable
+    F[3] = 1
+    Vf = orc.gvf_fmg(F, 0.1, 100, prec="f64")
+    Ve = orc.gvf_euler(F, 0.1, 4000, prec="f64")
+    assert np.abs(Vf[:3] - Ve[:3]).max() < 1e-8
+
+
+def test_fmg_float_close_to_double(orc):
+    shape = (16, 16, 16)
+    F = np.zeros((4,) + shape, np.float32)
+    F[:3] = rng(7).uniform(-1, 1, (3,) + shape)
+    F[3] = 1
+    a = orc.gvf_fmg(F, 0.15, 3, prec="f32")
+    b = orc.gvf_fmg(F, 0.15, 3, prec="f64")
+    assert np.linalg.norm(a[:3] - b[:3]) / np.linalg.norm(b[:3]) < 1e-5
+
+
+# ------------------------------------------------------------------ A9 eigen
+def _check_eigen(orc, A, prec, tol):
+    d, V = orc.eigen3(A, prec)
+    w, U = np.linalg.eigh(A)
+    order = np.argsort(np.abs(w), kind="stable")
+    assert np.allclose(d, w[order], atol=tol * max(1, np.abs(w).max()))
+    assert np.all(np.diff(np.abs(d)) >= -tol)
+    assert np.allclose(V.T @ V, np.eye(3), atol=tol * 10)
+    assert np.allclose(A @ V, V * d[None, :], atol=tol * 10 * max(1, np.abs(w).max()))
+
+
+def test_eigen_random_symmetric(orc):
+    g = rng(8)
+    for _ in range(200):
+        B = g.standard_normal((3, 3))
+        A = B + B.T
+        _check_eigen(orc, A, "f64", 1e-12)
+        _check_eigen(orc, A.astype(np.float32), "f32", 3e-5)
+
+
+def test_eigen_known_spectra(orc):
+    g = rng(9)
+    for lam in ([1.0, -2.0, 3.0], [0.0, 0.0, 1.0], [-1e-3, 1e-3, 5.0], [2.0, 2.0, 2.0], [0.0, 0.0, 0.0]):
+        Q, _ = np.linalg.qr(g.standard_normal((3, 3)))
+        A = Q @ np.diag(lam) @ Q.T
+        A = 0.5 * (A + A.T)
+        d, V = orc.eigen3(A, "f64")
+        assert np.allclose(np.sort(d), np.sort(lam), atol=1e-12)
+        assert np.allclose(np.abs(d), np.sort(np.abs(lam)), atol=1e-12)
+
+
+def test_eigen_diagonal_exact(orc):
+    d, V = orc.eigen3(np.diag([3.0, -1.0, 2.0]), "f64")
+    assert np.array_equal(d, [-1.0, 2.0, 3.0])
+    # eigenvectors are +-unit axes: lambda -1 -> y, 2 -> z, 3 -> x
+    assert np.allclose(np.abs(V), [[0, 0, 1], [1, 0, 0], [0, 1, 0]])
+
+
+# ------------------------------------------------------------------ A8 Hessian
+def test_hessian_linear_field(orc):
+    Z, Y, X = 7, 8, 9
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    J = np.array([[0.02, 0.01, -0.03], [0.01, 0.05, 0.02], [-0.03, 0.02, -0.04]])  # symmetric
+    P = np.stack([x, y, z]).astype(np.float64)
+    F = np.zeros((4, Z, Y, X))
+    F[:3] = np.einsum("ij,jzyx->izyx", J, P)
+    F[3] = 2.0
+    lam, vec = orc.hessian_eigen(F, 0, prec="f64")
+    w = np.linalg.eigvalsh(J)
+    exp = w[np.argsort(np.abs(w))]
+    assert np.allclose(lam[:, 3, 4, 4], exp, atol=1e-12)
+    lam_n, _ = orc.hessian_eigen(F, 1, prec="f64")  # every sample / w = 2
+    assert np.allclose(lam_n[:, 3, 4, 4], exp / 2, atol=1e-12)
+
+
+# ------------------------------------------------------------------ trilinear
+def test_trilinear_linear_field_and_shift(orc):
+    Z, Y, X = 6, 7, 8
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    F = np.zeros((4, Z, Y, X))
+    F[0] = 1.5 * x + 0.25
+    F[1] = -0.5 * y
+    F[2] = 2.0 * z - 1
+    for (u, v, w) in [(3.2, 2.9, 3.0), (1.5, 1.5, 1.5), (4.75, 3.1, 2.6)]:
+        s = orc.trilinear(F, u, v, w)
+        # the linear filter samples texel centres at i + 0.5: value at u is f(u - 0.5)
+        assert np.allclose(s, [1.5 * (u - 0.5) + 0.25, -0.5 * (v - 0.5), 2.0 * (w - 0.5) - 1], atol=1e-12)
+    # integer coordinate -> mean of two neighbours; far outside -> clamped edge value
+    s = orc.trilinear(F, 3.0, 3.5, 2.5)
+    assert np.isclose(s[0], 0.5 * (F[0, 2, 3, 2] + F[0, 2, 3, 3]))
+    s = orc.trilinear(F, -5.0, 3.5, 2.5)
+    assert np.isclose(s[0], F[0, 2, 3, 0])
+
+
+# ------------------------------------------------------------------ A11 TDF
+def test_circle_tables_match_reference_literals(orc):
+    c, s = orc.circle_tables()
+    # literal entries of the reference's tables (kernels_no_3d_write.cl:1500-1501)
+    assert c[0] == np.float32(1.0) and c[1] == np.float32(0.540302) and c[11] == np.float32(0.0044257)
+    assert c[22] == np.float32(-0.999961) and c[31] == np.float32(0.914742)
+    assert s[1] == np.float32(0.841471) and s[22] == np.float32(-0.00885131) and s[11] == np.float32(-0.99999)
+    assert np.allclose(c, np.cos(np.arange(32)), atol=5e-6) and np.allclose(s, np.sin(np.arange(32)), atol=5e-6)
+    assert abs(float(c.astype(np.float64).sum()) - 0.5875767) < 1e-6
+    assert abs(float(s.astype(np.float64).sum()) - (-0.1239863)) < 1e-6
+
+
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_tdf_uniform_field(orc, sign):
+    shape = (8, 8, 8)
+    cvec = sign * np.array([0.3, -0.4, 0.2])
+    F = np.zeros((4,) + shape)
+    F[:3] = cvec[:, None, None, None]
+    F[3] = 1
+    tdf, rad = orc.tdf_circle(F, 1.0, 3.0, 0.5, "f64")
+    # zero Jacobian -> tred2/tql2 of the zero matrix returns the identity basis
+    lam, vec = orc.hessian_eigen(F, 0, prec="f64")
+    e2 = vec[3:6, 4, 4, 4]
+    e3 = vec[6:9, 4, 4, 4]
+    c, s = orc.circle_tables()
+    S = -(cvec @ e3 * c.astype(np.float64).sum() + cvec @ e2 * s.astype(np.float64).sum()) / 32
+    if S > 0:
+        assert np.isclose(tdf[4, 4, 4], S, rtol=1e-12) and rad[4, 4, 4] == 1.0
+    else:
+        assert tdf[4, 4, 4] == 0 and rad[4, 4, 4] == 0
+    assert np.all(tdf == tdf[4, 4, 4])
+
+
+def test_tdf_linear_inward_field(orc):
+    Z, Y, X = 9, 24, 24
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    k = 0.02
+    c0 = 11.5  # the sampled field is f(p - 0.5): centre it between voxels 11 and 12 -> at voxel 12 the
+    F = np.zeros((4, Z, Y, X))  # sampled inward field is -k (p - 12)
+    F[0] = -k * (x - c0)
+    F[1] = -k * (y - c0)
+    F[3] = 1
+    tdf, rad = orc.tdf_circle(F, 1.0, 5.0, 1.0, "f64")
+    c, s = orc.circle_tables()
+    m = np.mean(c.astype(np.float64) ** 2 + s.astype(np.float64) ** 2)
+    # Jacobian diag(-k, -k, 0): e1 = z, e2/e3 span xy; S(r) = k r m grows with r -> radius rmax
+    assert rad[4, 12, 12] == 5.0
+    assert np.isclose(tdf[4, 12, 12], k * 5.0 * m, rtol=1e-9)
+
+
+# ------------------------------------------------------------------ A12, A13
+def test_combine_and_candidates(orc):
+    g = rng(10)
+    ts, rs, tl, rl = (g.uniform(0, 1, 100) for _ in range(4))
+    a, b = orc.combine(ts, rs, tl, rl, "f64")
+    take = tl < ts
+    assert np.array_equal(a, np.where(take, ts, tl)) and np.array_equal(b, np.where(take, rs, rl))
+    m = orc.candidates(ts, 0.5, "f64")
+    assert np.array_equal(m, (ts >= 0.5).astype(np.uint8))
+
+
+# ------------------------------------------------------------------ A14, A15
+def test_hp_key_digits(orc):
+    assert orc.hp_key(1, 0, 0) == 1 and orc.hp_key(0, 0, 1) == 2 and orc.hp_key(0, 1, 0) == 4
+    assert orc.hp_key(2, 0, 0) == 8 and orc.hp_key(0, 2, 0) == 32 and orc.hp_key(1, 1, 1) == 7
+
+
+@pytest.mark.parametrize("shape,p", [((16, 16, 16), 0.1), ((20, 12, 28), 0.3), ((5, 9, 7), 0.5), ((40, 36, 32), 0.02),
+                                     ((8, 8, 8), 1.0), ((8, 8, 8), 0.0)])
+def test_hp_sort_equals_pyramid(orc, shape, p):
+    mask = (rng(11).uniform(0, 1, shape) < p).astype(np.uint8)
+    a = orc.hp_compact(mask, "sort")
+    b = orc.hp_compact(mask, "pyramid")
+    assert a.shape[0] == mask.sum() == b.shape[0]
+    assert np.array_equal(a, b)
+    if a.shape[0]:
+        keys = [orc.hp_key(*t) for t in a]
+        assert all(k1 < k2 for k1, k2 in zip(keys, keys[1:]))
+        assert np.all(mask[a[:, 2], a[:, 1], a[:, 0]] == 1)
+
+
+def test_hp_single_voxel(orc):
+    mask = np.zeros((12, 10, 14), np.uint8)
+    mask[7, 3, 11] = 1
+    assert orc.hp_compact(mask, "pyramid").tolist() == [[11, 3, 7]]
+
+
+# ------------------------------------------------------------------ A16 NMS
+def test_candidate_nms_cross_section(orc):
+    Z, Y, X = 15, 15, 15
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    F = np.zeros((4, Z, Y, X))
+    F[0] = -0.05 * (x - 6.5)
+    F[1] = -0.05 * (y - 6.5)
+    F[3] = 1.0  # |v| plane (the NMS reads it)
+    radius = np.full((Z, Y, X), 2.0)
+    tdf = np.ones((Z, Y, X))
+    p = np.array([[7, 7, 7]], np.int32)
+    assert orc.candidate_nms(tdf, radius, F, p, 5, "f64")[7, 7, 7] == 1
+    G = F.copy()
+    G[3, 7, 7, 8] = 0.5  # in the cross-section (x neighbour), smaller |v| -> reject
+    assert orc.candidate_nms(tdf, radius, G, p, 5, "f64")[7, 7, 7] == 0
+    H = F.copy()
+    H[3, 8, 7, 7] = 0.5  # along the tube axis e1 = z -> theta = pi/2 -> keep
+    assert orc.candidate_nms(tdf, radius, H, p, 5, "f64")[7, 7, 7] == 1
+    K = F.copy()
+    K[3, 7, 7, 9] = 0.5  # |r| = 2 = maxD -> outside the open disk -> keep
+    assert orc.candidate_nms(tdf, radius, K, p, 5, "f64")[7, 7, 7] == 1
+
+
+# ------------------------------------------------------------------ A17 cube max
+def test_cube_max_numpy(orc):
+    shape = (10, 11, 13)
+    g = rng(12)
+    tdf = g.uniform(0, 1, shape)
+    tdf[g.uniform(0, 1, shape) < 0.05] = 0.0
+    cand = (g.uniform(0, 1, shape) < 0.3).astype(np.uint8)
+    cube = 3
+    out = orc.cube_max(tdf, cand, cube, "f64")
+    exp = np.zeros(shape, np.uint8)
+    Z, Y, X = shape
+    for gz in range(0, Z, cube):
+        for gy in range(0, Y, cube):
+            for gx in range(0, X, cube):
+                blk = np.where(cand[gz:gz + cube, gy:gy + cube, gx:gx + cube] == 1,
+                               tdf[gz:gz + cube, gy:gy + cube, gx:gx + cube], -1.0)
+                # loop order x outer, z inner: first maximum in (x, y, z) lexicographic order
+                t = np.transpose(blk, (2, 1, 0)).ravel()
+                if t.max() > 0:
+                    k = int(np.argmax(t))
+                    bx, by, bz = np.unravel_index(k, np.transpose(blk, (2, 1, 0)).shape)
+                    exp[gz + bz, gy + by, gx + bx] = 1
+    assert np.array_equal(out, exp)
+
+
+# ------------------------------------------------------------------ A10 doEigen
+def test_do_eigen_equals_normalised_hessian(orc):
+    shape = (8, 9, 10)
+    F = np.zeros((4,) + shape)
+    F[:3] = rng(13).uniform(-1, 1, (3,) + shape)
+    F[3] = np.sqrt((F[:3] ** 2).sum(0))
+    lam, vec = orc.hessian_eigen(F, 1, prec="f64")
+    for p in [(3, 4, 5), (1, 1, 1), (8, 7, 6)]:
+        l, e1, e2, e3 = orc.do_eigen(F[0], F[1], F[2], p, "f64")
+        x, y, z = p
+        assert np.allclose(l, lam[:, z, y, x], atol=1e-12)
+        assert np.allclose(e1, vec[0:3, z, y, x], atol=1e-12)
+        assert np.allclose(e3, vec[6:9, z, y, x], atol=1e-12)

@@ -1,0 +1,17 @@
+"""Python binding of libtsf_hip.so (the gfx950 volume-filter library).
+
+This is a thin ctypes layer over the C ABI declared in include/tsf.h; all
+computation happens in the HIP kernels.  Loading fails loudly when the
+library has not been built -- there is no CPU fallback.
+
+Volumes are numpy arrays of shape (Z, Y, X); a vector field on the device is
+a `Field` (four SoA planes x, y, z, w).
+"""
+from ._capi import (  # noqa: F401
+    TSFError, lib, library_path, Dims, FieldDesc, DeviceArray, Field, Stream, Event,
+    STORAGE_F32, STORAGE_F16, STORAGE_BF16, storage_dtype,
+    blur_gaussian, create_blur_mask, to_float, create_vector_field, gvf_euler, gvf_fmg,
+    hessian_eigen, do_eigen_points, tdf_circle, combine, candidates, hp_compact,
+    candidate_nms, cube_max, gvf_euler_workspace_size, hp_compact_workspace_size,
+    gvf_fmg_workspace_size, device_count, exported_symbols,
+)

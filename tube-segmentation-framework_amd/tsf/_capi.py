@@ -1,0 +1,352 @@
+"""ctypes declarations of include/tsf.h and small numpy-facing helpers."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(os.path.dirname(_HERE), "lib", "libtsf_hip.so")
+
+STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+_STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DIMENSIONS", 3: "HIP", 4: "OUT_OF_MEMORY", 5: "CAPACITY",
+           6: "RANGE"}
+
+
+class TSFError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("z", ctypes.c_int32)]
+
+
+class FieldDesc(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("w", ctypes.c_void_p), ("storage", ctypes.c_int32), ("scale", ctypes.c_float)]
+
+
+_lib = None
+_vp, _i32, _i64, _f32, _sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_size_t
+_PF = ctypes.POINTER(FieldDesc)
+
+_SIGS = {
+    "tsf_last_error": ([], ctypes.c_char_p),
+    "tsf_version": ([], ctypes.c_char_p),
+    "tsf_malloc": ([ctypes.POINTER(_vp), _sz], _i32),
+    "tsf_free": ([_vp], _i32),
+    "tsf_memcpy_htod": ([_vp, _vp, _sz, _vp], _i32),
+    "tsf_memcpy_dtoh": ([_vp, _vp, _sz, _vp], _i32),
+    "tsf_memcpy_dtod": ([_vp, _vp, _sz, _vp], _i32),
+    "tsf_memset": ([_vp, ctypes.c_int, _sz, _vp], _i32),
+    "tsf_stream_create": ([ctypes.POINTER(_vp)], _i32),
+    "tsf_stream_destroy": ([_vp], _i32),
+    "tsf_stream_synchronize": ([_vp], _i32),
+    "tsf_event_create": ([ctypes.POINTER(_vp)], _i32),
+    "tsf_event_destroy": ([_vp], _i32),
+    "tsf_event_record": ([_vp, _vp], _i32),
+    "tsf_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(_f32)], _i32),
+    "tsf_device_count": ([ctypes.POINTER(ctypes.c_int)], _i32),
+    "tsf_create_blur_mask": ([_f32, _vp, ctypes.POINTER(_i32)], _i32),
+    "tsf_blur_gaussian": ([_vp, _vp, Dims, _f32, _vp], _i32),
+    "tsf_to_float": ([_vp, _i32, _vp, Dims, _f32, _f32, _vp], _i32),
+    "tsf_create_vector_field": ([_vp, _PF, Dims, _f32, _i32, _vp], _i32),
+    "tsf_gvf_euler": ([_PF, _PF, Dims, _f32, _i32, _i32, _vp, _sz, _vp], _i32),
+    "tsf_gvf_euler_workspace_size": ([Dims, _i32], _sz),
+    "tsf_gvf_fmg": ([_PF, _PF, Dims, _f32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _sz, _vp], _i32),
+    "tsf_gvf_fmg_workspace_size": ([Dims, _i32], _sz),
+    "tsf_hessian_eigen": ([_PF, Dims, _i32, _vp, _vp, _vp], _i32),
+    "tsf_do_eigen_points": ([_vp, _vp, _vp, Dims, _vp, _i64, _vp, _vp], _i32),
+    "tsf_tdf_circle": ([_PF, Dims, _f32, _f32, _f32, _vp, _vp, _vp], _i32),
+    "tsf_combine": ([_vp, _vp, _vp, _vp, _i64, _vp], _i32),
+    "tsf_candidates": ([_vp, _vp, _i64, _f32, _vp], _i32),
+    "tsf_hp_compact": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp, _sz, _vp], _i32),
+    "tsf_hp_compact_workspace_size": ([Dims], _sz),
+    "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
+    "tsf_cube_max": ([_vp, _vp, Dims, _i32, _vp, _vp], _i32),
+}
+
+
+def lib():
+    """Load libtsf_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError(f"{library_path} is missing: build it with "
+                               f"`python tube-segmentation-framework_amd/build.py` (no CPU fallback exists)")
+        L = ctypes.CDLL(library_path)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().tsf_last_error().decode(errors="replace")
+        raise TSFError(status, msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    st = lib().tsf_device_count(ctypes.byref(n))
+    return n.value if st == 0 else 0
+
+
+def storage_dtype(storage):
+    return {STORAGE_F32: np.float32, STORAGE_F16: np.float16, STORAGE_BF16: np.uint16}[storage]
+
+
+def _to_bf16_bits(a):
+    """float32 -> bfloat16 bit patterns, round to nearest even."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32)
+    r = ((u >> 16) & 1) + 0x7FFF
+    out = ((u + r) >> 16).astype(np.uint16)
+    nan = np.isnan(a)
+    out[nan] = 0x7FC0
+    return out
+
+
+def _from_bf16_bits(b):
+    return (b.astype(np.uint32) << 16).view(np.float32)
+
+
+class DeviceArray:
+    """A device allocation owned by Python (freed on garbage collection)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        _check(lib().tsf_malloc(ctypes.byref(p), max(self.nbytes, 1)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None, stream=None):
+        a = np.ascontiguousarray(a, dtype)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().tsf_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes, _sp(stream)))
+
+    def download(self, stream=None):
+        a = np.empty(self.shape, self.dtype)
+        _check(lib().tsf_memcpy_dtoh(a.ctypes.data, self.ptr, self.nbytes, _sp(stream)))
+        return a
+
+    def zero(self, stream=None):
+        _check(lib().tsf_memset(self.ptr, 0, self.nbytes, _sp(stream)))
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.tsf_free(self.ptr)
+            self.ptr = None
+
+
+def _sp(stream):
+    return None if stream is None else stream.handle
+
+
+class Stream:
+    def __init__(self):
+        h = _vp()
+        _check(lib().tsf_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        _check(lib().tsf_stream_synchronize(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.tsf_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        h = _vp()
+        _check(lib().tsf_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        _check(lib().tsf_event_record(self.handle, _sp(stream)))
+
+    def elapsed_ms(self, later):
+        ms = _f32(0)
+        _check(lib().tsf_event_elapsed_ms(self.handle, later.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None) and _lib is not None:
+            _lib.tsf_event_destroy(self.handle)
+            self.handle = None
+
+
+class Field:
+    """Four device planes (x, y, z, w) of one storage type."""
+
+    def __init__(self, shape, storage=STORAGE_F32, scale=1.0, with_w=True):
+        self.shape = tuple(shape)
+        self.storage = storage
+        self.scale = float(scale)
+        dt = storage_dtype(storage)
+        self.planes = [DeviceArray(shape, dt) for _ in range(4 if with_w else 3)]
+
+    def desc(self):
+        p = [a.ptr for a in self.planes] + [None] * (4 - len(self.planes))
+        return FieldDesc(p[0], p[1], p[2], p[3], self.storage, self.scale)
+
+    @classmethod
+    def from_host(cls, F, storage=STORAGE_F32, scale=1.0, stream=None):
+        """F: array (4 or 3, Z, Y, X) of float values (scaled/rounded here)."""
+        F = np.asarray(F, np.float32)
+        f = cls(F.shape[1:], storage, scale, with_w=F.shape[0] == 4)
+        for k in range(F.shape[0]):
+            f.planes[k].upload(_encode(F[k], storage, scale), stream)
+        return f
+
+    def download(self, stream=None):
+        return np.stack([_decode(p.download(stream), self.storage, self.scale) for p in self.planes])
+
+
+def _encode(a, storage, scale):
+    a = np.asarray(a, np.float32) * np.float32(scale)
+    if storage == STORAGE_F32:
+        return a
+    if storage == STORAGE_F16:
+        return a.astype(np.float16)
+    return _to_bf16_bits(a)
+
+
+def _decode(a, storage, scale):
+    if storage == STORAGE_F32:
+        v = a.astype(np.float32)
+    elif storage == STORAGE_F16:
+        v = a.astype(np.float32)
+    else:
+        v = _from_bf16_bits(a)
+    return v / np.float32(scale)
+
+
+def _dims(shape):
+    Z, Y, X = shape[-3:]
+    return Dims(int(X), int(Y), int(Z))
+
+
+def _ws(ws):
+    return (None, 0) if ws is None else (ws.ptr, ws.nbytes)
+
+
+# ------------------------------------------------------------------- operators
+def create_blur_mask(sigma):
+    mask = np.zeros(1331, np.float32)
+    m = _i32(0)
+    _check(lib().tsf_create_blur_mask(sigma, mask.ctypes.data, ctypes.byref(m)))
+    M = 2 * m.value + 1
+    return m.value, mask[:M ** 3].copy()
+
+
+def blur_gaussian(src, dst, sigma, stream=None):
+    _check(lib().tsf_blur_gaussian(src.ptr, dst.ptr, _dims(src.shape), sigma, _sp(stream)))
+
+
+_TYPE_CODES = {np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16): 2, np.dtype(np.int16): 3,
+               np.dtype(np.float32): 4}
+
+
+def to_float(src, dst, minimum, maximum, stream=None):
+    _check(lib().tsf_to_float(src.ptr, _TYPE_CODES[src.dtype], dst.ptr, _dims(src.shape), minimum, maximum,
+                              _sp(stream)))
+
+
+def create_vector_field(vol, field, fmax, sign=1, stream=None):
+    d = field.desc()
+    _check(lib().tsf_create_vector_field(vol.ptr, ctypes.byref(d), _dims(vol.shape), fmax, sign, _sp(stream)))
+
+
+def gvf_euler_workspace_size(shape, storage):
+    return lib().tsf_gvf_euler_workspace_size(_dims(shape), storage)
+
+
+def gvf_euler(f, out, mu, iterations, fuse=1, workspace=None, stream=None):
+    fd, od = f.desc(), out.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_gvf_euler(ctypes.byref(fd), ctypes.byref(od), _dims(f.shape), mu, iterations, fuse, w, wb,
+                               _sp(stream)))
+
+
+def gvf_fmg_workspace_size(shape, levels):
+    return lib().tsf_gvf_fmg_workspace_size(_dims(shape), levels)
+
+
+def gvf_fmg(f, out, mu, iterations, levels=2, v0=1, v1=2, v2=2, norms=False, workspace=None, stream=None):
+    fd, od = f.desc(), out.desc()
+    w, wb = _ws(workspace)
+    nr = np.zeros(3 * max(iterations, 1), np.float64) if norms else None
+    _check(lib().tsf_gvf_fmg(ctypes.byref(fd), ctypes.byref(od), _dims(f.shape), mu, iterations, levels, v0, v1, v2,
+                             nr.ctypes.data if norms else None, w, wb, _sp(stream)))
+    return nr[:3 * iterations].reshape(3, iterations) if norms else None
+
+
+def hessian_eigen(field, normalize, lam, vec=None, stream=None):
+    d = field.desc()
+    _check(lib().tsf_hessian_eigen(ctypes.byref(d), _dims(field.shape), int(normalize), lam.ptr,
+                                   vec.ptr if vec is not None else None, _sp(stream)))
+
+
+def do_eigen_points(fx, fy, fz, positions, out, stream=None):
+    n = positions.shape[0]
+    _check(lib().tsf_do_eigen_points(fx.ptr, fy.ptr, fz.ptr, _dims(fx.shape), positions.ptr, n, out.ptr,
+                                     _sp(stream)))
+
+
+def tdf_circle(field, rmin, rmax, rstep, tdf, radius, stream=None):
+    d = field.desc()
+    _check(lib().tsf_tdf_circle(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, tdf.ptr, radius.ptr,
+                                _sp(stream)))
+
+
+def combine(ts, rs, tl, rl, stream=None):
+    n = int(np.prod(ts.shape))
+    _check(lib().tsf_combine(ts.ptr, rs.ptr, tl.ptr, rl.ptr, n, _sp(stream)))
+
+
+def candidates(tdf, mask, threshold, stream=None):
+    n = int(np.prod(tdf.shape))
+    _check(lib().tsf_candidates(tdf.ptr, mask.ptr, n, threshold, _sp(stream)))
+
+
+def hp_compact_workspace_size(shape):
+    return lib().tsf_hp_compact_workspace_size(_dims(shape))
+
+
+def hp_compact(mask, positions=None, workspace=None, stream=None):
+    """Returns (count, positions DeviceArray of shape (capacity, 3)); raises on overflow."""
+    cnt = _i64(0)
+    cap = positions.shape[0] if positions is not None else 0
+    w, wb = _ws(workspace)
+    _check(lib().tsf_hp_compact(mask.ptr, _dims(mask.shape), positions.ptr if positions is not None else None, cap,
+                                ctypes.byref(cnt), w, wb, _sp(stream)))
+    return cnt.value
+
+
+def candidate_nms(tdf, radius, field, positions, count, maxd_cap, out, stream=None):
+    d = field.desc()
+    _check(lib().tsf_candidate_nms(tdf.ptr, radius.ptr, ctypes.byref(d), _dims(field.shape), positions.ptr, count,
+                                   maxd_cap, out.ptr, _sp(stream)))
+
+
+def cube_max(tdf, cand, cube, out, stream=None):
+    _check(lib().tsf_cube_max(tdf.ptr, cand.ptr, _dims(tdf.shape), cube, out.ptr, _sp(stream)))
