@@ -76,6 +76,7 @@ def main():
     ws = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, S),), np.uint8)
     t_s, r_s = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
     t_l, r_l = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    tws = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, S),), np.uint8)
     ev = [tsf.Event() for _ in range(4)]
 
     def step():
@@ -83,14 +84,14 @@ def main():
         # small-radius branch (radius-min < 2.5)
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, F_small, 0.2, 1, stream)
-        tsf.tdf_circle(F_small, 0.5, 3.0, 0.5, t_s, r_s, stream)
+        tsf.tdf_circle(F_small, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
         # large-radius branch
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
         ev[1].record(stream)
         tsf.gvf_euler(F, V, args.mu, args.iters, args.fuse, ws, stream)
         ev[2].record(stream)
-        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, stream)
+        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
         tsf.combine(t_s, r_s, t_l, r_l, stream)
         ev[3].record(stream)
 

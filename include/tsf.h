@@ -177,9 +177,11 @@ tsf_status tsf_do_eigen_points(const float* fx, const float* fy, const float* fz
  * sampling with the OpenCL linear-filter convention (-0.5 shift,
  * clamp-to-edge); keep the first maximum, stop at the first r that does not
  * increase it.  tdf = max(0, best), radius = its r (0 if none).  The field
- * needs plane w when rmax >= 4. */
+ * needs plane w when rmax >= 4.  Workspace: the field repacked as one texel
+ * (x, y, z, w) per voxel for the trilinear gathers. */
 tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep, float* tdf,
-                          float* radius, hipStream_t stream);
+                          float* radius, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage);
 
 /* combine (kernels_no_3d_write.cl:1087-1098): where tdf_large < tdf_small,
  * take the small-scale TDF and radius.  In place on the large buffers. */

@@ -61,7 +61,7 @@ def test_invalid_arguments_rejected_on_host():
     assert st == 1 and b"power of two" in L.tsf_last_error()
     st = L.tsf_gvf_euler(ctypes.byref(f), ctypes.byref(f), d, 0.1, -1, 1, None, 0, None)
     assert st == 1
-    st = L.tsf_tdf_circle(ctypes.byref(f), d, 1.0, 5.0, 0.0, 1, 1, None)
+    st = L.tsf_tdf_circle(ctypes.byref(f), d, 1.0, 5.0, 0.0, 1, 1, None, 0, None)
     assert st == 1  # rstep must be > 0
     cnt = ctypes.c_int64(0)
     st = L.tsf_hp_compact(None, d, None, 0, ctypes.byref(cnt), None, 0, None)

@@ -76,7 +76,8 @@ def _field_from(orc, vol, fmax=0.2, sigma=0.5):
     return orc.vector_field(orc.blur(vol, sigma, "f32"), fmax, 1, "f32")
 
 
-@pytest.mark.parametrize("shape,iters", [((64, 64, 64), 250), ((20, 24, 36), 7), ((16, 17, 18), 1), ((8, 8, 8), 0)])
+@pytest.mark.parametrize("shape,iters", [((64, 64, 64), 250), ((20, 24, 36), 7), ((16, 17, 18), 1), ((8, 8, 8), 0),
+                                         ((5, 13, 130), 3), ((4, 9, 66), 4), ((37, 23, 65), 5), ((70, 6, 4), 2)])
 def test_gvf_euler_f32(tsf, orc, shape, iters):
     F = _field_from(orc, phantom(shape, 4))
     f = tsf.Field.from_host(F)

@@ -1,15 +1,28 @@
 // Gradient vector flow, explicit Euler (Jacobi) iterations.
 //
-// Layout: f (the initial field) and v (the iterate) are SoA planes.  One
-// launch advances the iterate by one step; every thread owns one (x, y)
-// column and marches a run of ZRUN planes in z, keeping the three z-planes
-// of its column in registers, so each v value is fetched from memory once per
-// step by its owner plus the x/y neighbour reads that hit L1/L2.
+// Layout: f (the initial field) and v (the iterate) are SoA planes in HBM.
+//
+// Kernel k_gvf_march (one Jacobi step per launch): a workgroup of 64 x GTY
+// threads owns a (64*VEC) x GTY tile of (x, y) columns, each thread VEC
+// x-adjacent columns, and marches a chunk of ZC planes in z.  Per column the
+// thread keeps planes q-1 .. q+2 of v (a four-slot register ring, the loop
+// unrolled by four so the ring never needs register moves) and planes q,
+// q+1 of f; every load is issued two planes before its use and is
+// unconditional (out-of-range lanes read a safe address and discard it), so
+// no wave waits on a load it has just issued.  The current plane of the tile
+// plus a one-voxel x/y apron is staged in a double-buffered LDS plane: one
+// barrier per plane.  Every v and f value is read from HBM about once per
+// step.
+//
+// Boundary rule of the reference (kernels_no_3d_write.cl:1757-1760): the
+// value written at w is the update computed at q = mirror(w), 0 -> 2 and
+// N-1 -> N-3.  The kernel computes each interior plane q once and also writes
+// it to plane 0 (q == 2) or plane Z-1 (q == Z-3); boundary columns in x/y copy
+// the result of their mirror column through LDS.  Tiles are placed so that a
+// boundary column and its mirror column always share a tile.
 #include "tsf_common.hpp"
 
 namespace tsf {
-
-constexpr int GX = 64, GY = 4, ZRUN = 8;
 
 template <class S>
 struct Planes3 {
@@ -25,52 +38,229 @@ struct Out4 {
     typename S::T* w;
 };
 
-// One Jacobi step of GVF3DIteration (kernels_no_3d_write.cl:1746-1787).  The
-// arithmetic order follows the reference:
+constexpr int GTY = 4, ZC = 64;
+
+__device__ __forceinline__ int tile_origin(int b, int T, int N) {
+    int o = b * T;
+    return N >= T ? min(o, N - T) : 0;
+}
+
+// The update of GVF3DIteration (kernels_no_3d_write.cl:1746-1787) in the
+// reference's arithmetic order:
 //   lap = -6 v + v(x+1) + v(x-1) + v(y+1) + v(y-1) + v(z+1) + v(z-1)
 //   v'  = v + (mu lap - (v - f) |f|^2)
-// With FINISH the step also writes w = |v'| (or 1), GVF3DFinish 1798-1807.
-template <class S, bool FINISH>
-__global__ void __launch_bounds__(GX* GY) k_gvf_step(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d, float mu,
-                                                     float inv_f, float inv_v, float sc_v) {
-    const int x = blockIdx.x * GX + threadIdx.x;
-    const int y = blockIdx.y * GY + threadIdx.y;
-    if (x >= d.X || y >= d.Y) return;
-    const int qx = remap(x, d.X), qy = remap(y, d.Y);
-    const i64 sx = 1, sy = d.X, sz = (i64)d.X * d.Y;
-    const int z0 = blockIdx.z * ZRUN;
-    const int z1 = min(z0 + ZRUN, d.Z);
-    for (int z = z0; z < z1; z++) {
-        const int qz = remap(z, d.Z);
-        const i64 q = d.at(qx, qy, qz);
-        const float ix = S::ld(f.x, q, inv_f), iy = S::ld(f.y, q, inv_f), iz = S::ld(f.z, q, inv_f);
-        const float m2 = ix * ix + iy * iy + iz * iz;
-        float r[3];
-        const typename S::T* P[3] = {vin.x, vin.y, vin.z};
-        const float F[3] = {ix, iy, iz};
+__device__ __forceinline__ float gvf_update(float vc, float xp, float xm, float yp, float ym, float zp, float zm,
+                                            float fc, float m2, float mu) {
+    float lap = -6.0f * vc;
+    lap = lap + xp;
+    lap = lap + xm;
+    lap = lap + yp;
+    lap = lap + ym;
+    lap = lap + zp;
+    lap = lap + zm;
+    return vc + (mu * lap - (vc - fc) * m2);
+}
+
+// Unconditional load of VEC consecutive elements at i (i even when VEC == 2);
+// `ok` false loads element 0 instead and returns zeros.
+template <class S, int VEC>
+__device__ __forceinline__ void ldv(const typename S::T* p, i64 i, bool ok, float inv, float* o) {
+    const i64 j = ok ? i : 0;
+    if (VEC == 2) {
+        S::ld2(p, j, inv, o[0], o[1]);
+    } else {
+        o[0] = S::ld(p, j, inv);
+    }
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const typename S::T* p = P[c];
-            const float vc = S::ld(p, q, inv_v);
-            float lap = -6.0f * vc;
-            lap = lap + S::ld(p, q + sx, inv_v);
-            lap = lap + S::ld(p, q - sx, inv_v);
-            lap = lap + S::ld(p, q + sy, inv_v);
-            lap = lap + S::ld(p, q - sy, inv_v);
-            lap = lap + S::ld(p, q + sz, inv_v);
-            lap = lap + S::ld(p, q - sz, inv_v);
-            r[c] = vc + (mu * lap - (vc - F[c]) * m2);
+    for (int k = 0; k < VEC; k++) o[k] = ok ? o[k] : 0.0f;
+}
+template <class S, int VEC>
+__device__ __forceinline__ void stv(typename S::T* p, i64 i, const float* v, float s) {
+    if (VEC == 2) S::st2(p, i, v[0], v[1], s);
+    else S::st(p, i, v[0], s);
+}
+
+template <class S, int VEC>
+struct MarchCtx {
+    Planes3<S> f, vin;
+    Out4<S> vout;
+    Dims d;
+    float mu, inv_f, inv_v, sc_v;
+    int lane, ty, x0, y0, xb, y, qe;
+    bool act, blk_edge;
+    i64 P, col;
+    // apron: this thread loads VEC elements at plane offset hidx and writes
+    // element hsel (or all of them for a row apron) to LDS (hly, hlx)
+    i64 hidx;
+    bool hok, hrow, hcolcell;
+    int hsel, hly, hlx;
+};
+
+template <class S, int VEC, bool FINISH, int A, int B, int C, int E>
+__device__ __forceinline__ void march_plane(const MarchCtx<S, VEC>& m, int q, float (&ring)[4][3][VEC],
+                                            float (&fr)[2][3][VEC], float (&hr)[2][3][VEC],
+                                            float (&lds)[2][3][GTY + 2][64 * VEC + 2],
+                                            float (&res)[3][GTY][64 * VEC]) {
+    constexpr int TXV = 64 * VEC;
+    const int b = q & 1;
+    const int lane = m.lane, ty = m.ty;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int k = 0; k < VEC; k++) lds[b][c][ty + 1][1 + lane * VEC + k] = ring[B][c][k];
+        if (m.hrow) {
+#pragma unroll
+            for (int k = 0; k < VEC; k++) lds[b][c][m.hly][m.hlx + k] = hr[E][c][k];
+        } else if (m.hcolcell) {
+            // bitwise select: a select between two elements of the register
+            // array would be folded into a dynamic index and sent to scratch
+            const unsigned lo = __float_as_uint(hr[E][c][0]), hi = __float_as_uint(hr[E][c][VEC - 1]);
+            const unsigned msk = (VEC == 2 && m.hsel) ? 0xffffffffu : 0u;
+            lds[b][c][m.hly][m.hlx] = __uint_as_float(lo ^ ((lo ^ hi) & msk));
         }
-        const i64 w = d.at(x, y, z);
-        S::st(vout.x, w, r[0], sc_v);
-        S::st(vout.y, w, r[1], sc_v);
-        S::st(vout.z, w, r[2], sc_v);
+    }
+    __syncthreads();
+    float r[3][VEC];
+    float m2[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; k++)
+        m2[k] = fr[E][0][k] * fr[E][0][k] + fr[E][1][k] * fr[E][1][k] + fr[E][2][k] * fr[E][2][k];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            const int lx = 1 + lane * VEC + k;
+            r[c][k] = gvf_update(ring[B][c][k], lds[b][c][ty + 1][lx + 1], lds[b][c][ty + 1][lx - 1],
+                                 lds[b][c][ty + 2][lx], lds[b][c][ty][lx], ring[C][c][k], ring[A][c][k],
+                                 fr[E][c][k], m2[k], m.mu);
+        }
+    if (m.blk_edge) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int k = 0; k < VEC; k++) res[c][ty][lane * VEC + k] = r[c][k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            const int x = m.xb + k;
+            if (m.act && (x == 0 || x == m.d.X - 1 || m.y == 0 || m.y == m.d.Y - 1)) {
+                const int mx = remap(x, m.d.X) - m.x0, my = remap(m.y, m.d.Y) - m.y0;
+#pragma unroll
+                for (int c = 0; c < 3; c++) r[c][k] = res[c][my][mx];
+            }
+        }
+    }
+    if (m.act) {
+        float wv[VEC];
         if (FINISH) {
-            // the magnitude is taken of the stored (rounded) components
-            float a = S::ld(vout.x, w, inv_v), b = S::ld(vout.y, w, inv_v), c = S::ld(vout.z, w, inv_v);
-            float l = sqrtf(a * a + b * b + c * c);
-            S::st(vout.w, w, l > 0.0f ? l : 1.0f, sc_v);
+            // magnitude of the stored (rounded) components
+#pragma unroll
+            for (int k = 0; k < VEC; k++) {
+                typename S::T t0, t1, t2;
+                S::st(&t0, 0, r[0][k], m.sc_v);
+                S::st(&t1, 0, r[1][k], m.sc_v);
+                S::st(&t2, 0, r[2][k], m.sc_v);
+                const float ax = S::ld(&t0, 0, m.inv_v), ay = S::ld(&t1, 0, m.inv_v), az = S::ld(&t2, 0, m.inv_v);
+                const float l = sqrtf(ax * ax + ay * ay + az * az);
+                wv[k] = l > 0.0f ? l : 1.0f;
+            }
         }
+        const int nout = 1 + (q == 2 ? 1 : 0) + (q == m.d.Z - 3 ? 1 : 0);
+        const int planes[3] = {q, q == 2 ? 0 : m.d.Z - 1, m.d.Z - 1};
+        typename S::T* OP[3] = {m.vout.x, m.vout.y, m.vout.z};
+#pragma unroll
+        for (int o = 0; o < 3; o++) {
+            if (o >= nout) break;
+            const i64 i = planes[o] * m.P + m.col;
+#pragma unroll
+            for (int c = 0; c < 3; c++) stv<S, VEC>(OP[c], i, r[c], m.sc_v);
+            if (FINISH) stv<S, VEC>(m.vout.w, i, wv, m.sc_v);
+        }
+    }
+    // issue the loads two planes ahead: v(q+3) into the slot of v(q-1),
+    // f(q+2) and the apron of q+2 into the current f/apron slots
+    const typename S::T* VP[3] = {m.vin.x, m.vin.y, m.vin.z};
+    const typename S::T* FP[3] = {m.f.x, m.f.y, m.f.z};
+    const int zl = min(q + 3, m.d.Z - 1), zf = min(q + 2, m.d.Z - 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ldv<S, VEC>(VP[c], zl * m.P + m.col, m.act, m.inv_v, ring[A][c]);
+        ldv<S, VEC>(FP[c], zf * m.P + m.col, m.act, m.inv_f, fr[E][c]);
+        ldv<S, VEC>(VP[c], zf * m.P + m.hidx, m.hok, m.inv_v, hr[E][c]);
+    }
+    (void)TXV;
+}
+
+template <class S, int VEC, bool FINISH>
+__global__ void __launch_bounds__(64 * GTY) k_gvf_march(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d,
+                                                        float mu, float inv_f, float inv_v, float sc_v) {
+    constexpr int TXV = 64 * VEC;
+    __shared__ float lds[2][3][GTY + 2][TXV + 2];
+    __shared__ float res[3][GTY][TXV];
+    MarchCtx<S, VEC> m;
+    m.f = f;
+    m.vin = vin;
+    m.vout = vout;
+    m.d = d;
+    m.mu = mu;
+    m.inv_f = inv_f;
+    m.inv_v = inv_v;
+    m.sc_v = sc_v;
+    m.lane = threadIdx.x;
+    m.ty = threadIdx.y;
+    m.x0 = tile_origin(blockIdx.x, TXV, d.X);
+    m.y0 = tile_origin(blockIdx.y, GTY, d.Y);
+    m.xb = m.x0 + m.lane * VEC;
+    m.y = m.y0 + m.ty;
+    m.act = m.y < d.Y && m.xb < d.X;  // VEC == 2 only runs with X even
+    m.blk_edge = m.x0 == 0 || m.y0 == 0 || m.x0 + TXV >= d.X || m.y0 + GTY >= d.Y;
+    const int qs = 1 + blockIdx.z * ZC;
+    m.qe = min(qs + ZC, d.Z - 1);
+    m.P = (i64)d.X * d.Y;
+    m.col = m.act ? (i64)m.y * d.X + m.xb : 0;
+    // apron roles: rows y0-1 (ty 0) and y0+GTY (ty 1) for the thread's own
+    // columns; columns x0-1 (ty 2) and x0+TXV (ty 3) for row y0 + lane
+    m.hrow = m.ty < 2;
+    m.hcolcell = !m.hrow && m.lane < GTY;
+    m.hsel = 0;
+    int hx, hy;
+    if (m.hrow) {
+        hy = m.ty == 0 ? m.y0 - 1 : m.y0 + GTY;
+        hx = m.xb;
+        m.hly = m.ty == 0 ? 0 : GTY + 1;
+        m.hlx = 1 + m.lane * VEC;
+        m.hok = hy >= 0 && hy < d.Y && hx < d.X;
+    } else {
+        const int cx = m.ty == 2 ? m.x0 - 1 : m.x0 + TXV;
+        hy = m.y0 + m.lane;
+        hx = VEC == 2 ? (cx & ~1) : cx;  // aligned pair holding the apron cell
+        m.hsel = VEC == 2 ? (cx & 1) : 0;
+        m.hly = m.lane + 1;
+        m.hlx = m.ty == 2 ? 0 : TXV + 1;
+        m.hok = m.hcolcell && cx >= 0 && cx < d.X && hy < d.Y;
+    }
+    m.hidx = m.hok ? (i64)hy * d.X + hx : 0;
+
+    const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
+    const typename S::T* FP[3] = {f.x, f.y, f.z};
+    float ring[4][3][VEC], fr[2][3][VEC], hr[2][3][VEC];
+    const int z3 = min(qs + 2, d.Z - 1), zf1 = min(qs + 1, d.Z - 1);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ldv<S, VEC>(VP[c], (qs - 1) * m.P + m.col, m.act, inv_v, ring[0][c]);
+        ldv<S, VEC>(VP[c], qs * m.P + m.col, m.act, inv_v, ring[1][c]);
+        ldv<S, VEC>(VP[c], (qs + 1) * m.P + m.col, m.act, inv_v, ring[2][c]);
+        ldv<S, VEC>(VP[c], z3 * m.P + m.col, m.act, inv_v, ring[3][c]);
+        ldv<S, VEC>(FP[c], qs * m.P + m.col, m.act, inv_f, fr[0][c]);
+        ldv<S, VEC>(FP[c], zf1 * m.P + m.col, m.act, inv_f, fr[1][c]);
+        ldv<S, VEC>(VP[c], qs * m.P + m.hidx, m.hok, inv_v, hr[0][c]);
+        ldv<S, VEC>(VP[c], zf1 * m.P + m.hidx, m.hok, inv_v, hr[1][c]);
+    }
+    for (int q = qs; q < m.qe; q += 4) {
+        march_plane<S, VEC, FINISH, 0, 1, 2, 0>(m, q, ring, fr, hr, lds, res);
+        if (q + 1 < m.qe) march_plane<S, VEC, FINISH, 1, 2, 3, 1>(m, q + 1, ring, fr, hr, lds, res);
+        if (q + 2 < m.qe) march_plane<S, VEC, FINISH, 2, 3, 0, 0>(m, q + 2, ring, fr, hr, lds, res);
+        if (q + 3 < m.qe) march_plane<S, VEC, FINISH, 3, 0, 1, 1>(m, q + 3, ring, fr, hr, lds, res);
     }
 }
 
@@ -86,6 +276,20 @@ __global__ void k_gvf_copy_finish(Planes3<S> f, Out4<S> vout, i64 n, float inv_f
         S::st(vout.w, i, l > 0.0f ? l : 1.0f, sc_v);
     }
 }
+
+template <class S, int VEC>
+static void launch_step(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, Dims d, float mu, float inv_f,
+                        float inv_src, float sc_v, hipStream_t stream) {
+    constexpr int TXV = 64 * VEC;
+    dim3 grid((d.X + TXV - 1) / TXV, (d.Y + GTY - 1) / GTY, (d.Z - 2 + ZC - 1) / ZC);
+    dim3 block(64, GTY);
+    if (finish)
+        hipLaunchKernelGGL((k_gvf_march<S, VEC, true>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src, sc_v);
+    else
+        hipLaunchKernelGGL((k_gvf_march<S, VEC, false>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src, sc_v);
+}
+
+static bool aligned(const void* p, size_t a) { return ((uintptr_t)p % a) == 0; }
 
 template <class S>
 static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, float mu, int iters, void* ws,
@@ -103,20 +307,20 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
     Out4<S> Wk{A, A + N, A + 2 * N, nullptr};
     Planes3<S> Wr{A, A + N, A + 2 * N};
     Planes3<S> Or{O.x, O.y, O.z};
-    dim3 grid((d.X + GX - 1) / GX, (d.Y + GY - 1) / GY, (d.Z + ZRUN - 1) / ZRUN);
-    dim3 block(GX, GY);
+    // two-column vector accesses need even rows and aligned planes
+    const size_t al = 2 * sizeof(T);
+    bool vec2 = (d.X % 2) == 0;
+    const void* ptrs[8] = {f->x, f->y, f->z, out->x, out->y, out->z, out->w, A};
+    for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, al));
     for (int i = 1; i <= iters; i++) {
+        // ping-pong so that the last step lands in `out`
         const bool to_out = ((iters - i) % 2) == 0;
         Planes3<S> src = (i == 1) ? F : (to_out ? Wr : Or);
         const float inv_src = (i == 1) ? inv_f : inv_v;
-        if (to_out) {
-            if (i == iters)
-                hipLaunchKernelGGL((k_gvf_step<S, true>), grid, block, 0, stream, F, src, O, d, mu, inv_f, inv_src, sc_v);
-            else
-                hipLaunchKernelGGL((k_gvf_step<S, false>), grid, block, 0, stream, F, src, O, d, mu, inv_f, inv_src, sc_v);
-        } else {
-            hipLaunchKernelGGL((k_gvf_step<S, false>), grid, block, 0, stream, F, src, Wk, d, mu, inv_f, inv_src, sc_v);
-        }
+        Out4<S> dst = to_out ? O : Wk;
+        const bool fin = to_out && i == iters;
+        if (vec2) launch_step<S, 2>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+        else launch_step<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
         TSF_TRY(check_launch("tsf_gvf_euler"));
     }
     return TSF_OK;

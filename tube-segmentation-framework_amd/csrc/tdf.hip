@@ -1,146 +1,10 @@
-// Hessian eigenanalysis, circle-fitting TDF, candidate non-maximum
-// suppression and cube maximum.  Gather/VALU-bound per-voxel kernels; the
-// eigensolver runs in registers (eigen3.hpp).
-#include <math.h>
-#include <stdio.h>
-#include <stdlib.h>
-
+// Hessian eigenanalysis, candidate non-maximum suppression and cube maximum.
+// Gather/VALU-bound per-voxel kernels; the eigensolver runs in registers
+// (eigen3.hpp).  The circle-fitting TDF lives in circle.hip.
 #include "eigen3.hpp"
-#include "tsf_common.hpp"
+#include "field.hpp"
 
 namespace tsf {
-
-// Field planes as a kernel argument (any storage).
-template <class S>
-struct Field {
-    const typename S::T* x;
-    const typename S::T* y;
-    const typename S::T* z;
-    const typename S::T* w;
-    float inv;  // 1 / scale
-    __device__ __forceinline__ float c(int comp, i64 i) const {
-        return S::ld(comp == 0 ? x : (comp == 1 ? y : z), i, inv);
-    }
-    __device__ __forceinline__ float mag(i64 i) const { return S::ld(w, i, inv); }
-};
-
-// Jacobian "Hessian" of circleFittingTDF / findCandidateCenterpoints2
-// (gradient / gradientNormalized, kernels_no_3d_write.cl:90-200; layout
-// 1522-1526).  Samples clamp to the edge; with NORM each sample is divided by
-// the w plane of its own voxel.
-template <class S, bool NORM>
-__device__ __forceinline__ void hessian(const Field<S>& F, const Dims& d, int x, int y, int z, float H[3][3]) {
-    const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1);
-    const int ym = max(y - 1, 0), yp = min(y + 1, d.Y - 1);
-    const int zm = max(z - 1, 0), zp = min(z + 1, d.Z - 1);
-    const i64 ixp = d.at(xp, y, z), ixm = d.at(xm, y, z);
-    const i64 iyp = d.at(x, yp, z), iym = d.at(x, ym, z);
-    const i64 izp = d.at(x, y, zp), izm = d.at(x, y, zm);
-    float a0, a1, b0, b1, b2, b3, c0, c1, c2, c3, c4, c5;
-    a0 = F.c(0, ixp); a1 = F.c(0, ixm);
-    b0 = F.c(1, ixp); b1 = F.c(1, ixm); b2 = F.c(1, iyp); b3 = F.c(1, iym);
-    c0 = F.c(2, ixp); c1 = F.c(2, ixm); c2 = F.c(2, iyp); c3 = F.c(2, iym); c4 = F.c(2, izp); c5 = F.c(2, izm);
-    if (NORM) {
-        const float wxp = F.mag(ixp), wxm = F.mag(ixm), wyp = F.mag(iyp), wym = F.mag(iym);
-        const float wzp = F.mag(izp), wzm = F.mag(izm);
-        a0 = a0 / wxp; a1 = a1 / wxm;
-        b0 = b0 / wxp; b1 = b1 / wxm; b2 = b2 / wyp; b3 = b3 / wym;
-        c0 = c0 / wxp; c1 = c1 / wxm; c2 = c2 / wyp; c3 = c3 / wym; c4 = c4 / wzp; c5 = c5 / wzm;
-    }
-    const float fxx = 0.5f * (a0 - a1);
-    const float fyx = 0.5f * (b0 - b1), fyy = 0.5f * (b2 - b3);
-    const float fzx = 0.5f * (c0 - c1), fzy = 0.5f * (c2 - c3), fzz = 0.5f * (c4 - c5);
-    H[0][0] = fxx; H[0][1] = fyx; H[0][2] = fzx;
-    H[1][0] = fyx; H[1][1] = fyy; H[1][2] = fzy;
-    H[2][0] = fzx; H[2][1] = fzy; H[2][2] = fzz;
-}
-
-// The 32 circle directions at integer radians j = 0..31, rounded to six
-// significant digits as "%g" prints cos(j) and sin(j) (the tables of
-// kernels_no_3d_write.cl:1500-1501 hold exactly these values).
-struct CircleTables {
-    float c[32];
-    float s[32];
-};
-
-static CircleTables make_tables() {
-    CircleTables t;
-    char buf[64];
-    for (int j = 0; j < 32; j++) {
-        snprintf(buf, sizeof buf, "%g", cos((double)j));
-        t.c[j] = strtof(buf, nullptr);
-        snprintf(buf, sizeof buf, "%g", sin((double)j));
-        t.s[j] = strtof(buf, nullptr);
-    }
-    return t;
-}
-
-// OpenCL CLK_FILTER_LINEAR, unnormalised coordinates, clamp-to-edge: sample
-// at u - 0.5, eight terms in the OpenCL specification's order.
-template <class S>
-__device__ __forceinline__ void trilinear(const Field<S>& F, const Dims& d, float u, float v, float w, float out[3]) {
-    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
-    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
-    const float a = uu - fu, b = vv - fv, c = ww - fw;
-    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
-    const int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
-    i0 = clampi(i0, 0, d.X - 1);
-    j0 = clampi(j0, 0, d.Y - 1);
-    k0 = clampi(k0, 0, d.Z - 1);
-    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
-    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
-    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
-    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
-    const i64 r00 = (i64)d.X * ((i64)j0 + (i64)d.Y * k0), r10 = (i64)d.X * ((i64)j1 + (i64)d.Y * k0);
-    const i64 r01 = (i64)d.X * ((i64)j0 + (i64)d.Y * k1), r11 = (i64)d.X * ((i64)j1 + (i64)d.Y * k1);
-#pragma unroll
-    for (int comp = 0; comp < 3; comp++) {
-        float s = wt0 * F.c(comp, r00 + i0);
-        s = s + wt1 * F.c(comp, r00 + i1);
-        s = s + wt2 * F.c(comp, r10 + i0);
-        s = s + wt3 * F.c(comp, r10 + i1);
-        s = s + wt4 * F.c(comp, r01 + i0);
-        s = s + wt5 * F.c(comp, r01 + i1);
-        s = s + wt6 * F.c(comp, r11 + i0);
-        s = s + wt7 * F.c(comp, r11 + i1);
-        out[comp] = s;
-    }
-}
-
-// circleFittingTDF, kernels_no_3d_write.cl:1503-1572.
-template <class S, bool NORM>
-__global__ void __launch_bounds__(256) k_tdf(Field<S> F, Dims d, float rmin, float rmax, float rstep,
-                                             CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
-    if (x >= d.X || y >= d.Y) return;
-    float H[3][3], V[3][3], ev[3];
-    hessian<S, NORM>(F, d, x, y, z, H);
-    eigen3(H, V, ev);
-    const float e2x = V[0][1], e2y = V[1][1], e2z = V[2][1];
-    const float e3x = V[0][2], e3y = V[1][2], e3z = V[2][2];
-    float maxSum = 0.0f, maxRadius = 0.0f;
-    const float fx = (float)x, fy = (float)y, fz = (float)z;
-    for (float radius = rmin; radius <= rmax; radius += rstep) {
-        float sum = 0.0f;
-        for (int j = 0; j < 32; j++) {
-            const float cj = tab.c[j], sj = tab.s[j];
-            const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
-            float s[3];
-            trilinear<S>(F, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
-            sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
-        }
-        sum /= 32.0f;
-        if (sum > maxSum) {
-            maxSum = sum;
-            maxRadius = radius;
-        } else {
-            break;
-        }
-    }
-    const i64 i = d.at(x, y, z);
-    tdf[i] = maxSum;
-    rad[i] = maxRadius;
-}
 
 template <class S, bool NORM>
 __global__ void __launch_bounds__(256) k_hessian_eigen(Field<S> F, Dims d, float* __restrict__ lam,
@@ -148,7 +12,7 @@ __global__ void __launch_bounds__(256) k_hessian_eigen(Field<S> F, Dims d, float
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
     if (x >= d.X || y >= d.Y) return;
     float H[3][3], V[3][3], ev[3];
-    hessian<S, NORM>(F, d, x, y, z, H);
+    hessian<NORM>(F, d, x, y, z, H);
     eigen3(H, V, ev);
     const i64 i = d.at(x, y, z), N = d.n();
     lam[i] = ev[0];
@@ -211,7 +75,7 @@ __global__ void __launch_bounds__(256) k_candidate_nms(const float* __restrict__
     const float rr = radius[d.at(x, y, z)];
     const int maxD = (int)fmaxf(fminf(roundf(rr), (float)cap), 1.0f);
     float H[3][3], V[3][3], ev[3];
-    hessian<S, true>(F, d, x, y, z, H);
+    hessian<true>(F, d, x, y, z, H);
     eigen3(H, V, ev);
     const float e1x = V[0][0], e1y = V[1][0], e1z = V[2][0];
     const float wp = F.mag(d.at(x, y, z));
@@ -271,53 +135,12 @@ __global__ void k_cube_max(const float* __restrict__ tdf, const uint8_t* __restr
 
 using namespace tsf;
 
-static tsf_status field_norm_check(const tsf_field* f, bool need_w, const char* W) { return check_field(f, need_w, W); }
-
 extern "C" {
-
-tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep, float* tdf,
-                          float* radius, hipStream_t stream) {
-    const char* W = "tsf_tdf_circle";
-    const bool norm = !(rmax < 4.0f);
-    TSF_TRY(field_norm_check(field, norm, W));
-    TSF_TRY(check_dims(dims, 1, W));
-    if (!tdf || !radius) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL output", W);
-    if (!(rstep > 0.0f) || !__builtin_isfinite(rmin) || !__builtin_isfinite(rmax))
-        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: radius range must be finite with rstep > 0", W);
-    if ((double)(rmax - rmin) / rstep > 100000.0)
-        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: more than 1e5 radii", W);
-    static const CircleTables tab = make_tables();
-    Dims d = D(dims);
-    dim3 grid((d.X + 63) / 64, (d.Y + 3) / 4, d.Z), block(64, 4);
-    const float inv = 1.0f / field->scale;
-    switch (field->storage) {
-        case TSF_STORAGE_F32: {
-            Field<F32> F{(const float*)field->x, (const float*)field->y, (const float*)field->z, (const float*)field->w, inv};
-            if (norm) hipLaunchKernelGGL((k_tdf<F32, true>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-            else hipLaunchKernelGGL((k_tdf<F32, false>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-            break;
-        }
-        case TSF_STORAGE_F16: {
-            typedef __half T;
-            Field<F16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
-            if (norm) hipLaunchKernelGGL((k_tdf<F16, true>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-            else hipLaunchKernelGGL((k_tdf<F16, false>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-            break;
-        }
-        default: {
-            typedef __hip_bfloat16 T;
-            Field<BF16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
-            if (norm) hipLaunchKernelGGL((k_tdf<BF16, true>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-            else hipLaunchKernelGGL((k_tdf<BF16, false>), grid, block, 0, stream, F, d, rmin, rmax, rstep, tab, tdf, radius);
-        }
-    }
-    return check_launch(W);
-}
 
 tsf_status tsf_hessian_eigen(const tsf_field* field, tsf_dims dims, int32_t normalize, float* lambda, float* vectors,
                              hipStream_t stream) {
     const char* W = "tsf_hessian_eigen";
-    TSF_TRY(field_norm_check(field, normalize != 0, W));
+    TSF_TRY(check_field(field, normalize != 0, W));
     TSF_TRY(check_dims(dims, 1, W));
     if (!lambda) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL lambda", W);
     Dims d = D(dims);

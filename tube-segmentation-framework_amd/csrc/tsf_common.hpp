@@ -30,15 +30,33 @@ __host__ __device__ inline int remap(int p, int N) { return p == 0 ? 2 : (p >= N
 // Vector-field planes are stored as float, half or bfloat16.  Reduced
 // precision planes hold value * scale (scale is a power of two, so the
 // scaling itself is exact); arithmetic is always float.
+// ld2/st2 move two consecutive elements with one access (i must be even and
+// the plane base suitably aligned: 8 bytes for F32, 4 bytes otherwise).
 struct F32 {
     typedef float T;
     static __device__ __forceinline__ float ld(const T* p, i64 i, float) { return p[i]; }
     static __device__ __forceinline__ void st(T* p, i64 i, float v, float) { p[i] = v; }
+    static __device__ __forceinline__ void ld2(const T* p, i64 i, float, float& a, float& b) {
+        const float2 v = *reinterpret_cast<const float2*>(p + i);
+        a = v.x;
+        b = v.y;
+    }
+    static __device__ __forceinline__ void st2(T* p, i64 i, float a, float b, float) {
+        *reinterpret_cast<float2*>(p + i) = make_float2(a, b);
+    }
 };
 struct F16 {
     typedef __half T;
     static __device__ __forceinline__ float ld(const T* p, i64 i, float inv) { return __half2float(p[i]) * inv; }
     static __device__ __forceinline__ void st(T* p, i64 i, float v, float s) { p[i] = __float2half_rn(v * s); }
+    static __device__ __forceinline__ void ld2(const T* p, i64 i, float inv, float& a, float& b) {
+        const __half2 h = *reinterpret_cast<const __half2*>(p + i);
+        a = __low2float(h) * inv;
+        b = __high2float(h) * inv;
+    }
+    static __device__ __forceinline__ void st2(T* p, i64 i, float a, float b, float s) {
+        *reinterpret_cast<__half2*>(p + i) = __halves2half2(__float2half_rn(a * s), __float2half_rn(b * s));
+    }
 };
 struct BF16 {
     typedef __hip_bfloat16 T;
@@ -47,6 +65,16 @@ struct BF16 {
     }
     static __device__ __forceinline__ void st(T* p, i64 i, float v, float s) {
         p[i] = __float2bfloat16(v * s);
+    }
+    static __device__ __forceinline__ void ld2(const T* p, i64 i, float inv, float& a, float& b) {
+        const unsigned u = *reinterpret_cast<const unsigned*>(p + i);
+        a = __uint_as_float(u << 16) * inv;
+        b = __uint_as_float(u & 0xffff0000u) * inv;
+    }
+    static __device__ __forceinline__ void st2(T* p, i64 i, float a, float b, float s) {
+        const __hip_bfloat16 x = __float2bfloat16(a * s), y = __float2bfloat16(b * s);
+        const unsigned u = (unsigned)__bfloat16_as_ushort(x) | ((unsigned)__bfloat16_as_ushort(y) << 16);
+        *reinterpret_cast<unsigned*>(p + i) = u;
     }
 };
 

@@ -13,5 +13,5 @@ from ._capi import (  # noqa: F401
     blur_gaussian, create_blur_mask, to_float, create_vector_field, gvf_euler, gvf_fmg,
     hessian_eigen, do_eigen_points, tdf_circle, combine, candidates, hp_compact,
     candidate_nms, cube_max, gvf_euler_workspace_size, hp_compact_workspace_size,
-    gvf_fmg_workspace_size, device_count, exported_symbols,
+    gvf_fmg_workspace_size, tdf_circle_workspace_size, device_count, exported_symbols,
 )

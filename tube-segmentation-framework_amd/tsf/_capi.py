@@ -58,7 +58,8 @@ _SIGS = {
     "tsf_gvf_fmg_workspace_size": ([Dims, _i32], _sz),
     "tsf_hessian_eigen": ([_PF, Dims, _i32, _vp, _vp, _vp], _i32),
     "tsf_do_eigen_points": ([_vp, _vp, _vp, Dims, _vp, _i64, _vp, _vp], _i32),
-    "tsf_tdf_circle": ([_PF, Dims, _f32, _f32, _f32, _vp, _vp, _vp], _i32),
+    "tsf_tdf_circle": ([_PF, Dims, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
+    "tsf_tdf_circle_workspace_size": ([Dims, _i32], _sz),
     "tsf_combine": ([_vp, _vp, _vp, _vp, _i64, _vp], _i32),
     "tsf_candidates": ([_vp, _vp, _i64, _f32, _vp], _i32),
     "tsf_hp_compact": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp, _sz, _vp], _i32),
@@ -312,9 +313,14 @@ def do_eigen_points(fx, fy, fz, positions, out, stream=None):
                                      _sp(stream)))
 
 
-def tdf_circle(field, rmin, rmax, rstep, tdf, radius, stream=None):
+def tdf_circle_workspace_size(shape, storage):
+    return lib().tsf_tdf_circle_workspace_size(_dims(shape), storage)
+
+
+def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=None):
     d = field.desc()
-    _check(lib().tsf_tdf_circle(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, tdf.ptr, radius.ptr,
+    w, wb = _ws(workspace)
+    _check(lib().tsf_tdf_circle(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, tdf.ptr, radius.ptr, w, wb,
                                 _sp(stream)))
 
 
