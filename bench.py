@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--fuse", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run a short fixed workload for hardware-counter passes (no timing output)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -78,6 +80,23 @@ def main():
     t_l, r_l = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
     tws = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, S),), np.uint8)
     ev = [tsf.Event() for _ in range(4)]
+
+    if args.pmc_workload:
+        # Fixed launches for counter passes: k_gvf_copy_finish on F32 planes is
+        # the calibration copy (reads 3 x 4 B, writes 4 x 4 B per voxel), then
+        # GVF steps in the benchmark storage and both TDF passes.
+        F32c, V32c = tsf.Field(shape), tsf.Field(shape)
+        tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
+        tsf.create_vector_field(d_blur, F32c, 0.2, 1, stream)
+        tsf.gvf_euler(F32c, V32c, args.mu, 0, 1, None, stream)
+        tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
+        tsf.gvf_euler(F, V, args.mu, 4, 1, ws, stream)
+        tsf.gvf_euler(F32c, V32c, args.mu, 2, 1, None, stream)
+        tsf.tdf_circle(F, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
+        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
+        stream.synchronize()
+        log("pmc workload done")
+        return
 
     def step():
         ev[0].record(stream)
@@ -146,6 +165,18 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol)
 
+    # HBM bytes per launch of the dominant kernel, measured by the two
+    # rocprofv3 counter passes recorded in profiles/ (null when not measured
+    # for this storage type)
+    traffic = None
+    kname = {"f16": "k_gvf_march<F16, 2, false>", "f32": "k_gvf_march<F32, 2, false>"}.get(args.storage)
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_circle_forward_512cubed.json")
+    if kname and n == 512 and os.path.exists(tpath):
+        with open(tpath) as fh:
+            rec = json.load(fh)["kernels"].get(kname)
+        if rec:
+            traffic = round(rec["traffic_bytes"] / 1e9, 4)
+
     out = {
         "metric": "GVF Gvoxel-iter/s (FP16-stored field, 512^3, mu 0.05, 400 iterations)",
         "value": round(gvox, 3),
@@ -168,8 +199,9 @@ def main():
         "wall_s_timed": round(wall, 3),
         "f32_gvf": f32,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM, 4), "traffic": None,
-                     "kernel": "k_gvf_step", "bytes_per_voxel_per_launch": bytes_per_voxel,
+                     "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
+                     "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
+                     "kernel": "k_gvf_march", "bytes_per_voxel_per_launch": bytes_per_voxel,
                      "launch_ms": round(launch_ms, 4)},
         "cpu_baseline": cpu,
     }
@@ -186,10 +218,11 @@ def cpu_baseline(vol):
     o = (vol.shape[0] - c) // 2
     sub = np.ascontiguousarray(vol[o:o + c, o:o + c, o:o + c])
     F = oracle.vector_field(oracle.blur(sub, 0.5, "f32"), 0.2, 1, "f32")
+    oracle.gvf_euler(F, 0.05, 2, "f32")  # thread pool and page warm-up
     t0 = time.time()
-    oracle.gvf_euler(F, 0.05, 2, "f32")
+    oracle.gvf_euler(F, 0.05, 4, "f32")
     t1 = time.time() - t0
-    iters = max(2, int(10.0 / max(t1 / 2, 1e-3)))
+    iters = max(4, int(12.0 / max(t1 / 4, 1e-3)))
     t0 = time.time()
     oracle.gvf_euler(F, 0.05, iters, "f32")
     dt = time.time() - t0
