@@ -1,0 +1,278 @@
+// Tests of the C++ drop-in layer (tube-segmentation-framework_amd/host),
+// written in the style of the reference's gtest suites
+// (tests/parameterTests.cpp, tests/tubeSegmentationTests.cpp): the public
+// entry points are called with a paramList and the reference's presets, and
+// the results are compared with the CPU oracle (oracle/tsf_oracle.cpp) run on
+// the same input.
+//
+//   test_dropin params            parameter system (no device needed)
+//   test_dropin gpu <dataset_dir> runCircleFittingMethod / candidate
+//                                 compaction / doEigen on dataset_1
+// Prints one "PASS name" / "FAIL name: reason" line per case; exit status is
+// the number of failures.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../tube-segmentation-framework_amd/host/tsf_dropin.hpp"
+
+using namespace tsf;
+// HIP's headers declare global int3/float3 vector types; the reference's
+// SIPL::int3/float3 roles are taken by tsf::int3/float3
+using I3 = tsf::int3;
+using F3 = tsf::float3;
+
+// ----------------------------------------------------------------- oracle (extern "C" in libtsf_oracle.so)
+extern "C" {
+void orc_blur_mask(float sigma, float* mask, int* maskSize);
+void orc_blur_f32(const float* in, float* out, int X, int Y, int Z, int m, const float* mask);
+void orc_to_float_f32(const double* in, float* out, long long n, float mn, float mx);
+void orc_vector_field_f32(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign);
+void orc_gvf_euler_f32(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters);
+void orc_tdf_circle_f32(const float* F, int X, int Y, int Z, float rmin, float rmax, float rstep, float* tdf,
+                        float* rad);
+void orc_combine_f32(const float* ts, const float* rs, float* tl, float* rl, long long n);
+void orc_candidates_f32(const float* tdf, uint8_t* mask, long long n, float thr);
+long long orc_hp_compact_sort(const uint8_t* mask, int X, int Y, int Z, int32_t* pos, long long cap);
+void orc_candidate_nms_f32(const float* tdf, const float* rad, const float* F, int X, int Y, int Z,
+                           const int32_t* pos, long long count, int cap, uint8_t* out);
+void orc_cube_max_f32(const float* tdf, const uint8_t* cand, int X, int Y, int Z, int cube, uint8_t* out);
+void orc_do_eigen_f32(const float* Fx, const float* Fy, const float* Fz, int X, int Y, int Z, int x, int y, int z,
+                      float* lam, float* e1, float* e2, float* e3);
+}
+
+static int failures = 0;
+static void run(const char* name, const std::function<std::string()>& fn) {
+    std::string r;
+    try {
+        r = fn();
+    } catch (const std::exception& e) {
+        r = std::string("exception: ") + e.what();
+    }
+    if (r.empty()) std::printf("PASS %s\n", name);
+    else {
+        std::printf("FAIL %s: %s\n", name, r.c_str());
+        failures++;
+    }
+    std::fflush(stdout);
+}
+
+template <class F>
+static bool throws(F f, tsf_status* st = nullptr) {
+    try {
+        f();
+    } catch (const Error& e) {
+        if (st) *st = e.status;
+        return true;
+    }
+    return false;
+}
+
+// ----------------------------------------------------------------- parameters
+static void param_tests() {
+    run("defaults", [] {
+        paramList p = initParameters();
+        if (getParam(p, "gvf-iterations") != 250) return std::string("gvf-iterations");
+        if (getParamStr(p, "mode") != "white") return std::string("mode");
+        if (!getParamBool(p, "16bit-vectors")) return std::string("16bit-vectors");
+        if (std::fabs(getParam(p, "gvf-mu") - 0.05f) > 1e-7f) return std::string("gvf-mu");
+        return std::string();
+    });
+    run("set_and_validate", [] {
+        paramList p = initParameters();
+        setParameter(p, "gvf-iterations", "10");
+        setParameter(p, "mode", "black");
+        setParameter(p, "use-fmg-gvf", "true");
+        if (getParam(p, "gvf-iterations") != 10 || getParamStr(p, "mode") != "black" || !getParamBool(p, "use-fmg-gvf"))
+            return std::string("set values not read back");
+        if (!throws([&] { setParameter(p, "gvf-mu", "0.9"); })) return std::string("out-of-range numeric accepted");
+        if (!throws([&] { setParameter(p, "mode", "grey"); })) return std::string("invalid string accepted");
+        if (!throws([&] { setParameter(p, "no-such-parameter", "1"); })) return std::string("unknown name accepted");
+        if (!throws([&] { getParam(p, "mode"); })) return std::string("wrong-type get accepted");
+        return std::string();
+    });
+    run("preset", [] {
+        paramList p = initParameters();
+        const std::string dir = "/tmp/tsf_preset_test";
+        (void)!std::system(("mkdir -p " + dir + "/centerline-gpu").c_str());
+        std::ofstream(dir + "/centerline-gpu/Example") << "mode white\nsmall-blur 1.0\nfmax 0.1\nradius-max 4.0\n"
+                                                          "centerpoints-only\n";
+        setParameter(p, "parameters", "Example");
+        loadParameterPreset(p, dir);
+        if (std::fabs(getParam(p, "fmax") - 0.1f) > 1e-7f || getParam(p, "radius-max") != 4.0f ||
+            !getParamBool(p, "centerpoints-only"))
+            return std::string("preset values not applied");
+        setParameter(p, "parameters", "Missing");
+        if (!throws([&] { loadParameterPreset(p, dir); })) return std::string("missing preset accepted");
+        return std::string();
+    });
+}
+
+// ----------------------------------------------------------------- GPU
+static std::vector<uint8_t> read_raw(const std::string& fn, size_t n) {
+    std::vector<uint8_t> v(n);
+    std::ifstream f(fn, std::ios::binary);
+    if (!f.read((char*)v.data(), n)) throw std::runtime_error("cannot read " + fn);
+    return v;
+}
+
+static std::string compare(const std::vector<float>& a, const std::vector<float>& b, float rtol, float atol,
+                           const char* what) {
+    size_t bad = 0, worst = 0;
+    double wd = 0;
+    for (size_t i = 0; i < a.size(); i++) {
+        double d = std::fabs((double)a[i] - b[i]);
+        if (d > rtol * std::fabs(b[i]) + atol) {
+            bad++;
+            if (d > wd) {
+                wd = d;
+                worst = i;
+            }
+        }
+    }
+    if (!bad) return std::string();
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %zu of %zu differ, worst at %zu: %g vs %g", what, bad, a.size(), worst,
+                  a[worst], b[worst]);
+    return buf;
+}
+
+static void gpu_tests(const std::string& dsdir) {
+    const int X = 100, Y = 100, Z = 100;
+    const size_t n = (size_t)X * Y * Z;
+    const I3 size(X, Y, Z);
+    std::vector<uint8_t> raw = read_raw(dsdir + "/noisy0.raw", n);
+    // Synthetic-Vascusynth preset of the gpu centerline method, float vectors
+    paramList p = initParameters();
+    for (auto kv : std::vector<std::pair<const char*, const char*>>{
+             {"mode", "white"}, {"small-blur", "1.0"}, {"fmax", "0.1"}, {"minimum", "0"}, {"maximum", "75"},
+             {"min-mean-tdf", "0.3"}, {"tdf-high", "0.35"}, {"cube-size", "3"}, {"min-tree-length", "3"},
+             {"max-distance", "10.0"}, {"radius-max", "4.0"}, {"16bit-vectors", "false"}})
+        setParameter(p, kv.first, kv.second);
+
+    OpenCL ocl;
+    // toFloat on the device
+    Buffer draw(n);
+    draw.upload(ocl, raw.data(), n);
+    Image3D dataset(size, 1);
+    check(tsf_to_float(draw.get(), 0, (float*)dataset.plane(0), dataset.dims(), 0.0f, 75.0f, ocl.queue));
+
+    Image3D vf, tdf, rad;
+    Image3D ds = dataset;
+    runCircleFittingMethod(ocl, &ds, size, p, vf, tdf, rad);
+    std::vector<float> gT = tdf.downloadPlane(ocl, 0), gR = rad.downloadPlane(ocl, 0);
+
+    // oracle pipeline: toFloat, blur (small 1.0, large 1.0), vector field,
+    // TDF small 0.5..3, GVF 250 (mu 0.05), TDF large 2.5..4, combine
+    std::vector<double> rd(raw.begin(), raw.end());
+    std::vector<float> vol(n), blur(n), Fs(4 * n), F(4 * n), V(4 * n), ts(n), rs(n), tl(n), rl(n);
+    orc_to_float_f32(rd.data(), vol.data(), (long long)n, 0.0f, 75.0f);
+    float mask[1331];
+    int m;
+    orc_blur_mask(1.0f, mask, &m);
+    orc_blur_f32(vol.data(), blur.data(), X, Y, Z, m, mask);
+    orc_vector_field_f32(blur.data(), Fs.data(), X, Y, Z, 0.1f, 1);
+    orc_tdf_circle_f32(Fs.data(), X, Y, Z, 0.5f, 3.0f, 0.5f, ts.data(), rs.data());
+    orc_vector_field_f32(blur.data(), F.data(), X, Y, Z, 0.1f, 1);  // large-blur 1.0 (default)
+    orc_gvf_euler_f32(F.data(), V.data(), X, Y, Z, 0.05f, 250);
+    orc_tdf_circle_f32(V.data(), X, Y, Z, 2.5f, 4.0f, 1.0f, tl.data(), rl.data());
+    orc_combine_f32(ts.data(), rs.data(), tl.data(), rl.data(), (long long)n);
+
+    run("runCircleFittingMethod_field", [&] {
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++) {
+            std::vector<float> g = vf.downloadPlane(ocl, c), o(V.begin() + c * n, V.begin() + (c + 1) * n);
+            r = compare(g, o, 1e-4f, 1e-7f, "GVF field");
+        }
+        return r;
+    });
+    run("runCircleFittingMethod_tdf", [&] { return compare(gT, tl, 1e-4f, 1e-7f, "TDF"); });
+    run("runCircleFittingMethod_radius", [&] { return compare(gR, rl, 0.0f, 0.0f, "radius"); });
+
+    // candidate compaction, on the GPU's own TDF so the mask is shared
+    run("runNewCenterlineAlgCandidates", [&] {
+        Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
+        CenterpointVertices cv = runNewCenterlineAlgCandidates(ocl, size, p, vf2, tdf2, rad2);
+        std::vector<float> Vg(4 * n);
+        for (int c = 0; c < 4; c++) {
+            std::vector<float> g = vf.downloadPlane(ocl, c);
+            std::copy(g.begin(), g.end(), Vg.begin() + c * n);
+        }
+        std::vector<uint8_t> cand(n), nms(n, 0), cells(n, 0);
+        orc_candidates_f32(gT.data(), cand.data(), (long long)n, 0.35f);
+        std::vector<int32_t> pos(3 * n);
+        long long k = orc_hp_compact_sort(cand.data(), X, Y, Z, pos.data(), (long long)n);
+        if (k != cv.candidates) return std::string("candidate count differs");
+        orc_candidate_nms_f32(gT.data(), gR.data(), Vg.data(), X, Y, Z, pos.data(), k, 8, nms.data());
+        std::vector<uint8_t> gn(n);
+        cv.centerpoints.download(ocl, gn.data(), n);
+        if (gn != nms) return std::string("suppressed centerpoints differ");
+        orc_cube_max_f32(gT.data(), nms.data(), X, Y, Z, 3, cells.data());
+        long long nv = orc_hp_compact_sort(cells.data(), X, Y, Z, pos.data(), (long long)n);
+        if (nv != cv.count) return std::string("vertex count differs");
+        std::vector<int32_t> gv(3 * nv);
+        cv.vertices.download(ocl, gv.data(), gv.size() * 4);
+        for (long long i = 0; i < 3 * nv; i++)
+            if (gv[i] != pos[i]) return std::string("vertex list differs (order or position)");
+        std::printf("  candidates %d, vertices %d\n", cv.candidates, cv.count);
+        return std::string();
+    });
+    run("candidate_count_range_error", [&] {
+        paramList q = p;
+        setParameter(q, "tdf-high", "1.0");
+        Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
+        tsf_status st = TSF_OK;
+        if (!throws([&] { runNewCenterlineAlgCandidates(ocl, size, q, vf2, tdf2, rad2); }, &st))
+            return std::string("no exception for an empty candidate set");
+        return st == TSF_ERR_RANGE ? std::string() : std::string("wrong status");
+    });
+    run("doEigen", [&] {
+        TubeSegmentation T;
+        std::vector<float> fx = vf.downloadPlane(ocl, 0), fy = vf.downloadPlane(ocl, 1), fz = vf.downloadPlane(ocl, 2);
+        T.Fx = fx.data();
+        T.Fy = fy.data();
+        T.Fz = fz.data();
+        for (I3 q : {I3(50, 50, 50), I3(10, 20, 30), I3(1, 1, 1), I3(98, 98, 98)}) {
+            F3 l, a, b, c;
+            doEigen(T, q, size, &l, &a, &b, &c);
+            float ol[3], o1[3], o2[3], o3[3];
+            orc_do_eigen_f32(fx.data(), fy.data(), fz.data(), X, Y, Z, q.x, q.y, q.z, ol, o1, o2, o3);
+            const float g[12] = {l.x, l.y, l.z, a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+            const float o[12] = {ol[0], ol[1], ol[2], o1[0], o1[1], o1[2], o2[0], o2[1], o2[2], o3[0], o3[1], o3[2]};
+            for (int i = 0; i < 12; i++)
+                if (!(std::fabs(g[i] - o[i]) <= 1e-4f * std::fabs(o[i]) + 1e-5f)) return std::string("eigen mismatch");
+            F3 t = getTubeDirection(T, q, size);
+            if (t.x != a.x || t.y != a.y || t.z != a.z) return std::string("getTubeDirection != e1");
+        }
+        releaseEigenMirror(T);
+        return std::string();
+    });
+    run("runGVF_consumes_input_and_rounds_iterations", [&] {
+        paramList q = p;
+        setParameter(q, "gvf-iterations", "7");  // the reference returns iterate 6
+        Image3D f(size, 4);
+        for (int c = 0; c < 4; c++) f.uploadPlane(ocl, c, F.data() + c * n);
+        I3 s = size;
+        Image3D out = runGVF(ocl, &f, q, s, false);
+        if (f.valid()) return std::string("input field not released");
+        std::vector<float> o(4 * n);
+        orc_gvf_euler_f32(F.data(), o.data(), X, Y, Z, 0.05f, 6);
+        return compare(out.downloadPlane(ocl, 0), std::vector<float>(o.begin(), o.begin() + n), 1e-5f, 1e-8f, "x");
+    });
+}
+
+int main(int argc, char** argv) {
+    const std::string mode = argc > 1 ? argv[1] : "params";
+    if (mode == "params") param_tests();
+    else if (mode == "gpu" && argc > 2) gpu_tests(argv[2]);
+    else {
+        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir>\n");
+        return 2;
+    }
+    std::printf("%d failure(s)\n", failures);
+    return failures;
+}

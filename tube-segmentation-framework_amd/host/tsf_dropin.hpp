@@ -1,0 +1,230 @@
+// C++ drop-in layer over the C ABI (include/tsf.h).
+//
+// Mirrors the reference's host interface for the volume-filter path -- same
+// function names, argument order and meaning, ownership and error behaviour
+// -- with the OpenCL handle types replaced by HIP-backed ones of the same
+// role:
+//   OpenCL  (commons.hpp:8-15)        -> tsf::OpenCL: one in-order HIP stream
+//   cl::Image3D / cl::Buffer           -> tsf::Image3D / tsf::Buffer: device
+//                                         volumes / linear buffers (shared
+//                                         ownership, freed with the last copy)
+//   SIPL::int3 / SIPL::float3          -> tsf::int3 / tsf::float3
+//   paramList (parameters.hpp:78-91)   -> tsf::paramList with the same accessors
+//   cl::Error / SIPL::SIPLException    -> tsf::Error (carries the tsf_status)
+// Nothing crosses the C ABI as a C++ type; every call below is a thin
+// sequence of tsf_* calls on the context's stream.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tsf.h"
+
+namespace tsf {
+
+struct int3 {
+    int x = 0, y = 0, z = 0;
+    int3() = default;
+    int3(int a, int b, int c) : x(a), y(b), z(c) {}
+};
+struct float3 {
+    float x = 0, y = 0, z = 0;
+    float3() = default;
+    float3(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+
+class Error : public std::runtime_error {
+   public:
+    Error(tsf_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+    tsf_status status;
+};
+
+// Throws tsf::Error for a failed tsf_* call (the reference's calls throw
+// cl::Error from the OpenCL C++ bindings).
+void check(tsf_status st);
+
+// ----------------------------------------------------------------- parameters
+// parameters.hpp: typed parameters with ranges and allowed values.
+class BoolParameter {
+   public:
+    BoolParameter() = default;
+    BoolParameter(bool v, std::string description, std::string group);
+    bool get() const { return value; }
+    void set(bool v) { value = v; }
+    std::string getDescription() const { return description; }
+    std::string getGroup() const { return group; }
+
+   private:
+    bool value = false;
+    std::string description, group;
+};
+class NumericParameter {
+   public:
+    NumericParameter() = default;
+    NumericParameter(float v, float min, float max, float step, std::string description, std::string group);
+    float get() const { return value; }
+    void set(float v);  // throws tsf::Error outside [min, max]
+    bool validate(float v) const { return v >= min && v <= max; }
+    float getMin() const { return min; }
+    float getMax() const { return max; }
+    float getStep() const { return step; }
+    std::string getDescription() const { return description; }
+
+   private:
+    float value = 0, min = 0, max = 0, step = 0;
+    std::string description, group;
+};
+class StringParameter {
+   public:
+    StringParameter() = default;
+    StringParameter(std::string v, std::vector<std::string> possibilities, std::string description,
+                    std::string group);
+    std::string get() const { return value; }
+    void set(const std::string& v);  // throws tsf::Error if not one of the possibilities
+    void setWithoutValidation(const std::string& v) { value = v; }
+    bool validate(const std::string& v) const;
+    std::vector<std::string> getPossibilities() const { return possibilities; }
+
+   private:
+    std::string value;
+    std::vector<std::string> possibilities;
+    std::string description, group;
+};
+struct paramList {
+    std::map<std::string, BoolParameter> bools;
+    std::map<std::string, NumericParameter> numerics;
+    std::map<std::string, StringParameter> strings;
+};
+
+// Defaults of every parameter the volume-filter path reads (the reference's
+// parameters/parameters file), plus "3d_write" (set by run() from the device
+// in the reference) and "vector-storage" (f32 | f16 | bf16, new: the storage
+// of the GVF field; "auto" follows 16bit-vectors).
+paramList initParameters();
+// Reads a parameters file in the reference's format (name type default ...).
+paramList initParameters(const std::string& parameter_dir);
+// Applies parameter_dir/centerline-<method>/<preset> ("name value" lines) when
+// the "parameters" parameter is not "none" (parameters.cpp:67-106).
+void loadParameterPreset(paramList& parameters, const std::string& parameter_dir);
+void setParameter(paramList& parameters, const std::string& name, const std::string& value);
+float getParam(const paramList& parameters, const std::string& name);
+bool getParamBool(const paramList& parameters, const std::string& name);
+std::string getParamStr(const paramList& parameters, const std::string& name);
+
+// ----------------------------------------------------------------- context and memory
+struct OpenCL {
+    hipStream_t queue = nullptr;
+    OpenCL();
+    ~OpenCL();
+    OpenCL(const OpenCL&) = delete;
+    OpenCL& operator=(const OpenCL&) = delete;
+    void finish();
+};
+
+class Buffer {
+   public:
+    Buffer() = default;
+    explicit Buffer(size_t bytes);
+    void* get() const { return mem ? mem.get() : nullptr; }
+    size_t size() const { return bytes; }
+    void upload(OpenCL& ocl, const void* host, size_t n);
+    void download(OpenCL& ocl, void* host, size_t n) const;
+
+   private:
+    std::shared_ptr<void> mem;
+    size_t bytes = 0;
+};
+
+// A device volume: `channels` planes (1 = scalar volume, 4 = vector field
+// x, y, z, w) of one storage type, each size.x*size.y*size.z elements.
+class Image3D {
+   public:
+    Image3D() = default;
+    Image3D(int3 size, int channels, tsf_storage storage = TSF_STORAGE_F32, float scale = 1.0f);
+    int3 size() const { return sz; }
+    int channels() const { return nch; }
+    tsf_storage storage() const { return st; }
+    float scale() const { return sc; }
+    void* plane(int c) const { return planes.at(c).get(); }
+    bool valid() const { return !planes.empty(); }
+    tsf_field field() const;  // channels == 4
+    tsf_dims dims() const { return tsf_dims{sz.x, sz.y, sz.z}; }
+    // host transfer of plane c as float values (scaled/rounded as stored)
+    void uploadPlane(OpenCL& ocl, int c, const float* host);
+    std::vector<float> downloadPlane(OpenCL& ocl, int c) const;
+
+   private:
+    int3 sz;
+    int nch = 0;
+    tsf_storage st = TSF_STORAGE_F32;
+    float sc = 1.0f;
+    std::vector<Buffer> planes;
+};
+
+// Storage the path uses for the vector field under these parameters.
+tsf_storage vectorStorage(const paramList& parameters);
+float vectorScale(tsf_storage s);
+
+// ----------------------------------------------------------------- the volume-filter path
+// createBlurMask (tube-segmentation.cpp:188-211): caller frees with delete[].
+float* createBlurMask(float sigma, int* maskSizePointer);
+
+// gradientVectorFlow.hpp:8-10.  Consume *vectorField (it is released, as the
+// reference deletes it) and return the new field (channels 4, w = |v|).
+// runGVF returns iterate 2*floor(gvf-iterations/2): the reference's finish
+// step reads the first ping-pong buffer (gradientVectorFlow.cpp:1071, 1161).
+// useLessMemory selects the reference's per-component variant, which computes
+// the same values; here both run the same kernels.
+Image3D runGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size, bool useLessMemory);
+Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size);
+
+// tubeDetectionFilters.hpp:15.  TDF and radius are caller-allocated device
+// buffers of size.x*size.y*size.z floats.
+void runCircleFittingTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* TDF, Buffer* radius,
+                         float radiusMin, float radiusMax, float radiusStep);
+
+// tube-segmentation.cpp:212-939: blur -> vector field -> (small TDF) -> blur
+// -> vector field -> GVF (Euler or FMG) -> large TDF -> combine.  dataset is a
+// float scalar volume in [0, 1] (toFloat output); it is consumed.  Outputs
+// are set by reference as in the reference.
+void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList& parameters, Image3D& vectorField,
+                            Image3D& TDF, Image3D& radiusImage);
+
+// The candidate / compaction part of runNewCenterlineAlg
+// (parallelCenterlineExtraction.cpp:689-899): threshold -> pyramid compaction
+// -> cross-section non-maximum suppression -> cube maximum -> pyramid
+// compaction of the vertices.  Throws tsf::Error with TSF_ERR_RANGE when the
+// candidate count is <= 0 or > half the volume, or the vertex count is < 8 or
+// >= 16384, as the reference does.
+struct CenterpointVertices {
+    Buffer centerpoints;   // uint8[size.x*size.y*size.z]: candidates kept by the suppression
+    Buffer vertices;       // int32[3*count] in pyramid order
+    int count = 0;
+    int candidates = 0;
+};
+CenterpointVertices runNewCenterlineAlgCandidates(OpenCL& ocl, int3 size, paramList& parameters,
+                                                  Image3D& vectorField, Image3D& TDF, Image3D& radius);
+
+// eigenanalysisOfHessian.hpp:7-9 on the host arrays of TubeSegmentation
+// (tube-segmentation.hpp:22-30).  The arrays are mirrored to the device on
+// first use; pos must lie one voxel inside every border.
+struct TubeSegmentation {
+    float *Fx = nullptr, *Fy = nullptr, *Fz = nullptr;
+    float *FxSmall = nullptr, *FySmall = nullptr, *FzSmall = nullptr;
+    float* TDF = nullptr;
+    float* radius = nullptr;
+    char* centerline = nullptr;
+    char* segmentation = nullptr;
+    float* intensity = nullptr;
+};
+void doEigen(TubeSegmentation& T, int3 pos, int3 size, float3* lambda, float3* e1, float3* e2, float3* e3);
+float3 getTubeDirection(TubeSegmentation& T, int3 pos, int3 size);
+// Batched form: one launch for many points.
+void doEigenBatch(TubeSegmentation& T, int3 size, const std::vector<int3>& pos, std::vector<float>& out12);
+// Drops the device mirror of T's arrays (call when the host arrays change).
+void releaseEigenMirror(TubeSegmentation& T);
+
+}  // namespace tsf
