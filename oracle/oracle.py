@@ -209,3 +209,79 @@ def trilinear(F, u, v, w, prec="f64"):
     ct = ctypes.c_float if prec == "f32" else ctypes.c_double
     _fn("trilinear", prec)(_p(F), X, Y, Z, ct(u), ct(v), ct(w), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- adjoints
+_d = ctypes.c_double
+
+
+def blur_sigma(vol, sigma, prec="f64"):
+    """Blur with the mask computed in `prec` (the differentiable chain's blur)."""
+    vol = np.ascontiguousarray(vol, _dt(prec))
+    out = np.empty_like(vol)
+    X, Y, Z = _dims(vol)
+    _fn("blur_sigma", prec)(_p(vol), _p(out), X, Y, Z, _d(sigma))
+    return out
+
+
+def blur_adjoint(dB, I, sigma, prec="f64"):
+    dB = np.ascontiguousarray(dB, _dt(prec))
+    I = np.ascontiguousarray(I, _dt(prec))
+    X, Y, Z = _dims(dB)
+    dI = np.empty_like(dB)
+    ds = _d(0)
+    _fn("blur_adjoint", prec)(_p(dB), _p(I), X, Y, Z, _d(sigma), _p(dI), ctypes.byref(ds))
+    return dI, ds.value
+
+
+def vector_field_adjoint(B, dF, fmax, sign=1, prec="f64"):
+    B = np.ascontiguousarray(B, _dt(prec))
+    dF = np.ascontiguousarray(dF[:3], _dt(prec))
+    X, Y, Z = _dims(B)
+    dB = np.empty_like(B)
+    _fn("vector_field_adjoint", prec)(_p(B), _p(dF), X, Y, Z, ctypes.c_float(fmax), int(sign), _p(dB))
+    return dB
+
+
+def gvf_adjoint(F, mu, iters, dV, prec="f64"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    dV = np.ascontiguousarray(dV[:3], _dt(prec))
+    X, Y, Z = _dims(F)
+    dF = np.empty((3,) + F.shape[1:], _dt(prec))
+    dm = _d(0)
+    _fn("gvf_adjoint", prec)(_p(F), X, Y, Z, _d(mu), int(iters), _p(dV), _p(dF), ctypes.byref(dm))
+    return dF, dm.value
+
+
+def tdf_adjoint(F, rmin, rmax, rstep, gT, prec="f64"):
+    F = np.ascontiguousarray(F, _dt(prec))
+    gT = np.ascontiguousarray(gT, _dt(prec))
+    X, Y, Z = _dims(F)
+    dV = np.empty((3,) + F.shape[1:], _dt(prec))
+    dW = np.empty(F.shape[1:], _dt(prec))
+    _fn("tdf_adjoint", prec)(_p(F), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep),
+                             _p(gT), _p(dV), _p(dW))
+    return dV, dW
+
+
+def chain_forward(I, sigma, fmax, sign, mu, iters, rmin, rmax, rstep, prec="f64"):
+    I = np.ascontiguousarray(I, _dt(prec))
+    X, Y, Z = _dims(I)
+    t = np.empty_like(I)
+    r = np.empty_like(I)
+    _fn("chain_forward", prec)(_p(I), X, Y, Z, _d(sigma), ctypes.c_float(fmax), int(sign), _d(mu), int(iters),
+                               ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep), _p(t), _p(r))
+    return t, r
+
+
+def chain_gradient(I, sigma, fmax, sign, mu, iters, rmin, rmax, rstep, gT, prec="f64"):
+    I = np.ascontiguousarray(I, _dt(prec))
+    gT = np.ascontiguousarray(gT, _dt(prec))
+    X, Y, Z = _dims(I)
+    t = np.empty_like(I)
+    dI = np.empty_like(I)
+    dm, ds = _d(0), _d(0)
+    _fn("chain_gradient", prec)(_p(I), X, Y, Z, _d(sigma), ctypes.c_float(fmax), int(sign), _d(mu), int(iters),
+                                ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep), _p(gT), _p(t),
+                                _p(dI), ctypes.byref(dm), ctypes.byref(ds))
+    return t, dI, dm.value, ds.value

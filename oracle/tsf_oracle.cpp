@@ -906,6 +906,344 @@ static void do_eigen(const T* Fx, const T* Fy, const T* Fz, Dims d, int x, int y
     }
 }
 
+// ================================================================ adjoints
+// Reverse-mode derivatives of the differentiable chain
+//   B = blur_sigma(I), F = VF(B), V = GVF_N(F; mu), W = |V|, T = TDF(V, W)
+// for a scalar L = sum_p gT(p) T(p).  Discrete choices are held fixed: the
+// blur mask size, the Fmax branch of the vector field, the eigenvalue order,
+// and the TDF radius r* (T = 0 voxels carry no gradient).  These functions
+// are the FP64 reference gradient; they are pinned by central finite
+// differences of the forward chain in tests/test_oracle_adjoint.py.
+
+// Normalised Gaussian weights in precision T and their sigma-derivative.
+template <typename T>
+static int blur_weights(T sigma, std::vector<T>& w, std::vector<T>* dw) {
+    int m = (int)std::ceil((float)sigma / 0.5f);
+    m = std::min(5, std::max(1, m));
+    const int M = 2 * m + 1;
+    std::vector<T> g(M * M * M), gp(M * M * M);
+    T G = 0, Gp = 0;
+    for (int a = -m; a <= m; a++)
+        for (int b = -m; b <= m; b++)
+            for (int c = -m; c <= m; c++) {
+                const int i = (a + m) + (b + m) * M + (c + m) * M * M;
+                const T r2 = (T)(a * a + b * b + c * c);
+                g[i] = std::exp(-r2 / (2 * sigma * sigma));
+                gp[i] = g[i] * r2 / (sigma * sigma * sigma);
+                G += g[i];
+                Gp += gp[i];
+            }
+    w.resize(g.size());
+    if (dw) dw->resize(g.size());
+    for (size_t i = 0; i < g.size(); i++) {
+        w[i] = g[i] / G;
+        if (dw) (*dw)[i] = (gp[i] * G - g[i] * Gp) / (G * G);
+    }
+    return m;
+}
+
+template <typename T>
+static void blur_w(const T* in, T* out, Dims d, int m, const std::vector<T>& w) {
+    const int M = 2 * m + 1;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T s = 0;
+                for (int a = -m; a <= m; a++)
+                    for (int b = -m; b <= m; b++)
+                        for (int c = -m; c <= m; c++)
+                            s += w[(a + m) + (b + m) * M + (c + m) * M * M] * rd_edge(in, d, x + a, y + b, z + c);
+                out[d.at(x, y, z)] = s;
+            }
+}
+
+// dI = B^T dB (clamp-to-edge transpose), dsigma = sum_p dB(p) sum_k dw_k I(clamp(p+k)).
+template <typename T>
+static void blur_adjoint(const T* dB, const T* I, Dims d, T sigma, T* dI, double* dsigma) {
+    std::vector<T> w, dw;
+    const int m = blur_weights(sigma, w, &dw), M = 2 * m + 1;
+    std::fill(dI, dI + d.n(), (T)0);
+    double ds = 0;
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                const T g = dB[d.at(x, y, z)];
+                if (g == (T)0) continue;
+                T acc = 0;
+                for (int a = -m; a <= m; a++)
+                    for (int b = -m; b <= m; b++)
+                        for (int c = -m; c <= m; c++) {
+                            const int k = (a + m) + (b + m) * M + (c + m) * M * M;
+                            const i64 j = d.at(clampi(x + a, 0, d.X - 1), clampi(y + b, 0, d.Y - 1),
+                                               clampi(z + c, 0, d.Z - 1));
+                            dI[j] += w[k] * g;
+                            acc += dw[k] * I[j];
+                        }
+                ds += (double)g * (double)acc;
+            }
+    *dsigma = ds;
+}
+
+// Vector field adjoint (Fmax branch held fixed): dB from dF (planes x, y, z).
+template <typename T>
+static void vector_field_adjoint(const T* B, const T* dF, Dims d, float Fmax, int sign, T* dB) {
+    const i64 N = d.n();
+    std::fill(dB, dB + N, (T)0);
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                const int xm = std::max(x - 1, 0), xp = std::min(x + 1, d.X - 1);
+                const int ym = std::max(y - 1, 0), yp = std::min(y + 1, d.Y - 1);
+                const int zm = std::max(z - 1, 0), zp = std::min(z + 1, d.Z - 1);
+                T g[3] = {(T)sign * (T)0.5 * (B[d.at(xp, y, z)] - B[d.at(xm, y, z)]),
+                          (T)sign * (T)0.5 * (B[d.at(x, yp, z)] - B[d.at(x, ym, z)]),
+                          (T)sign * (T)0.5 * (B[d.at(x, y, zp)] - B[d.at(x, y, zm)])};
+                const i64 i = d.at(x, y, z);
+                const T df[3] = {dF[i], dF[N + i], dF[2 * N + i]};
+                const T l = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+                T dg[3];
+                if (l < (T)Fmax) {
+                    for (int k = 0; k < 3; k++) dg[k] = df[k] / (T)Fmax;
+                } else {
+                    const T f[3] = {g[0] / l, g[1] / l, g[2] / l};
+                    const T fd = f[0] * df[0] + f[1] * df[1] + f[2] * df[2];
+                    for (int k = 0; k < 3; k++) dg[k] = (df[k] - f[k] * fd) / l;
+                }
+                const T h = (T)sign * (T)0.5;
+                dB[d.at(xp, y, z)] += h * dg[0];
+                dB[d.at(xm, y, z)] -= h * dg[0];
+                dB[d.at(x, yp, z)] += h * dg[1];
+                dB[d.at(x, ym, z)] -= h * dg[1];
+                dB[d.at(x, y, zp)] += h * dg[2];
+                dB[d.at(x, y, zm)] -= h * dg[2];
+            }
+}
+
+// Mirror fold: Lam(q) = sum of lam(w) over all w with mirror(w) == q (zero
+// on boundary voxels, which are never the image of the mirror map).
+template <typename T>
+static void mirror_fold(const T* lam, T* out, Dims d) {
+    const i64 N = d.n();
+    std::fill(out, out + N, (T)0);
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++)
+                out[d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z))] += lam[d.at(x, y, z)];
+}
+
+// GVF adjoint: given lam = dL/dv^N (3 planes), the forward iterates
+// states[n] (v^n, n = 0..N-1, 3 planes each) and F, returns dF (3 planes,
+// including v^0 = F) and dmu.
+template <typename T>
+static void gvf_adjoint(const T* F, const T* states, Dims d, T mu, int N_it, const T* dV, T* dF, double* dmu) {
+    const i64 N = d.n();
+    std::vector<T> lam(dV, dV + 3 * N), Lam(3 * N), nl(3 * N);
+    std::fill(dF, dF + 3 * N, (T)0);
+    double gm = 0;
+    const i64 sx = 1, sy = d.X, sz = (i64)d.X * d.Y;
+    for (int n = N_it - 1; n >= 0; n--) {
+        const T* v = states + (i64)n * 3 * N;
+        for (int c = 0; c < 3; c++) mirror_fold(lam.data() + c * N, Lam.data() + c * N, d);
+        for (int z = 0; z < d.Z; z++)
+            for (int y = 0; y < d.Y; y++)
+                for (int x = 0; x < d.X; x++) {
+                    const i64 i = d.at(x, y, z);
+                    const T f[3] = {F[i], F[N + i], F[2 * N + i]};
+                    const T m2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+                    T s = 0;
+                    for (int c = 0; c < 3; c++) {
+                        const T* L = Lam.data() + c * N;
+                        T nb = 0;
+                        if (x + 1 < d.X) nb += L[i + sx];
+                        if (x > 0) nb += L[i - sx];
+                        if (y + 1 < d.Y) nb += L[i + sy];
+                        if (y > 0) nb += L[i - sy];
+                        if (z + 1 < d.Z) nb += L[i + sz];
+                        if (z > 0) nb += L[i - sz];
+                        nl[c * N + i] = ((T)1 - (T)6 * mu - m2) * L[i] + mu * nb;
+                        s += L[i] * (f[c] - v[c * N + i]);
+                    }
+                    for (int c = 0; c < 3; c++) dF[c * N + i] += Lam[c * N + i] * m2 + (T)2 * f[c] * s;
+                    const bool interior = x > 0 && y > 0 && z > 0 && x < d.X - 1 && y < d.Y - 1 && z < d.Z - 1;
+                    if (interior)
+                        for (int c = 0; c < 3; c++) {
+                            const T* p = v + c * N;
+                            const T lap = p[i + sx] + p[i - sx] + p[i + sy] + p[i - sy] + p[i + sz] + p[i - sz] -
+                                          (T)6 * p[i];
+                            gm += (double)Lam[c * N + i] * (double)lap;
+                        }
+                }
+        lam.swap(nl);
+    }
+    for (i64 k = 0; k < 3 * N; k++) dF[k] += lam[k];
+    *dmu = gm;
+}
+
+// Trilinear sample of planes 0..2 and its spatial Jacobian J[c][m] = d s_c / d x_m.
+template <typename T>
+static void trilinear_grad(const T* F, Dims d, T u, T v, T w, T s[3], T J[3][3], i64 idx[8], T wt[8]) {
+    const i64 N = d.n();
+    T uu = u - (T)0.5, vv = v - (T)0.5, ww = w - (T)0.5;
+    T fu = std::floor(uu), fv = std::floor(vv), fw = std::floor(ww);
+    T a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1); j0 = clampi(j0, 0, d.Y - 1); k0 = clampi(k0, 0, d.Z - 1);
+    const T A[8] = {1 - a, a, 1 - a, a, 1 - a, a, 1 - a, a}, Bv[8] = {1 - b, 1 - b, b, b, 1 - b, 1 - b, b, b};
+    const T C[8] = {1 - c, 1 - c, 1 - c, 1 - c, c, c, c, c};
+    const T dA[8] = {-1, 1, -1, 1, -1, 1, -1, 1}, dBv[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const T dC[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    const i64 id[8] = {d.at(i0, j0, k0), d.at(i1, j0, k0), d.at(i0, j1, k0), d.at(i1, j1, k0),
+                       d.at(i0, j0, k1), d.at(i1, j0, k1), d.at(i0, j1, k1), d.at(i1, j1, k1)};
+    for (int k = 0; k < 8; k++) {
+        idx[k] = id[k];
+        wt[k] = A[k] * Bv[k] * C[k];
+    }
+    for (int comp = 0; comp < 3; comp++) {
+        const T* p = F + comp * N;
+        T sv = 0, g0 = 0, g1 = 0, g2 = 0;
+        for (int k = 0; k < 8; k++) {
+            const T t = p[id[k]];
+            sv += wt[k] * t;
+            g0 += dA[k] * Bv[k] * C[k] * t;
+            g1 += A[k] * dBv[k] * C[k] * t;
+            g2 += A[k] * Bv[k] * dC[k] * t;
+        }
+        s[comp] = sv;
+        J[comp][0] = g0;
+        J[comp][1] = g1;
+        J[comp][2] = g2;
+    }
+}
+
+// TDF adjoint: dV (3 planes) and dW (w plane) from gT, radius r* held fixed.
+template <typename T>
+static void tdf_adjoint(const T* F, Dims d, float rMin, float rMax, float rStep, const T* gT, T* dV, T* dW) {
+    make_tables();
+    const i64 N = d.n();
+    const bool norm = !(rMax < 4);
+    std::vector<T> tdf(N), rad(N);
+    tdf_circle(F, d, rMin, rMax, rStep, tdf.data(), rad.data());
+    std::fill(dV, dV + 3 * N, (T)0);
+    std::fill(dW, dW + N, (T)0);
+    const T* W = F + 3 * N;
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                const i64 p = d.at(x, y, z);
+                const T g = gT[p];
+                if (g == (T)0 || !(tdf[p] > (T)0)) continue;
+                const T r = rad[p];
+                T H[3][3], E[3][3], lam[3];
+                hessian_at(F, d, x, y, z, norm, H);
+                eigen3(H, E, lam);
+                T A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0};
+                for (int j = 0; j < 32; j++) {
+                    T dj[3];
+                    for (int k = 0; k < 3; k++) dj[k] = (T)g_cos[j] * E[k][2] + (T)g_sin[j] * E[k][1];
+                    T s[3], J[3][3], wt[8];
+                    i64 idx[8];
+                    trilinear_grad(F, d, (T)x + r * dj[0], (T)y + r * dj[1], (T)z + r * dj[2], s, J, idx, wt);
+                    const T cf = -g / (T)32;
+                    for (int k = 0; k < 8; k++)
+                        for (int c = 0; c < 3; c++) dV[c * N + idx[k]] += cf * dj[c] * wt[k];
+                    T u[3];
+                    for (int mm = 0; mm < 3; mm++)
+                        u[mm] = s[mm] + r * (dj[0] * J[0][mm] + dj[1] * J[1][mm] + dj[2] * J[2][mm]);
+                    for (int mm = 0; mm < 3; mm++) {
+                        A3[mm] += cf * (T)g_cos[j] * u[mm];
+                        A2[mm] += cf * (T)g_sin[j] * u[mm];
+                    }
+                }
+                // eigenvector perturbation: de_k = sum_{i != k} e_i (e_i^T dH e_k) / (lam_k - lam_i)
+                T G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                const T* Ak[2] = {A2, A3};
+                for (int kk = 0; kk < 2; kk++) {
+                    const int k = kk + 1;
+                    for (int i = 0; i < 3; i++) {
+                        if (i == k || lam[k] == lam[i]) continue;
+                        const T ae = Ak[kk][0] * E[0][i] + Ak[kk][1] * E[1][i] + Ak[kk][2] * E[2][i];
+                        const T coef = ae / (lam[k] - lam[i]);
+                        for (int a = 0; a < 3; a++)
+                            for (int b = 0; b < 3; b++) G[a][b] += coef * E[a][i] * E[b][k];
+                    }
+                }
+                const T gxx = G[0][0], gyx = G[0][1] + G[1][0], gzx = G[0][2] + G[2][0];
+                const T gyy = G[1][1], gzy = G[1][2] + G[2][1], gzz = G[2][2];
+                // H entries are 0.5 (s(+) - s(-)) of component samples, each divided by w when normalised
+                auto put = [&](int comp, int xx, int yy, int zz, T coefv) {
+                    const i64 q = d.at(clampi(xx, 0, d.X - 1), clampi(yy, 0, d.Y - 1), clampi(zz, 0, d.Z - 1));
+                    if (norm) {
+                        const T wq = W[q];
+                        dV[comp * N + q] += coefv / wq;
+                        dW[q] -= coefv * F[comp * N + q] / (wq * wq);
+                    } else {
+                        dV[comp * N + q] += coefv;
+                    }
+                };
+                put(0, x + 1, y, z, (T)0.5 * gxx);
+                put(0, x - 1, y, z, -(T)0.5 * gxx);
+                put(1, x + 1, y, z, (T)0.5 * gyx);
+                put(1, x - 1, y, z, -(T)0.5 * gyx);
+                put(1, x, y + 1, z, (T)0.5 * gyy);
+                put(1, x, y - 1, z, -(T)0.5 * gyy);
+                put(2, x + 1, y, z, (T)0.5 * gzx);
+                put(2, x - 1, y, z, -(T)0.5 * gzx);
+                put(2, x, y + 1, z, (T)0.5 * gzy);
+                put(2, x, y - 1, z, -(T)0.5 * gzy);
+                put(2, x, y, z + 1, (T)0.5 * gzz);
+                put(2, x, y, z - 1, -(T)0.5 * gzz);
+            }
+}
+
+// The differentiable chain forward (T-precision mask) and its gradient.
+template <typename T>
+static void chain_forward(const T* I, Dims d, T sigma, float Fmax, int sign, T mu, int iters, float rMin, float rMax,
+                          float rStep, T* tdf, T* rad, std::vector<T>* B_out, std::vector<T>* F_out,
+                          std::vector<T>* states_out, std::vector<T>* V_out) {
+    const i64 N = d.n();
+    std::vector<T> w;
+    const int m = blur_weights<T>(sigma, w, (std::vector<T>*)nullptr);
+    std::vector<T> B(N), F(4 * N);
+    blur_w(I, B.data(), d, m, w);
+    vector_field(B.data(), F.data(), d, Fmax, sign);
+    std::vector<T> states;
+    if (states_out) states.resize((size_t)iters * 3 * N);
+    std::vector<T> a(F.begin(), F.begin() + 3 * N), b(3 * N);
+    for (int it = 0; it < iters; it++) {
+        if (states_out) std::copy(a.begin(), a.end(), states.begin() + (i64)it * 3 * N);
+        gvf_step(F.data(), a.data(), b.data(), d, mu);
+        a.swap(b);
+    }
+    std::vector<T> V(4 * N);
+    std::copy(a.begin(), a.end(), V.begin());
+    gvf_finish(V.data(), d);
+    tdf_circle(V.data(), d, rMin, rMax, rStep, tdf, rad);
+    if (B_out) B_out->swap(B);
+    if (F_out) F_out->swap(F);
+    if (states_out) states_out->swap(states);
+    if (V_out) V_out->swap(V);
+}
+
+template <typename T>
+static void chain_gradient(const T* I, Dims d, T sigma, float Fmax, int sign, T mu, int iters, float rMin, float rMax,
+                           float rStep, const T* gT, T* tdf, T* dI, double* dmu, double* dsigma) {
+    const i64 N = d.n();
+    std::vector<T> B, F, states, V, rad(N);
+    chain_forward(I, d, sigma, Fmax, sign, mu, iters, rMin, rMax, rStep, tdf, rad.data(), &B, &F, &states, &V);
+    std::vector<T> dV(3 * N), dW(N), dF(3 * N), dB(N);
+    tdf_adjoint(V.data(), d, rMin, rMax, rStep, gT, dV.data(), dW.data());
+    for (i64 i = 0; i < N; i++) {  // w = |v| (1 where |v| == 0)
+        const T l = std::sqrt(V[i] * V[i] + V[N + i] * V[N + i] + V[2 * N + i] * V[2 * N + i]);
+        if (l > (T)0)
+            for (int c = 0; c < 3; c++) dV[c * N + i] += dW[i] * V[c * N + i] / l;
+    }
+    gvf_adjoint(F.data(), states.data(), d, mu, iters, dV.data(), dF.data(), dmu);
+    vector_field_adjoint(B.data(), dF.data(), d, Fmax, sign, dB.data());
+    blur_adjoint(dB.data(), I, d, sigma, dI, dsigma);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C exports
@@ -963,6 +1301,48 @@ static void do_eigen(const T* Fx, const T* Fy, const T* Fz, Dims d, int x, int y
     }                                                                                                        \
     extern "C" void orc_trilinear_##S(const T* F, int X, int Y, int Z, T u, T v, T w, T* out) {             \
         trilinear3<T>(F, DIMS, u, v, w, out);                                                                \
+    }                                                                                                        \
+    extern "C" void orc_blur_sigma_##S(const T* in, T* out, int X, int Y, int Z, double sigma) {            \
+        std::vector<T> w;                                                                                    \
+        int m = blur_weights<T>((T)sigma, w, (std::vector<T>*)nullptr);                                      \
+        blur_w<T>(in, out, DIMS, m, w);                                                                      \
+    }                                                                                                        \
+    extern "C" void orc_blur_adjoint_##S(const T* dB, const T* I, int X, int Y, int Z, double sigma, T* dI, \
+                                         double* dsigma) {                                                   \
+        blur_adjoint<T>(dB, I, DIMS, (T)sigma, dI, dsigma);                                                  \
+    }                                                                                                        \
+    extern "C" void orc_vector_field_adjoint_##S(const T* B, const T* dF, int X, int Y, int Z, float Fmax,  \
+                                                 int sign, T* dB) {                                          \
+        vector_field_adjoint<T>(B, dF, DIMS, Fmax, sign, dB);                                                \
+    }                                                                                                        \
+    extern "C" void orc_gvf_adjoint_##S(const T* F, int X, int Y, int Z, double mu, int iters, const T* dV, \
+                                        T* dF, double* dmu) {                                                \
+        Dims d = DIMS;                                                                                       \
+        const i64 N = d.n();                                                                                 \
+        std::vector<T> states((size_t)iters * 3 * N), a(F, F + 3 * N), b(3 * N);                             \
+        for (int it = 0; it < iters; it++) {                                                                 \
+            std::copy(a.begin(), a.end(), states.begin() + (i64)it * 3 * N);                                 \
+            gvf_step<T>(F, a.data(), b.data(), d, (T)mu);                                                    \
+            a.swap(b);                                                                                       \
+        }                                                                                                    \
+        gvf_adjoint<T>(F, states.data(), d, (T)mu, iters, dV, dF, dmu);                                      \
+    }                                                                                                        \
+    extern "C" void orc_tdf_adjoint_##S(const T* F, int X, int Y, int Z, float rmin, float rmax,            \
+                                        float rstep, const T* gT, T* dV, T* dW) {                            \
+        tdf_adjoint<T>(F, DIMS, rmin, rmax, rstep, gT, dV, dW);                                              \
+    }                                                                                                        \
+    extern "C" void orc_chain_forward_##S(const T* I, int X, int Y, int Z, double sigma, float Fmax,        \
+                                          int sign, double mu, int iters, float rmin, float rmax,           \
+                                          float rstep, T* tdf, T* rad) {                                     \
+        chain_forward<T>(I, DIMS, (T)sigma, Fmax, sign, (T)mu, iters, rmin, rmax, rstep, tdf, rad, nullptr, \
+                         nullptr, nullptr, nullptr);                                                         \
+    }                                                                                                        \
+    extern "C" void orc_chain_gradient_##S(const T* I, int X, int Y, int Z, double sigma, float Fmax,       \
+                                           int sign, double mu, int iters, float rmin, float rmax,          \
+                                           float rstep, const T* gT, T* tdf, T* dI, double* dmu,            \
+                                           double* dsigma) {                                                 \
+        chain_gradient<T>(I, DIMS, (T)sigma, Fmax, sign, (T)mu, iters, rmin, rmax, rstep, gT, tdf, dI, dmu, \
+                          dsigma);                                                                           \
     }
 
 EXPORT_T(float, f32)
