@@ -226,6 +226,58 @@ tsf_status tsf_candidate_nms(const float* tdf, const float* radius, const tsf_fi
 tsf_status tsf_cube_max(const float* tdf, const uint8_t* candidates, tsf_dims dims, int32_t cube, uint8_t* out,
                         hipStream_t stream);
 
+/* ------------------------------------------------------------ adjoints
+ * Reverse-mode derivatives of the chain
+ *   B = blur_sigma(I), F = createVectorField(B), V = GVF_N(F; mu), W = |V|,
+ *   T = circleFittingTDF(V, W),   L = sum_p gT(p) T(p).
+ * No reference counterpart: the reference has no gradients.  Discrete
+ * choices are held fixed (blur mask size, the Fmax branch of the vector
+ * field, the eigenvalue order, the TDF radius r*).  Float arithmetic; the
+ * scalar gradients (d/dsigma, d/dmu) are accumulated in double on the
+ * device.  Entry points that return a host double synchronise the stream. */
+
+/* dI = B^T dB for the blur of tsf_blur_gaussian(sigma) (clamp-to-edge
+ * transpose); dsigma (optional, needs `in`) = sum_p dB(p) sum_k dw_k/dsigma
+ * in(clamp(p+k)) with the mask size held fixed. */
+tsf_status tsf_blur_adjoint(const float* dB, const float* in, float* dI, tsf_dims dims, float sigma, double* dsigma,
+                            void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_blur_adjoint_workspace_size(void);
+
+/* dvolume = (d createVectorField / d volume)^T dF (planes x, y, z of dF). */
+tsf_status tsf_vector_field_adjoint(const float* volume, const tsf_field* dF, tsf_dims dims, float fmax, int32_t sign,
+                                    float* dvolume, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_vector_field_adjoint_workspace_size(tsf_dims dims);
+
+/* Forward GVF (F32) that stores v^0, v^k, v^2k, ... (k = checkpoint_every)
+ * into `checkpoints` (tsf_gvf_checkpoint_count(...) states of 3 float
+ * planes, contiguous) and the result (with w) into out.  Same values as
+ * tsf_gvf_euler. */
+int64_t tsf_gvf_checkpoint_count(int32_t iterations, int32_t checkpoint_every);
+tsf_status tsf_gvf_euler_checkpointed(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu,
+                                      int32_t iterations, int32_t checkpoint_every, float* checkpoints,
+                                      size_t checkpoint_bytes, hipStream_t stream);
+
+/* Adjoint of the GVF iterations: from dV (3 float planes, dL/dv^N) returns
+ * dF (3 planes: through v^0 = f and the data term of every step; mirror
+ * boundary transposed) and dmu (host).  Each checkpoint segment is
+ * recomputed once; memory (checkpoint_every + 3) states + the checkpoints. */
+tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
+                                 int32_t checkpoint_every, const float* checkpoints, const float* dV, float* dF,
+                                 double* dmu, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every);
+
+/* Adjoint of the finish step w = |v|: dv.xyz += dv.w * v / |v| (|v| > 0). */
+tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_dims dims, hipStream_t stream);
+
+/* Adjoint of circleFittingTDF at the forward's radius r*: accumulates
+ * (+=) dL/dV into dfield.xyz and, when rmax >= 4 (normalised Hessian),
+ * dL/dW into dfield.w.  Includes the sampled values, the sample positions
+ * and the eigenvector directions (first-order perturbation of e2, e3).
+ * Voxels with T = 0 carry no gradient.  Scatter with float atomics, so the
+ * summation order (last bits) is not reproducible between runs. */
+tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
+                           const float* tdf_grad, const tsf_field* dfield, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,675 @@
+// Hand-written adjoints of the differentiable chain
+//   B = blur_sigma(I), F = VF(B), V = GVF_N(F; mu), W = |V|, T = TDF(V, W)
+// for L = sum_p gT(p) T(p).  Discrete choices are held fixed (blur mask
+// size, Fmax branch, eigenvalue order, TDF radius r*).  All adjoints are
+// float with double accumulation of the scalar parameter gradients.
+//
+// Kernels and their data movement:
+//   k_blur_adjoint     gather B^T dB with clamp-to-edge folding (deterministic)
+//   k_blur_dsigma      per-voxel dB * (dw * I) -> block partial sums (double)
+//   k_vf_dg / k_vf_gather   the vector-field adjoint as two gathers
+//   k_gvf_adjoint_step the transposed Jacobi step: mirror fold of the
+//                      adjoint, transposed 7-point stencil and data term,
+//                      dF accumulation, d/dmu block partials (double)
+//   k_tdf_adjoint      per-voxel recomputation of the circle fit at r*,
+//                      then atomic scatter of the sample, direction
+//                      (eigenvector perturbation) and Hessian contributions
+// The GVF backward runs over checkpoints of the forward taken every k
+// iterations; each segment is recomputed into a k-state buffer.
+#include <math.h>
+#include <vector>
+
+#include "eigen3.hpp"
+#include "field.hpp"
+
+namespace tsf {
+
+tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], float* const out[3], Dims d, float mu,
+                         int iters, float* ws, hipStream_t stream);
+
+constexpr int RB = 256;     // reduction block
+constexpr int NPART = 1024; // partial sums per launch
+
+__device__ __forceinline__ void block_sum_double(double v, double* out) {
+    __shared__ double sh[RB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = RB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// out[0] += sum of n partials (one block, deterministic order)
+__global__ void __launch_bounds__(RB) k_fold_partials(const double* __restrict__ part, int n, double* __restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += RB) s += part[i];
+    __shared__ double sh[RB];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = RB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] += sh[0];
+}
+
+__global__ void k_zero_double(double* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- blur
+// Per axis, the positions p with clamp(p + a) == y: one position inside,
+// a run of them at the clamped ends.
+__device__ __forceinline__ void pre_range(int y, int a, int N, int& lo, int& hi) {
+    if (N == 1) {
+        lo = hi = 0;
+    } else if (y == 0) {
+        lo = 0;
+        hi = min(-a, N - 1);
+    } else if (y == N - 1) {
+        lo = max(0, N - 1 - a);
+        hi = N - 1;
+    } else {
+        lo = hi = y - a;
+        if (lo < 0 || lo > N - 1) hi = lo - 1;  // empty
+    }
+}
+
+__global__ void k_blur_adjoint(const float* __restrict__ dB, float* __restrict__ dI, Dims d, int m,
+                               const float* __restrict__ w) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= d.X || y >= d.Y) return;
+    const int M = 2 * m + 1;
+    float acc = 0.0f;
+    for (int a = -m; a <= m; a++) {
+        int xl, xh;
+        pre_range(x, a, d.X, xl, xh);
+        if (xl > xh) continue;
+        for (int b = -m; b <= m; b++) {
+            int yl, yh;
+            pre_range(y, b, d.Y, yl, yh);
+            if (yl > yh) continue;
+            for (int c = -m; c <= m; c++) {
+                int zl, zh;
+                pre_range(z, c, d.Z, zl, zh);
+                if (zl > zh) continue;
+                float s = 0.0f;
+                for (int pz = zl; pz <= zh; pz++)
+                    for (int py = yl; py <= yh; py++)
+                        for (int px = xl; px <= xh; px++) s += dB[d.at(px, py, pz)];
+                acc += w[(a + m) + (b + m) * M + (c + m) * M * M] * s;
+            }
+        }
+    }
+    dI[d.at(x, y, z)] = acc;
+}
+
+__global__ void __launch_bounds__(RB) k_blur_dsigma(const float* __restrict__ dB, const float* __restrict__ I, Dims d,
+                                                    int m, const double* __restrict__ dw, double* __restrict__ part) {
+    const i64 n = d.n();
+    const int M = 2 * m + 1;
+    double s = 0.0;
+    for (i64 i = blockIdx.x * (i64)RB + threadIdx.x; i < n; i += (i64)gridDim.x * RB) {
+        const float g = dB[i];
+        if (g == 0.0f) continue;
+        const int x = (int)(i % d.X), y = (int)((i / d.X) % d.Y), z = (int)(i / ((i64)d.X * d.Y));
+        double acc = 0.0;
+        for (int a = -m; a <= m; a++)
+            for (int b = -m; b <= m; b++)
+                for (int c = -m; c <= m; c++)
+                    acc += dw[(a + m) + (b + m) * M + (c + m) * M * M] *
+                           (double)I[d.at(clampi(x + a, 0, d.X - 1), clampi(y + b, 0, d.Y - 1), clampi(z + c, 0, d.Z - 1))];
+        s += (double)g * acc;
+    }
+    block_sum_double(s, &part[blockIdx.x]);
+}
+
+// ---------------------------------------------------------------- vector field
+__global__ void k_vf_dg(const float* __restrict__ B, const float* __restrict__ dFx, const float* __restrict__ dFy,
+                        const float* __restrict__ dFz, Dims d, float fmax, float sign, float* __restrict__ dg) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= d.X || y >= d.Y) return;
+    const i64 N = d.n(), i = d.at(x, y, z);
+    const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1), ym = max(y - 1, 0), yp = min(y + 1, d.Y - 1);
+    const int zm = max(z - 1, 0), zp = min(z + 1, d.Z - 1);
+    float g0 = sign * (0.5f * (B[d.at(xp, y, z)] - B[d.at(xm, y, z)]));
+    float g1 = sign * (0.5f * (B[d.at(x, yp, z)] - B[d.at(x, ym, z)]));
+    float g2 = sign * (0.5f * (B[d.at(x, y, zp)] - B[d.at(x, y, zm)]));
+    const float l = sqrtf(g0 * g0 + g1 * g1 + g2 * g2);
+    const float a = dFx[i], b = dFy[i], c = dFz[i];
+    float r0, r1, r2;
+    if (l < fmax) {
+        r0 = a / fmax;
+        r1 = b / fmax;
+        r2 = c / fmax;
+    } else {
+        const float f0 = g0 / l, f1 = g1 / l, f2 = g2 / l;
+        const float fd = f0 * a + f1 * b + f2 * c;
+        r0 = (a - f0 * fd) / l;
+        r1 = (b - f1 * fd) / l;
+        r2 = (c - f2 * fd) / l;
+    }
+    const float h = sign * 0.5f;
+    dg[i] = h * r0;
+    dg[N + i] = h * r1;
+    dg[2 * N + i] = h * r2;
+}
+
+// dB(y) = sum over axes of dg(y - e)[y >= 1] + dg(y)[y == N-1] - dg(y + e)[y <= N-2] - dg(y)[y == 0]
+__global__ void k_vf_gather(const float* __restrict__ dg, Dims d, float* __restrict__ dB) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= d.X || y >= d.Y) return;
+    const i64 N = d.n(), i = d.at(x, y, z);
+    const int p[3] = {x, y, z}, n[3] = {d.X, d.Y, d.Z};
+    const i64 st[3] = {1, (i64)d.X, (i64)d.X * d.Y};
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float* g = dg + a * N;
+        if (p[a] >= 1) s += g[i - st[a]];
+        if (p[a] == n[a] - 1) s += g[i];
+        if (p[a] <= n[a] - 2) s -= g[i + st[a]];
+        if (p[a] == 0) s -= g[i];
+    }
+    dB[i] = s;
+}
+
+// ---------------------------------------------------------------- GVF
+// Mirror fold: the adjoint that reaches the update at interior position q
+// is the sum over every w whose mirror image is q (per axis: q itself, plus
+// 0 when q == 2, plus N-1 when q == N-3); boundary positions receive none.
+__device__ __forceinline__ int pre_list(int q, int N, int* o) {
+    if (q <= 0 || q >= N - 1) return 0;
+    int k = 0;
+    o[k++] = q;
+    if (q == 2) o[k++] = 0;
+    if (q == N - 3) o[k++] = N - 1;
+    return k;
+}
+__device__ __forceinline__ float fold(const float* __restrict__ lam, const Dims& d, int x, int y, int z) {
+    if (x < 0 || y < 0 || z < 0 || x >= d.X || y >= d.Y || z >= d.Z) return 0.0f;
+    int px[3], py[3], pz[3];
+    const int nx = pre_list(x, d.X, px), ny = pre_list(y, d.Y, py), nz = pre_list(z, d.Z, pz);
+    float s = 0.0f;
+    for (int c = 0; c < nz; c++)
+        for (int b = 0; b < ny; b++)
+            for (int a = 0; a < nx; a++) s += lam[d.at(px[a], py[b], pz[c])];
+    return s;
+}
+
+struct P3 {
+    const float* c[3];
+};
+struct M3 {
+    float* c[3];
+};
+
+__global__ void __launch_bounds__(RB) k_gvf_adjoint_step(P3 lam, P3 v, P3 F, Dims d, float mu, M3 lam_out, M3 dF,
+                                                         double* __restrict__ part) {
+    const i64 n = d.n();
+    const i64 sy = d.X, sz = (i64)d.X * d.Y;
+    double gm = 0.0;
+    for (i64 i = blockIdx.x * (i64)RB + threadIdx.x; i < n; i += (i64)gridDim.x * RB) {
+        const int x = (int)(i % d.X), y = (int)((i / d.X) % d.Y), z = (int)(i / sz);
+        const float f0 = F.c[0][i], f1 = F.c[1][i], f2 = F.c[2][i];
+        const float fc[3] = {f0, f1, f2};
+        const float m2 = f0 * f0 + f1 * f1 + f2 * f2;
+        const bool interior = x > 0 && y > 0 && z > 0 && x < d.X - 1 && y < d.Y - 1 && z < d.Z - 1;
+        float L[3];
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float* la = lam.c[c];
+            L[c] = fold(la, d, x, y, z);
+            const float nb = fold(la, d, x + 1, y, z) + fold(la, d, x - 1, y, z) + fold(la, d, x, y + 1, z) +
+                             fold(la, d, x, y - 1, z) + fold(la, d, x, y, z + 1) + fold(la, d, x, y, z - 1);
+            lam_out.c[c][i] = (1.0f - 6.0f * mu - m2) * L[c] + mu * nb;
+            s += L[c] * (fc[c] - v.c[c][i]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) dF.c[c][i] += L[c] * m2 + 2.0f * fc[c] * s;
+        if (interior) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float* p = v.c[c];
+                const float lap = p[i + 1] + p[i - 1] + p[i + sy] + p[i - sy] + p[i + sz] + p[i - sz] - 6.0f * p[i];
+                gm += (double)L[c] * (double)lap;
+            }
+        }
+    }
+    block_sum_double(gm, &part[blockIdx.x]);
+}
+
+// W = |V| (1 where |V| == 0): dV += dW V / |V|
+__global__ void k_finish_adjoint(const float* __restrict__ vx, const float* __restrict__ vy,
+                                 const float* __restrict__ vz, const float* __restrict__ dW, float* __restrict__ dx,
+                                 float* __restrict__ dy, float* __restrict__ dz, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const float a = vx[i], b = vy[i], c = vz[i];
+        const float l = sqrtf(a * a + b * b + c * c);
+        if (l > 0.0f) {
+            const float g = dW[i] / l;
+            dx[i] += g * a;
+            dy[i] += g * b;
+            dz[i] += g * c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- TDF
+// Trilinear sample (the forward's order of terms) with its spatial Jacobian.
+__device__ __forceinline__ void tri_grad(const Field<F32>& F, const Dims& d, float u, float v, float w, float s[3],
+                                         float J[3][3], i64 idx[8], float wt[8]) {
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    const int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1);
+    j0 = clampi(j0, 0, d.Y - 1);
+    k0 = clampi(k0, 0, d.Z - 1);
+    const float A[8] = {1 - a, a, 1 - a, a, 1 - a, a, 1 - a, a};
+    const float Bv[8] = {1 - b, 1 - b, b, b, 1 - b, 1 - b, b, b};
+    const float C[8] = {1 - c, 1 - c, 1 - c, 1 - c, c, c, c, c};
+    const float dA[8] = {-1, 1, -1, 1, -1, 1, -1, 1}, dB[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const float dC[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    idx[0] = d.at(i0, j0, k0); idx[1] = d.at(i1, j0, k0); idx[2] = d.at(i0, j1, k0); idx[3] = d.at(i1, j1, k0);
+    idx[4] = d.at(i0, j0, k1); idx[5] = d.at(i1, j0, k1); idx[6] = d.at(i0, j1, k1); idx[7] = d.at(i1, j1, k1);
+#pragma unroll
+    for (int k = 0; k < 8; k++) wt[k] = A[k] * Bv[k] * C[k];
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+        float sv = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float t = F.c(comp, idx[k]);
+            sv = sv + wt[k] * t;
+            g0 += dA[k] * Bv[k] * C[k] * t;
+            g1 += A[k] * dB[k] * C[k] * t;
+            g2 += A[k] * Bv[k] * dC[k] * t;
+        }
+        s[comp] = sv;
+        J[comp][0] = g0;
+        J[comp][1] = g1;
+        J[comp][2] = g2;
+    }
+}
+
+template <bool NORM>
+__global__ void __launch_bounds__(256) k_tdf_adjoint(Field<F32> F, Dims d, float rmin, float rmax, float rstep,
+                                                     CircleTables tab, const float* __restrict__ gT,
+                                                     float* __restrict__ dVx, float* __restrict__ dVy,
+                                                     float* __restrict__ dVz, float* __restrict__ dW) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= d.X || y >= d.Y) return;
+    const float g = gT[d.at(x, y, z)];
+    if (g == 0.0f) return;
+    float H[3][3], E[3][3], lam[3];
+    hessian<NORM>(F, d, x, y, z, H);
+    eigen3(H, E, lam);
+    const float e2x = E[0][1], e2y = E[1][1], e2z = E[2][1];
+    const float e3x = E[0][2], e3y = E[1][2], e3z = E[2][2];
+    // forward radius search, as circleFittingTDF computes it
+    float maxSum = 0.0f, rstar = 0.0f;
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    for (float radius = rmin; radius <= rmax; radius += rstep) {
+        float sum = 0.0f;
+        for (int j = 0; j < 32; j++) {
+            const float vx = tab.c[j] * e3x + tab.s[j] * e2x, vy = tab.c[j] * e3y + tab.s[j] * e2y;
+            const float vz = tab.c[j] * e3z + tab.s[j] * e2z;
+            float s[3], J[3][3], wt[8];
+            i64 idx[8];
+            tri_grad(F, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s, J, idx, wt);
+            sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
+        }
+        sum /= 32.0f;
+        if (sum > maxSum) {
+            maxSum = sum;
+            rstar = radius;
+        } else {
+            break;
+        }
+    }
+    if (!(maxSum > 0.0f)) return;
+    const float cf = -g / 32.0f;
+    float A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0};
+    float* dV[3] = {dVx, dVy, dVz};
+    for (int j = 0; j < 32; j++) {
+        const float dj[3] = {tab.c[j] * e3x + tab.s[j] * e2x, tab.c[j] * e3y + tab.s[j] * e2y,
+                             tab.c[j] * e3z + tab.s[j] * e2z};
+        float s[3], J[3][3], wt[8];
+        i64 idx[8];
+        tri_grad(F, d, fx + rstar * dj[0], fy + rstar * dj[1], fz + rstar * dj[2], s, J, idx, wt);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) unsafeAtomicAdd(dV[c] + idx[k], cf * dj[c] * wt[k]);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const float u = s[m] + rstar * (dj[0] * J[0][m] + dj[1] * J[1][m] + dj[2] * J[2][m]);
+            A3[m] += cf * tab.c[j] * u;
+            A2[m] += cf * tab.s[j] * u;
+        }
+    }
+    // eigenvector perturbation: de_k = sum_{i != k} e_i (e_i^T dH e_k) / (lam_k - lam_i)
+    float G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+        const int k = kk + 1;
+        const float* A = kk == 0 ? A2 : A3;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            if (i == k || lam[k] == lam[i]) continue;
+            const float coef = (A[0] * E[0][i] + A[1] * E[1][i] + A[2] * E[2][i]) / (lam[k] - lam[i]);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) G[a][b] += coef * E[a][i] * E[b][k];
+        }
+    }
+    const float gxx = G[0][0], gyx = G[0][1] + G[1][0], gzx = G[0][2] + G[2][0];
+    const float gyy = G[1][1], gzy = G[1][2] + G[2][1], gzz = G[2][2];
+    auto put = [&](int comp, int xx, int yy, int zz, float cv) {
+        const i64 q = d.at(clampi(xx, 0, d.X - 1), clampi(yy, 0, d.Y - 1), clampi(zz, 0, d.Z - 1));
+        if (NORM) {
+            const float wq = F.mag(q);
+            unsafeAtomicAdd(dV[comp] + q, cv / wq);
+            unsafeAtomicAdd(dW + q, -cv * F.c(comp, q) / (wq * wq));
+        } else {
+            unsafeAtomicAdd(dV[comp] + q, cv);
+        }
+    };
+    put(0, x + 1, y, z, 0.5f * gxx);
+    put(0, x - 1, y, z, -0.5f * gxx);
+    put(1, x + 1, y, z, 0.5f * gyx);
+    put(1, x - 1, y, z, -0.5f * gyx);
+    put(1, x, y + 1, z, 0.5f * gyy);
+    put(1, x, y - 1, z, -0.5f * gyy);
+    put(2, x + 1, y, z, 0.5f * gzx);
+    put(2, x - 1, y, z, -0.5f * gzx);
+    put(2, x, y + 1, z, 0.5f * gzy);
+    put(2, x, y - 1, z, -0.5f * gzy);
+    put(2, x, y, z + 1, 0.5f * gzz);
+    put(2, x, y, z - 1, -0.5f * gzz);
+}
+
+__global__ void k_add_inplace_f(float* __restrict__ a, const float* __restrict__ b, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) a[i] = a[i] + b[i];
+}
+
+// w = |v| where nonzero, else 1 (GVF3DFinish)
+__global__ void k_mag_plane(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                            float* __restrict__ w, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const float l = sqrtf(x[i] * x[i] + y[i] * y[i] + z[i] * z[i]);
+        w[i] = l > 0.0f ? l : 1.0f;
+    }
+}
+
+static unsigned g1d(i64 n) {
+    i64 g = (n + 255) / 256;
+    return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+static tsf_status sync_read_double(const double* dptr, double* host, hipStream_t s, const char* W) {
+    hipError_t e = hipMemcpyAsync(host, dptr, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    return TSF_OK;
+}
+
+}  // namespace tsf
+
+using namespace tsf;
+
+#define STENCIL_GRID(d) dim3(((d).X + 63) / 64, ((d).Y + 3) / 4, (d).Z), dim3(64, 4)
+
+extern "C" {
+
+size_t tsf_blur_adjoint_workspace_size(void) { return 1331 * (sizeof(float) + sizeof(double)) + (NPART + 1) * sizeof(double) + 512; }
+
+tsf_status tsf_blur_adjoint(const float* dB, const float* in, float* dI, tsf_dims dims, float sigma, double* dsigma,
+                            void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    const char* W = "tsf_blur_adjoint";
+    if (!dB || !dI || (dsigma && !in)) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL argument", W);
+    if (dB == dI) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: dB and dI must differ", W);
+    TSF_TRY(check_dims(dims, 1, W));
+    float mask[1331];
+    int32_t m;
+    TSF_TRY(tsf_create_blur_mask(sigma, mask, &m));
+    const int M = 2 * m + 1, K = M * M * M;
+    // d w_k / d sigma of the float mask of tsf_create_blur_mask, in double
+    std::vector<double> dw(K);
+    {
+        double G = 0, Gp = 0;
+        std::vector<double> g(K), gp(K);
+        for (int a = -m; a <= m; a++)
+            for (int b = -m; b <= m; b++)
+                for (int c = -m; c <= m; c++) {
+                    const int i = (a + m) + (b + m) * M + (c + m) * M * M;
+                    const double r2 = a * a + b * b + c * c, s = sigma;
+                    g[i] = exp(-r2 / (2 * s * s));
+                    gp[i] = g[i] * r2 / (s * s * s);
+                    G += g[i];
+                    Gp += gp[i];
+                }
+        for (int i = 0; i < K; i++) dw[i] = (gp[i] * G - g[i] * Gp) / (G * G);
+    }
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_blur_adjoint_workspace_size(), W));
+    double* part = (double*)sc.ptr;
+    double* acc = part + NPART;
+    double* ddw = acc + 1;
+    float* dmask = (float*)(ddw + 1331);
+    hipError_t e = hipMemcpyAsync(dmask, mask, sizeof(float) * K, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ddw, dw.data(), sizeof(double) * K, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    Dims d = D(dims);
+    hipLaunchKernelGGL(k_blur_adjoint, STENCIL_GRID(d), 0, stream, dB, dI, d, m, dmask);
+    TSF_TRY(check_launch(W));
+    if (dsigma) {
+        hipLaunchKernelGGL(k_zero_double, 1, 64, 0, stream, acc, 1);
+        hipLaunchKernelGGL(k_blur_dsigma, NPART, RB, 0, stream, dB, in, d, m, ddw, part);
+        hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part, NPART, acc);
+        TSF_TRY(check_launch(W));
+        TSF_TRY(sync_read_double(acc, dsigma, stream, W));
+    } else {
+        e = hipStreamSynchronize(stream);  // the staged mask lives in scratch
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    }
+    return TSF_OK;
+}
+
+size_t tsf_vector_field_adjoint_workspace_size(tsf_dims dims) { return (size_t)3 * D(dims).n() * sizeof(float); }
+
+tsf_status tsf_vector_field_adjoint(const float* volume, const tsf_field* dF, tsf_dims dims, float fmax, int32_t sign,
+                                    float* dvolume, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    const char* W = "tsf_vector_field_adjoint";
+    if (!volume || !dvolume) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL volume", W);
+    TSF_TRY(check_field(dF, false, W));
+    if (dF->storage != TSF_STORAGE_F32) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: dF must be F32", W);
+    TSF_TRY(check_dims(dims, 1, W));
+    if (!(fmax > 0.0f)) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: fmax must be positive", W);
+    if (sign != 1 && sign != -1) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: sign must be +1 or -1", W);
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_vector_field_adjoint_workspace_size(dims), W));
+    Dims d = D(dims);
+    hipLaunchKernelGGL(k_vf_dg, STENCIL_GRID(d), 0, stream, volume, (const float*)dF->x, (const float*)dF->y,
+                       (const float*)dF->z, d, fmax, (float)sign, (float*)sc.ptr);
+    hipLaunchKernelGGL(k_vf_gather, STENCIL_GRID(d), 0, stream, (const float*)sc.ptr, d, dvolume);
+    TSF_TRY(check_launch(W));
+    if (sc.owned) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    }
+    return TSF_OK;
+}
+
+int64_t tsf_gvf_checkpoint_count(int32_t iterations, int32_t checkpoint_every) {
+    if (iterations <= 0 || checkpoint_every <= 0) return 0;
+    return (iterations + checkpoint_every - 1) / checkpoint_every;
+}
+
+tsf_status tsf_gvf_euler_checkpointed(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu,
+                                      int32_t iterations, int32_t checkpoint_every, float* checkpoints,
+                                      size_t checkpoint_bytes, hipStream_t stream) {
+    const char* W = "tsf_gvf_euler_checkpointed";
+    TSF_TRY(check_field(f, false, W));
+    TSF_TRY(check_field(out, true, W));
+    if (f->storage != TSF_STORAGE_F32 || out->storage != TSF_STORAGE_F32)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 fields only", W);
+    TSF_TRY(check_dims(dims, 4, W));
+    if (iterations < 1 || checkpoint_every < 1)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: iterations and checkpoint_every must be >= 1", W);
+    if (!(mu >= 0.0f)) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: mu must be >= 0", W);
+    Dims d = D(dims);
+    const i64 N = d.n();
+    const i64 C = tsf_gvf_checkpoint_count(iterations, checkpoint_every);
+    if (!checkpoints || checkpoint_bytes < (size_t)C * 3 * N * sizeof(float))
+        return set_error(TSF_ERR_OUT_OF_MEMORY, "%s: checkpoint buffer too small", W);
+    Scratch sc;
+    TSF_TRY(sc.get(nullptr, 0, (size_t)3 * N * sizeof(float), W));
+    const float* fp[3] = {(const float*)f->x, (const float*)f->y, (const float*)f->z};
+    for (i64 s = 0; s < C; s++) {
+        float* cp = checkpoints + s * 3 * N;
+        float* cps[3] = {cp, cp + N, cp + 2 * N};
+        if (s == 0) {
+            for (int c = 0; c < 3; c++) {
+                hipError_t e = hipMemcpyAsync(cps[c], fp[c], N * sizeof(float), hipMemcpyDeviceToDevice, stream);
+                if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+            }
+        } else {
+            const float* prev = checkpoints + (s - 1) * 3 * N;
+            const float* pv[3] = {prev, prev + N, prev + 2 * N};
+            TSF_TRY(gvf_steps_f32(fp, pv, cps, d, mu, checkpoint_every, (float*)sc.ptr, stream));
+        }
+    }
+    // last segment into out, then the magnitude plane
+    const float* last = checkpoints + (C - 1) * 3 * N;
+    const float* lv[3] = {last, last + N, last + 2 * N};
+    float* op[3] = {(float*)out->x, (float*)out->y, (float*)out->z};
+    const int rem = iterations - (int)((C - 1) * checkpoint_every);
+    TSF_TRY(gvf_steps_f32(fp, lv, op, d, mu, rem, (float*)sc.ptr, stream));
+    hipLaunchKernelGGL(k_mag_plane, g1d(N), 256, 0, stream, op[0], op[1], op[2], (float*)out->w, N);
+    TSF_TRY(check_launch(W));
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    return TSF_OK;
+}
+
+size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every) {
+    const size_t N = (size_t)D(dims).n();
+    return ((size_t)checkpoint_every + 3) * 3 * N * sizeof(float) + (NPART + 1) * sizeof(double);
+}
+
+tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
+                                 int32_t checkpoint_every, const float* checkpoints, const float* dV, float* dF,
+                                 double* dmu, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+    const char* W = "tsf_gvf_euler_adjoint";
+    TSF_TRY(check_field(f, false, W));
+    if (f->storage != TSF_STORAGE_F32) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 field only", W);
+    TSF_TRY(check_dims(dims, 4, W));
+    if (!checkpoints || !dV || !dF) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL buffer", W);
+    if (iterations < 1 || checkpoint_every < 1)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: iterations and checkpoint_every must be >= 1", W);
+    Dims d = D(dims);
+    const i64 N = d.n();
+    const int k = checkpoint_every;
+    const i64 C = tsf_gvf_checkpoint_count(iterations, k);
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_gvf_euler_adjoint_workspace_size(dims, k), W));
+    float* seg = (float*)sc.ptr;              // k states of 3 planes
+    float* lamA = seg + (i64)k * 3 * N;       // adjoint ping-pong
+    float* lamB = lamA + 3 * N;
+    float* tmp = lamB + 3 * N;                // ping-pong of the recomputation
+    double* part = (double*)(tmp + 3 * N);
+    double* acc = part + NPART;
+    const float* fp[3] = {(const float*)f->x, (const float*)f->y, (const float*)f->z};
+    hipError_t e = hipMemcpyAsync(lamA, dV, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(dF, 0, 3 * N * sizeof(float), stream);
+    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    hipLaunchKernelGGL(k_zero_double, 1, 64, 0, stream, acc, 1);
+    P3 Fp{{fp[0], fp[1], fp[2]}};
+    M3 dFp{{dF, dF + N, dF + 2 * N}};
+    float* lin = lamA;
+    float* lout = lamB;
+    for (i64 s = C - 1; s >= 0; s--) {
+        const int start = (int)(s * k), end = (int)std::min<i64>((s + 1) * k, iterations);
+        // recompute v^start .. v^(end-1) into seg
+        const float* cp = checkpoints + s * 3 * N;
+        e = hipMemcpyAsync(seg, cp, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+        for (int t = 1; t < end - start; t++) {
+            const float* pv = seg + (i64)(t - 1) * 3 * N;
+            float* nv = seg + (i64)t * 3 * N;
+            const float* pvv[3] = {pv, pv + N, pv + 2 * N};
+            float* nvv[3] = {nv, nv + N, nv + 2 * N};
+            TSF_TRY(gvf_steps_f32(fp, pvv, nvv, d, mu, 1, tmp, stream));
+        }
+        for (int n = end - 1; n >= start; n--) {
+            const float* vn = seg + (i64)(n - start) * 3 * N;
+            P3 L{{lin, lin + N, lin + 2 * N}};
+            P3 Vn{{vn, vn + N, vn + 2 * N}};
+            M3 Lo{{lout, lout + N, lout + 2 * N}};
+            hipLaunchKernelGGL(k_gvf_adjoint_step, NPART, RB, 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
+            hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part, NPART, acc);
+            TSF_TRY(check_launch(W));
+            std::swap(lin, lout);
+        }
+    }
+    // v^0 = f: the remaining adjoint flows into dF
+    for (int c = 0; c < 3; c++) {
+        hipLaunchKernelGGL(k_add_inplace_f, g1d(N), 256, 0, stream, dF + c * N, lin + c * N, N);
+    }
+    TSF_TRY(check_launch(W));
+    double h = 0.0;
+    TSF_TRY(sync_read_double(acc, &h, stream, W));
+    if (dmu) *dmu = h;
+    return TSF_OK;
+}
+
+tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_dims dims, hipStream_t stream) {
+    const char* W = "tsf_gvf_finish_adjoint";
+    TSF_TRY(check_field(v, false, W));
+    TSF_TRY(check_field(dv, true, W));
+    if (v->storage != TSF_STORAGE_F32 || dv->storage != TSF_STORAGE_F32)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 fields only", W);
+    TSF_TRY(check_dims(dims, 1, W));
+    const i64 N = D(dims).n();
+    hipLaunchKernelGGL(k_finish_adjoint, g1d(N), 256, 0, stream, (const float*)v->x, (const float*)v->y,
+                       (const float*)v->z, (const float*)dv->w, (float*)dv->x, (float*)dv->y, (float*)dv->z, N);
+    return check_launch(W);
+}
+
+tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
+                           const float* tdf_grad, const tsf_field* dfield, hipStream_t stream) {
+    const char* W = "tsf_tdf_adjoint";
+    const bool norm = !(rmax < 4.0f);
+    TSF_TRY(check_field(field, norm, W));
+    TSF_TRY(check_field(dfield, norm, W));
+    if (field->storage != TSF_STORAGE_F32 || dfield->storage != TSF_STORAGE_F32)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 fields only", W);
+    TSF_TRY(check_dims(dims, 1, W));
+    if (!tdf_grad) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL gradient", W);
+    if (!(rstep > 0.0f) || !__builtin_isfinite(rmin) || !__builtin_isfinite(rmax))
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: radius range must be finite with rstep > 0", W);
+    if ((double)(rmax - rmin) / rstep > 100000.0) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: too many radii", W);
+    static const CircleTables tab = make_circle_tables();
+    Dims d = D(dims);
+    Field<F32> F{(const float*)field->x, (const float*)field->y, (const float*)field->z, (const float*)field->w, 1.0f};
+    float* dx = (float*)dfield->x;
+    float* dy = (float*)dfield->y;
+    float* dz = (float*)dfield->z;
+    float* dw = (float*)dfield->w;
+    if (norm)
+        hipLaunchKernelGGL(k_tdf_adjoint<true>, STENCIL_GRID(d), 0, stream, F, d, rmin, rmax, rstep, tab, tdf_grad, dx,
+                           dy, dz, dw);
+    else
+        hipLaunchKernelGGL(k_tdf_adjoint<false>, STENCIL_GRID(d), 0, stream, F, d, rmin, rmax, rstep, tab, tdf_grad, dx,
+                           dy, dz, dw);
+    return check_launch(W);
+}
+
+}  // extern "C"

@@ -326,6 +326,32 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
     return TSF_OK;
 }
 
+// `iters` float Euler steps from v0 (which need not be f) into out (planes
+// x, y, z; w untouched).  ws: 3 planes, used when iters > 1.  For the
+// checkpointed forward and the adjoint's segment recomputation.
+tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], float* const out[3], Dims d, float mu,
+                         int iters, float* ws, hipStream_t stream) {
+    const i64 N = d.n();
+    Planes3<F32> F{f[0], f[1], f[2]};
+    Planes3<F32> V0{v0[0], v0[1], v0[2]};
+    Out4<F32> O{out[0], out[1], out[2], nullptr};
+    Out4<F32> Wk{ws, ws + N, ws + 2 * N, nullptr};
+    Planes3<F32> Wr{ws, ws + N, ws + 2 * N};
+    Planes3<F32> Or{out[0], out[1], out[2]};
+    bool vec2 = (d.X % 2) == 0;
+    const void* ptrs[10] = {f[0], f[1], f[2], v0[0], v0[1], v0[2], out[0], out[1], out[2], ws};
+    for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, 8));
+    for (int i = 1; i <= iters; i++) {
+        const bool to_out = ((iters - i) % 2) == 0;
+        Planes3<F32> src = (i == 1) ? V0 : (to_out ? Wr : Or);
+        Out4<F32> dst = to_out ? O : Wk;
+        if (vec2) launch_step<F32, 2>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+        else launch_step<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+        TSF_TRY(check_launch("gvf_steps_f32"));
+    }
+    return TSF_OK;
+}
+
 }  // namespace tsf
 
 using namespace tsf;

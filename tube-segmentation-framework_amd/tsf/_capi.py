@@ -66,6 +66,17 @@ _SIGS = {
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
     "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
     "tsf_cube_max": ([_vp, _vp, Dims, _i32, _vp, _vp], _i32),
+    "tsf_blur_adjoint": ([_vp, _vp, _vp, Dims, _f32, ctypes.POINTER(ctypes.c_double), _vp, _sz, _vp], _i32),
+    "tsf_blur_adjoint_workspace_size": ([], _sz),
+    "tsf_vector_field_adjoint": ([_vp, _PF, Dims, _f32, _i32, _vp, _vp, _sz, _vp], _i32),
+    "tsf_vector_field_adjoint_workspace_size": ([Dims], _sz),
+    "tsf_gvf_checkpoint_count": ([_i32, _i32], _i64),
+    "tsf_gvf_euler_checkpointed": ([_PF, _PF, Dims, _f32, _i32, _i32, _vp, _sz, _vp], _i32),
+    "tsf_gvf_euler_adjoint": ([_PF, Dims, _f32, _i32, _i32, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double), _vp,
+                               _sz, _vp], _i32),
+    "tsf_gvf_euler_adjoint_workspace_size": ([Dims, _i32], _sz),
+    "tsf_gvf_finish_adjoint": ([_PF, _PF, Dims, _vp], _i32),
+    "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _PF, _vp], _i32),
 }
 
 
@@ -356,3 +367,53 @@ def candidate_nms(tdf, radius, field, positions, count, maxd_cap, out, stream=No
 
 def cube_max(tdf, cand, cube, out, stream=None):
     _check(lib().tsf_cube_max(tdf.ptr, cand.ptr, _dims(tdf.shape), cube, out.ptr, _sp(stream)))
+
+
+# ------------------------------------------------------------------- adjoints
+def blur_adjoint(dB, src, dI, sigma, want_dsigma=True, stream=None):
+    ds = ctypes.c_double(0.0)
+    _check(lib().tsf_blur_adjoint(dB.ptr, src.ptr if src is not None else None, dI.ptr, _dims(dB.shape), sigma,
+                                  ctypes.byref(ds) if want_dsigma else None, None, 0, _sp(stream)))
+    return ds.value if want_dsigma else None
+
+
+def vector_field_adjoint(vol, dF, fmax, sign, dvol, workspace=None, stream=None):
+    d = dF.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_vector_field_adjoint(vol.ptr, ctypes.byref(d), _dims(vol.shape), fmax, sign, dvol.ptr, w, wb,
+                                          _sp(stream)))
+
+
+def gvf_checkpoint_count(iterations, every):
+    return lib().tsf_gvf_checkpoint_count(iterations, every)
+
+
+def gvf_euler_checkpointed(f, out, mu, iterations, every, checkpoints, stream=None):
+    fd, od = f.desc(), out.desc()
+    _check(lib().tsf_gvf_euler_checkpointed(ctypes.byref(fd), ctypes.byref(od), _dims(f.shape), mu, iterations, every,
+                                            checkpoints.ptr, checkpoints.nbytes, _sp(stream)))
+
+
+def gvf_euler_adjoint_workspace_size(shape, every):
+    return lib().tsf_gvf_euler_adjoint_workspace_size(_dims(shape), every)
+
+
+def gvf_euler_adjoint(f, mu, iterations, every, checkpoints, dV, dF, workspace=None, stream=None):
+    """dV, dF: DeviceArray (3, Z, Y, X) float32.  Returns dmu."""
+    fd = f.desc()
+    dm = ctypes.c_double(0.0)
+    w, wb = _ws(workspace)
+    _check(lib().tsf_gvf_euler_adjoint(ctypes.byref(fd), _dims(f.shape), mu, iterations, every, checkpoints.ptr,
+                                       dV.ptr, dF.ptr, ctypes.byref(dm), w, wb, _sp(stream)))
+    return dm.value
+
+
+def gvf_finish_adjoint(v, dv, stream=None):
+    a, b = v.desc(), dv.desc()
+    _check(lib().tsf_gvf_finish_adjoint(ctypes.byref(a), ctypes.byref(b), _dims(v.shape), _sp(stream)))
+
+
+def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None):
+    a, b = field.desc(), dfield.desc()
+    _check(lib().tsf_tdf_adjoint(ctypes.byref(a), _dims(field.shape), rmin, rmax, rstep, gT.ptr, ctypes.byref(b),
+                                 _sp(stream)))
