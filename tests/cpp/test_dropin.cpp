@@ -8,6 +8,8 @@
 //   test_dropin params            parameter system (no device needed)
 //   test_dropin gpu <dataset_dir> runCircleFittingMethod / candidate
 //                                 compaction / doEigen on dataset_1
+//   test_dropin autograd          TubeFilterFunction forward/backward against
+//                                 the oracle's FP64 chain gradient
 // Prints one "PASS name" / "FAIL name: reason" line per case; exit status is
 // the number of failures.
 #include <cmath>
@@ -18,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "../../tube-segmentation-framework_amd/host/tsf_autograd.hpp"
 #include "../../tube-segmentation-framework_amd/host/tsf_dropin.hpp"
 
 using namespace tsf;
@@ -43,6 +46,11 @@ void orc_candidate_nms_f32(const float* tdf, const float* rad, const float* F, i
 void orc_cube_max_f32(const float* tdf, const uint8_t* cand, int X, int Y, int Z, int cube, uint8_t* out);
 void orc_do_eigen_f32(const float* Fx, const float* Fy, const float* Fz, int X, int Y, int Z, int x, int y, int z,
                       float* lam, float* e1, float* e2, float* e3);
+void orc_chain_forward_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
+                           int iters, float rmin, float rmax, float rstep, double* tdf, double* rad);
+void orc_chain_gradient_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
+                            int iters, float rmin, float rmax, float rstep, const double* gT, double* tdf,
+                            double* dI, double* dmu, double* dsigma);
 }
 
 static int failures = 0;
@@ -265,12 +273,137 @@ static void gpu_tests(const std::string& dsdir) {
     });
 }
 
+// ----------------------------------------------------------------- autograd
+// A 32^3 phantom: three cylinders (along x, y and a diagonal) of intensity 1
+// over 0, plus uniform noise in [-0.05, 0.05) from a fixed LCG.
+static std::vector<float> phantom(int n) {
+    std::vector<float> v((size_t)n * n * n);
+    uint32_t s = 12345u;
+    for (int z = 0; z < n; z++)
+        for (int y = 0; y < n; y++)
+            for (int x = 0; x < n; x++) {
+                const float a = (float)(y - 11) * (y - 11) + (float)(z - 12) * (z - 12);
+                const float b = (float)(x - 20) * (x - 20) + (float)(z - 19) * (z - 19);
+                const float dx = (float)(x - z), dy = (float)(y - 22);
+                const float c = 0.5f * dx * dx + dy * dy;
+                float val = (a <= 9.0f || b <= 4.0f || c <= 6.25f) ? 1.0f : 0.0f;
+                s = s * 1664525u + 1013904223u;
+                val += ((float)(s >> 8) / 16777216.0f - 0.5f) * 0.1f;
+                v[((size_t)z * n + y) * n + x] = val;
+            }
+    return v;
+}
+
+static void autograd_tests() {
+    const int n = 32;
+    const size_t N = (size_t)n * n * n;
+    const float mu = 0.06f, sigma = 0.8f;
+    TubeFilterOptions o;
+    o.iterations = 40;
+    o.radiusMin = 1.0f;
+    o.radiusMax = 5.0f;
+    o.radiusStep = 1.0f;
+    o.checkpoint_every = 10;
+    const std::vector<float> vol = phantom(n);
+    std::vector<double> vol64(vol.begin(), vol.end());
+    OpenCL ocl;
+    const I3 size(n, n, n);
+    Image3D I(size, 1);
+    I.uploadPlane(ocl, 0, vol.data());
+    AutogradContext ctx;
+    std::vector<Image3D> out;
+    run("autograd_forward", [&] {
+        out = TubeFilterFunction::forward(&ctx, ocl, I, mu, sigma, o);
+        std::vector<double> t64(N), r64(N);
+        orc_chain_forward_f64(vol64.data(), n, n, n, sigma, o.Fmax, o.sign, mu, o.iterations, o.radiusMin, o.radiusMax,
+                              o.radiusStep, t64.data(), r64.data());
+        const std::vector<float> t = out[0].downloadPlane(ocl, 0);
+        double num = 0, den = 0;
+        for (size_t i = 0; i < N; i++) {
+            num += (t[i] - t64[i]) * (t[i] - t64[i]);
+            den += t64[i] * t64[i];
+        }
+        if (!(std::sqrt(num / den) < 1e-3)) return std::string("TDF differs from the FP64 chain");
+        return std::string();
+    });
+    run("autograd_backward_vs_fp64", [&] {
+        if (out.size() != 2) return std::string("no forward");
+        const std::vector<float> t = out[0].downloadPlane(ocl, 0), r = out[1].downloadPlane(ocl, 0);
+        std::vector<double> t64(N), r64(N);
+        orc_chain_forward_f64(vol64.data(), n, n, n, sigma, o.Fmax, o.sign, mu, o.iterations, o.radiusMin, o.radiusMax,
+                              o.radiusStep, t64.data(), r64.data());
+        // Eigenvalues of the (normalised) Hessian of the forward's field: where
+        // two of them nearly coincide the eigenvector derivative (~1/gap) turns
+        // the float/double difference of the field into a large gradient
+        // difference, so such voxels are left out of the loss.
+        const Image3D& V = ctx.get_saved_variables()[3];
+        Buffer lam(3 * N * sizeof(float));
+        const tsf_field vf = V.field();
+        check(tsf_hessian_eigen(&vf, V.dims(), 1, (float*)lam.get(), nullptr, ocl.queue));
+        std::vector<float> L(3 * N);
+        lam.download(ocl, L.data(), L.size() * sizeof(float));
+        // the loss weights: voxels with a response whose radius the FP64
+        // forward also picks (the TDF is piecewise smooth in its radius) and
+        // whose eigenvalues are at least 1e-2 apart
+        std::vector<float> gT(N);
+        std::vector<double> gT64(N);
+        size_t used = 0;
+        for (size_t i = 0; i < N; i++) {
+            const float l0 = L[i], l1 = L[N + i], l2 = L[2 * N + i];
+            const float gap = std::fmin(std::fabs(l1 - l0), std::fmin(std::fabs(l1 - l2), std::fabs(l2 - l0)));
+            const bool use = t[i] > 0.05f && r[i] == (float)r64[i] && gap >= 1e-2f;
+            gT[i] = use ? 0.5f + (float)((i * 2654435761u) % 1000u) / 1000.0f : 0.0f;
+            gT64[i] = gT[i];
+            used += use;
+        }
+        std::printf("  voxels in the loss: %zu\n", used);
+        Image3D G(size, 1);
+        G.uploadPlane(ocl, 0, gT.data());
+        std::vector<double> dI64(N), tt(N);
+        double dmu64 = 0, dsig64 = 0;
+        orc_chain_gradient_f64(vol64.data(), n, n, n, sigma, o.Fmax, o.sign, mu, o.iterations, o.radiusMin,
+                               o.radiusMax, o.radiusStep, gT64.data(), tt.data(), dI64.data(), &dmu64, &dsig64);
+        for (int rep = 0; rep < 2; rep++) {  // one context, two backwards
+            TubeFilterGrads g = TubeFilterFunction::backward(&ctx, ocl, {G});
+            const std::vector<float> dI = g.volume.downloadPlane(ocl, 0);
+            double num = 0, den = 0;
+            for (size_t i = 0; i < N; i++) {
+                num += (dI[i] - dI64[i]) * (dI[i] - dI64[i]);
+                den += dI64[i] * dI64[i];
+            }
+            const double e = std::sqrt(num / den);
+            std::printf("  backward %d: dI rel %.2e, dmu %.8g vs %.8g, dsigma %.8g vs %.8g\n", rep, e, g.mu, dmu64,
+                        g.sigma, dsig64);
+            if (!(e < 1e-3)) return std::string("dI differs from the FP64 gradient");
+            if (!(std::fabs(g.mu - dmu64) <= 1e-3 * std::fabs(dmu64))) return std::string("dmu differs");
+            if (!(std::fabs(g.sigma - dsig64) <= 1e-3 * std::fabs(dsig64))) return std::string("dsigma differs");
+        }
+        return std::string();
+    });
+    run("autograd_errors", [&] {
+        AutogradContext empty;
+        Image3D G(size, 1);
+        if (!throws([&] { TubeFilterFunction::backward(&empty, ocl, {G}); })) return std::string("empty context");
+        Image3D wrong(I3(n, n, n - 1), 1);
+        tsf_status st = TSF_OK;
+        if (!throws([&] { TubeFilterFunction::backward(&ctx, ocl, {wrong}); }, &st) || st != TSF_ERR_INVALID_ARGUMENT)
+            return std::string("mismatched gradient size accepted");
+        TubeFilterOptions bad = o;
+        bad.iterations = 0;
+        AutogradContext c2;
+        if (!throws([&] { TubeFilterFunction::forward(&c2, ocl, I, mu, sigma, bad); }))
+            return std::string("zero iterations accepted");
+        return std::string();
+    });
+}
+
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "params";
     if (mode == "params") param_tests();
     else if (mode == "gpu" && argc > 2) gpu_tests(argv[2]);
+    else if (mode == "autograd") autograd_tests();
     else {
-        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir>\n");
+        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir> | autograd\n");
         return 2;
     }
     std::printf("%d failure(s)\n", failures);

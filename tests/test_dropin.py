@@ -33,3 +33,11 @@ def test_dropin_pipeline_on_dataset_1(tsf):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
     assert r.stdout.count("PASS") == 7
+
+
+@pytest.mark.gpu
+def test_dropin_autograd_function(tsf):
+    r = _run("autograd", timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 3

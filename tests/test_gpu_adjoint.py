@@ -174,7 +174,10 @@ def test_chain_finite_differences_128(tsf, orc):
     Central differences, h = 1e-3: in mu and sigma on the float GPU forward;
     along a signed random subset of 64 voxels on the FP64 forward, because the
     float forward's rounding noise (~5e-5 in L over this volume) is of the
-    order of 1e-3 of that directional difference."""
+    order of 1e-3 of that directional difference.  Voxels whose radius
+    switches, whose response has a kink inside [-h, h], or (for the float /
+    double comparison) whose Hessian eigenvalues lie within 1e-2 of each
+    other are left out of the loss."""
     from tsf import synth
     vol, _ = synth.config4(seeds=(401,))[0]
     mu, sigma, h = 0.05, 0.8, 1e-3
@@ -212,9 +215,18 @@ def test_chain_finite_differences_128(tsf, orc):
     tp, rp = orc.chain_forward(V64 + h * D, *args)
     tm, rm = orc.chain_forward(V64 - h * D, *args)
     smooth &= (r64 == r0) & kinks("volume (FP64)", tp, tm, rp, rm, t64, r64)
+    # float against double: leave out voxels whose Hessian eigenvalues nearly
+    # coincide (the eigenvector derivative ~ 1/gap amplifies the float/double
+    # difference of the field there)
+    lam = tsf.DeviceArray((3,) + vol.shape, np.float32)
+    tsf.hessian_eigen(ctx.saved["V"], True, lam)
+    L = lam.download()
+    gap = np.minimum(np.abs(L[1] - L[0]), np.minimum(np.abs(L[1] - L[2]), np.abs(L[2] - L[0])))
+    print(f"eigenvalue gap < 1e-2: {int(((gap < 1e-2) & (gT != 0)).sum())} voxels")
+    smooth &= gap >= 1e-2
     diffs["volume"] = (tp - tm, None)
     print(f"voxels in loss: {int(smooth.sum())} of {int((gT != 0).sum())}")
-    assert smooth.sum() > 0.9 * (gT != 0).sum()
+    assert smooth.sum() > 0.8 * (gT != 0).sum()
     gTm = np.where(smooth, gT, 0.0).astype(np.float32).astype(np.float64)
     dI, dmu, dsig = _backward(tsf, ctx, gTm)  # the saved forward serves any number of backwards
     an = {"mu": dmu, "sigma": dsig, "volume": float((dI.astype(np.float64) * (2 * h * D)).sum())}

@@ -206,8 +206,33 @@ class Event:
             self.handle = None
 
 
+class _PlaneView:
+    """Plane c of a multi-plane F32 DeviceArray (keeps the base alive)."""
+
+    def __init__(self, base, c):
+        self.base = base
+        self.shape = base.shape[1:]
+        self.dtype = base.dtype
+        self.nbytes = base.nbytes // base.shape[0]
+        self.ptr = base.ptr + c * self.nbytes
+
+    def download(self, stream=None):
+        a = np.empty(self.shape, self.dtype)
+        _check(lib().tsf_memcpy_dtoh(a.ctypes.data, self.ptr, self.nbytes, _sp(stream)))
+        return a
+
+
 class Field:
     """Four device planes (x, y, z, w) of one storage type."""
+
+    @classmethod
+    def over(cls, base):
+        """F32 field over the planes of a (3 or 4, Z, Y, X) DeviceArray."""
+        assert base.dtype == np.float32 and base.shape[0] in (3, 4)
+        f = cls.__new__(cls)
+        f.shape, f.storage, f.scale = tuple(base.shape[1:]), STORAGE_F32, 1.0
+        f.planes = [_PlaneView(base, c) for c in range(base.shape[0])]
+        return f
 
     def __init__(self, shape, storage=STORAGE_F32, scale=1.0, with_w=True):
         self.shape = tuple(shape)

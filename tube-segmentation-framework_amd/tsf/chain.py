@@ -44,25 +44,19 @@ class TubeFilterFunction:
     def backward(ctx, gT, stream=None):
         """gT: DeviceArray of dL/dT.  Returns (dI DeviceArray, dmu, dsigma)."""
         s = ctx.saved
-        shape = s["shape"]
-        dV = C.Field(shape)
-        for p in dV.planes:
-            p.zero(stream)
-        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dV, stream)
-        C.gvf_finish_adjoint(s["V"], dV, stream)
-        dVx = C.DeviceArray((3,) + tuple(shape), np.float32)
-        for c in range(3):
-            C.lib().tsf_memcpy_dtod(dVx.ptr + c * dV.planes[c].nbytes, dV.planes[c].ptr, dV.planes[c].nbytes,
-                                    C._sp(stream))
-        dF = C.DeviceArray((3,) + tuple(shape), np.float32)
+        shape = tuple(s["shape"])
+        # dV: x, y, z, w planes in one allocation -- a field for the TDF and
+        # finish adjoints, the 3-plane input of the GVF adjoint
+        dV = C.DeviceArray((4,) + shape, np.float32)
+        dV.zero(stream)
+        dVf = C.Field.over(dV)
+        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dVf, stream)
+        C.gvf_finish_adjoint(s["V"], dVf, stream)
+        dF = C.DeviceArray((3,) + shape, np.float32)
         ws = C.DeviceArray((C.gvf_euler_adjoint_workspace_size(shape, s["k"]),), np.uint8)
-        dmu = C.gvf_euler_adjoint(s["F"], s["mu"], s["iterations"], s["k"], s["cps"], dVx, dF, ws, stream)
-        dFf = C.Field(shape, with_w=False)
-        n = int(np.prod(shape)) * 4
-        for c in range(3):
-            C.lib().tsf_memcpy_dtod(dFf.planes[c].ptr, dF.ptr + c * n, n, C._sp(stream))
+        dmu = C.gvf_euler_adjoint(s["F"], s["mu"], s["iterations"], s["k"], s["cps"], dV, dF, ws, stream)
         dB = C.DeviceArray(shape, np.float32)
-        C.vector_field_adjoint(s["B"], dFf, s["fmax"], s["sign"], dB, None, stream)
+        C.vector_field_adjoint(s["B"], C.Field.over(dF), s["fmax"], s["sign"], dB, ws, stream)
         dI = C.DeviceArray(shape, np.float32)
         dsigma = C.blur_adjoint(dB, s["I"], dI, s["sigma"], True, stream)
         return dI, dmu, dsigma
