@@ -10,7 +10,16 @@ float arithmetic -> TDF r = 2.5..4.0, step 0.5) -> combine.
 Headline value: GVF throughput in Gvoxel-iter/s (voxels x iterations / GVF
 time) of the FP16-stored field.  Also reported: forward ms per step, the same
 GVF in plain FP32 storage (speed-up), the GVF kernel's achieved HBM bandwidth
-against the 8 TB/s peak, and the oracle's FP32 GVF on the host CPU.
+against the 8 TB/s peak, the oracle's FP32 GVF on the host CPU, and one
+forward + adjoint step of the differentiable FP32 chain at the same size
+(blur 0.5 -> vector field -> GVF 400 iterations checkpointed every 40 ->
+TDF r = 0.5..4.0, backward to dL/dvolume, dL/dmu, dL/dsigma) with its
+device memory at the backward's peak.
+
+--fit-steps K runs K gradient-descent steps fitting mu and sigma of that
+chain on the noisy volume to the TDF of the noise-free rendering of the
+same tree (mu 0.05, sigma 0.5), starting from mu 0.03, sigma 0.8, and
+prints one JSON line for the fit.
 
 All inputs are resident in device memory before the timed region; timings use
 HIP events recorded on the stream the kernels run on.
@@ -45,6 +54,10 @@ def main():
     ap.add_argument("--fuse", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--checkpoint-every", type=int, default=40)
+    ap.add_argument("--fit-steps", type=int, default=0,
+                    help="run this many mu/sigma fitting steps instead of the forward benchmark")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="run a short fixed workload for hardware-counter passes (no timing output)")
     args = ap.parse_args()
@@ -71,6 +84,9 @@ def main():
 
     stream = tsf.Stream()
     d_vol = tsf.DeviceArray.from_host(vol, stream=stream)
+    if args.fit_steps > 0:
+        fit(args, vol, d_vol, shape, stream)
+        return
     d_blur = tsf.DeviceArray(shape, np.float32)
     F_small = tsf.Field(shape, S, scale)
     F = tsf.Field(shape, S, scale)
@@ -161,6 +177,11 @@ def main():
                "hbm_frac": round(36 * N / (f32_ms / launches * 1e-3) / PEAK_HBM, 4)}
         del F32, V32, ws32
 
+    adjoint = None
+    if not args.no_adjoint:
+        del F_small, F, V, ws, tws, t_s, r_s, t_l, r_l  # the forward's buffers
+        adjoint = adjoint_step(args, d_vol, shape, stream)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol)
@@ -198,6 +219,7 @@ def main():
         "forward_ms": round(fwd_ms, 3),
         "wall_s_timed": round(wall, 3),
         "f32_gvf": f32,
+        "adjoint": adjoint,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
                      "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
@@ -206,6 +228,78 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+CHAIN = dict(sigma=0.5, rmin=0.5, rmax=4.0, rstep=0.5)
+
+
+def adjoint_step(args, d_vol, shape, stream):
+    """One forward + backward of the FP32 chain, timed with events on the
+    chain's stream; a first step warms up.  dL/dT = 1 where T > 0 (every
+    responding voxel carries gradient)."""
+    import tsf
+    from tsf.chain import Ctx, TubeFilterFunction as TF
+    free0, total = tsf.mem_info()
+    ev = [tsf.Event() for _ in range(3)]
+    kw = dict(mu=args.mu, sigma=CHAIN["sigma"], iterations=args.iters, rmin=CHAIN["rmin"], rmax=CHAIN["rmax"],
+              rstep=CHAIN["rstep"], checkpoint_every=args.checkpoint_every, stream=stream)
+    res = []
+    for rep in range(2):
+        ctx = Ctx()
+        ev[0].record(stream)
+        T, R = TF.forward(ctx, d_vol, **kw)
+        ev[1].record(stream)
+        stream.synchronize()
+        if rep == 0:
+            gT = tsf.DeviceArray.from_host((T.download() > 0).astype(np.float32))
+        ev[1].record(stream)
+        tb = time.time()
+        _, dmu, dsigma = TF.backward(ctx, gT, stream)
+        ev[2].record(stream)
+        stream.synchronize()
+        res.append((ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2]), (time.time() - tb) * 1e3,
+                    ctx.backward_used_bytes))
+        log(f"adjoint step {rep}: forward {res[-1][0]:.1f} ms, backward {res[-1][1]:.1f} ms, "
+            f"dmu {dmu:.5g}, dsigma {dsigma:.5g}")
+        del ctx, T, R
+    fwd_ms, bwd_ms, bwd_wall, used = res[-1]
+    return {"workload": f"FP32 chain at {shape[0]}^3: blur {CHAIN['sigma']}, vector field, GVF {args.iters} it "
+                        f"mu {args.mu} checkpointed every {args.checkpoint_every}, TDF {CHAIN['rmin']}-"
+                        f"{CHAIN['rmax']} step {CHAIN['rstep']}; backward to dvolume, dmu, dsigma",
+            "forward_ms": round(fwd_ms, 2), "backward_ms": round(bwd_ms, 2),
+            "step_ms": round(fwd_ms + bwd_ms, 2), "backward_wall_ms": round(bwd_wall, 2),
+            "hbm_peak_gb": round(used / 1e9, 2), "hbm_chain_gb": round((used - (total - free0)) / 1e9, 2),
+            "hbm_total_gb": round(total / 1e9, 1)}
+
+
+def fit(args, vol, d_vol, shape, stream):
+    import tsf
+    from tsf import synth
+    from tsf.chain import Ctx, TubeFilterFunction as TF, fit_mu_sigma
+    clean, _ = synth.mra_thin(shape, 120, 505, scale=0.0)
+    d_clean = tsf.DeviceArray.from_host(clean, stream=stream)
+    ctx = Ctx()
+    target, _ = TF.forward(ctx, d_clean, 0.05, 0.5, args.iters, CHAIN["rmin"], CHAIN["rmax"], CHAIN["rstep"],
+                           checkpoint_every=args.iters, stream=stream)
+    stream.synchronize()
+    del ctx, d_clean
+    t0 = time.time()
+    hist, mu, sigma = fit_mu_sigma(d_vol, target, 0.03, 0.8, args.fit_steps, args.iters, CHAIN["rmin"],
+                                   CHAIN["rmax"], CHAIN["rstep"], checkpoint_every=args.checkpoint_every,
+                                   stream=stream, log=log)
+    wall = time.time() - t0
+    ms = [h["ms"] for h in hist]
+    print(json.dumps({
+        "metric": "mu/sigma fit: ms per gradient-descent step (FP32 chain forward + adjoint)",
+        "value": round(float(np.median(ms)), 1), "unit": "ms/step", "higher_is_better": False,
+        "steps": args.fit_steps, "wall_s": round(wall, 1),
+        "config": {"workload": f"fit mu, sigma at {shape[0]}^3 to the noise-free TDF (mu 0.05, sigma 0.5); "
+                               f"GVF {args.iters} it checkpointed every {args.checkpoint_every}, TDF "
+                               f"{CHAIN['rmin']}-{CHAIN['rmax']}", "start": {"mu": 0.03, "sigma": 0.8}},
+        "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"], "mu": mu, "sigma": sigma,
+        "hbm_peak_gb": round(max(h["hbm_bytes"] or 0 for h in hist) / 1e9, 2),
+        "history": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in h.items() if k != "hbm_bytes"}
+                    for h in hist]}), flush=True)
 
 
 def cpu_baseline(vol):

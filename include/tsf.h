@@ -82,6 +82,8 @@ tsf_status tsf_event_destroy(hipEvent_t ev);
 tsf_status tsf_event_record(hipEvent_t ev, hipStream_t stream);
 tsf_status tsf_event_elapsed_ms(hipEvent_t start, hipEvent_t stop, float* ms);
 tsf_status tsf_device_count(int* count);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+tsf_status tsf_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ------------------------------------------------------------ volume prep */
 
@@ -277,6 +279,14 @@ tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_d
  * summation order (last bits) is not reproducible between runs. */
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
                            const float* tdf_grad, const tsf_field* dfield, hipStream_t stream);
+
+/* Mean squared error of t against target over n floats: *loss (host) =
+ * (1/n) sum (t - target)^2 and grad = dL/dt = 2 (t - target) / n, the
+ * gradient that starts a backward of the chain when fitting mu and sigma to
+ * a target response.  Synchronises the stream. */
+tsf_status tsf_mse(const float* t, const float* target, int64_t n, float* grad, double* loss, void* workspace,
+                   size_t workspace_bytes, hipStream_t stream);
+size_t tsf_mse_workspace_size(void);
 
 #ifdef __cplusplus
 }

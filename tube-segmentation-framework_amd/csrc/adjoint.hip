@@ -408,6 +408,19 @@ __global__ void k_mag_plane(const float* __restrict__ x, const float* __restrict
     }
 }
 
+// L = (1/n) sum (t - target)^2 -> grad = 2 (t - target) / n, block partials of L
+__global__ void __launch_bounds__(RB) k_mse(const float* __restrict__ t, const float* __restrict__ target, i64 n,
+                                            float* __restrict__ grad, double* __restrict__ part) {
+    const float inv = 2.0f / (float)n;
+    double s = 0.0;
+    for (i64 i = blockIdx.x * (i64)RB + threadIdx.x; i < n; i += (i64)gridDim.x * RB) {
+        const float e = t[i] - target[i];
+        grad[i] = inv * e;
+        s += (double)e * (double)e;
+    }
+    block_sum_double(s, &part[blockIdx.x]);
+}
+
 static unsigned g1d(i64 n) {
     i64 g = (n + 255) / 256;
     return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -641,6 +654,27 @@ tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_d
     hipLaunchKernelGGL(k_finish_adjoint, g1d(N), 256, 0, stream, (const float*)v->x, (const float*)v->y,
                        (const float*)v->z, (const float*)dv->w, (float*)dv->x, (float*)dv->y, (float*)dv->z, N);
     return check_launch(W);
+}
+
+size_t tsf_mse_workspace_size(void) { return (NPART + 1) * sizeof(double); }
+
+tsf_status tsf_mse(const float* t, const float* target, int64_t n, float* grad, double* loss, void* workspace,
+                   size_t workspace_bytes, hipStream_t stream) {
+    const char* W = "tsf_mse";
+    if (!t || !target || !grad || !loss) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL argument", W);
+    if (n < 1) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: n must be >= 1", W);
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_mse_workspace_size(), W));
+    double* part = (double*)sc.ptr;
+    double* acc = part + NPART;
+    hipLaunchKernelGGL(k_zero_double, 1, 64, 0, stream, acc, 1);
+    hipLaunchKernelGGL(k_mse, NPART, RB, 0, stream, t, target, (i64)n, grad, part);
+    hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part, NPART, acc);
+    TSF_TRY(check_launch(W));
+    double h = 0.0;
+    TSF_TRY(sync_read_double(acc, &h, stream, W));
+    *loss = h / (double)n;
+    return TSF_OK;
 }
 
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,

@@ -159,5 +159,10 @@ tsf_status tsf_device_count(int* count) {
     }
     return TSF_OK;
 }
+tsf_status tsf_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return set_error(TSF_ERR_INVALID_ARGUMENT, "tsf_mem_info: NULL");
+    HIPCALL(hipMemGetInfo(free_bytes, total_bytes), "tsf_mem_info");
+    return TSF_OK;
+}
 
 }  // extern "C"

@@ -48,6 +48,7 @@ _SIGS = {
     "tsf_event_record": ([_vp, _vp], _i32),
     "tsf_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(_f32)], _i32),
     "tsf_device_count": ([ctypes.POINTER(ctypes.c_int)], _i32),
+    "tsf_mem_info": ([ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], _i32),
     "tsf_create_blur_mask": ([_f32, _vp, ctypes.POINTER(_i32)], _i32),
     "tsf_blur_gaussian": ([_vp, _vp, Dims, _f32, _vp], _i32),
     "tsf_to_float": ([_vp, _i32, _vp, Dims, _f32, _f32, _vp], _i32),
@@ -76,6 +77,8 @@ _SIGS = {
                                _sz, _vp], _i32),
     "tsf_gvf_euler_adjoint_workspace_size": ([Dims, _i32], _sz),
     "tsf_gvf_finish_adjoint": ([_PF, _PF, Dims, _vp], _i32),
+    "tsf_mse": ([_vp, _vp, ctypes.c_int64, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp], _i32),
+    "tsf_mse_workspace_size": ([], ctypes.c_size_t),
     "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _PF, _vp], _i32),
 }
 
@@ -110,6 +113,13 @@ def device_count():
     n = ctypes.c_int(0)
     st = lib().tsf_device_count(ctypes.byref(n))
     return n.value if st == 0 else 0
+
+
+def mem_info():
+    """(free, total) bytes of device memory."""
+    f, t = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _check(lib().tsf_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def storage_dtype(storage):
@@ -442,3 +452,12 @@ def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None):
     a, b = field.desc(), dfield.desc()
     _check(lib().tsf_tdf_adjoint(ctypes.byref(a), _dims(field.shape), rmin, rmax, rstep, gT.ptr, ctypes.byref(b),
                                  _sp(stream)))
+
+
+def mse(t, target, grad, workspace=None, stream=None):
+    """Returns (1/n) sum (t - target)^2 and writes grad = 2 (t - target) / n."""
+    loss = ctypes.c_double(0.0)
+    w, wb = _ws(workspace)
+    _check(lib().tsf_mse(t.ptr, target.ptr, int(np.prod(t.shape)), grad.ptr, ctypes.byref(loss), w, wb,
+                         _sp(stream)))
+    return loss.value
