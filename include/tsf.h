@@ -21,7 +21,8 @@
  *  - Entry points that need temporary device memory take (workspace,
  *    workspace_bytes).  Pass NULL/0 to let the library allocate it (the call
  *    then synchronises the stream before freeing); query the size with the
- *    matching *_workspace_size function to supply it yourself.
+ *    matching *_workspace_size function to supply it yourself.  A supplied
+ *    workspace must be 256-byte aligned (tsf_malloc / hipMalloc memory is).
  */
 #ifndef TSF_H
 #define TSF_H
@@ -275,10 +276,15 @@ tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_d
  * (+=) dL/dV into dfield.xyz and, when rmax >= 4 (normalised Hessian),
  * dL/dW into dfield.w.  Includes the sampled values, the sample positions
  * and the eigenvector directions (first-order perturbation of e2, e3).
- * Voxels with T = 0 carry no gradient.  Scatter with float atomics, so the
- * summation order (last bits) is not reproducible between runs. */
+ * Voxels with T = 0 carry no gradient.  The radius search is the forward
+ * filter's own code on the same packed texels, so r* is the forward's.
+ * Contributions are summed in an LDS tile per 16x4x4 brick and then added
+ * with float atomics, so the summation order (last bits) is not
+ * reproducible between runs.  Workspace: the packed field (16 B/voxel). */
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
-                           const float* tdf_grad, const tsf_field* dfield, hipStream_t stream);
+                           const float* tdf_grad, const tsf_field* dfield, void* workspace, size_t workspace_bytes,
+                           hipStream_t stream);
+size_t tsf_tdf_adjoint_workspace_size(tsf_dims dims);
 
 /* Mean squared error of t against target over n floats: *loss (host) =
  * (1/n) sum (t - target)^2 and grad = dL/dt = 2 (t - target) / n, the

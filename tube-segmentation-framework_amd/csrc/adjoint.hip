@@ -8,9 +8,10 @@
 //   k_blur_adjoint     gather B^T dB with clamp-to-edge folding (deterministic)
 //   k_blur_dsigma      per-voxel dB * (dw * I) -> block partial sums (double)
 //   k_vf_dg / k_vf_gather   the vector-field adjoint as two gathers
-//   k_gvf_adjoint_step the transposed Jacobi step: mirror fold of the
-//                      adjoint, transposed 7-point stencil and data term,
-//                      dF accumulation, d/dmu block partials (double)
+//   k_gvf_adj          the transposed Jacobi step on the folded adjoint:
+//                      transposed 7-point stencil and data term, dF
+//                      accumulation, d/dmu block partials (double), and the
+//                      mirror fold of its output for the next reverse step
 //   k_tdf_adjoint      per-voxel recomputation of the circle fit at r*,
 //                      then atomic scatter of the sample, direction
 //                      (eigenvector perturbation) and Hessian contributions
@@ -21,6 +22,7 @@
 
 #include "eigen3.hpp"
 #include "field.hpp"
+#include "texel.hpp"
 
 namespace tsf {
 
@@ -206,40 +208,114 @@ struct M3 {
     float* c[3];
 };
 
-__global__ void __launch_bounds__(RB) k_gvf_adjoint_step(P3 lam, P3 v, P3 F, Dims d, float mu, M3 lam_out, M3 dF,
-                                                         double* __restrict__ part) {
-    const i64 n = d.n();
-    const i64 sy = d.X, sz = (i64)d.X * d.Y;
+// Lf = fold(lam) on all three planes (the adjoint that enters the first
+// reverse step).
+__global__ void k_fold3(P3 lam, Dims d, M3 out) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (x >= d.X || y >= d.Y) return;
+    const i64 i = d.at(x, y, z);
+#pragma unroll
+    for (int c = 0; c < 3; c++) out.c[c][i] = fold(lam.c[c], d, x, y, z);
+}
+
+// lam^n at a position w with a boundary coordinate: there Lf(w) = 0, so only
+// the mu-weighted neighbours remain.
+__device__ __forceinline__ float lam_boundary(const float* __restrict__ Lf, const Dims& d, int x, int y, int z,
+                                              float mu) {
+    const i64 i = d.at(x, y, z);
+    float nb = 0.0f;
+    if (x + 1 < d.X) nb += Lf[i + 1];
+    if (x > 0) nb += Lf[i - 1];
+    if (y + 1 < d.Y) nb += Lf[i + d.X];
+    if (y > 0) nb += Lf[i - d.X];
+    if (z + 1 < d.Z) nb += Lf[i + (i64)d.X * d.Y];
+    if (z > 0) nb += Lf[i - (i64)d.X * d.Y];
+    return mu * nb;
+}
+
+// One reverse Euler step.  In: Lf = fold(lam^{n+1}) (zero on every boundary
+// plane), v = v^n, F.  lam^n(y) = (1 - 6 mu - |f(y)|^2) Lf(y) + mu sum_e Lf(y+e).
+// FOLD: out = fold(lam^n), the input of the next reverse step; otherwise
+// (the step from v^0 = f) dF += lam^n.  Always: dF += data-term gradient
+// Lf m2 + 2 f sum_c Lf_c (f_c - v_c); part[block] = sum Lf . lap(v) (d/dmu).
+template <bool FOLD>
+__global__ void __launch_bounds__(256) k_gvf_adj(P3 Lf, P3 v, P3 F, Dims d, float mu, M3 out, M3 dF,
+                                                 double* __restrict__ part) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
     double gm = 0.0;
-    for (i64 i = blockIdx.x * (i64)RB + threadIdx.x; i < n; i += (i64)gridDim.x * RB) {
-        const int x = (int)(i % d.X), y = (int)((i / d.X) % d.Y), z = (int)(i / sz);
+    if (x < d.X && y < d.Y) {
+        const i64 sy = d.X, sz = (i64)d.X * d.Y;
+        const i64 i = d.at(x, y, z);
+        const bool bxm = x > 0, bxp = x < d.X - 1, bym = y > 0, byp = y < d.Y - 1, bzm = z > 0, bzp = z < d.Z - 1;
+        const i64 ixp = bxp ? i + 1 : i, ixm = bxm ? i - 1 : i, iyp = byp ? i + sy : i, iym = bym ? i - sy : i;
+        const i64 izp = bzp ? i + sz : i, izm = bzm ? i - sz : i;
+        const bool interior = bxm && bxp && bym && byp && bzm && bzp;
         const float f0 = F.c[0][i], f1 = F.c[1][i], f2 = F.c[2][i];
         const float fc[3] = {f0, f1, f2};
         const float m2 = f0 * f0 + f1 * f1 + f2 * f2;
-        const bool interior = x > 0 && y > 0 && z > 0 && x < d.X - 1 && y < d.Y - 1 && z < d.Z - 1;
-        float L[3];
+        const float diag = 1.0f - 6.0f * mu - m2;
+        float L[3], lam[3];
         float s = 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float* la = lam.c[c];
-            L[c] = fold(la, d, x, y, z);
-            const float nb = fold(la, d, x + 1, y, z) + fold(la, d, x - 1, y, z) + fold(la, d, x, y + 1, z) +
-                             fold(la, d, x, y - 1, z) + fold(la, d, x, y, z + 1) + fold(la, d, x, y, z - 1);
-            lam_out.c[c][i] = (1.0f - 6.0f * mu - m2) * L[c] + mu * nb;
-            s += L[c] * (fc[c] - v.c[c][i]);
+            const float* la = Lf.c[c];
+            const float* vc = v.c[c];
+            const float l0 = la[i], a = la[ixp], b = la[ixm], e = la[iyp], g = la[iym], h = la[izp], k = la[izm];
+            const float v0 = vc[i], va = vc[ixp], vb = vc[ixm], ve = vc[iyp], vg = vc[iym], vh = vc[izp], vk = vc[izm];
+            const float nb = (bxp ? a : 0.0f) + (bxm ? b : 0.0f) + (byp ? e : 0.0f) + (bym ? g : 0.0f) +
+                             (bzp ? h : 0.0f) + (bzm ? k : 0.0f);
+            L[c] = l0;
+            lam[c] = diag * l0 + mu * nb;
+            s += l0 * (fc[c] - v0);
+            const float lap = va + vb + ve + vg + vh + vk - 6.0f * v0;
+            if (interior) gm += (double)l0 * (double)lap;
         }
 #pragma unroll
-        for (int c = 0; c < 3; c++) dF.c[c][i] += L[c] * m2 + 2.0f * fc[c] * s;
-        if (interior) {
+        for (int c = 0; c < 3; c++) {
+            float g = dF.c[c][i] + (L[c] * m2 + 2.0f * fc[c] * s);
+            if (!FOLD) g += lam[c];
+            dF.c[c][i] = g;
+        }
+        if (FOLD) {
+            // fold(lam^n)(q): zero on boundary planes; at q == 2 or q == N-3
+            // (per axis) the mirrored boundary positions add their lam^n
+            if (!interior) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float* p = v.c[c];
-                const float lap = p[i + 1] + p[i - 1] + p[i + sy] + p[i - sy] + p[i + sz] + p[i - sz] - 6.0f * p[i];
-                gm += (double)L[c] * (double)lap;
+                for (int c = 0; c < 3; c++) lam[c] = 0.0f;
+            } else if (x == 2 || x == d.X - 3 || y == 2 || y == d.Y - 3 || z == 2 || z == d.Z - 3) {
+                int px[3], py[3], pz[3];
+                const int nx = pre_list(x, d.X, px), ny = pre_list(y, d.Y, py), nz = pre_list(z, d.Z, pz);
+                for (int cz = 0; cz < nz; cz++)
+                    for (int cy = 0; cy < ny; cy++)
+                        for (int cx = 0; cx < nx; cx++) {
+                            if (cx == 0 && cy == 0 && cz == 0) continue;  // q itself
+#pragma unroll
+                            for (int c = 0; c < 3; c++) lam[c] += lam_boundary(Lf.c[c], d, px[cx], py[cy], pz[cz], mu);
+                        }
             }
+#pragma unroll
+            for (int c = 0; c < 3; c++) out.c[c][i] = lam[c];
         }
     }
-    block_sum_double(gm, &part[blockIdx.x]);
+    // block partial of d/dmu
+    __shared__ double sh[256];
+    const int t = threadIdx.y * 64 + threadIdx.x;
+    sh[t] = gm;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) sh[t] += sh[t + o];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x + (i64)gridDim.x * (blockIdx.y + (i64)gridDim.y * blockIdx.z)] = sh[0];
+}
+
+// part2[b] = sum of a contiguous chunk of n partials (fixed order)
+__global__ void __launch_bounds__(RB) k_reduce_parts(const double* __restrict__ part, i64 n, double* __restrict__ part2) {
+    const i64 chunk = (n + gridDim.x - 1) / gridDim.x;
+    const i64 lo = blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0.0;
+    for (i64 j = lo + threadIdx.x; j < hi; j += RB) s += part[j];
+    block_sum_double(s, &part2[blockIdx.x]);
 }
 
 // W = |V| (1 where |V| == 0): dV += dW V / |V|
@@ -259,9 +335,11 @@ __global__ void k_finish_adjoint(const float* __restrict__ vx, const float* __re
 }
 
 // ---------------------------------------------------------------- TDF
-// Trilinear sample (the forward's order of terms) with its spatial Jacobian.
-__device__ __forceinline__ void tri_grad(const Field<F32>& F, const Dims& d, float u, float v, float w, float s[3],
-                                         float J[3][3], i64 idx[8], float wt[8]) {
+// The filter's trilinear sample (trilinear_tex: identical terms and order,
+// so the recomputed radius is the forward's) together with its spatial
+// Jacobian and the eight corner indices and weights.
+__device__ __forceinline__ void tri_grad(const Tex<F32>& T, const Dims& d, float u, float v, float w, float s[3],
+                                         float J[3][3], int ci[8], int cj[8], int ck[8], float wt[8]) {
     const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
     const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
     const float a = uu - fu, b = vv - fv, c = ww - fw;
@@ -270,133 +348,198 @@ __device__ __forceinline__ void tri_grad(const Field<F32>& F, const Dims& d, flo
     i0 = clampi(i0, 0, d.X - 1);
     j0 = clampi(j0, 0, d.Y - 1);
     k0 = clampi(k0, 0, d.Z - 1);
+    const int I[8] = {i0, i1, i0, i1, i0, i1, i0, i1}, Jj[8] = {j0, j0, j1, j1, j0, j0, j1, j1};
+    const int K[8] = {k0, k0, k0, k0, k1, k1, k1, k1};
     const float A[8] = {1 - a, a, 1 - a, a, 1 - a, a, 1 - a, a};
     const float Bv[8] = {1 - b, 1 - b, b, b, 1 - b, 1 - b, b, b};
     const float C[8] = {1 - c, 1 - c, 1 - c, 1 - c, c, c, c, c};
     const float dA[8] = {-1, 1, -1, 1, -1, 1, -1, 1}, dB[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
     const float dC[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-    idx[0] = d.at(i0, j0, k0); idx[1] = d.at(i1, j0, k0); idx[2] = d.at(i0, j1, k0); idx[3] = d.at(i1, j1, k0);
-    idx[4] = d.at(i0, j0, k1); idx[5] = d.at(i1, j0, k1); idx[6] = d.at(i0, j1, k1); idx[7] = d.at(i1, j1, k1);
+    float4 t[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) wt[k] = A[k] * Bv[k] * C[k];
-#pragma unroll
-    for (int comp = 0; comp < 3; comp++) {
-        float sv = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float t = F.c(comp, idx[k]);
-            sv = sv + wt[k] * t;
-            g0 += dA[k] * Bv[k] * C[k] * t;
-            g1 += A[k] * dB[k] * C[k] * t;
-            g2 += A[k] * Bv[k] * dC[k] * t;
-        }
-        s[comp] = sv;
-        J[comp][0] = g0;
-        J[comp][1] = g1;
-        J[comp][2] = g2;
+    for (int k = 0; k < 8; k++) {
+        t[k] = T.at(d.at(I[k], Jj[k], K[k]));
+        ci[k] = I[k];
+        cj[k] = Jj[k];
+        ck[k] = K[k];
+        wt[k] = A[k] * Bv[k] * C[k];
     }
-}
-
-template <bool NORM>
-__global__ void __launch_bounds__(256) k_tdf_adjoint(Field<F32> F, Dims d, float rmin, float rmax, float rstep,
-                                                     CircleTables tab, const float* __restrict__ gT,
-                                                     float* __restrict__ dVx, float* __restrict__ dVy,
-                                                     float* __restrict__ dVz, float* __restrict__ dW) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
-    if (x >= d.X || y >= d.Y) return;
-    const float g = gT[d.at(x, y, z)];
-    if (g == 0.0f) return;
-    float H[3][3], E[3][3], lam[3];
-    hessian<NORM>(F, d, x, y, z, H);
-    eigen3(H, E, lam);
-    const float e2x = E[0][1], e2y = E[1][1], e2z = E[2][1];
-    const float e3x = E[0][2], e3y = E[1][2], e3z = E[2][2];
-    // forward radius search, as circleFittingTDF computes it
-    float maxSum = 0.0f, rstar = 0.0f;
-    const float fx = (float)x, fy = (float)y, fz = (float)z;
-    for (float radius = rmin; radius <= rmax; radius += rstep) {
-        float sum = 0.0f;
-        for (int j = 0; j < 32; j++) {
-            const float vx = tab.c[j] * e3x + tab.s[j] * e2x, vy = tab.c[j] * e3y + tab.s[j] * e2y;
-            const float vz = tab.c[j] * e3z + tab.s[j] * e2z;
-            float s[3], J[3][3], wt[8];
-            i64 idx[8];
-            tri_grad(F, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s, J, idx, wt);
-            sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
-        }
-        sum /= 32.0f;
-        if (sum > maxSum) {
-            maxSum = sum;
-            rstar = radius;
-        } else {
-            break;
-        }
+    // the filter's sum, term by term
+    float sx = wt[0] * t[0].x, sy = wt[0] * t[0].y, sz = wt[0] * t[0].z;
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        sx = sx + wt[k] * t[k].x;
+        sy = sy + wt[k] * t[k].y;
+        sz = sz + wt[k] * t[k].z;
     }
-    if (!(maxSum > 0.0f)) return;
-    const float cf = -g / 32.0f;
-    float A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0};
-    float* dV[3] = {dVx, dVy, dVz};
-    for (int j = 0; j < 32; j++) {
-        const float dj[3] = {tab.c[j] * e3x + tab.s[j] * e2x, tab.c[j] * e3y + tab.s[j] * e2y,
-                             tab.c[j] * e3z + tab.s[j] * e2z};
-        float s[3], J[3][3], wt[8];
-        i64 idx[8];
-        tri_grad(F, d, fx + rstar * dj[0], fy + rstar * dj[1], fz + rstar * dj[2], s, J, idx, wt);
+    s[0] = sx;
+    s[1] = sy;
+    s[2] = sz;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+    for (int m = 0; m < 3; m++) J[m][0] = J[m][1] = J[m][2] = 0.0f;
 #pragma unroll
-            for (int c = 0; c < 3; c++) unsafeAtomicAdd(dV[c] + idx[k], cf * dj[c] * wt[k]);
+    for (int k = 0; k < 8; k++) {
+        const float gx = dA[k] * Bv[k] * C[k], gy = A[k] * dB[k] * C[k], gz = A[k] * Bv[k] * dC[k];
+        const float tv[3] = {t[k].x, t[k].y, t[k].z};
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-            const float u = s[m] + rstar * (dj[0] * J[0][m] + dj[1] * J[1][m] + dj[2] * J[2][m]);
-            A3[m] += cf * tab.c[j] * u;
-            A2[m] += cf * tab.s[j] * u;
+            J[m][0] += gx * tv[m];
+            J[m][1] += gy * tv[m];
+            J[m][2] += gz * tv[m];
         }
     }
-    // eigenvector perturbation: de_k = sum_{i != k} e_i (e_i^T dH e_k) / (lam_k - lam_i)
-    float G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-    for (int kk = 0; kk < 2; kk++) {
-        const int k = kk + 1;
-        const float* A = kk == 0 ? A2 : A3;
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            if (i == k || lam[k] == lam[i]) continue;
-            const float coef = (A[0] * E[0][i] + A[1] * E[1][i] + A[2] * E[2][i]) / (lam[k] - lam[i]);
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) G[a][b] += coef * E[a][i] * E[b][k];
-        }
-    }
-    const float gxx = G[0][0], gyx = G[0][1] + G[1][0], gzx = G[0][2] + G[2][0];
-    const float gyy = G[1][1], gzy = G[1][2] + G[2][1], gzz = G[2][2];
-    auto put = [&](int comp, int xx, int yy, int zz, float cv) {
-        const i64 q = d.at(clampi(xx, 0, d.X - 1), clampi(yy, 0, d.Y - 1), clampi(zz, 0, d.Z - 1));
-        if (NORM) {
-            const float wq = F.mag(q);
-            unsafeAtomicAdd(dV[comp] + q, cv / wq);
-            unsafeAtomicAdd(dW + q, -cv * F.c(comp, q) / (wq * wq));
-        } else {
-            unsafeAtomicAdd(dV[comp] + q, cv);
-        }
-    };
-    put(0, x + 1, y, z, 0.5f * gxx);
-    put(0, x - 1, y, z, -0.5f * gxx);
-    put(1, x + 1, y, z, 0.5f * gyx);
-    put(1, x - 1, y, z, -0.5f * gyx);
-    put(1, x, y + 1, z, 0.5f * gyy);
-    put(1, x, y - 1, z, -0.5f * gyy);
-    put(2, x + 1, y, z, 0.5f * gzx);
-    put(2, x - 1, y, z, -0.5f * gzx);
-    put(2, x, y + 1, z, 0.5f * gzy);
-    put(2, x, y - 1, z, -0.5f * gzy);
-    put(2, x, y, z + 1, 0.5f * gzz);
-    put(2, x, y, z - 1, -0.5f * gzz);
 }
 
-__global__ void k_add_inplace_f(float* __restrict__ a, const float* __restrict__ b, i64 n) {
-    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) a[i] = a[i] + b[i];
+// Accumulation tile of dV (3 planes) over one brick plus a halo of `h`
+// voxels, in LDS; contributions outside it go to global atomics.
+struct AdjTile {
+    float* a;  // 3 * cells floats
+    int ox, oy, oz, tx, ty, tz, cells;
+    __device__ __forceinline__ void add(float* const dV[3], const Dims& d, int c, int x, int y, int z, float v) const {
+        const int lx = x - ox, ly = y - oy, lz = z - oz;
+        if (a && (unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty && (unsigned)lz < (unsigned)tz)
+            atomicAdd(&a[c * cells + lx + tx * (ly + ty * lz)], v);
+        else
+            unsafeAtomicAdd(dV[c] + d.at(x, y, z), v);
+    }
+};
+
+// Adjoint of circleFittingTDF at the forward's radius.  Block = one 16x4x4
+// brick; halo > 0 selects the LDS tile (dynamic shared memory of
+// 3 * (16 + 2 halo)(4 + 2 halo)^2 floats), halo == 0 global atomics only.
+template <bool NORM>
+__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims d, float rmin, float rmax, float rstep,
+                                                               CircleTables tab, const float* __restrict__ gT,
+                                                               float* __restrict__ dVx, float* __restrict__ dVy,
+                                                               float* __restrict__ dVz, float* __restrict__ dW,
+                                                               int halo) {
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x + CBX * (threadIdx.y + CBY * threadIdx.z);
+    AdjTile tile;
+    tile.a = halo > 0 ? lds : nullptr;
+    tile.tx = CBX + 2 * halo;
+    tile.ty = CBY + 2 * halo;
+    tile.tz = CBZ + 2 * halo;
+    tile.cells = tile.tx * tile.ty * tile.tz;
+    tile.ox = blockIdx.x * CBX - halo;
+    tile.oy = blockIdx.y * CBY - halo;
+    tile.oz = blockIdx.z * CBZ - halo;
+    if (halo > 0) {
+        for (int k = tid; k < 3 * tile.cells; k += CBX * CBY * CBZ) lds[k] = 0.0f;
+        __syncthreads();
+    }
+    float* const dV[3] = {dVx, dVy, dVz};
+    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
+    const int z = blockIdx.z * CBZ + threadIdx.z;
+    const float g = (x < d.X && y < d.Y && z < d.Z) ? gT[d.at(x, y, z)] : 0.0f;
+    if (g != 0.0f) {
+        float H[3][3], E[3][3], lam[3];
+        hessian_tex<F32, NORM>(T, d, x, y, z, H);
+        eigen3(H, E, lam);
+        const float e2x = E[0][1], e2y = E[1][1], e2z = E[2][1];
+        const float e3x = E[0][2], e3y = E[1][2], e3z = E[2][2];
+        // the forward's radius search
+        float maxSum = 0.0f, rstar = 0.0f;
+        const float fx = (float)x, fy = (float)y, fz = (float)z;
+        for (float radius = rmin; radius <= rmax; radius += rstep) {
+            float sum = 0.0f;
+            for (int j = 0; j < 32; j++) {
+                const float cj = tab.c[j], sj = tab.s[j];
+                const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
+                float s[3];
+                trilinear_tex<F32>(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+                sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
+            }
+            sum /= 32.0f;
+            if (sum > maxSum) {
+                maxSum = sum;
+                rstar = radius;
+            } else {
+                break;
+            }
+        }
+        if (maxSum > 0.0f) {
+            const float cf = -g / 32.0f;
+            float A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0};
+            for (int j = 0; j < 32; j++) {
+                const float dj[3] = {tab.c[j] * e3x + tab.s[j] * e2x, tab.c[j] * e3y + tab.s[j] * e2y,
+                                     tab.c[j] * e3z + tab.s[j] * e2z};
+                float s[3], J[3][3], wt[8];
+                int ci[8], cj[8], ck[8];
+                tri_grad(T, d, fx + rstar * dj[0], fy + rstar * dj[1], fz + rstar * dj[2], s, J, ci, cj, ck, wt);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) tile.add(dV, d, c, ci[k], cj[k], ck[k], cf * dj[c] * wt[k]);
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const float u = s[m] + rstar * (dj[0] * J[0][m] + dj[1] * J[1][m] + dj[2] * J[2][m]);
+                    A3[m] += cf * tab.c[j] * u;
+                    A2[m] += cf * tab.s[j] * u;
+                }
+            }
+            // eigenvector perturbation: de_k = sum_{i != k} e_i (e_i^T dH e_k) / (lam_k - lam_i)
+            float G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const int k = kk + 1;
+                const float* A = kk == 0 ? A2 : A3;
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if (i == k || lam[k] == lam[i]) continue;
+                    const float coef = (A[0] * E[0][i] + A[1] * E[1][i] + A[2] * E[2][i]) / (lam[k] - lam[i]);
+#pragma unroll
+                    for (int a = 0; a < 3; a++)
+#pragma unroll
+                        for (int b = 0; b < 3; b++) G[a][b] += coef * E[a][i] * E[b][k];
+                }
+            }
+            const float gxx = G[0][0], gyx = G[0][1] + G[1][0], gzx = G[0][2] + G[2][0];
+            const float gyy = G[1][1], gzy = G[1][2] + G[2][1], gzz = G[2][2];
+            // H entries are central differences (clamp-to-edge) of the samples
+            // V_c / W (NORM) or V_c
+            auto put = [&](int comp, int xx, int yy, int zz, float cv) {
+                xx = clampi(xx, 0, d.X - 1);
+                yy = clampi(yy, 0, d.Y - 1);
+                zz = clampi(zz, 0, d.Z - 1);
+                if (NORM) {
+                    const float4 tq = T.at(d.at(xx, yy, zz));
+                    const float vq = comp == 0 ? tq.x : (comp == 1 ? tq.y : tq.z);
+                    tile.add(dV, d, comp, xx, yy, zz, cv / tq.w);
+                    unsafeAtomicAdd(dW + d.at(xx, yy, zz), -cv * vq / (tq.w * tq.w));
+                } else {
+                    tile.add(dV, d, comp, xx, yy, zz, cv);
+                }
+            };
+            put(0, x + 1, y, z, 0.5f * gxx);
+            put(0, x - 1, y, z, -0.5f * gxx);
+            put(1, x + 1, y, z, 0.5f * gyx);
+            put(1, x - 1, y, z, -0.5f * gyx);
+            put(1, x, y + 1, z, 0.5f * gyy);
+            put(1, x, y - 1, z, -0.5f * gyy);
+            put(2, x + 1, y, z, 0.5f * gzx);
+            put(2, x - 1, y, z, -0.5f * gzx);
+            put(2, x, y + 1, z, 0.5f * gzy);
+            put(2, x, y - 1, z, -0.5f * gzy);
+            put(2, x, y, z + 1, 0.5f * gzz);
+            put(2, x, y, z - 1, -0.5f * gzz);
+        }
+    }
+    if (halo > 0) {
+        // one global atomic per touched tile cell inside the volume
+        __syncthreads();
+        for (int k = tid; k < tile.cells; k += CBX * CBY * CBZ) {
+            const int lx = k % tile.tx, ly = (k / tile.tx) % tile.ty, lz = k / (tile.tx * tile.ty);
+            const int gx = tile.ox + lx, gy = tile.oy + ly, gz = tile.oz + lz;
+            if (gx < 0 || gy < 0 || gz < 0 || gx >= d.X || gy >= d.Y || gz >= d.Z) continue;
+            const i64 q = d.at(gx, gy, gz);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float v = lds[c * tile.cells + k];
+                if (v != 0.0f) unsafeAtomicAdd(dV[c] + q, v);
+            }
+        }
+    }
 }
 
 // w = |v| where nonzero, else 1 (GVF3DFinish)
@@ -572,9 +715,17 @@ tsf_status tsf_gvf_euler_checkpointed(const tsf_field* f, const tsf_field* out, 
     return TSF_OK;
 }
 
+static size_t stencil_blocks(const Dims& d) { return (size_t)((d.X + 63) / 64) * ((d.Y + 3) / 4) * d.Z; }
+
+// float states first, then the double partial sums at a 256-byte boundary
+static size_t adj_float_bytes(const Dims& d, int k) {
+    return (((size_t)k + 3) * 3 * (size_t)d.n() * sizeof(float) + 255) & ~(size_t)255;
+}
+
 size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every) {
-    const size_t N = (size_t)D(dims).n();
-    return ((size_t)checkpoint_every + 3) * 3 * N * sizeof(float) + (NPART + 1) * sizeof(double);
+    const Dims d = D(dims);
+    if (checkpoint_every < 1) return 0;
+    return adj_float_bytes(d, checkpoint_every) + (NPART + 1 + stencil_blocks(d)) * sizeof(double);
 }
 
 tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
@@ -594,16 +745,20 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
     Scratch sc;
     TSF_TRY(sc.get(workspace, workspace_bytes, tsf_gvf_euler_adjoint_workspace_size(dims, k), W));
     float* seg = (float*)sc.ptr;              // k states of 3 planes
-    float* lamA = seg + (i64)k * 3 * N;       // adjoint ping-pong
+    float* lamA = seg + (i64)k * 3 * N;       // folded adjoint, ping-pong
     float* lamB = lamA + 3 * N;
     float* tmp = lamB + 3 * N;                // ping-pong of the recomputation
-    double* part = (double*)(tmp + 3 * N);
-    double* acc = part + NPART;
+    double* part2 = (double*)((char*)sc.ptr + adj_float_bytes(d, k));  // NPART
+    double* acc = part2 + NPART;
+    double* part = acc + 1;                   // one per stencil block
+    const i64 nblk = (i64)stencil_blocks(d);
     const float* fp[3] = {(const float*)f->x, (const float*)f->y, (const float*)f->z};
-    hipError_t e = hipMemcpyAsync(lamA, dV, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, stream);
-    if (e == hipSuccess) e = hipMemsetAsync(dF, 0, 3 * N * sizeof(float), stream);
+    hipError_t e = hipMemsetAsync(dF, 0, 3 * N * sizeof(float), stream);
     if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
     hipLaunchKernelGGL(k_zero_double, 1, 64, 0, stream, acc, 1);
+    hipLaunchKernelGGL(k_fold3, STENCIL_GRID(d), 0, stream, P3{{dV, dV + N, dV + 2 * N}}, d,
+                       M3{{lamA, lamA + N, lamA + 2 * N}});
+    TSF_TRY(check_launch(W));
     P3 Fp{{fp[0], fp[1], fp[2]}};
     M3 dFp{{dF, dF + N, dF + 2 * N}};
     float* lin = lamA;
@@ -626,17 +781,16 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
             P3 L{{lin, lin + N, lin + 2 * N}};
             P3 Vn{{vn, vn + N, vn + 2 * N}};
             M3 Lo{{lout, lout + N, lout + 2 * N}};
-            hipLaunchKernelGGL(k_gvf_adjoint_step, NPART, RB, 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
-            hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part, NPART, acc);
+            if (n > 0)
+                hipLaunchKernelGGL(k_gvf_adj<true>, STENCIL_GRID(d), 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
+            else  // v^0 = f: the last adjoint flows into dF
+                hipLaunchKernelGGL(k_gvf_adj<false>, STENCIL_GRID(d), 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
+            hipLaunchKernelGGL(k_reduce_parts, NPART, RB, 0, stream, part, nblk, part2);
+            hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part2, NPART, acc);
             TSF_TRY(check_launch(W));
             std::swap(lin, lout);
         }
     }
-    // v^0 = f: the remaining adjoint flows into dF
-    for (int c = 0; c < 3; c++) {
-        hipLaunchKernelGGL(k_add_inplace_f, g1d(N), 256, 0, stream, dF + c * N, lin + c * N, N);
-    }
-    TSF_TRY(check_launch(W));
     double h = 0.0;
     TSF_TRY(sync_read_double(acc, &h, stream, W));
     if (dmu) *dmu = h;
@@ -677,8 +831,11 @@ tsf_status tsf_mse(const float* t, const float* target, int64_t n, float* grad, 
     return TSF_OK;
 }
 
+size_t tsf_tdf_adjoint_workspace_size(tsf_dims dims) { return (size_t)D(dims).n() * sizeof(float4); }
+
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
-                           const float* tdf_grad, const tsf_field* dfield, hipStream_t stream) {
+                           const float* tdf_grad, const tsf_field* dfield, void* workspace, size_t workspace_bytes,
+                           hipStream_t stream) {
     const char* W = "tsf_tdf_adjoint";
     const bool norm = !(rmax < 4.0f);
     TSF_TRY(check_field(field, norm, W));
@@ -692,18 +849,38 @@ tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, fl
     if ((double)(rmax - rmin) / rstep > 100000.0) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: too many radii", W);
     static const CircleTables tab = make_circle_tables();
     Dims d = D(dims);
-    Field<F32> F{(const float*)field->x, (const float*)field->y, (const float*)field->z, (const float*)field->w, 1.0f};
+    const i64 N = d.n();
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_adjoint_workspace_size(dims), W));
+    pack_texels_f32(field, (float4*)sc.ptr, N, stream);
+    Tex<F32> T{(const float4*)sc.ptr, 1.0f};
+    // LDS tile: samples reach ceil(rmax + 0.5) voxels from the centre (the
+    // trilinear corner below u - 0.5), the Hessian one; at most 64 KiB so
+    // two bricks fit a compute unit, else global atomics only
+    int halo = (int)ceilf(fmaxf(fabsf(rmin), fabsf(rmax)) + 0.5f);
+    if (halo < 1) halo = 1;
+    size_t lds = (size_t)3 * (CBX + 2 * halo) * (CBY + 2 * halo) * (CBZ + 2 * halo) * sizeof(float);
+    if (lds > 65536) {
+        halo = 0;
+        lds = 0;
+    }
     float* dx = (float*)dfield->x;
     float* dy = (float*)dfield->y;
     float* dz = (float*)dfield->z;
     float* dw = (float*)dfield->w;
+    dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
     if (norm)
-        hipLaunchKernelGGL(k_tdf_adjoint<true>, STENCIL_GRID(d), 0, stream, F, d, rmin, rmax, rstep, tab, tdf_grad, dx,
-                           dy, dz, dw);
+        hipLaunchKernelGGL(k_tdf_adjoint<true>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad, dx,
+                           dy, dz, dw, halo);
     else
-        hipLaunchKernelGGL(k_tdf_adjoint<false>, STENCIL_GRID(d), 0, stream, F, d, rmin, rmax, rstep, tab, tdf_grad, dx,
-                           dy, dz, dw);
-    return check_launch(W);
+        hipLaunchKernelGGL(k_tdf_adjoint<false>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
+                           dx, dy, dz, dw, halo);
+    TSF_TRY(check_launch(W));
+    if (sc.owned) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    }
+    return TSF_OK;
 }
 
 }  // extern "C"

@@ -63,6 +63,8 @@ tsf_status Scratch::get(void* ws, size_t ws_bytes, size_t need, const char* what
     if (ws) {
         if (ws_bytes < need)
             return set_error(TSF_ERR_OUT_OF_MEMORY, "%s: workspace of %zu bytes, %zu needed", what, ws_bytes, need);
+        if ((uintptr_t)ws % 256 != 0)
+            return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: workspace must be 256-byte aligned", what);
         ptr = ws;
         return TSF_OK;
     }

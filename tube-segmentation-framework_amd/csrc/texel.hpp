@@ -1,0 +1,104 @@
+// Packed field texels and the TDF's sampling primitives, shared by the
+// forward filter (circle.hip) and its adjoint (adjoint.hip).
+#pragma once
+
+#include "tsf_common.hpp"
+
+namespace tsf {
+
+template <class S>
+struct Texel;
+template <>
+struct Texel<F32> {
+    typedef float4 P;
+    static __device__ __forceinline__ float4 unpack(P p, float) { return p; }
+};
+template <>
+struct Texel<F16> {
+    typedef uint2 P;
+    static __device__ __forceinline__ float h(unsigned b) { return __half2float(__ushort_as_half((unsigned short)b)); }
+    static __device__ __forceinline__ float4 unpack(P p, float inv) {
+        return make_float4(h(p.x & 0xffffu) * inv, h(p.x >> 16) * inv, h(p.y & 0xffffu) * inv, h(p.y >> 16) * inv);
+    }
+};
+template <>
+struct Texel<BF16> {
+    typedef uint2 P;
+    static __device__ __forceinline__ float b(unsigned v) { return __uint_as_float(v << 16); }
+    static __device__ __forceinline__ float4 unpack(P p, float inv) {
+        return make_float4(b(p.x & 0xffffu) * inv, b(p.x >> 16) * inv, b(p.y & 0xffffu) * inv, b(p.y >> 16) * inv);
+    }
+};
+
+template <class S>
+struct Tex {
+    const typename Texel<S>::P* __restrict__ p;
+    float inv;
+    __device__ __forceinline__ float4 at(i64 i) const { return Texel<S>::unpack(p[i], inv); }
+};
+
+template <class S, bool NORM>
+__device__ __forceinline__ void hessian_tex(const Tex<S>& T, const Dims& d, int x, int y, int z, float H[3][3]) {
+    const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1);
+    const int ym = max(y - 1, 0), yp = min(y + 1, d.Y - 1);
+    const int zm = max(z - 1, 0), zp = min(z + 1, d.Z - 1);
+    const float4 XP = T.at(d.at(xp, y, z)), XM = T.at(d.at(xm, y, z));
+    const float4 YP = T.at(d.at(x, yp, z)), YM = T.at(d.at(x, ym, z));
+    const float4 ZP = T.at(d.at(x, y, zp)), ZM = T.at(d.at(x, y, zm));
+    float a0 = XP.x, a1 = XM.x;
+    float b0 = XP.y, b1 = XM.y, b2 = YP.y, b3 = YM.y;
+    float c0 = XP.z, c1 = XM.z, c2 = YP.z, c3 = YM.z, c4 = ZP.z, c5 = ZM.z;
+    if (NORM) {
+        a0 = a0 / XP.w; a1 = a1 / XM.w;
+        b0 = b0 / XP.w; b1 = b1 / XM.w; b2 = b2 / YP.w; b3 = b3 / YM.w;
+        c0 = c0 / XP.w; c1 = c1 / XM.w; c2 = c2 / YP.w; c3 = c3 / YM.w; c4 = c4 / ZP.w; c5 = c5 / ZM.w;
+    }
+    const float fxx = 0.5f * (a0 - a1);
+    const float fyx = 0.5f * (b0 - b1), fyy = 0.5f * (b2 - b3);
+    const float fzx = 0.5f * (c0 - c1), fzy = 0.5f * (c2 - c3), fzz = 0.5f * (c4 - c5);
+    H[0][0] = fxx; H[0][1] = fyx; H[0][2] = fzx;
+    H[1][0] = fyx; H[1][1] = fyy; H[1][2] = fzy;
+    H[2][0] = fzx; H[2][1] = fzy; H[2][2] = fzz;
+}
+
+// OpenCL CLK_FILTER_LINEAR, unnormalised coordinates, clamp-to-edge: sample
+// at u - 0.5, the eight terms in the OpenCL specification's order.
+template <class S>
+__device__ __forceinline__ void trilinear_tex(const Tex<S>& T, const Dims& d, float u, float v, float w,
+                                              float out[3]) {
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    const int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1);
+    j0 = clampi(j0, 0, d.Y - 1);
+    k0 = clampi(k0, 0, d.Z - 1);
+    const i64 r00 = (i64)d.X * ((i64)j0 + (i64)d.Y * k0), r10 = (i64)d.X * ((i64)j1 + (i64)d.Y * k0);
+    const i64 r01 = (i64)d.X * ((i64)j0 + (i64)d.Y * k1), r11 = (i64)d.X * ((i64)j1 + (i64)d.Y * k1);
+    const float4 t0 = T.at(r00 + i0), t1 = T.at(r00 + i1), t2 = T.at(r10 + i0), t3 = T.at(r10 + i1);
+    const float4 t4 = T.at(r01 + i0), t5 = T.at(r01 + i1), t6 = T.at(r11 + i0), t7 = T.at(r11 + i1);
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    float sx = wt0 * t0.x, sy = wt0 * t0.y, sz = wt0 * t0.z;
+    sx = sx + wt1 * t1.x; sy = sy + wt1 * t1.y; sz = sz + wt1 * t1.z;
+    sx = sx + wt2 * t2.x; sy = sy + wt2 * t2.y; sz = sz + wt2 * t2.z;
+    sx = sx + wt3 * t3.x; sy = sy + wt3 * t3.y; sz = sz + wt3 * t3.z;
+    sx = sx + wt4 * t4.x; sy = sy + wt4 * t4.y; sz = sz + wt4 * t4.z;
+    sx = sx + wt5 * t5.x; sy = sy + wt5 * t5.y; sz = sz + wt5 * t5.z;
+    sx = sx + wt6 * t6.x; sy = sy + wt6 * t6.y; sz = sz + wt6 * t6.z;
+    sx = sx + wt7 * t7.x; sy = sy + wt7 * t7.y; sz = sz + wt7 * t7.z;
+    out[0] = sx;
+    out[1] = sy;
+    out[2] = sz;
+}
+
+constexpr int CBX = 16, CBY = 4, CBZ = 4;  // TDF brick
+
+// Packs the field's planes into float4 texels (x, y, z, w; w = 1 when the
+// field has no w plane).  Defined in circle.hip.
+void pack_texels_f32(const tsf_field* field, float4* out, i64 n, hipStream_t stream);
+
+}  // namespace tsf

@@ -24,6 +24,7 @@ static size_t backward_workspace(int3 size, int k) {
     size_t w = tsf_gvf_euler_adjoint_workspace_size(d, k);
     w = std::max(w, tsf_vector_field_adjoint_workspace_size(d));
     w = std::max(w, tsf_blur_adjoint_workspace_size());
+    w = std::max(w, tsf_tdf_adjoint_workspace_size(d));
     return w;
 }
 
@@ -87,7 +88,7 @@ TubeFilterGrads TubeFilterFunction::backward(AutogradContext* ctx, OpenCL& ocl,
     const tsf_field dv = planes_field(dV.get(), N, true);
     const tsf_field v = V.field();
     check(tsf_tdf_adjoint(&v, d, o.radiusMin, o.radiusMax, o.radiusStep, (const float*)grad_outputs[0].plane(0), &dv,
-                          ocl.queue));
+                          ws, wsb, ocl.queue));
     check(tsf_gvf_finish_adjoint(&v, &dv, d, ocl.queue));
     Buffer dF(3 * N * sizeof(float));
     const tsf_field f = F.field();

@@ -79,7 +79,8 @@ _SIGS = {
     "tsf_gvf_finish_adjoint": ([_PF, _PF, Dims, _vp], _i32),
     "tsf_mse": ([_vp, _vp, ctypes.c_int64, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp], _i32),
     "tsf_mse_workspace_size": ([], ctypes.c_size_t),
-    "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _PF, _vp], _i32),
+    "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _PF, _vp, ctypes.c_size_t, _vp], _i32),
+    "tsf_tdf_adjoint_workspace_size": ([Dims], ctypes.c_size_t),
 }
 
 
@@ -448,10 +449,15 @@ def gvf_finish_adjoint(v, dv, stream=None):
     _check(lib().tsf_gvf_finish_adjoint(ctypes.byref(a), ctypes.byref(b), _dims(v.shape), _sp(stream)))
 
 
-def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None):
+def tdf_adjoint_workspace_size(shape):
+    return lib().tsf_tdf_adjoint_workspace_size(_dims(shape))
+
+
+def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None, workspace=None):
     a, b = field.desc(), dfield.desc()
+    w, wb = _ws(workspace)
     _check(lib().tsf_tdf_adjoint(ctypes.byref(a), _dims(field.shape), rmin, rmax, rstep, gT.ptr, ctypes.byref(b),
-                                 _sp(stream)))
+                                 w, wb, _sp(stream)))
 
 
 def mse(t, target, grad, workspace=None, stream=None):

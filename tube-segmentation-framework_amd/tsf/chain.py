@@ -50,10 +50,11 @@ class TubeFilterFunction:
         dV = C.DeviceArray((4,) + shape, np.float32)
         dV.zero(stream)
         dVf = C.Field.over(dV)
-        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dVf, stream)
+        ws = C.DeviceArray((max(C.gvf_euler_adjoint_workspace_size(shape, s["k"]),
+                                C.tdf_adjoint_workspace_size(shape)),), np.uint8)
+        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dVf, stream, ws)
         C.gvf_finish_adjoint(s["V"], dVf, stream)
         dF = C.DeviceArray((3,) + shape, np.float32)
-        ws = C.DeviceArray((C.gvf_euler_adjoint_workspace_size(shape, s["k"]),), np.uint8)
         dmu = C.gvf_euler_adjoint(s["F"], s["mu"], s["iterations"], s["k"], s["cps"], dV, dF, ws, stream)
         dB = C.DeviceArray(shape, np.float32)
         C.vector_field_adjoint(s["B"], C.Field.over(dF), s["fmax"], s["sign"], dB, ws, stream)
