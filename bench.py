@@ -51,7 +51,7 @@ def main():
     ap.add_argument("--iters", type=int, default=400)
     ap.add_argument("--mu", type=float, default=0.05)
     ap.add_argument("--storage", choices=["f16", "bf16", "f32"], default="f16")
-    ap.add_argument("--fuse", type=int, default=1)
+    ap.add_argument("--fuse", type=int, default=2, help="GVF steps per launch (1 or 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -149,7 +149,8 @@ def main():
     gvf_ms = float(np.median(gvf))
     fwd_ms = float(np.median(fwd))
     gvox = N * args.iters / (gvf_ms * 1e-3) / 1e9
-    launches = max(1, -(-args.iters // args.fuse))
+    pairs = args.iters // 2 if args.fuse >= 2 else 0  # launches of two steps (an odd count adds one single)
+    launches = max(1, pairs + (args.iters - 2 * pairs))
     bytes_per_voxel = 9 * (4 if S == tsf.STORAGE_F32 else 2)  # read v, f; write v (3 components each)
     launch_ms = gvf_ms / launches
     achieved = bytes_per_voxel * N / (launch_ms * 1e-3)

@@ -264,6 +264,364 @@ __global__ void __launch_bounds__(64 * GTY) k_gvf_march(Planes3<S> f, Planes3<S>
     }
 }
 
+// ---------------------------------------------------------------- two steps per launch
+// k_gvf_fused2 runs two Jacobi steps per launch, so v and f cross HBM once
+// per two iterations.  A workgroup of FBX x FBY threads (each VEC adjacent
+// columns) owns an extended column tile E of (FBX*VEC) x FBY columns; it
+// computes v1 = step(v0) on E and v2 = step(v1) on the inner tile O (E less
+// VEC columns at each x edge and one row at each y edge), marching z.  Per
+// step t (one barrier): LDS <- v0 plane t (E plus a one-column ring) and v1
+// plane t-1; v1(t) from LDS x/y neighbours and the thread's register ring
+// of v0 planes; v2(t-1) from LDS v1 plane t-1 and the register ring of v1
+// planes t-2..t.  Loads run three planes ahead into rings of raw stored
+// values that are converted at the top of the step that uses them.  v1 is
+// kept in float.
+// Mirror rule: v1 at a boundary column / plane equals v1 at its mirror
+// (0 -> 2, N-1 -> N-3), so v1 reads there are redirected; v2 computed at q
+// is also written to every w with mirror(w) = q.  Tiles are placed so a
+// boundary column and its mirror lie in the same O (the last tile along x and
+// y is shifted to end at the volume's edge).
+constexpr int FBX = 64, FBY = 8;
+constexpr int FZC = 64;  // output planes per z chunk, a multiple of 4
+
+// raw stored values of VEC consecutive elements; get() converts element k
+template <class S, int VEC>
+struct RawV;
+template <>
+struct RawV<F32, 1> {
+    typedef float T;
+    static __device__ __forceinline__ T load(const float* p, i64 i) { return p[i]; }
+    static __device__ __forceinline__ float get(T r, int, float) { return r; }
+};
+template <>
+struct RawV<F32, 2> {
+    typedef float2 T;
+    static __device__ __forceinline__ T load(const float* p, i64 i) { return *reinterpret_cast<const float2*>(p + i); }
+    static __device__ __forceinline__ float get(T r, int k, float) { return k ? r.y : r.x; }
+};
+template <>
+struct RawV<F16, 1> {
+    typedef unsigned short T;
+    static __device__ __forceinline__ T load(const __half* p, i64 i) {
+        return reinterpret_cast<const unsigned short*>(p)[i];
+    }
+    static __device__ __forceinline__ float get(T r, int, float inv) { return __half2float(__ushort_as_half(r)) * inv; }
+};
+template <>
+struct RawV<F16, 2> {
+    typedef unsigned T;
+    static __device__ __forceinline__ T load(const __half* p, i64 i) { return *reinterpret_cast<const unsigned*>(p + i); }
+    static __device__ __forceinline__ float get(T r, int k, float inv) {
+        return __half2float(__ushort_as_half((unsigned short)(k ? (r >> 16) : (r & 0xffffu)))) * inv;
+    }
+};
+template <>
+struct RawV<BF16, 1> {
+    typedef unsigned short T;
+    static __device__ __forceinline__ T load(const __hip_bfloat16* p, i64 i) {
+        return reinterpret_cast<const unsigned short*>(p)[i];
+    }
+    static __device__ __forceinline__ float get(T r, int, float inv) { return __uint_as_float((unsigned)r << 16) * inv; }
+};
+template <>
+struct RawV<BF16, 2> {
+    typedef unsigned T;
+    static __device__ __forceinline__ T load(const __hip_bfloat16* p, i64 i) {
+        return *reinterpret_cast<const unsigned*>(p + i);
+    }
+    static __device__ __forceinline__ float get(T r, int k, float inv) {
+        return __uint_as_float(k ? (r & 0xffff0000u) : (r << 16)) * inv;
+    }
+};
+
+template <int VEC>
+struct FGeo {
+    static constexpr int EW = FBX * VEC;    // E width (columns)
+    static constexpr int RX = VEC;          // E columns beyond O at each x edge (keeps pairs aligned)
+    static constexpr int OW = EW - 2 * RX;  // O width
+    static constexpr int OH = FBY - 2;      // O height
+    static constexpr int LW = EW + 2 * VEC; // LDS row of v0: own columns at VEC + x, ring at VEC-1 and VEC+EW
+};
+
+template <class S, int VEC>
+struct Fused2 {
+    Planes3<S> f, vin;
+    Out4<S> vout;
+    Dims d;
+    float mu, inv_f, inv_v, sc_v;
+    int lane, ty, ex, ey, cy, qs, qe;
+    int cx[VEC];
+    bool act, icol[VEC], inO, hrow, hcol, hok, hsel;
+    i64 P, col, hidx;
+    int lxp[VEC], lxm[VEC], lyp, lym;  // LDS coordinates of v1 neighbours (mirror-redirected)
+    int xw1[VEC], xw2[VEC], yw1, yw2;  // mirror images that receive this column's v2 (-1 = none)
+    int hly, hlx;
+};
+
+__device__ __forceinline__ int mirror_of(int p, int N) { return p == 0 ? 2 : (p == N - 1 ? N - 3 : p); }
+
+template <class S, int VEC, bool FINISH>
+__device__ __forceinline__ void f2_store(const Fused2<S, VEC>& m, int p, const float (&r)[VEC][3],
+                                         const bool (&ok)[VEC]) {
+    float wv[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; k++) {
+        wv[k] = 1.0f;
+        if (FINISH) {
+            // magnitude of the stored (rounded) components
+            typename S::T t0, t1, t2;
+            S::st(&t0, 0, r[k][0], m.sc_v);
+            S::st(&t1, 0, r[k][1], m.sc_v);
+            S::st(&t2, 0, r[k][2], m.sc_v);
+            const float ax = S::ld(&t0, 0, m.inv_v), ay = S::ld(&t1, 0, m.inv_v), az = S::ld(&t2, 0, m.inv_v);
+            const float l = sqrtf(ax * ax + ay * ay + az * az);
+            wv[k] = l > 0.0f ? l : 1.0f;
+        }
+    }
+    typename S::T* OP[3] = {m.vout.x, m.vout.y, m.vout.z};
+    // this position and its mirror images (-1 = none); fully unrolled with
+    // constant indices so the candidates stay in registers
+    const int zw[3] = {p, p == 2 ? 0 : -1, p == m.d.Z - 3 ? m.d.Z - 1 : -1};
+    const int yw[3] = {m.cy, m.yw1, m.yw2};
+#pragma unroll
+    for (int k = 0; k < VEC; k++) {
+        if (!ok[k]) continue;
+        const int xw[3] = {m.cx[k], m.xw1[k], m.xw2[k]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (zw[a] < 0) continue;
+#pragma unroll
+            for (int bb = 0; bb < 3; bb++) {
+                if (yw[bb] < 0) continue;
+#pragma unroll
+                for (int cc = 0; cc < 3; cc++) {
+                    if (xw[cc] < 0) continue;
+                    const i64 i = zw[a] * m.P + (i64)yw[bb] * m.d.X + xw[cc];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) S::st(OP[c], i, r[k][c], m.sc_v);
+                    if (FINISH) S::st(m.vout.w, i, wv[k], m.sc_v);
+                }
+            }
+        }
+    }
+}
+
+template <class S, int VEC, bool FINISH, int T4>
+__device__ __forceinline__ void f2_step(const Fused2<S, VEC>& m, int t, typename RawV<S, VEC>::T (&v0)[4][3],
+                                        float (&v1)[4][VEC][3], typename RawV<S, VEC>::T (&fr)[4][3],
+                                        typename RawV<S, VEC>::T (&hr)[2][3],
+                                        float (&L0)[2][3][FBY + 2][FGeo<VEC>::LW],
+                                        float (&L1)[2][3][FBY][FGeo<VEC>::EW]) {
+    typedef RawV<S, VEC> R;
+    constexpr int SM1 = (T4 + 3) & 3, S0 = T4, SP1 = (T4 + 1) & 3, SM2 = (T4 + 2) & 3;
+    constexpr int B = T4 & 1, BP = (T4 + 1) & 1;
+    const int lane = m.lane, ty = m.ty;
+    const float iv = m.inv_v, ifl = m.inv_f;
+    // every conversion in straight-line code, before any branch, so the
+    // waits on the loads issued two to four planes ago stay exact
+    float c0[VEC][3], cp[VEC][3], cm[VEC][3], f1[VEC][3], f2[VEC][3], ch[VEC][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            c0[k][c] = R::get(v0[S0][c], k, iv);
+            cp[k][c] = R::get(v0[SP1][c], k, iv);
+            cm[k][c] = R::get(v0[SM1][c], k, iv);
+            f1[k][c] = R::get(fr[S0][c], k, ifl);
+            f2[k][c] = R::get(fr[SM1][c], k, ifl);
+            ch[k][c] = R::get(hr[B][c], k, iv);
+            // left alone, the compiler sinks these into the branches that
+            // use them, where the wait becomes a full drain
+            asm volatile("" : "+v"(ch[k][c]), "+v"(f2[k][c]));
+        }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            L0[B][c][ty + 1][VEC + lane * VEC + k] = c0[k][c];
+            L1[BP][c][ty][lane * VEC + k] = v1[SM1][k][c];
+        }
+    if (m.hrow) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int k = 0; k < VEC; k++) L0[B][c][m.hly][VEC + lane * VEC + k] = ch[k][c];
+    } else if (m.hcol) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) L0[B][c][m.hly][m.hlx] = m.hsel ? ch[VEC - 1][c] : ch[0][c];
+    }
+    __syncthreads();
+    // v1(t) at this thread's columns
+    const bool tin = t >= 1 && t <= m.d.Z - 2;
+#pragma unroll
+    for (int k = 0; k < VEC; k++) {
+        const int lx = VEC + lane * VEC + k;
+        const float m2 = f1[k][0] * f1[k][0] + f1[k][1] * f1[k][1] + f1[k][2] * f1[k][2];
+        const bool do1 = m.act && m.icol[k] && tin;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float r = gvf_update(c0[k][c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1],
+                                       L0[B][c][ty + 2][lx], L0[B][c][ty][lx], cp[k][c], cm[k][c], f1[k][c], m2, m.mu);
+            v1[S0][k][c] = do1 ? r : 0.0f;
+        }
+    }
+    // v2(t - 1) at this thread's columns
+    const int p = t - 1;
+    if (m.inO && p >= m.qs && p < m.qe) {
+        const bool lo = (p - 1) == 0, hi = t == m.d.Z - 1;  // plane 0 / Z-1 alias plane 2 / Z-3
+        float r[VEC][3];
+        bool ok[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; k++) {
+            ok[k] = m.icol[k];
+            const float m2 = f2[k][0] * f2[k][0] + f2[k][1] * f2[k][1] + f2[k][2] * f2[k][2];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float zm = lo ? v1[S0][k][c] : v1[SM2][k][c];
+                const float zp = hi ? v1[SM2][k][c] : v1[S0][k][c];
+                r[k][c] = gvf_update(v1[SM1][k][c], L1[BP][c][ty][m.lxp[k]], L1[BP][c][ty][m.lxm[k]],
+                                     L1[BP][c][m.lyp][lane * VEC + k], L1[BP][c][m.lym][lane * VEC + k], zp, zm,
+                                     f2[k][c], m2, m.mu);
+            }
+        }
+        f2_store<S, VEC, FINISH>(m, p, r, ok);
+    }
+    // loads: v0 and f of plane t+3 into the slots of t-1, the ring cells of
+    // plane t+2 into this step's halo slot
+    const int z3 = min(t + 3, m.d.Z - 1), z2 = min(t + 2, m.d.Z - 1);
+    const i64 iv3 = m.act ? z3 * m.P + m.col : 0, ih2 = m.hok ? z2 * m.P + m.hidx : 0;
+    const typename S::T* VP[3] = {m.vin.x, m.vin.y, m.vin.z};
+    const typename S::T* FP[3] = {m.f.x, m.f.y, m.f.z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v0[SM1][c] = R::load(VP[c], iv3);
+        fr[SM1][c] = R::load(FP[c], iv3);
+        hr[B][c] = R::load(VP[c], ih2);
+    }
+}
+
+template <class S, int VEC, bool FINISH>
+__global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4) k_gvf_fused2(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d,
+                                                            float mu, float inv_f, float inv_v, float sc_v) {
+    typedef FGeo<VEC> G;
+    typedef RawV<S, VEC> R;
+    __shared__ float L0[2][3][FBY + 2][G::LW];
+    __shared__ float L1[2][3][FBY][G::EW];
+    Fused2<S, VEC> m;
+    m.f = f;
+    m.vin = vin;
+    m.vout = vout;
+    m.d = d;
+    m.mu = mu;
+    m.inv_f = inv_f;
+    m.inv_v = inv_v;
+    m.sc_v = sc_v;
+    m.lane = threadIdx.x;
+    m.ty = threadIdx.y;
+    const int ox = tile_origin(blockIdx.x, G::OW, d.X), oy = tile_origin(blockIdx.y, G::OH, d.Y);
+    m.ex = ox - G::RX;
+    m.ey = oy - 1;
+    m.cy = m.ey + m.ty;
+    const int x0 = m.ex + m.lane * VEC;  // even when VEC == 2: ox, X and OW are even
+#pragma unroll
+    for (int k = 0; k < VEC; k++) m.cx[k] = x0 + k;
+    // VEC == 2 runs only with X even, so a pair is wholly inside or outside
+    m.act = x0 >= 0 && x0 < d.X && m.cy >= 0 && m.cy < d.Y;
+    const bool yint = m.cy >= 1 && m.cy <= d.Y - 2;
+#pragma unroll
+    for (int k = 0; k < VEC; k++) m.icol[k] = m.act && yint && m.cx[k] >= 1 && m.cx[k] <= d.X - 2;
+    m.inO = m.act && m.lane * VEC >= G::RX && m.lane * VEC < G::RX + G::OW && m.ty >= 1 && m.ty <= G::OH;
+    m.P = (i64)d.X * d.Y;
+    m.col = m.act ? (i64)m.cy * d.X + x0 : 0;
+    // v1 neighbours of an O column, redirected at the volume's boundary
+    // columns (their mirror lies inside E for every O column that needs it)
+#pragma unroll
+    for (int k = 0; k < VEC; k++) {
+        m.lxp[k] = clampi(mirror_of(m.cx[k] + 1, d.X) - m.ex, 0, G::EW - 1);
+        m.lxm[k] = clampi(mirror_of(m.cx[k] - 1, d.X) - m.ex, 0, G::EW - 1);
+        m.xw1[k] = m.cx[k] == 2 ? 0 : -1;
+        m.xw2[k] = m.cx[k] == d.X - 3 ? d.X - 1 : -1;
+    }
+    m.lyp = clampi(mirror_of(m.cy + 1, d.Y) - m.ey, 0, FBY - 1);
+    m.lym = clampi(mirror_of(m.cy - 1, d.Y) - m.ey, 0, FBY - 1);
+    m.yw1 = m.cy == 2 ? 0 : -1;
+    m.yw2 = m.cy == d.Y - 3 ? d.Y - 1 : -1;
+    // ring of E: rows ey-1 (ty 0) and ey+FBY (ty 1) at this thread's
+    // columns; columns ex-1 and ex+EW (ty 2, lanes 0..2*FBY-1), read as the
+    // aligned VEC-group holding the cell
+    m.hrow = m.ty == 0 || m.ty == 1;
+    m.hcol = m.ty == 2 && m.lane < 2 * FBY;
+    m.hsel = false;
+    m.hly = 0;
+    m.hlx = 0;
+    int hx = 0, hy = 0;
+    bool hin = false;
+    if (m.hrow) {
+        hy = m.ty == 0 ? m.ey - 1 : m.ey + FBY;
+        hx = x0;
+        m.hly = m.ty == 0 ? 0 : FBY + 1;
+        hin = x0 >= 0 && x0 < d.X && hy >= 0 && hy < d.Y;
+    } else if (m.hcol) {
+        const bool left = m.lane < FBY;
+        const int cx = left ? m.ex - 1 : m.ex + G::EW;
+        hy = m.ey + (left ? m.lane : m.lane - FBY);
+        hx = VEC == 2 ? (cx & ~1) : cx;
+        m.hsel = VEC == 2 && (cx & 1);
+        m.hly = (left ? m.lane : m.lane - FBY) + 1;
+        m.hlx = left ? VEC - 1 : VEC + G::EW;
+        hin = cx >= 0 && cx < d.X && hy >= 0 && hy < d.Y;
+    }
+    m.hok = hin;
+    m.hidx = hin ? (i64)hy * d.X + hx : 0;
+    m.qs = 1 + blockIdx.z * FZC;
+    m.qe = min(m.qs + FZC, d.Z - 1);
+    const int t0 = m.qs - 1, te = m.qe;
+
+    typename R::T v0[4][3], fr[4][3], hr[2][3];
+    float v1[4][VEC][3];
+    const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
+    const typename S::T* FP[3] = {f.x, f.y, f.z};
+    // the loads of the four steps before t0, in the order the steps issue
+    // them (plane t0-1, t0, t0+1 with the ring cells of t0, t0+2 with those
+    // of t0+1), so the loop is entered with the steady state's load order
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const int z = min(max(t0 - 1 + g, 0), d.Z - 1), k = (g + 3) & 3;
+        const i64 i = m.act ? z * m.P + m.col : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v0[k][c] = R::load(VP[c], i);
+            fr[k][c] = R::load(FP[c], i);
+#pragma unroll
+            for (int q = 0; q < VEC; q++) v1[g][q][c] = 0.0f;
+            if (g >= 2) hr[g & 1][c] = R::load(VP[c], m.hok ? min(t0 + g - 2, d.Z - 1) * m.P + m.hidx : 0);
+        }
+    }
+    // four unconditional steps per iteration (up to three past te, which
+    // store nothing: their v2 plane is >= qe and their loads are clamped) so
+    // the loop body has no merge points that would blur the load counters
+    for (int t = t0; t <= te; t += 4) {
+        f2_step<S, VEC, FINISH, 0>(m, t, v0, v1, fr, hr, L0, L1);
+        f2_step<S, VEC, FINISH, 1>(m, t + 1, v0, v1, fr, hr, L0, L1);
+        f2_step<S, VEC, FINISH, 2>(m, t + 2, v0, v1, fr, hr, L0, L1);
+        f2_step<S, VEC, FINISH, 3>(m, t + 3, v0, v1, fr, hr, L0, L1);
+    }
+}
+
+template <class S, int VEC>
+static void launch_fused2(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, Dims d, float mu, float inv_f,
+                          float inv_src, float sc_v, hipStream_t stream) {
+    typedef FGeo<VEC> G;
+    dim3 grid((d.X + G::OW - 1) / G::OW, (d.Y + G::OH - 1) / G::OH, (d.Z - 2 + FZC - 1) / FZC);
+    dim3 block(FBX, FBY);
+    if (finish)
+        hipLaunchKernelGGL((k_gvf_fused2<S, VEC, true>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src,
+                           sc_v);
+    else
+        hipLaunchKernelGGL((k_gvf_fused2<S, VEC, false>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src,
+                           sc_v);
+}
+
 // iterations == 0: v = f, w = |f| or 1.
 template <class S>
 __global__ void k_gvf_copy_finish(Planes3<S> f, Out4<S> vout, i64 n, float inv_f, float inv_v, float sc_v) {
@@ -292,7 +650,7 @@ static void launch_step(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, 
 static bool aligned(const void* p, size_t a) { return ((uintptr_t)p % a) == 0; }
 
 template <class S>
-static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, float mu, int iters, void* ws,
+static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, float mu, int iters, int fuse, void* ws,
                             hipStream_t stream) {
     typedef typename S::T T;
     const i64 N = d.n();
@@ -312,15 +670,24 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
     bool vec2 = (d.X % 2) == 0;
     const void* ptrs[8] = {f->x, f->y, f->z, out->x, out->y, out->z, out->w, A};
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, al));
-    for (int i = 1; i <= iters; i++) {
-        // ping-pong so that the last step lands in `out`
-        const bool to_out = ((iters - i) % 2) == 0;
-        Planes3<S> src = (i == 1) ? F : (to_out ? Wr : Or);
-        const float inv_src = (i == 1) ? inv_f : inv_v;
+    // launches of one or two steps: with fuse >= 2 an odd count starts with
+    // a single step, then pairs
+    const int pairs = fuse >= 2 ? iters / 2 : 0;
+    const int singles = iters - 2 * pairs;
+    const int launches = pairs + singles;
+    for (int j = 0; j < launches; j++) {
+        // ping-pong so that the last launch lands in `out`
+        const bool to_out = ((launches - 1 - j) % 2) == 0;
+        Planes3<S> src = (j == 0) ? F : (to_out ? Wr : Or);
+        const float inv_src = (j == 0) ? inv_f : inv_v;
         Out4<S> dst = to_out ? O : Wk;
-        const bool fin = to_out && i == iters;
-        if (vec2) launch_step<S, 2>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
-        else launch_step<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+        const bool fin = j == launches - 1;
+        if (j < singles) {
+            if (vec2) launch_step<S, 2>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+            else launch_step<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+        } else {
+            launch_fused2<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+        }
         TSF_TRY(check_launch("tsf_gvf_euler"));
     }
     return TSF_OK;
@@ -381,9 +748,9 @@ tsf_status tsf_gvf_euler(const tsf_field* f, const tsf_field* out, tsf_dims dims
     if (iterations > 1) TSF_TRY(sc.get(workspace, workspace_bytes, tsf_gvf_euler_workspace_size(dims, out->storage), W));
     tsf_status st;
     switch (out->storage) {
-        case TSF_STORAGE_F32: st = run_euler<F32>(f, out, d, mu, iterations, sc.ptr, stream); break;
-        case TSF_STORAGE_F16: st = run_euler<F16>(f, out, d, mu, iterations, sc.ptr, stream); break;
-        default: st = run_euler<BF16>(f, out, d, mu, iterations, sc.ptr, stream); break;
+        case TSF_STORAGE_F32: st = run_euler<F32>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
+        case TSF_STORAGE_F16: st = run_euler<F16>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
+        default: st = run_euler<BF16>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
     }
     if (st == TSF_OK && sc.owned) {
         hipError_t e = hipStreamSynchronize(stream);
