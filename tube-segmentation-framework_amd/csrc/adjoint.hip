@@ -358,7 +358,7 @@ __device__ __forceinline__ void tri_grad(const Tex<F32>& T, const Dims& d, float
     float4 t[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        t[k] = T.at(d.at(I[k], Jj[k], K[k]));
+        t[k] = T.at(T.row(d, Jj[k], K[k]), I[k]);
         ci[k] = I[k];
         cj[k] = Jj[k];
         ck[k] = K[k];
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
     const float g = (x < d.X && y < d.Y && z < d.Z) ? gT[d.at(x, y, z)] : 0.0f;
     if (g != 0.0f) {
         float H[3][3], E[3][3], lam[3];
-        hessian_tex<F32, NORM>(T, d, x, y, z, H);
+        hessian_tex<NORM>(T, d, x, y, z, H);
         eigen3(H, E, lam);
         const float e2x = E[0][1], e2y = E[1][1], e2z = E[2][1];
         const float e3x = E[0][2], e3y = E[1][2], e3z = E[2][2];
@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
                 const float cj = tab.c[j], sj = tab.s[j];
                 const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
                 float s[3];
-                trilinear_tex<F32>(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+                trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
                 sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
             }
             sum /= 32.0f;
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
                 yy = clampi(yy, 0, d.Y - 1);
                 zz = clampi(zz, 0, d.Z - 1);
                 if (NORM) {
-                    const float4 tq = T.at(d.at(xx, yy, zz));
+                    const float4 tq = T.at(T.row(d, yy, zz), xx);
                     const float vq = comp == 0 ? tq.x : (comp == 1 ? tq.y : tq.z);
                     tile.add(dV, d, comp, xx, yy, zz, cv / tq.w);
                     unsafeAtomicAdd(dW + d.at(xx, yy, zz), -cv * vq / (tq.w * tq.w));

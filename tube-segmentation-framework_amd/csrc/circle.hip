@@ -32,20 +32,19 @@ __global__ void k_pack_16(const uint16_t* __restrict__ x, const uint16_t* __rest
 
 
 
-// circleFittingTDF, kernels_no_3d_write.cl:1503-1572.
-template <class S, bool NORM>
-__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float rmin, float rmax, float rstep,
-                                                       CircleTables tab, float* __restrict__ tdf,
-                                                       float* __restrict__ rad) {
-    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
-    const int z = blockIdx.z * CBZ + threadIdx.z;
-    if (x >= d.X || y >= d.Y || z >= d.Z) return;
+// circleFittingTDF, kernels_no_3d_write.cl:1503-1572, at one voxel: the
+// (normalised) Hessian, its eigenvectors e2, e3, and the radius search with
+// float accumulation and a first-maximum break.
+template <bool NORM, class A>
+__device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int y, int z, float rmin, float rmax,
+                                          float rstep, const CircleTables& tab, float& maxSum, float& maxRadius) {
     float H[3][3], V[3][3], ev[3];
-    hessian_tex<S, NORM>(T, d, x, y, z, H);
+    hessian_tex<NORM>(T, d, x, y, z, H);
     eigen3(H, V, ev);
     const float e2x = V[0][1], e2y = V[1][1], e2z = V[2][1];
     const float e3x = V[0][2], e3y = V[1][2], e3z = V[2][2];
-    float maxSum = 0.0f, maxRadius = 0.0f;
+    maxSum = 0.0f;
+    maxRadius = 0.0f;
     const float fx = (float)x, fy = (float)y, fz = (float)z;
     for (float radius = rmin; radius <= rmax; radius += rstep) {
         float sum = 0.0f;
@@ -53,7 +52,7 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float r
             const float cj = tab.c[j], sj = tab.s[j];
             const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
             float s[3];
-            trilinear_tex<S>(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+            trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
             sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
         }
         sum /= 32.0f;
@@ -64,9 +63,52 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float r
             break;
         }
     }
+}
+
+// Texels gathered from device memory (halos too wide for the LDS tile).
+template <class S, bool NORM>
+__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float rmin, float rmax, float rstep,
+                                                       CircleTables tab, float* __restrict__ tdf,
+                                                       float* __restrict__ rad) {
+    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
+    const int z = blockIdx.z * CBZ + threadIdx.z;
+    if (x >= d.X || y >= d.Y || z >= d.Z) return;
+    float t, r;
+    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r);
     const i64 i = d.at(x, y, z);
-    tdf[i] = maxSum;
-    rad[i] = maxRadius;
+    tdf[i] = t;
+    rad[i] = r;
+}
+
+// The brick's footprint -- the brick plus a halo of h voxels, clamp-to-edge
+// -- staged in LDS once; the Hessian and every trilinear tap of the radius
+// search then read LDS.  h covers the Hessian's +-1 and the farthest sample
+// corner, floor(rmax + 0.5) + 1.
+template <class S, bool NORM, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>::P* __restrict__ tex, float inv,
+                                                        Dims d, float rmin, float rmax, float rstep, CircleTables tab,
+                                                        float* __restrict__ tdf, float* __restrict__ rad, int h) {
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    const int bx0 = blockIdx.x * BX, by0 = blockIdx.y * BY, bz0 = blockIdx.z * BZ;
+    LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    const int cells = T.tx * T.ty * T.tz;
+    for (int c = tid; c < cells; c += BX * BY * BZ) {
+        const int lx = c % T.tx, rr = c / T.tx, ly = rr % T.ty, lz = rr / T.ty;
+        const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
+        const int gz = clampi(T.oz + lz, 0, d.Z - 1);
+        tile[c] = tex[d.at(gx, gy, gz)];
+    }
+    __syncthreads();
+    const int x = bx0 + threadIdx.x, y = by0 + threadIdx.y, z = bz0 + threadIdx.z;
+    if (x >= d.X || y >= d.Y || z >= d.Z) return;
+    float t, r;
+    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r);
+    const i64 i = d.at(x, y, z);
+    tdf[i] = t;
+    rad[i] = r;
 }
 
 static unsigned grid1d(i64 n) {
@@ -79,11 +121,29 @@ void pack_texels_f32(const tsf_field* field, float4* out, i64 n, hipStream_t str
                        (const float*)field->z, (const float*)field->w, out, n);
 }
 
+// LDS tile of the brick plus halo h: bricks of 16x4x4 voxels for 8-byte
+// texels, 8x4x4 for 16-byte ones (F32), up to 64 KiB; wider halos gather
+// from device memory.
+static int tdf_halo(float rmin, float rmax) {
+    const float r = fmaxf(fabsf(rmin), fabsf(rmax));
+    return (int)floorf(r + 0.5f) + 1;
+}
+
 template <class S, bool NORM>
 static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rmax, float rstep, float* tdf,
                        float* radius, hipStream_t stream) {
     static const CircleTables tab = make_circle_tables();
-    Tex<S> T{(const typename Texel<S>::P*)tex, inv};
+    typedef typename Texel<S>::P P;
+    constexpr int BX = sizeof(P) == 16 ? 8 : 16, BY = 4, BZ = 4;
+    const int h = tdf_halo(rmin, rmax);
+    const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
+    if (h <= 64 && lds <= 65536) {
+        dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
+        hipLaunchKernelGGL((k_tdf_lds<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, d, rmin,
+                           rmax, rstep, tab, tdf, radius, h);
+        return;
+    }
+    Tex<S> T{(const P*)tex, inv};
     dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
     hipLaunchKernelGGL((k_tdf<S, NORM>), grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf, radius);
 }

@@ -30,21 +30,46 @@ struct Texel<BF16> {
     }
 };
 
+// Texel sources of the sampling functions below: row(d, y, z) locates row
+// (y, z) and at(row, x) is the texel at (x, y, z), for coordinates already
+// clamped into the volume.
+// Tex: the packed texel array in device memory.
 template <class S>
 struct Tex {
     const typename Texel<S>::P* __restrict__ p;
     float inv;
-    __device__ __forceinline__ float4 at(i64 i) const { return Texel<S>::unpack(p[i], inv); }
+    __device__ __forceinline__ i64 row(const Dims& d, int y, int z) const {
+        return (i64)d.X * ((i64)y + (i64)d.Y * z);
+    }
+    __device__ __forceinline__ float4 at(i64 row, int x) const { return Texel<S>::unpack(p[row + x], inv); }
 };
 
-template <class S, bool NORM>
-__device__ __forceinline__ void hessian_tex(const Tex<S>& T, const Dims& d, int x, int y, int z, float H[3][3]) {
+// LdsTex: a tile of texels staged in LDS covering [ox, ox+tx) x [oy, oy+ty) x
+// [oz, ...).  Local coordinates are clamped into the tile, so a coordinate
+// outside it (only possible for non-finite sample positions, whose weights
+// are non-finite too) still reads LDS in bounds.
+template <class S>
+struct LdsTex {
+    const typename Texel<S>::P* p;
+    float inv;
+    int ox, oy, oz, tx, ty, tz;
+    __device__ __forceinline__ int row(const Dims&, int y, int z) const {
+        return tx * (clampi(y - oy, 0, ty - 1) + ty * clampi(z - oz, 0, tz - 1));
+    }
+    __device__ __forceinline__ float4 at(int row, int x) const {
+        return Texel<S>::unpack(p[row + clampi(x - ox, 0, tx - 1)], inv);
+    }
+};
+
+template <bool NORM, class A>
+__device__ __forceinline__ void hessian_tex(const A& T, const Dims& d, int x, int y, int z, float H[3][3]) {
     const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1);
     const int ym = max(y - 1, 0), yp = min(y + 1, d.Y - 1);
     const int zm = max(z - 1, 0), zp = min(z + 1, d.Z - 1);
-    const float4 XP = T.at(d.at(xp, y, z)), XM = T.at(d.at(xm, y, z));
-    const float4 YP = T.at(d.at(x, yp, z)), YM = T.at(d.at(x, ym, z));
-    const float4 ZP = T.at(d.at(x, y, zp)), ZM = T.at(d.at(x, y, zm));
+    const auto r0 = T.row(d, y, z);
+    const float4 XP = T.at(r0, xp), XM = T.at(r0, xm);
+    const float4 YP = T.at(T.row(d, yp, z), x), YM = T.at(T.row(d, ym, z), x);
+    const float4 ZP = T.at(T.row(d, y, zp), x), ZM = T.at(T.row(d, y, zm), x);
     float a0 = XP.x, a1 = XM.x;
     float b0 = XP.y, b1 = XM.y, b2 = YP.y, b3 = YM.y;
     float c0 = XP.z, c1 = XM.z, c2 = YP.z, c3 = YM.z, c4 = ZP.z, c5 = ZM.z;
@@ -63,21 +88,20 @@ __device__ __forceinline__ void hessian_tex(const Tex<S>& T, const Dims& d, int 
 
 // OpenCL CLK_FILTER_LINEAR, unnormalised coordinates, clamp-to-edge: sample
 // at u - 0.5, the eight terms in the OpenCL specification's order.
-template <class S>
-__device__ __forceinline__ void trilinear_tex(const Tex<S>& T, const Dims& d, float u, float v, float w,
-                                              float out[3]) {
+template <class A>
+__device__ __forceinline__ void trilinear_tex(const A& T, const Dims& d, float u, float v, float w, float out[3]) {
     const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
     const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
     const float a = uu - fu, b = vv - fv, c = ww - fw;
     int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
-    const int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    int i1 = clampi(i0 + 1, 0, d.X - 1);
+    const int j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
     i0 = clampi(i0, 0, d.X - 1);
     j0 = clampi(j0, 0, d.Y - 1);
     k0 = clampi(k0, 0, d.Z - 1);
-    const i64 r00 = (i64)d.X * ((i64)j0 + (i64)d.Y * k0), r10 = (i64)d.X * ((i64)j1 + (i64)d.Y * k0);
-    const i64 r01 = (i64)d.X * ((i64)j0 + (i64)d.Y * k1), r11 = (i64)d.X * ((i64)j1 + (i64)d.Y * k1);
-    const float4 t0 = T.at(r00 + i0), t1 = T.at(r00 + i1), t2 = T.at(r10 + i0), t3 = T.at(r10 + i1);
-    const float4 t4 = T.at(r01 + i0), t5 = T.at(r01 + i1), t6 = T.at(r11 + i0), t7 = T.at(r11 + i1);
+    const auto r00 = T.row(d, j0, k0), r10 = T.row(d, j1, k0), r01 = T.row(d, j0, k1), r11 = T.row(d, j1, k1);
+    const float4 t0 = T.at(r00, i0), t1 = T.at(r00, i1), t2 = T.at(r10, i0), t3 = T.at(r10, i1);
+    const float4 t4 = T.at(r01, i0), t5 = T.at(r01, i1), t6 = T.at(r11, i0), t7 = T.at(r11, i1);
     const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
     const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
     const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
