@@ -12,6 +12,7 @@ template <>
 struct Texel<F32> {
     typedef float4 P;
     static __device__ __forceinline__ float4 unpack(P p, float) { return p; }
+    static __device__ __forceinline__ float4 raw(P p) { return p; }
 };
 template <>
 struct Texel<F16> {
@@ -20,6 +21,9 @@ struct Texel<F16> {
     static __device__ __forceinline__ float4 unpack(P p, float inv) {
         return make_float4(h(p.x & 0xffffu) * inv, h(p.x >> 16) * inv, h(p.y & 0xffffu) * inv, h(p.y >> 16) * inv);
     }
+    static __device__ __forceinline__ float4 raw(P p) {
+        return make_float4(h(p.x & 0xffffu), h(p.x >> 16), h(p.y & 0xffffu), h(p.y >> 16));
+    }
 };
 template <>
 struct Texel<BF16> {
@@ -27,6 +31,9 @@ struct Texel<BF16> {
     static __device__ __forceinline__ float b(unsigned v) { return __uint_as_float(v << 16); }
     static __device__ __forceinline__ float4 unpack(P p, float inv) {
         return make_float4(b(p.x & 0xffffu) * inv, b(p.x >> 16) * inv, b(p.y & 0xffffu) * inv, b(p.y >> 16) * inv);
+    }
+    static __device__ __forceinline__ float4 raw(P p) {
+        return make_float4(b(p.x & 0xffffu), b(p.x >> 16), b(p.y & 0xffffu), b(p.y >> 16));
     }
 };
 
@@ -42,6 +49,7 @@ struct Tex {
         return (i64)d.X * ((i64)y + (i64)d.Y * z);
     }
     __device__ __forceinline__ float4 at(i64 row, int x) const { return Texel<S>::unpack(p[row + x], inv); }
+    __device__ __forceinline__ float4 at_raw(i64 row, int x) const { return Texel<S>::raw(p[row + x]); }
 };
 
 // LdsTex: a tile of texels staged in LDS covering [ox, ox+tx) x [oy, oy+ty) x
@@ -58,6 +66,9 @@ struct LdsTex {
     }
     __device__ __forceinline__ float4 at(int row, int x) const {
         return Texel<S>::unpack(p[row + clampi(x - ox, 0, tx - 1)], inv);
+    }
+    __device__ __forceinline__ float4 at_raw(int row, int x) const {
+        return Texel<S>::raw(p[row + clampi(x - ox, 0, tx - 1)]);
     }
 };
 
@@ -87,7 +98,10 @@ __device__ __forceinline__ void hessian_tex(const A& T, const Dims& d, int x, in
 }
 
 // OpenCL CLK_FILTER_LINEAR, unnormalised coordinates, clamp-to-edge: sample
-// at u - 0.5, the eight terms in the OpenCL specification's order.
+// at u - 0.5, the eight terms in the OpenCL specification's order.  The
+// texels' storage scale (a power of two) is applied once to each sum: with
+// no overflow or underflow, scaling by a power of two commutes exactly with
+// every product and sum, so this equals scaling each tap.
 template <class A>
 __device__ __forceinline__ void trilinear_tex(const A& T, const Dims& d, float u, float v, float w, float out[3]) {
     const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
@@ -100,8 +114,8 @@ __device__ __forceinline__ void trilinear_tex(const A& T, const Dims& d, float u
     j0 = clampi(j0, 0, d.Y - 1);
     k0 = clampi(k0, 0, d.Z - 1);
     const auto r00 = T.row(d, j0, k0), r10 = T.row(d, j1, k0), r01 = T.row(d, j0, k1), r11 = T.row(d, j1, k1);
-    const float4 t0 = T.at(r00, i0), t1 = T.at(r00, i1), t2 = T.at(r10, i0), t3 = T.at(r10, i1);
-    const float4 t4 = T.at(r01, i0), t5 = T.at(r01, i1), t6 = T.at(r11, i0), t7 = T.at(r11, i1);
+    const float4 t0 = T.at_raw(r00, i0), t1 = T.at_raw(r00, i1), t2 = T.at_raw(r10, i0), t3 = T.at_raw(r10, i1);
+    const float4 t4 = T.at_raw(r01, i0), t5 = T.at_raw(r01, i1), t6 = T.at_raw(r11, i0), t7 = T.at_raw(r11, i1);
     const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
     const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
     const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
@@ -114,9 +128,9 @@ __device__ __forceinline__ void trilinear_tex(const A& T, const Dims& d, float u
     sx = sx + wt5 * t5.x; sy = sy + wt5 * t5.y; sz = sz + wt5 * t5.z;
     sx = sx + wt6 * t6.x; sy = sy + wt6 * t6.y; sz = sz + wt6 * t6.z;
     sx = sx + wt7 * t7.x; sy = sy + wt7 * t7.y; sz = sz + wt7 * t7.z;
-    out[0] = sx;
-    out[1] = sy;
-    out[2] = sz;
+    out[0] = sx * T.inv;
+    out[1] = sy * T.inv;
+    out[2] = sz * T.inv;
 }
 
 constexpr int CBX = 16, CBY = 4, CBZ = 4;  // TDF brick
