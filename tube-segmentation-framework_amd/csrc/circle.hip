@@ -80,35 +80,144 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float r
     rad[i] = r;
 }
 
+// The mean of dot(-F(p + r V_j), V_j) over the 32 circle directions.
+template <class A>
+__device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx, float fy, float fz, float radius,
+                                            const float (&e2)[3], const float (&e3)[3], const CircleTables& tab) {
+    float sum = 0.0f;
+    for (int j = 0; j < 32; j++) {
+        const float cj = tab.c[j], sj = tab.s[j];
+        const float vx = cj * e3[0] + sj * e2[0], vy = cj * e3[1] + sj * e2[1], vz = cj * e3[2] + sj * e2[2];
+        float s[3];
+        trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+        sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
+    }
+    return sum / 32.0f;
+}
+
 // The brick's footprint -- the brick plus a halo of h voxels, clamp-to-edge
-// -- staged in LDS once; the Hessian and every trilinear tap of the radius
-// search then read LDS.  h covers the Hessian's +-1 and the farthest sample
-// corner, floor(rmax + 0.5) + 1.
+// -- staged in LDS once; the Hessian and every trilinear tap read LDS.  h
+// covers the Hessian's +-1 and the farthest sample corner,
+// floor(rmax + 0.5) + 1.
+// The radius search runs in rounds, one radius per round for every voxel
+// of the brick still improving: after each round the survivors (voxel, e2,
+// e3, best sum) are compacted to the front of the workgroup through LDS, so
+// a wave's lanes stay busy and waves with no survivor retire.  The radius of
+// round k is the same float accumulation for every voxel, so each voxel's
+// arithmetic is that of the sequential loop.
 template <class S, bool NORM, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>::P* __restrict__ tex, float inv,
                                                         Dims d, float rmin, float rmax, float rstep, CircleTables tab,
                                                         float* __restrict__ tdf, float* __restrict__ rad, int h) {
+    constexpr int NT = BX * BY * BZ, NW = NT / 64;
     typedef typename Texel<S>::P P;
     extern __shared__ float4 lds_f4[];
+    __shared__ float st_e[6][NT];
+    __shared__ float st_m[NT];
+    __shared__ int st_v[NT];
+    __shared__ int wcount[NW];
     P* tile = reinterpret_cast<P*>(lds_f4);
     const int bx0 = blockIdx.x * BX, by0 = blockIdx.y * BY, bz0 = blockIdx.z * BZ;
     LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
     const int cells = T.tx * T.ty * T.tz;
-    for (int c = tid; c < cells; c += BX * BY * BZ) {
+    for (int c = tid; c < cells; c += NT) {
         const int lx = c % T.tx, rr = c / T.tx, ly = rr % T.ty, lz = rr / T.ty;
         const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
         const int gz = clampi(T.oz + lz, 0, d.Z - 1);
         tile[c] = tex[d.at(gx, gy, gz)];
     }
     __syncthreads();
-    const int x = bx0 + threadIdx.x, y = by0 + threadIdx.y, z = bz0 + threadIdx.z;
-    if (x >= d.X || y >= d.Y || z >= d.Z) return;
-    float t, r;
-    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r);
-    const i64 i = d.at(x, y, z);
-    tdf[i] = t;
-    rad[i] = r;
+    // round 0 at the thread's own voxel
+    int v = tid;  // voxel of the brick this thread works on
+    float e2[3], e3[3], best = 0.0f;
+    bool act = false;
+    {
+        const int x = bx0 + threadIdx.x, y = by0 + threadIdx.y, z = bz0 + threadIdx.z;
+        if (x < d.X && y < d.Y && z < d.Z) {
+            float H[3][3], V[3][3], ev[3];
+            hessian_tex<NORM>(T, d, x, y, z, H);
+            eigen3(H, V, ev);
+            e2[0] = V[0][1]; e2[1] = V[1][1]; e2[2] = V[2][1];
+            e3[0] = V[0][2]; e3[1] = V[1][2]; e3[2] = V[2][2];
+            float r0 = 0.0f;
+            if (rmin <= rmax) {
+                const float sum = circle_sum(T, d, (float)x, (float)y, (float)z, rmin, e2, e3, tab);
+                if (sum > 0.0f) {
+                    best = sum;
+                    r0 = rmin;
+                    act = true;
+                }
+            }
+            if (!act) {
+                const i64 i = d.at(x, y, z);
+                tdf[i] = best;
+                rad[i] = r0;
+            }
+        }
+    }
+    const int lane = tid & 63, wid = tid >> 6;
+    float rprev = rmin;
+    for (float radius = rmin + rstep; ; radius += rstep) {
+        // voxel of this thread's state -> coordinates
+        auto coords = [&](int vv, int& x, int& y, int& z) {
+            x = bx0 + vv % BX;
+            y = by0 + (vv / BX) % BY;
+            z = bz0 + vv / (BX * BY);
+        };
+        if (!(radius <= rmax)) {
+            if (act) {
+                int x, y, z;
+                coords(v, x, y, z);
+                const i64 i = d.at(x, y, z);
+                tdf[i] = best;
+                rad[i] = rprev;
+            }
+            break;
+        }
+        // compact the survivors: slot = rank among active threads
+        const unsigned long long bal = __ballot(act);
+        if (lane == 0) wcount[wid] = __popcll(bal);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const int cnt = wcount[w];
+            off += w < wid ? cnt : 0;
+            total += cnt;
+        }
+        if (act) {
+            const int slot = off + __popcll(bal & ((1ull << lane) - 1ull));
+            st_e[0][slot] = e2[0]; st_e[1][slot] = e2[1]; st_e[2][slot] = e2[2];
+            st_e[3][slot] = e3[0]; st_e[4][slot] = e3[1]; st_e[5][slot] = e3[2];
+            st_m[slot] = best;
+            st_v[slot] = v;
+        }
+        __syncthreads();
+        if (total == 0) break;
+        act = tid < total;
+        if (act) {
+            e2[0] = st_e[0][tid]; e2[1] = st_e[1][tid]; e2[2] = st_e[2][tid];
+            e3[0] = st_e[3][tid]; e3[1] = st_e[4][tid]; e3[2] = st_e[5][tid];
+            best = st_m[tid];
+            v = st_v[tid];
+        }
+        __syncthreads();  // the state area is rewritten next round
+        if (act) {
+            int x, y, z;
+            coords(v, x, y, z);
+            const float sum = circle_sum(T, d, (float)x, (float)y, (float)z, radius, e2, e3, tab);
+            if (sum > best) {
+                best = sum;
+            } else {
+                const i64 i = d.at(x, y, z);
+                tdf[i] = best;
+                rad[i] = rprev;
+                act = false;
+            }
+        }
+        rprev = radius;
+    }
 }
 
 static unsigned grid1d(i64 n) {
@@ -137,7 +246,8 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     constexpr int BX = sizeof(P) == 16 ? 8 : 16, BY = 4, BZ = 4;
     const int h = tdf_halo(rmin, rmax);
     const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
-    if (h <= 64 && lds <= 65536) {
+    const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // k_tdf_lds's static LDS
+    if (h <= 64 && lds + state <= 65536) {
         dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
         hipLaunchKernelGGL((k_tdf_lds<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, d, rmin,
                            rmax, rstep, tab, tdf, radius, h);
