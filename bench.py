@@ -106,8 +106,10 @@ def main():
         tsf.create_vector_field(d_blur, F32c, 0.2, 1, stream)
         tsf.gvf_euler(F32c, V32c, args.mu, 0, 1, None, stream)
         tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
-        tsf.gvf_euler(F, V, args.mu, 4, 1, ws, stream)
+        tsf.gvf_euler(F, V, args.mu, 4, 1, ws, stream)  # single steps
+        tsf.gvf_euler(F, V, args.mu, 8, 2, ws, stream)  # two steps per launch
         tsf.gvf_euler(F32c, V32c, args.mu, 2, 1, None, stream)
+        tsf.gvf_euler(F32c, V32c, args.mu, 4, 2, None, stream)
         tsf.tdf_circle(F, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
         tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
         stream.synchronize()
@@ -173,9 +175,21 @@ def main():
             stream.synchronize()
             ts.append(ev[1].elapsed_ms(ev[2]))
         f32_ms = float(np.median(ts))
+        # the plain FP32 path: one step per launch
+        ts1 = []
+        for _ in range(max(1, args.steps)):
+            ev[1].record(stream)
+            tsf.gvf_euler(F32, V32, args.mu, args.iters, 1, ws32, stream)
+            ev[2].record(stream)
+            stream.synchronize()
+            ts1.append(ev[1].elapsed_ms(ev[2]))
+        f32_1 = float(np.median(ts1))
         f32 = {"gvf_ms": round(f32_ms, 3), "gvoxel_iter_per_s": round(N * args.iters / (f32_ms * 1e-3) / 1e9, 2),
                "speedup_reduced_vs_f32": round(f32_ms / gvf_ms, 3),
-               "hbm_frac": round(36 * N / (f32_ms / launches * 1e-3) / PEAK_HBM, 4)}
+               "hbm_frac": round(36 * N / (f32_ms / launches * 1e-3) / PEAK_HBM, 4),
+               "single_step_gvf_ms": round(f32_1, 3),
+               "single_step_gvoxel_iter_per_s": round(N * args.iters / (f32_1 * 1e-3) / 1e9, 2),
+               "speedup_reduced_vs_f32_single_step": round(f32_1 / gvf_ms, 3)}
         del F32, V32, ws32
 
     adjoint = None
@@ -191,7 +205,8 @@ def main():
     # rocprofv3 counter passes recorded in profiles/ (null when not measured
     # for this storage type)
     traffic = None
-    kname = {"f16": "k_gvf_march<F16, 2, false>", "f32": "k_gvf_march<F32, 2, false>"}.get(args.storage)
+    gk = "k_gvf_fused2<{}, 1, false>" if args.fuse >= 2 else "k_gvf_march<{}, 2, false>"
+    kname = {"f16": gk.format("F16"), "f32": gk.format("F32")}.get(args.storage)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_circle_forward_512cubed.json")
     if kname and n == 512 and os.path.exists(tpath):
         with open(tpath) as fh:
@@ -224,7 +239,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
                      "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
-                     "kernel": "k_gvf_march", "bytes_per_voxel_per_launch": bytes_per_voxel,
+                     "kernel": kname or "k_gvf", "steps_per_launch": 2 if args.fuse >= 2 else 1,
+                     "bytes_per_voxel_per_launch": bytes_per_voxel,
                      "launch_ms": round(launch_ms, 4)},
         "cpu_baseline": cpu,
     }
