@@ -117,7 +117,9 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>:
     __shared__ int st_v[NT];
     __shared__ int wcount[NW];
     P* tile = reinterpret_cast<P*>(lds_f4);
-    const int bx0 = blockIdx.x * BX, by0 = blockIdx.y * BY, bz0 = blockIdx.z * BZ;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
     LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
     const int cells = T.tx * T.ty * T.tz;

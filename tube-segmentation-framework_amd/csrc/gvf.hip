@@ -518,7 +518,9 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4) k_gvf_fused2(Planes3
     m.sc_v = sc_v;
     m.lane = threadIdx.x;
     m.ty = threadIdx.y;
-    const int ox = tile_origin(blockIdx.x, G::OW, d.X), oy = tile_origin(blockIdx.y, G::OH, d.Y);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int ox = tile_origin(tbx, G::OW, d.X), oy = tile_origin(tby, G::OH, d.Y);
     m.ex = ox - G::RX;
     m.ey = oy - 1;
     m.cy = m.ey + m.ty;
@@ -573,7 +575,7 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4) k_gvf_fused2(Planes3
     }
     m.hok = hin;
     m.hidx = hin ? (i64)hy * d.X + hx : 0;
-    m.qs = 1 + blockIdx.z * FZC;
+    m.qs = 1 + tbz * FZC;
     m.qe = min(m.qs + FZC, d.Z - 1);
     const int t0 = m.qs - 1, te = m.qe;
 

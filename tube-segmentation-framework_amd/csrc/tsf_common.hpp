@@ -23,6 +23,22 @@ struct Dims {
 
 __host__ __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round-robin
+// by linear workgroup id.  xcd_tile() renumbers them so that XCD k works
+// through a contiguous range of tiles: tiles that share halos then meet in
+// the same L2.  A bijection on [0, gridDim.x * gridDim.y * gridDim.z).
+constexpr int XCDS = 8;
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned total = gx * gy * gridDim.z;
+    const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = total / XCDS, r = total % XCDS, k = id % XCDS, j = id / XCDS;
+    const unsigned t = k * q + (k < r ? k : r) + j;
+    bx = (int)(t % gx);
+    by = (int)((t / gx) % gy);
+    bz = (int)(t / (gx * gy));
+}
+
 // Mirror remap of the GVF/FMG kernels: 0 -> 2, >= N-1 -> N-3.
 __host__ __device__ inline int remap(int p, int N) { return p == 0 ? 2 : (p >= N - 1 ? N - 3 : p); }
 
