@@ -241,7 +241,9 @@ __device__ __forceinline__ float lam_boundary(const float* __restrict__ Lf, cons
 template <bool FOLD>
 __global__ void __launch_bounds__(256) k_gvf_adj(P3 Lf, P3 v, P3 F, Dims d, float mu, M3 out, M3 dF,
                                                  double* __restrict__ part) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);  // z-neighbouring tiles share an L2
+    const int x = tbx * 64 + threadIdx.x, y = tby * 4 + threadIdx.y, z = tbz;
     double gm = 0.0;
     if (x < d.X && y < d.Y) {
         const i64 sy = d.X, sz = (i64)d.X * d.Y;
@@ -421,16 +423,18 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
     tile.ty = CBY + 2 * halo;
     tile.tz = CBZ + 2 * halo;
     tile.cells = tile.tx * tile.ty * tile.tz;
-    tile.ox = blockIdx.x * CBX - halo;
-    tile.oy = blockIdx.y * CBY - halo;
-    tile.oz = blockIdx.z * CBZ - halo;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    tile.ox = tbx * CBX - halo;
+    tile.oy = tby * CBY - halo;
+    tile.oz = tbz * CBZ - halo;
     if (halo > 0) {
         for (int k = tid; k < 3 * tile.cells; k += CBX * CBY * CBZ) lds[k] = 0.0f;
         __syncthreads();
     }
     float* const dV[3] = {dVx, dVy, dVz};
-    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
-    const int z = blockIdx.z * CBZ + threadIdx.z;
+    const int x = tbx * CBX + threadIdx.x, y = tby * CBY + threadIdx.y;
+    const int z = tbz * CBZ + threadIdx.z;
     const float g = (x < d.X && y < d.Y && z < d.Z) ? gT[d.at(x, y, z)] : 0.0f;
     if (g != 0.0f) {
         float H[3][3], E[3][3], lam[3];

@@ -20,7 +20,9 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_blur(const float* __restrict__ i
                                                      int m, const float* __restrict__ mask) {
     __shared__ float tile[TILE_MAX];
     const int TX = BX + 2 * m, TY = BY + 2 * m, TZ = BZ + 2 * m;
-    const int x0 = blockIdx.x * BX - m, y0 = blockIdx.y * BY - m, z0 = blockIdx.z * BZ - m;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);  // tiles sharing aprons meet in one L2
+    const int x0 = tbx * BX - m, y0 = tby * BY - m, z0 = tbz * BZ - m;
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
     const int total = TX * TY * TZ;
     for (int i = tid; i < total; i += BX * BY * BZ) {
@@ -29,7 +31,7 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_blur(const float* __restrict__ i
         tile[i] = in[d.at(gx, gy, gz)];
     }
     __syncthreads();
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, z = blockIdx.z * BZ + threadIdx.z;
+    const int x = tbx * BX + threadIdx.x, y = tby * BY + threadIdx.y, z = tbz * BZ + threadIdx.z;
     if (x >= d.X || y >= d.Y || z >= d.Z) return;
     const int M = 2 * m + 1;
     float sum = 0.0f;
