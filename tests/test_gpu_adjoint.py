@@ -59,10 +59,11 @@ def test_vector_field_adjoint(tsf, orc, sign):
     assert rel(out.download(), exp) < 1e-5
 
 
-@pytest.mark.parametrize("iters,every", [(10, 3), (12, 12), (7, 50), (1, 1)])
-def test_gvf_adjoint_checkpointed(tsf, orc, iters, every):
+@pytest.mark.parametrize("iters,every,sh", [(10, 3, (18, 20, 22)), (12, 12, (18, 20, 22)), (7, 50, (18, 20, 22)),
+                                           (1, 1, (18, 20, 22)), (5, 2, (70, 19, 130)), (4, 4, (5, 9, 4))])
+def test_gvf_adjoint_checkpointed(tsf, orc, iters, every, sh):
+    # shapes (Z, Y, X): one tile; several x/y tiles and z chunks; the smallest
     g = rng(3)
-    sh = (18, 20, 22)
     F = np.zeros((4,) + sh, np.float32)
     F[:3] = g.uniform(-1, 1, (3,) + sh) / np.sqrt(3)
     F[3] = 1

@@ -8,10 +8,10 @@
 //   k_blur_adjoint     gather B^T dB with clamp-to-edge folding (deterministic)
 //   k_blur_dsigma      per-voxel dB * (dw * I) -> block partial sums (double)
 //   k_vf_dg / k_vf_gather   the vector-field adjoint as two gathers
-//   k_gvf_adj          the transposed Jacobi step on the folded adjoint:
-//                      transposed 7-point stencil and data term, dF
-//                      accumulation, d/dmu block partials (double), and the
-//                      mirror fold of its output for the next reverse step
+//   k_gvf_adj_march    the transposed Jacobi step on the folded adjoint,
+//                      z-marching: transposed 7-point stencil and data term,
+//                      dF accumulation, d/dmu block partials (double), and
+//                      the mirror fold of its output for the next reverse step
 //   k_tdf_adjoint      per-voxel recomputation of the circle fit at r*,
 //                      then atomic scatter of the sample, direction
 //                      (eigenvector perturbation) and Hessian contributions
@@ -218,97 +218,197 @@ __global__ void k_fold3(P3 lam, Dims d, M3 out) {
     for (int c = 0; c < 3; c++) out.c[c][i] = fold(lam.c[c], d, x, y, z);
 }
 
-// lam^n at a position w with a boundary coordinate: there Lf(w) = 0, so only
-// the mu-weighted neighbours remain.
-__device__ __forceinline__ float lam_boundary(const float* __restrict__ Lf, const Dims& d, int x, int y, int z,
-                                              float mu) {
-    const i64 i = d.at(x, y, z);
-    float nb = 0.0f;
-    if (x + 1 < d.X) nb += Lf[i + 1];
-    if (x > 0) nb += Lf[i - 1];
-    if (y + 1 < d.Y) nb += Lf[i + d.X];
-    if (y > 0) nb += Lf[i - d.X];
-    if (z + 1 < d.Z) nb += Lf[i + (i64)d.X * d.Y];
-    if (z > 0) nb += Lf[i - (i64)d.X * d.Y];
-    return mu * nb;
-}
+// ---------------------------------------------------------------- GVF reverse step, z-marching
+// One reverse Euler step as a z-march.  In: Lf = fold(lam^{n+1}) (zero on every
+// boundary plane), v = v^n, F.  lam^n(y) = (1 - 6 mu - |f(y)|^2) Lf(y) + mu sum_e
+// Lf(y+e).  FOLD: out = fold(lam^n), the input of the next reverse step;
+// otherwise (the step from v^0 = f) dF += lam^n.  Always: dF += data-term
+// gradient Lf m2 + 2 f sum_c Lf_c (f_c - v_c); part[block] = sum Lf . lap(v).
+// Structure: a workgroup of ABX x
+// ABY threads owns that tile of columns and a chunk of AZC planes; per plane
+// (one barrier) Lf and v of the plane, with a one-column ring, go to LDS for
+// the x/y neighbours, and each thread keeps its column's planes in register
+// rings loaded three planes ahead.
+// The fold of the output needs no gather: Lf is zero on every boundary
+// plane, so lam at a boundary position w is mu * Lf at w's single interior
+// neighbour when w has one boundary coordinate, and zero when it has more.
+// Hence fold(lam)(q) = lam(q) + mu * sum over axes of [q_a == 2] Lf(q - e_a)
+// + [q_a == N_a - 3] Lf(q + e_a) at interior q, and zero on boundary planes.
+constexpr int ABX = 64, ABY = 8, AZC = 64;
 
-// One reverse Euler step.  In: Lf = fold(lam^{n+1}) (zero on every boundary
-// plane), v = v^n, F.  lam^n(y) = (1 - 6 mu - |f(y)|^2) Lf(y) + mu sum_e Lf(y+e).
-// FOLD: out = fold(lam^n), the input of the next reverse step; otherwise
-// (the step from v^0 = f) dF += lam^n.  Always: dF += data-term gradient
-// Lf m2 + 2 f sum_c Lf_c (f_c - v_c); part[block] = sum Lf . lap(v) (d/dmu).
-template <bool FOLD>
-__global__ void __launch_bounds__(256) k_gvf_adj(P3 Lf, P3 v, P3 F, Dims d, float mu, M3 out, M3 dF,
-                                                 double* __restrict__ part) {
-    int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz);  // z-neighbouring tiles share an L2
-    const int x = tbx * 64 + threadIdx.x, y = tby * 4 + threadIdx.y, z = tbz;
-    double gm = 0.0;
-    if (x < d.X && y < d.Y) {
-        const i64 sy = d.X, sz = (i64)d.X * d.Y;
-        const i64 i = d.at(x, y, z);
-        const bool bxm = x > 0, bxp = x < d.X - 1, bym = y > 0, byp = y < d.Y - 1, bzm = z > 0, bzp = z < d.Z - 1;
-        const i64 ixp = bxp ? i + 1 : i, ixm = bxm ? i - 1 : i, iyp = byp ? i + sy : i, iym = bym ? i - sy : i;
-        const i64 izp = bzp ? i + sz : i, izm = bzm ? i - sz : i;
-        const bool interior = bxm && bxp && bym && byp && bzm && bzp;
-        const float f0 = F.c[0][i], f1 = F.c[1][i], f2 = F.c[2][i];
+struct AdjCtx {
+    P3 Lf, v, F;
+    M3 out, dF;
+    Dims d;
+    float mu;
+    int lane, ty, cx, cy, z1;
+    bool act, icol, hrole, hok, fx2, fxN, fy2, fyN;
+    i64 P, col, hidx;
+    int hly, hlx;
+};
+
+template <bool FOLD, int T4>
+__device__ __forceinline__ void adj_step(const AdjCtx& m, int t, float (&lf)[4][3], float (&vv)[4][3],
+                                         float (&fr)[4][3], float (&dfr)[4][3], float (&hl)[2][3], float (&hv)[2][3],
+                                         float (&LL)[2][3][ABY + 2][ABX + 2], float (&LV)[2][3][ABY + 2][ABX + 2],
+                                         double& gm) {
+    constexpr int SM1 = (T4 + 3) & 3, S0 = T4, SP1 = (T4 + 1) & 3, B = T4 & 1;
+    const int lane = m.lane, ty = m.ty;
+    // out-of-volume cells read as zero
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        LL[B][c][ty + 1][lane + 1] = m.act ? lf[S0][c] : 0.0f;
+        LV[B][c][ty + 1][lane + 1] = m.act ? vv[S0][c] : 0.0f;
+    }
+    if (m.hrole) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            LL[B][c][m.hly][m.hlx] = m.hok ? hl[B][c] : 0.0f;
+            LV[B][c][m.hly][m.hlx] = m.hok ? hv[B][c] : 0.0f;
+        }
+    }
+    __syncthreads();
+    if (m.act && t < m.z1) {
+        const int z = t;
+        const bool zm_ok = z > 0, zp_ok = z < m.d.Z - 1;
+        const bool interior = m.icol && zm_ok && zp_ok;
+        const bool fz2 = z == 2, fzN = z == m.d.Z - 3;
+        const float f0 = fr[S0][0], f1 = fr[S0][1], f2 = fr[S0][2];
         const float fc[3] = {f0, f1, f2};
         const float m2 = f0 * f0 + f1 * f1 + f2 * f2;
-        const float diag = 1.0f - 6.0f * mu - m2;
-        float L[3], lam[3];
+        const float diag = 1.0f - 6.0f * m.mu - m2;
+        float lam[3], fold[3];
         float s = 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float* la = Lf.c[c];
-            const float* vc = v.c[c];
-            const float l0 = la[i], a = la[ixp], b = la[ixm], e = la[iyp], g = la[iym], h = la[izp], k = la[izm];
-            const float v0 = vc[i], va = vc[ixp], vb = vc[ixm], ve = vc[iyp], vg = vc[iym], vh = vc[izp], vk = vc[izm];
-            const float nb = (bxp ? a : 0.0f) + (bxm ? b : 0.0f) + (byp ? e : 0.0f) + (bym ? g : 0.0f) +
-                             (bzp ? h : 0.0f) + (bzm ? k : 0.0f);
-            L[c] = l0;
-            lam[c] = diag * l0 + mu * nb;
+            const float l0 = lf[S0][c];
+            const float lxp = LL[B][c][ty + 1][lane + 2], lxm = LL[B][c][ty + 1][lane];
+            const float lyp = LL[B][c][ty + 2][lane + 1], lym = LL[B][c][ty][lane + 1];
+            const float lzp = zp_ok ? lf[SP1][c] : 0.0f, lzm = zm_ok ? lf[SM1][c] : 0.0f;
+            const float nb = lxp + lxm + lyp + lym + lzp + lzm;
+            lam[c] = diag * l0 + m.mu * nb;
+            const float v0 = vv[S0][c];
             s += l0 * (fc[c] - v0);
-            const float lap = va + vb + ve + vg + vh + vk - 6.0f * v0;
-            if (interior) gm += (double)l0 * (double)lap;
+            if (interior) {
+                const float lap = LV[B][c][ty + 1][lane + 2] + LV[B][c][ty + 1][lane] + LV[B][c][ty + 2][lane + 1] +
+                                  LV[B][c][ty][lane + 1] + vv[SP1][c] + vv[SM1][c] - 6.0f * v0;
+                gm += (double)l0 * (double)lap;
+            }
+            const float e = (m.fx2 ? lxm : 0.0f) + (m.fxN ? lxp : 0.0f) + (m.fy2 ? lym : 0.0f) +
+                            (m.fyN ? lyp : 0.0f) + (fz2 ? lzm : 0.0f) + (fzN ? lzp : 0.0f);
+            fold[c] = interior ? lam[c] + m.mu * e : 0.0f;
         }
+        const i64 i = z * m.P + m.col;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            float g = dF.c[c][i] + (L[c] * m2 + 2.0f * fc[c] * s);
+            float g = dfr[S0][c] + (lf[S0][c] * m2 + 2.0f * fc[c] * s);
             if (!FOLD) g += lam[c];
-            dF.c[c][i] = g;
-        }
-        if (FOLD) {
-            // fold(lam^n)(q): zero on boundary planes; at q == 2 or q == N-3
-            // (per axis) the mirrored boundary positions add their lam^n
-            if (!interior) {
-#pragma unroll
-                for (int c = 0; c < 3; c++) lam[c] = 0.0f;
-            } else if (x == 2 || x == d.X - 3 || y == 2 || y == d.Y - 3 || z == 2 || z == d.Z - 3) {
-                int px[3], py[3], pz[3];
-                const int nx = pre_list(x, d.X, px), ny = pre_list(y, d.Y, py), nz = pre_list(z, d.Z, pz);
-                for (int cz = 0; cz < nz; cz++)
-                    for (int cy = 0; cy < ny; cy++)
-                        for (int cx = 0; cx < nx; cx++) {
-                            if (cx == 0 && cy == 0 && cz == 0) continue;  // q itself
-#pragma unroll
-                            for (int c = 0; c < 3; c++) lam[c] += lam_boundary(Lf.c[c], d, px[cx], py[cy], pz[cz], mu);
-                        }
-            }
-#pragma unroll
-            for (int c = 0; c < 3; c++) out.c[c][i] = lam[c];
+            m.dF.c[c][i] = g;
+            if (FOLD) m.out.c[c][i] = fold[c];
         }
     }
-    // block partial of d/dmu
-    __shared__ double sh[256];
-    const int t = threadIdx.y * 64 + threadIdx.x;
-    sh[t] = gm;
+    // loads: plane t+3 of the rings into the slots of t-1, the ring cells of
+    // plane t+2 into this step's halo slot
+    const int z3 = min(t + 3, m.d.Z - 1), z2 = min(t + 2, m.d.Z - 1);
+    const i64 i3 = m.act ? z3 * m.P + m.col : 0, ih = m.hok ? z2 * m.P + m.hidx : 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        lf[SM1][c] = m.Lf.c[c][i3];
+        vv[SM1][c] = m.v.c[c][i3];
+        fr[SM1][c] = m.F.c[c][i3];
+        dfr[SM1][c] = m.dF.c[c][i3];
+        hl[B][c] = m.Lf.c[c][ih];
+        hv[B][c] = m.v.c[c][ih];
+    }
+}
+
+template <bool FOLD>
+__global__ void __launch_bounds__(ABX* ABY, 4) k_gvf_adj_march(P3 Lf, P3 v, P3 F, Dims d, float mu, M3 out, M3 dF,
+                                                            double* __restrict__ part) {
+    __shared__ float LL[2][3][ABY + 2][ABX + 2];
+    __shared__ float LV[2][3][ABY + 2][ABX + 2];
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    AdjCtx m;
+    m.Lf = Lf;
+    m.v = v;
+    m.F = F;
+    m.out = out;
+    m.dF = dF;
+    m.d = d;
+    m.mu = mu;
+    m.lane = threadIdx.x;
+    m.ty = threadIdx.y;
+    const int x0 = tbx * ABX, y0 = tby * ABY;
+    m.cx = x0 + m.lane;
+    m.cy = y0 + m.ty;
+    m.act = m.cx < d.X && m.cy < d.Y;
+    m.icol = m.cx >= 1 && m.cx <= d.X - 2 && m.cy >= 1 && m.cy <= d.Y - 2;
+    m.fx2 = m.cx == 2;
+    m.fxN = m.cx == d.X - 3;
+    m.fy2 = m.cy == 2;
+    m.fyN = m.cy == d.Y - 3;
+    m.P = (i64)d.X * d.Y;
+    m.col = m.act ? (i64)m.cy * d.X + m.cx : 0;
+    // ring of the tile: rows y0-1 (ty 0) and y0+ABY (ty 1) at this thread's
+    // column; columns x0-1 and x0+ABX (ty 2, lanes 0..2*ABY-1)
+    int hx = 0, hy = 0;
+    m.hrole = false;
+    m.hly = 0;
+    m.hlx = 0;
+    if (m.ty == 0 || m.ty == 1) {
+        m.hrole = true;
+        hx = m.cx;
+        hy = m.ty == 0 ? y0 - 1 : y0 + ABY;
+        m.hly = m.ty == 0 ? 0 : ABY + 1;
+        m.hlx = m.lane + 1;
+    } else if (m.ty == 2 && m.lane < 2 * ABY) {
+        m.hrole = true;
+        const bool left = m.lane < ABY;
+        hx = left ? x0 - 1 : x0 + ABX;
+        hy = y0 + (left ? m.lane : m.lane - ABY);
+        m.hly = (left ? m.lane : m.lane - ABY) + 1;
+        m.hlx = left ? 0 : ABX + 1;
+    }
+    m.hok = m.hrole && hx >= 0 && hx < d.X && hy >= 0 && hy < d.Y;
+    m.hidx = m.hok ? (i64)hy * d.X + hx : 0;
+    const int z0 = tbz * AZC;
+    m.z1 = min(z0 + AZC, d.Z);
+    float lf[4][3], vv[4][3], fr[4][3], dfr[4][3], hl[2][3], hv[2][3];
+    // the loads of the four steps before z0 in the steps' order
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const int z = min(max(z0 - 1 + g, 0), d.Z - 1), k = (g + 3) & 3;
+        const i64 i = m.act ? z * m.P + m.col : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            lf[k][c] = Lf.c[c][i];
+            vv[k][c] = v.c[c][i];
+            fr[k][c] = F.c[c][i];
+            dfr[k][c] = dF.c[c][i];
+            if (g >= 2) {
+                const i64 ih = m.hok ? min(z0 + g - 2, d.Z - 1) * m.P + m.hidx : 0;
+                hl[g & 1][c] = Lf.c[c][ih];
+                hv[g & 1][c] = v.c[c][ih];
+            }
+        }
+    }
+    double gm = 0.0;
+    for (int t = z0; t < m.z1; t += 4) {
+        adj_step<FOLD, 0>(m, t, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
+        adj_step<FOLD, 1>(m, t + 1, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
+        adj_step<FOLD, 2>(m, t + 2, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
+        adj_step<FOLD, 3>(m, t + 3, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
+    }
+    __shared__ double sh[ABX * ABY];
+    const int tid = m.ty * ABX + m.lane;
+    sh[tid] = gm;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o) sh[t] += sh[t + o];
+    for (int o = ABX * ABY / 2; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
         __syncthreads();
     }
-    if (t == 0) part[blockIdx.x + (i64)gridDim.x * (blockIdx.y + (i64)gridDim.y * blockIdx.z)] = sh[0];
+    if (tid == 0) part[blockIdx.x + (i64)gridDim.x * (blockIdx.y + (i64)gridDim.y * blockIdx.z)] = sh[0];
 }
 
 // part2[b] = sum of a contiguous chunk of n partials (fixed order)
@@ -719,7 +819,11 @@ tsf_status tsf_gvf_euler_checkpointed(const tsf_field* f, const tsf_field* out, 
     return TSF_OK;
 }
 
-static size_t stencil_blocks(const Dims& d) { return (size_t)((d.X + 63) / 64) * ((d.Y + 3) / 4) * d.Z; }
+static dim3 adj_grid(const Dims& d) { return dim3((d.X + ABX - 1) / ABX, (d.Y + ABY - 1) / ABY, (d.Z + AZC - 1) / AZC); }
+static size_t adj_blocks(const Dims& d) {
+    const dim3 g = adj_grid(d);
+    return (size_t)g.x * g.y * g.z;
+}
 
 // float states first, then the double partial sums at a 256-byte boundary
 static size_t adj_float_bytes(const Dims& d, int k) {
@@ -729,7 +833,7 @@ static size_t adj_float_bytes(const Dims& d, int k) {
 size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every) {
     const Dims d = D(dims);
     if (checkpoint_every < 1) return 0;
-    return adj_float_bytes(d, checkpoint_every) + (NPART + 1 + stencil_blocks(d)) * sizeof(double);
+    return adj_float_bytes(d, checkpoint_every) + (NPART + 1 + adj_blocks(d)) * sizeof(double);
 }
 
 tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
@@ -755,7 +859,7 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
     double* part2 = (double*)((char*)sc.ptr + adj_float_bytes(d, k));  // NPART
     double* acc = part2 + NPART;
     double* part = acc + 1;                   // one per stencil block
-    const i64 nblk = (i64)stencil_blocks(d);
+    const i64 nblk = (i64)adj_blocks(d);
     const float* fp[3] = {(const float*)f->x, (const float*)f->y, (const float*)f->z};
     hipError_t e = hipMemsetAsync(dF, 0, 3 * N * sizeof(float), stream);
     if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
@@ -786,9 +890,11 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
             P3 Vn{{vn, vn + N, vn + 2 * N}};
             M3 Lo{{lout, lout + N, lout + 2 * N}};
             if (n > 0)
-                hipLaunchKernelGGL(k_gvf_adj<true>, STENCIL_GRID(d), 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
+                hipLaunchKernelGGL(k_gvf_adj_march<true>, adj_grid(d), dim3(ABX, ABY), 0, stream, L, Vn, Fp, d, mu, Lo,
+                                   dFp, part);
             else  // v^0 = f: the last adjoint flows into dF
-                hipLaunchKernelGGL(k_gvf_adj<false>, STENCIL_GRID(d), 0, stream, L, Vn, Fp, d, mu, Lo, dFp, part);
+                hipLaunchKernelGGL(k_gvf_adj_march<false>, adj_grid(d), dim3(ABX, ABY), 0, stream, L, Vn, Fp, d, mu,
+                                   Lo, dFp, part);
             hipLaunchKernelGGL(k_reduce_parts, NPART, RB, 0, stream, part, nblk, part2);
             hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part2, NPART, acc);
             TSF_TRY(check_launch(W));

@@ -710,12 +710,19 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
     bool vec2 = (d.X % 2) == 0;
     const void* ptrs[10] = {f[0], f[1], f[2], v0[0], v0[1], v0[2], out[0], out[1], out[2], ws};
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, 8));
-    for (int i = 1; i <= iters; i++) {
-        const bool to_out = ((iters - i) % 2) == 0;
-        Planes3<F32> src = (i == 1) ? V0 : (to_out ? Wr : Or);
+    // two steps per launch (the same float operations as single steps), an
+    // odd count starting with one single step
+    const int pairs = iters / 2, singles = iters - 2 * pairs, launches = pairs + singles;
+    for (int j = 0; j < launches; j++) {
+        const bool to_out = ((launches - 1 - j) % 2) == 0;
+        Planes3<F32> src = (j == 0) ? V0 : (to_out ? Wr : Or);
         Out4<F32> dst = to_out ? O : Wk;
-        if (vec2) launch_step<F32, 2>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
-        else launch_step<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+        if (j < singles) {
+            if (vec2) launch_step<F32, 2>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+            else launch_step<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+        } else {
+            launch_fused2<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+        }
         TSF_TRY(check_launch("gvf_steps_f32"));
     }
     return TSF_OK;
