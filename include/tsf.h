@@ -48,7 +48,11 @@ typedef enum tsf_status {
 typedef enum tsf_storage {
     TSF_STORAGE_F32 = 0,  /* float planes */
     TSF_STORAGE_F16 = 1,  /* IEEE half planes holding value * scale */
-    TSF_STORAGE_BF16 = 2  /* bfloat16 planes holding value * scale */
+    TSF_STORAGE_BF16 = 2, /* bfloat16 planes holding value * scale */
+    TSF_STORAGE_SN16 = 3  /* int16 SNORM planes, the reference's 16-bit vector format
+                             (kernels_no_3d_write.cl:9-21): stored as
+                             convert_short_sat_rte(v * 32767), read as
+                             max(-1, s / 32767); scale must be 1 */
 } tsf_storage;
 
 typedef struct tsf_dims {
@@ -190,6 +194,14 @@ size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage);
  * take the small-scale TDF and radius.  In place on the large buffers. */
 tsf_status tsf_combine(const float* tdf_small, const float* radius_small, float* tdf_large, float* radius_large,
                        int64_t count, hipStream_t stream);
+
+/* The reference's 16-bit TDF buffers (TDF_TYPE ushort with 16bit-vectors,
+ * kernels_no_3d_write.cl:19-21, written at 1570 and 1663): rounds each value
+ * in place to what that format holds, convert_ushort_sat_rte(t * 65535) / 65535
+ * (NaN -> 0).  Applied after tsf_tdf_circle on an SN16 field it gives the
+ * reference's 16-bit-mode TDF values; tsf_combine and tsf_candidates then
+ * compare them as the reference compares the codes. */
+tsf_status tsf_tdf_round_unorm16(float* tdf, int64_t count, hipStream_t stream);
 
 /* ------------------------------------------------------------ candidates */
 

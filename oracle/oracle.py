@@ -285,3 +285,49 @@ def chain_gradient(I, sigma, fmax, sign, mu, iters, rmin, rmax, rstep, gT, prec=
                                 ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep), _p(gT), _p(t),
                                 _p(dI), ctypes.byref(dm), ctypes.byref(ds))
     return t, dI, dm.value, ds.value
+
+
+# ------------------------------------------------------------- 16-bit mode
+def sn16_round(a):
+    """Values rounded to the reference's SNORM16 vector format."""
+    a = np.array(a, np.float32, copy=True, order="C")
+    lib().orc_sn16_round(_p(a), ctypes.c_longlong(a.size))
+    return a
+
+
+def unorm16_round(a):
+    """Values rounded to the reference's 16-bit TDF format."""
+    a = np.array(a, np.float32, copy=True, order="C")
+    lib().orc_unorm16_round(_p(a), ctypes.c_longlong(a.size))
+    return a
+
+
+def sn16_store(a):
+    a = np.ascontiguousarray(a, np.float32)
+    s = np.empty(a.shape, np.int16)
+    lib().orc_sn16_store(_p(a), _p(s), ctypes.c_longlong(a.size))
+    return s
+
+
+def sn16_load(s):
+    s = np.ascontiguousarray(s, np.int16)
+    a = np.empty(s.shape, np.float32)
+    lib().orc_sn16_load(_p(s), _p(a), ctypes.c_longlong(s.size))
+    return a
+
+
+def vector_field_sn16(vol, fmax, sign=1):
+    vol = np.ascontiguousarray(vol, np.float32)
+    X, Y, Z = _dims(vol)
+    F = np.empty((4,) + vol.shape, np.float32)
+    lib().orc_vector_field_sn16(_p(vol), _p(F), X, Y, Z, ctypes.c_float(fmax), int(sign))
+    return F
+
+
+def gvf_euler_sn16(F, mu, iters):
+    """F: (4 or 3, Z, Y, X) float32 holding SNORM16 values."""
+    F = np.ascontiguousarray(F, np.float32)
+    X, Y, Z = _dims(F)
+    V = np.empty((4,) + F.shape[1:], np.float32)
+    lib().orc_gvf_euler_sn16(_p(F), _p(V), X, Y, Z, ctypes.c_float(mu), int(iters))
+    return V

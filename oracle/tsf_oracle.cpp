@@ -201,6 +201,42 @@ static void gvf_euler(const T* f4, T* v4, Dims d, float mu, int iters) {
     gvf_finish(v4, d);
 }
 
+// ---------------------------------------------------------------- 16-bit mode
+// The reference's 16-bit vector format (kernels_no_3d_write.cl:9-21, the
+// VECTORS_16BIT branch): every store is convert_short_sat_rte(v * 32767.0f)
+// (round half to even, saturate, NaN -> 0), every load max(-1.0f,
+// (float)s / 32767.0f).
+static inline int16_t sn16_store(float v) {
+    const float x = v * 32767.0f;
+    if (x != x) return 0;
+    const float r = std::nearbyint(x);  // default rounding mode: to nearest even
+    return (int16_t)(r > 32767.0f ? 32767 : (r < -32768.0f ? -32768 : (int)r));
+}
+static inline float sn16_load(int16_t s) { return std::max(-1.0f, (float)s / 32767.0f); }
+static inline float sn16_round(float v) { return sn16_load(sn16_store(v)); }
+
+// runFastGVF in the 16-bit mode: f arrives already in the format (its
+// values are SNORM16 values); every iterate is stored in the format
+// (GVF3DIteration, kernels_no_3d_write.cl:1746-1787) and the finish stores
+// w = |v| (1 where 0) in the format too (GVF3DFinish 1798-1807), so w
+// saturates at 1.
+static void gvf_euler_sn16(const float* f4, float* v4, Dims d, float mu, int iters) {
+    const i64 N = d.n();
+    std::vector<float> a(f4, f4 + 3 * N), b(3 * N);
+    for (int it = 0; it < iters; it++) {
+        gvf_step<float>(f4, a.data(), b.data(), d, mu);
+#pragma omp parallel for schedule(static)
+        for (i64 i = 0; i < 3 * N; i++) b[i] = sn16_round(b[i]);
+        a.swap(b);
+    }
+    std::memcpy(v4, a.data(), sizeof(float) * 3 * N);
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < N; i++) {
+        const float l = std::sqrt(v4[i] * v4[i] + v4[N + i] * v4[N + i] + v4[2 * N + i] * v4[2 * N + i]);
+        v4[3 * N + i] = sn16_round(l > 0.0f ? l : 1.0f);
+    }
+}
+
 // ---------------------------------------------------------------- FMG GVF (A7)
 // Multigrid solver of gradientVectorFlow.cpp:5-441, 631-1005 with the kernels
 // of kernels_no_3d_write.cl:2071-2354 (FP32 path: no SNORM16 quantisation).
@@ -1349,6 +1385,33 @@ EXPORT_T(float, f32)
 EXPORT_T(double, f64)
 
 extern "C" void orc_blur_mask(float sigma, float* mask, int* maskSize) { blur_mask(sigma, mask, maskSize); }
+// 16-bit mode: round values to the SNORM16 format in place; store codes; the
+// vector field and GVF with every store in the format.
+extern "C" void orc_sn16_round(float* a, long long n) {
+    for (long long i = 0; i < n; i++) a[i] = sn16_round(a[i]);
+}
+extern "C" void orc_sn16_store(const float* a, int16_t* s, long long n) {
+    for (long long i = 0; i < n; i++) s[i] = sn16_store(a[i]);
+}
+extern "C" void orc_sn16_load(const int16_t* s, float* a, long long n) {
+    for (long long i = 0; i < n; i++) a[i] = sn16_load(s[i]);
+}
+// 16-bit TDF buffers (kernels_no_3d_write.cl:19-20): convert_ushort_sat_rte(t *
+// 65535) stored, (float)c / 65535.0f read.
+extern "C" void orc_unorm16_round(float* a, long long n) {
+    for (long long i = 0; i < n; i++) {
+        const float x = a[i] * 65535.0f;
+        const float c = x != x ? 0.0f : std::nearbyint(std::min(std::max(x, 0.0f), 65535.0f));
+        a[i] = c / 65535.0f;
+    }
+}
+extern "C" void orc_vector_field_sn16(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign) {
+    vector_field<float>(I, F, DIMS, Fmax, sign);
+    orc_sn16_round(F, 4LL * X * Y * Z);
+}
+extern "C" void orc_gvf_euler_sn16(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters) {
+    gvf_euler_sn16(f4, v4, DIMS, mu, iters);
+}
 extern "C" void orc_circle_tables(float* c, float* s) {
     make_tables();
     for (int j = 0; j < 32; j++) {

@@ -46,6 +46,9 @@ void orc_candidate_nms_f32(const float* tdf, const float* rad, const float* F, i
 void orc_cube_max_f32(const float* tdf, const uint8_t* cand, int X, int Y, int Z, int cube, uint8_t* out);
 void orc_do_eigen_f32(const float* Fx, const float* Fy, const float* Fz, int X, int Y, int Z, int x, int y, int z,
                       float* lam, float* e1, float* e2, float* e3);
+void orc_vector_field_sn16(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign);
+void orc_gvf_euler_sn16(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters);
+void orc_unorm16_round(float* a, long long n);
 void orc_chain_forward_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
                            int iters, float rmin, float rmax, float rstep, double* tdf, double* rad);
 void orc_chain_gradient_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
@@ -258,6 +261,32 @@ static void gpu_tests(const std::string& dsdir) {
         }
         releaseEigenMirror(T);
         return std::string();
+    });
+    // The reference's default parameters: 16bit-vectors on.  Vector fields and
+    // every GVF iterate are SNORM16, TDF buffers UNORM16.
+    run("runCircleFittingMethod_16bit_vectors", [&] {
+        paramList q = p;
+        setParameter(q, "16bit-vectors", "true");
+        Image3D vf16, tdf16, rad16;
+        Image3D ds2 = dataset;
+        runCircleFittingMethod(ocl, &ds2, size, q, vf16, tdf16, rad16);
+        if (vf16.storage() != TSF_STORAGE_SN16) return std::string("field not stored as SNORM16");
+        std::vector<float> S(4 * n), W(4 * n), t1(n), r1(n), t2(n), r2(n);
+        orc_vector_field_sn16(blur.data(), S.data(), X, Y, Z, 0.1f, 1);
+        orc_tdf_circle_f32(S.data(), X, Y, Z, 0.5f, 3.0f, 0.5f, t1.data(), r1.data());
+        orc_unorm16_round(t1.data(), (long long)n);
+        orc_gvf_euler_sn16(S.data(), W.data(), X, Y, Z, 0.05f, 250);
+        orc_tdf_circle_f32(W.data(), X, Y, Z, 2.5f, 4.0f, 1.0f, t2.data(), r2.data());
+        orc_unorm16_round(t2.data(), (long long)n);
+        orc_combine_f32(t1.data(), r1.data(), t2.data(), r2.data(), (long long)n);
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++)  // SNORM16 iterates: bit-exact
+            r = compare(vf16.downloadPlane(ocl, c), std::vector<float>(W.begin() + c * n, W.begin() + (c + 1) * n),
+                        0.0f, 0.0f, "16-bit GVF field");
+        // TDF: FP32 bar, then at most one UNORM16 code (1/65535) of rounding
+        if (r.empty()) r = compare(tdf16.downloadPlane(ocl, 0), t2, 1e-4f, 1.6e-5f, "16-bit TDF");
+        if (r.empty()) r = compare(rad16.downloadPlane(ocl, 0), r2, 0.0f, 0.0f, "16-bit radius");
+        return r;
     });
     run("runGVF_consumes_input_and_rounds_iterations", [&] {
         paramList q = p;

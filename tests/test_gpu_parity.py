@@ -160,6 +160,72 @@ def test_gvf_euler_reduced_precision(tsf, orc, storage, bound):
     assert rel < bound
 
 
+# ------------------------------------------------------------------ 16-bit mode
+# The reference's default 16-bit vectors (kernels_no_3d_write.cl:9-21): the
+# field and every GVF iterate are SNORM16; float arithmetic between stores.
+# Parity is bit-exact: the oracle runs the same float operations with the same
+# rounding to codes.
+SN16 = 3
+
+
+@pytest.mark.parametrize("sign,fmax", [(1, 0.2), (-1, 0.1)])
+def test_vector_field_sn16(tsf, orc, sign, fmax):
+    v = phantom((24, 28, 32), 3)
+    F = tsf.Field(v.shape, SN16)
+    tsf.create_vector_field(dev(tsf, v), F, fmax, sign)
+    assert np.array_equal(F.download(), orc.vector_field_sn16(v, fmax, sign))
+
+
+@pytest.mark.parametrize("shape,iters,mu,fuse", [
+    ((48, 48, 48), 100, 0.1, 1), ((48, 48, 48), 100, 0.1, 2), ((64, 64, 64), 250, 0.05, 2),
+    ((20, 24, 36), 7, 0.05, 1), ((37, 23, 65), 6, 0.1, 2), ((5, 13, 130), 5, 0.1, 2), ((4, 4, 4), 3, 0.05, 2),
+    ((70, 6, 4), 2, 0.1, 1), ((130, 20, 125), 10, 0.05, 2), ((16, 17, 18), 0, 0.05, 2)])
+def test_gvf_euler_sn16(tsf, orc, shape, iters, mu, fuse):
+    F = orc.vector_field_sn16(orc.blur(phantom(shape, 4), 0.5, "f32"), 0.2, 1)
+    f = tsf.Field.from_host(F, SN16)
+    out = tsf.Field(shape, SN16)
+    tsf.gvf_euler(f, out, mu, iters, fuse=fuse)
+    got = out.download()
+    exp = orc.gvf_euler_sn16(F, mu, iters)
+    print(f"sn16 gvf {shape} iters={iters} mu={mu} fuse={fuse}: identical {np.mean(got == exp):.6f}, "
+          f"saturated {np.mean(np.abs(exp[:3]) == 1.0):.5f}")
+    assert np.array_equal(got, exp)
+
+
+def test_gvf_euler_sn16_error_vs_fp64(tsf, orc):
+    # mu = 0.05 (stable): the 16-bit iterate's error against the FP64 oracle on
+    # the same (already 16-bit) input field, reported and bounded
+    shape = (48, 48, 48)
+    F = orc.vector_field_sn16(orc.blur(phantom(shape, 6), 0.5, "f32"), 0.2, 1)
+    f = tsf.Field.from_host(F, SN16)
+    out = tsf.Field(shape, SN16)
+    tsf.gvf_euler(f, out, 0.05, 100, fuse=2)
+    got = out.download()
+    ref = orc.gvf_euler(F.astype(np.float64), 0.05, 100, "f64")
+    rel = np.linalg.norm(got[:3] - ref[:3]) / np.linalg.norm(ref[:3])
+    print(f"sn16: L2 rel error vs FP64 {rel:.3e}")
+    assert rel < 2e-3
+
+
+def test_tdf_sn16_field_and_unorm16(tsf, orc):
+    shape = (32, 32, 32)
+    V = orc.gvf_euler_sn16(orc.vector_field_sn16(orc.blur(phantom(shape, 9), 0.5, "f32"), 0.2, 1), 0.05, 40)
+    fld = tsf.Field.from_host(V, SN16)
+    assert np.array_equal(fld.download(), V)
+    t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    tsf.tdf_circle(fld, 1.0, 5.0, 0.5, t, r)
+    et, er = orc.tdf_circle(V, 1.0, 5.0, 0.5, "f32")
+    nbad, rel = close(t.download(), et)
+    assert nbad == 0, rel
+    assert np.array_equal(r.download(), er)
+    # UNORM16 rounding of the TDF buffer: bit-exact on the same values
+    x = np.concatenate([t.download().ravel(), rng(30).uniform(-0.5, 1.5, 10000).astype(np.float32),
+                        np.array([np.nan, np.inf, -np.inf, 0.5, 1.0, 0.0], np.float32)])
+    d = dev(tsf, x)
+    tsf.tdf_round_unorm16(d)
+    assert np.array_equal(d.download(), orc.unorm16_round(x))
+
+
 # ------------------------------------------------------------------ FMG
 @pytest.mark.parametrize("shape,levels", [((32, 32, 32), 2), ((24, 20, 28), 2), ((16, 16, 16), 1)])
 def test_gvf_fmg(tsf, orc, shape, levels):

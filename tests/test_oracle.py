@@ -30,6 +30,14 @@ Pinning status per oracle function:
   candidate_nms ........... constructed cross-section / axial neighbours
   cube_max ................ independent numpy reduction
   do_eigen ................ equals hessian_eigen(normalize) with w = |F|
+  sn16_store/load/round ... codes of anchor values (ties to even, saturation,
+                            NaN), every one of the 65536 codes, numpy
+                            restatement on random values, monotonicity
+  unorm16_round ........... anchor codes, numpy restatement, order preserved
+  vector_field_sn16 ....... equals the pinned float field rounded to codes
+  gvf_euler_sn16 .......... equals the pinned float step with every iterate
+                            rounded (and w = round(|v|)); constant fixed
+                            point; bounded where the float scheme diverges
 """
 import numpy as np
 import pytest
@@ -465,3 +473,104 @@ def test_do_eigen_equals_normalised_hessian(orc):
         assert np.allclose(l, lam[:, z, y, x], atol=1e-12)
         assert np.allclose(e1, vec[0:3, z, y, x], atol=1e-12)
         assert np.allclose(e3, vec[6:9, z, y, x], atol=1e-12)
+
+
+# ------------------------------------------------------------------ 16-bit formats
+# The reference's 16-bit mode (kernels_no_3d_write.cl:9-21): vectors stored as
+# convert_short_sat_rte(v * 32767) and read as max(-1, s / 32767); TDF values
+# stored as convert_ushort_sat_rte(t * 65535) and read as t / 65535.
+def _np_sn16(a):
+    x = np.asarray(a, np.float32) * np.float32(32767)
+    s = np.where(np.isnan(x), 0, np.rint(np.clip(np.nan_to_num(x), -32768, 32767))).astype(np.int16)
+    return s
+
+
+def test_sn16_codes_of_anchor_values(orc):
+    v = np.array([0.0, -0.0, 1.0, -1.0, 0.5, -0.5, 2.0, -2.0, np.inf, -np.inf, np.nan, 1.0 / 32767, 1e-30],
+                 np.float32)
+    exp = np.array([0, 0, 32767, -32767, 16384, -16384, 32767, -32768, 32767, -32768, 0, 1, 0], np.int16)
+    # 0.5 * 32767 = 16383.5 is a tie: round half to even gives 16384
+    assert np.array_equal(orc.sn16_store(v), exp)
+    # the most negative code reads back as -1 (the max(-1, .) of the format)
+    assert orc.sn16_load(np.array([-32768, -32767, 32767, 0], np.int16)).tolist() == [-1.0, -1.0, 1.0, 0.0]
+
+
+def test_sn16_ties_round_to_even(orc):
+    k = np.arange(-32767, 32767)
+    v = ((k + 0.5) / 32767).astype(np.float32)
+    tie = v * np.float32(32767) == (k + 0.5).astype(np.float32)
+    assert tie.sum() > 1000
+    s = orc.sn16_store(v[tie]).astype(np.int64)
+    assert np.all(s % 2 == 0) and np.all(np.abs(s - (k[tie] + 0.5)) == 0.5)
+
+
+def test_sn16_all_codes_and_idempotence(orc):
+    codes = np.arange(-32768, 32768).astype(np.int16)
+    vals = orc.sn16_load(codes)
+    # float32 division is correctly rounded in numpy
+    assert np.array_equal(vals, np.maximum(np.float32(-1), codes.astype(np.float32) / np.float32(32767)))
+    # every code survives a store of its value (except -32768, which reads as -1)
+    back = orc.sn16_store(vals)
+    assert np.array_equal(back[1:], codes[1:]) and back[0] == -32767
+    assert np.array_equal(orc.sn16_round(vals), vals)
+    x = rng(20).uniform(-1.5, 1.5, 100000).astype(np.float32)
+    assert np.array_equal(orc.sn16_store(x), _np_sn16(x))
+    assert np.all(np.diff(orc.sn16_round(np.sort(x))) >= 0)
+
+
+def test_unorm16_anchor_values(orc):
+    v = np.array([0.0, 1.0, 0.5, -0.25, 1.5, np.nan, np.inf, 1.0 / 65535, 0.49 / 65535], np.float32)
+    exp = np.array([0, 65535, 32768, 0, 65535, 0, 65535, 1, 0], np.float64)
+    assert np.array_equal(orc.unorm16_round(v), (exp.astype(np.float32) / np.float32(65535)))
+    x = rng(21).uniform(-0.2, 1.2, 100000).astype(np.float32)
+    c = np.rint(np.clip(x * np.float32(65535), 0, 65535)).astype(np.float32)
+    assert np.array_equal(orc.unorm16_round(x), c / np.float32(65535))
+    # combine and candidates see the same order on rounded values as on codes
+    r = orc.unorm16_round(x)
+    assert np.array_equal(np.argsort(r, kind="stable"), np.argsort(c, kind="stable"))
+
+
+def test_vector_field_sn16_is_rounded_float_field(orc):
+    vol = rng(22).uniform(0, 1, (10, 11, 12)).astype(np.float32)
+    F = orc.vector_field_sn16(vol, 0.2, 1)
+    Ff = orc.vector_field(vol, 0.2, 1, "f32")
+    assert np.array_equal(orc.sn16_store(F), _np_sn16(Ff))
+    assert np.all(F[3] == 1.0)
+
+
+def test_gvf_euler_sn16_rounds_every_iterate(orc):
+    shape = (9, 10, 11)
+    F = orc.sn16_round(rng(23).uniform(-0.8, 0.8, (4,) + shape))
+    F[3] = 1
+    mu = np.float32(0.05)
+    V = F[:3].copy()
+    for it in range(1, 6):
+        V = orc.sn16_round(orc.gvf_step(F[:3], V, mu, "f32"))
+        got = orc.gvf_euler_sn16(F, mu, it)
+        assert np.array_equal(got[:3], V)
+        # w = |v| in float, stored in the format (saturates at 1, 1 where 0)
+        m = np.sqrt((V.astype(np.float32) ** 2).sum(0, dtype=np.float32))
+        assert np.array_equal(got[3], orc.sn16_round(np.where(m > 0, m, 1).astype(np.float32)))
+    # without the per-iterate rounding the iterates differ
+    assert not np.array_equal(orc.gvf_euler(F, mu, 5, "f32")[:3], V)
+
+
+def test_gvf_euler_sn16_fixed_point_and_saturation(orc):
+    shape = (8, 8, 8)
+    c = orc.sn16_round(np.array([0.3, -0.2, 0.9], np.float32))
+    F = np.concatenate([np.broadcast_to(c[:, None, None, None], (3,) + shape), np.ones((1,) + shape)]).astype(
+        np.float32)
+    V = orc.gvf_euler_sn16(F, 0.1, 7)
+    assert np.array_equal(V[:3], F[:3])
+    m = np.sqrt((c.astype(np.float32) ** 2).sum(dtype=np.float32))
+    assert np.all(V[3] == orc.sn16_round(np.array([m], np.float32))[0])
+    # mu = 0.1 with |f| = 1 is outside the explicit scheme's stability range:
+    # float iterates grow without bound, the 16-bit iterates stay within [-1, 1]
+    g = rng(24)
+    d = g.normal(size=(3,) + shape)
+    F2 = np.concatenate([d / np.linalg.norm(d, axis=0), np.ones((1,) + shape)]).astype(np.float32)
+    F2 = orc.sn16_round(F2)
+    f32 = orc.gvf_euler(F2, 0.1, 300, "f32")
+    s16 = orc.gvf_euler_sn16(F2, 0.1, 300)
+    assert np.abs(f32[:3]).max() > 10
+    assert np.abs(s16[:3]).max() <= 1.0 and np.all(np.isfinite(s16))

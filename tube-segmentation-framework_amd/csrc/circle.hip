@@ -267,7 +267,7 @@ using namespace tsf;
 extern "C" {
 
 size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage) {
-    if (storage < 0 || storage > 2) return 0;
+    if (storage < 0 || storage > 3) return 0;
     return (size_t)D(dims).n() * (storage == TSF_STORAGE_F32 ? 16 : 8);
 }
 
@@ -302,6 +302,15 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
                                (uint2*)sc.ptr, n);
             if (norm) launch_tdf<F16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             else launch_tdf<F16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
+            break;
+        }
+        case TSF_STORAGE_SN16: {
+            const uint16_t one = 32767;
+            hipLaunchKernelGGL(k_pack_16, grid1d(n), 256, 0, stream, (const uint16_t*)field->x,
+                               (const uint16_t*)field->y, (const uint16_t*)field->z, (const uint16_t*)field->w, one,
+                               (uint2*)sc.ptr, n);
+            if (norm) launch_tdf<SN16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
+            else launch_tdf<SN16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             break;
         }
         default: {

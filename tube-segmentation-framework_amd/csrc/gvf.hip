@@ -334,6 +334,33 @@ struct RawV<BF16, 2> {
     }
 };
 
+template <>
+struct RawV<SN16, 1> {
+    typedef short T;
+    static __device__ __forceinline__ T load(const short* p, i64 i) { return p[i]; }
+    static __device__ __forceinline__ float get(T r, int, float) { return sn16_to_f(r); }
+};
+template <>
+struct RawV<SN16, 2> {
+    typedef unsigned T;
+    static __device__ __forceinline__ T load(const short* p, i64 i) { return *reinterpret_cast<const unsigned*>(p + i); }
+    static __device__ __forceinline__ float get(T r, int k, float) {
+        return sn16_to_f((short)(k ? (r >> 16) : (r & 0xffffu)));
+    }
+};
+
+// Whether the intermediate iterate of a two-step launch is rounded to the
+// storage format: yes for the reference's SNORM16 format, which stores every
+// iterate; the FP16/BF16 precision forms keep it in float.
+template <class S>
+struct RoundMid {
+    static constexpr bool value = false;
+};
+template <>
+struct RoundMid<SN16> {
+    static constexpr bool value = true;
+};
+
 template <int VEC>
 struct FGeo {
     static constexpr int EW = FBX * VEC;    // E width (columns)
@@ -462,7 +489,8 @@ __device__ __forceinline__ void f2_step(const Fused2<S, VEC>& m, int t, typename
         for (int c = 0; c < 3; c++) {
             const float r = gvf_update(c0[k][c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1],
                                        L0[B][c][ty + 2][lx], L0[B][c][ty][lx], cp[k][c], cm[k][c], f1[k][c], m2, m.mu);
-            v1[S0][k][c] = do1 ? r : 0.0f;
+            const float rm = RoundMid<S>::value ? sn16_to_f(f_to_sn16(r)) : r;
+            v1[S0][k][c] = do1 ? rm : 0.0f;
         }
     }
     // v2(t - 1) at this thread's columns
@@ -735,7 +763,7 @@ using namespace tsf;
 extern "C" {
 
 size_t tsf_gvf_euler_workspace_size(tsf_dims dims, int32_t storage) {
-    if (storage < 0 || storage > 2) return 0;
+    if (storage < 0 || storage > 3) return 0;
     return (size_t)3 * (size_t)D(dims).n() * storage_bytes(storage);
 }
 
@@ -759,6 +787,7 @@ tsf_status tsf_gvf_euler(const tsf_field* f, const tsf_field* out, tsf_dims dims
     switch (out->storage) {
         case TSF_STORAGE_F32: st = run_euler<F32>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
         case TSF_STORAGE_F16: st = run_euler<F16>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
+        case TSF_STORAGE_SN16: st = run_euler<SN16>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
         default: st = run_euler<BF16>(f, out, d, mu, iterations, fuse, sc.ptr, stream); break;
     }
     if (st == TSF_OK && sc.owned) {

@@ -94,6 +94,23 @@ __global__ void k_combine(const float* __restrict__ ts, const float* __restrict_
     }
 }
 
+// The reference's 16-bit TDF format (kernels_no_3d_write.cl:19-20, 1570):
+// stored as convert_ushort_sat_rte(t * 65535), read as c / 65535.  The
+// quotient uses one fma correction of c * (1/65535), which equals the
+// correctly rounded c / 65535 for every code.
+__device__ __forceinline__ float unorm16_round(float t) {
+    const float x = t * 65535.0f;
+    const float c = x == x ? rintf(fminf(fmaxf(x, 0.0f), 65535.0f)) : 0.0f;
+    const float r = 1.0f / 65535.0f;
+    const float q0 = c * r;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 65535.0f, c), r, q0);
+}
+
+__global__ void k_unorm16_round(float* __restrict__ t, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        t[i] = unorm16_round(t[i]);
+}
+
 __global__ void k_candidates(const float* __restrict__ tdf, uint8_t* __restrict__ mask, i64 n, float thr) {
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
         mask[i] = tdf[i] < thr ? 0 : 1;
@@ -201,6 +218,10 @@ tsf_status tsf_create_vector_field(const float* volume, const tsf_field* out, ts
             hipLaunchKernelGGL(k_vector_field<F16>, grid, dim3(64, 4), 0, stream, volume, (__half*)out->x,
                                (__half*)out->y, (__half*)out->z, (__half*)out->w, d, fmax, sg, out->scale);
             break;
+        case TSF_STORAGE_SN16:
+            hipLaunchKernelGGL(k_vector_field<SN16>, grid, dim3(64, 4), 0, stream, volume, (short*)out->x,
+                               (short*)out->y, (short*)out->z, (short*)out->w, d, fmax, sg, out->scale);
+            break;
         default:
             hipLaunchKernelGGL(k_vector_field<BF16>, grid, dim3(64, 4), 0, stream, volume,
                                (__hip_bfloat16*)out->x, (__hip_bfloat16*)out->y, (__hip_bfloat16*)out->z,
@@ -215,6 +236,15 @@ tsf_status tsf_combine(const float* ts, const float* rs, float* tl, float* rl, i
     if (count < 0) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: negative count", W);
     if (count == 0) return TSF_OK;
     hipLaunchKernelGGL(k_combine, grid1d(count, 256), 256, 0, stream, ts, rs, tl, rl, (i64)count);
+    return check_launch(W);
+}
+
+tsf_status tsf_tdf_round_unorm16(float* tdf, int64_t count, hipStream_t stream) {
+    const char* W = "tsf_tdf_round_unorm16";
+    if (!tdf) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL buffer", W);
+    if (count < 0) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: negative count", W);
+    if (count == 0) return TSF_OK;
+    hipLaunchKernelGGL(k_unorm16_round, grid1d(count, 256), 256, 0, stream, tdf, (i64)count);
     return check_launch(W);
 }
 

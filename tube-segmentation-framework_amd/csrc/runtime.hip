@@ -40,12 +40,12 @@ tsf_status check_field(const tsf_field* f, bool need_w, const char* what) {
     if (!f) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field descriptor is NULL", what);
     if (!f->x || !f->y || !f->z) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field plane pointer is NULL", what);
     if (need_w && !f->w) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field plane w is required", what);
-    if (f->storage < TSF_STORAGE_F32 || f->storage > TSF_STORAGE_BF16)
+    if (f->storage < TSF_STORAGE_F32 || f->storage > TSF_STORAGE_SN16)
         return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: unknown storage type %d", what, f->storage);
     if (!(f->scale > 0.0f) || f->scale > 1e30f)
         return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field scale must be positive", what);
-    if (f->storage == TSF_STORAGE_F32 && f->scale != 1.0f)
-        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 fields must have scale 1", what);
+    if ((f->storage == TSF_STORAGE_F32 || f->storage == TSF_STORAGE_SN16) && f->scale != 1.0f)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: F32 and SN16 fields must have scale 1", what);
     int ex;
     if (frexpf(f->scale, &ex) != 0.5f)
         return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field scale must be a power of two", what);

@@ -160,6 +160,13 @@ tsf_status tsf_hessian_eigen(const tsf_field* field, tsf_dims dims, int32_t norm
             else hipLaunchKernelGGL((k_hessian_eigen<F16, false>), grid, block, 0, stream, F, d, lambda, vectors);
             break;
         }
+        case TSF_STORAGE_SN16: {
+            typedef short T;
+            Field<SN16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
+            if (normalize) hipLaunchKernelGGL((k_hessian_eigen<SN16, true>), grid, block, 0, stream, F, d, lambda, vectors);
+            else hipLaunchKernelGGL((k_hessian_eigen<SN16, false>), grid, block, 0, stream, F, d, lambda, vectors);
+            break;
+        }
         default: {
             typedef __hip_bfloat16 T;
             Field<BF16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
@@ -208,6 +215,12 @@ tsf_status tsf_candidate_nms(const float* tdf, const float* radius, const tsf_fi
             typedef __half T;
             Field<F16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
             hipLaunchKernelGGL(k_candidate_nms<F16>, g, 256, 0, stream, radius, F, d, positions, (i64)count, maxd_cap, out);
+            break;
+        }
+        case TSF_STORAGE_SN16: {
+            typedef short T;
+            Field<SN16> F{(const T*)field->x, (const T*)field->y, (const T*)field->z, (const T*)field->w, inv};
+            hipLaunchKernelGGL(k_candidate_nms<SN16>, g, 256, 0, stream, radius, F, d, positions, (i64)count, maxd_cap, out);
             break;
         }
         default: {

@@ -37,6 +37,16 @@ struct Texel<BF16> {
     }
 };
 
+template <>
+struct Texel<SN16> {
+    typedef uint2 P;
+    static __device__ __forceinline__ float v(unsigned b) { return sn16_to_f((short)(b & 0xffffu)); }
+    static __device__ __forceinline__ float4 unpack(P p, float) {
+        return make_float4(v(p.x), v(p.x >> 16), v(p.y), v(p.y >> 16));
+    }
+    static __device__ __forceinline__ float4 raw(P p) { return unpack(p, 1.0f); }
+};
+
 // Texel sources of the sampling functions below: row(d, y, z) locates row
 // (y, z) and at(row, x) is the texel at (x, y, z), for coordinates already
 // clamped into the volume.

@@ -94,6 +94,38 @@ struct BF16 {
     }
 };
 
+// The reference's 16-bit vector format (kernels_no_3d_write.cl:9-21):
+// stored as convert_short_sat_rte(v * 32767), read as max(-1, s / 32767).
+// The quotient is computed as q0 = s * r, e = fma(-q0, 32767, s), q0 +
+// e * r (one fma) with r = 1/32767 rounded: for every one of the 65536
+// codes this equals the correctly rounded s / 32767.
+__device__ __forceinline__ float sn16_to_f(int s) {
+    const float sf = (float)s, r = 1.0f / 32767.0f;
+    const float q0 = sf * r;
+    const float e = __builtin_fmaf(-q0, 32767.0f, sf);
+    return fmaxf(-1.0f, __builtin_fmaf(e, r, q0));
+}
+__device__ __forceinline__ short f_to_sn16(float v) {
+    const float x = v * 32767.0f;
+    // round to nearest even, saturate; NaN converts to 0
+    const float r = rintf(fminf(fmaxf(x, -32768.0f), 32767.0f));
+    return (short)(x == x ? (int)r : 0);
+}
+struct SN16 {
+    typedef short T;
+    static __device__ __forceinline__ float ld(const T* p, i64 i, float) { return sn16_to_f(p[i]); }
+    static __device__ __forceinline__ void st(T* p, i64 i, float v, float) { p[i] = f_to_sn16(v); }
+    static __device__ __forceinline__ void ld2(const T* p, i64 i, float, float& a, float& b) {
+        const unsigned u = *reinterpret_cast<const unsigned*>(p + i);
+        a = sn16_to_f((short)(u & 0xffffu));
+        b = sn16_to_f((short)(u >> 16));
+    }
+    static __device__ __forceinline__ void st2(T* p, i64 i, float a, float b, float) {
+        const unsigned u = (unsigned)(unsigned short)f_to_sn16(a) | ((unsigned)(unsigned short)f_to_sn16(b) << 16);
+        *reinterpret_cast<unsigned*>(p + i) = u;
+    }
+};
+
 // Host-side error plumbing.  Every entry point returns a tsf_status and leaves
 // a message retrievable with tsf_last_error().
 tsf_status set_error(tsf_status st, const char* fmt, ...);

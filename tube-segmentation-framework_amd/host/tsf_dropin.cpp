@@ -101,7 +101,7 @@ paramList initParameters() {
     B("use-spline-tdf", false, "tube-detection-filter");
     B("use-fmg-gvf", false, "gradient-vector-flow");
     B("3d_write", true, "advanced");
-    S("vector-storage", "auto", {"auto", "f32", "f16", "bf16"}, "advanced");
+    S("vector-storage", "auto", {"auto", "f32", "f16", "bf16", "sn16"}, "advanced");
     N("gvf-fuse", 1, 1, 16, 1, "gradient-vector-flow");
     return p;
 }
@@ -216,6 +216,15 @@ tsf_field Image3D::field() const {
     return tsf_field{plane(0), plane(1), plane(2), plane(3), (int32_t)st, sc};
 }
 
+// The reference's 16-bit vector format (kernels_no_3d_write.cl:9-21).
+static int16_t f32_to_sn16(float v) {
+    const float x = v * 32767.0f;
+    if (x != x) return 0;
+    const float r = std::nearbyint(x);
+    return (int16_t)(r > 32767.0f ? 32767 : (r < -32768.0f ? -32768 : (int)r));
+}
+static float sn16_to_f32(int16_t s) { return std::max(-1.0f, (float)s / 32767.0f); }
+
 static uint16_t f32_to_f16(float f) {
     // IEEE half, round to nearest even (host side conversion for uploads)
     uint32_t u;
@@ -282,7 +291,10 @@ void Image3D::uploadPlane(OpenCL& ocl, int c, const float* host) {
         return;
     }
     std::vector<uint16_t> t(n);
-    for (size_t i = 0; i < n; i++) t[i] = st == TSF_STORAGE_F16 ? f32_to_f16(host[i] * sc) : f32_to_bf16(host[i] * sc);
+    for (size_t i = 0; i < n; i++)
+        t[i] = st == TSF_STORAGE_F16    ? f32_to_f16(host[i] * sc)
+               : st == TSF_STORAGE_SN16 ? (uint16_t)f32_to_sn16(host[i])
+                                        : f32_to_bf16(host[i] * sc);
     planes.at(c).upload(ocl, t.data(), n * 2);
 }
 std::vector<float> Image3D::downloadPlane(OpenCL& ocl, int c) const {
@@ -294,7 +306,9 @@ std::vector<float> Image3D::downloadPlane(OpenCL& ocl, int c) const {
     }
     std::vector<uint16_t> t(n);
     planes.at(c).download(ocl, t.data(), n * 2);
-    for (size_t i = 0; i < n; i++) out[i] = (st == TSF_STORAGE_F16 ? f16_to_f32(t[i]) : bf16_to_f32(t[i])) / sc;
+    for (size_t i = 0; i < n; i++)
+        out[i] = st == TSF_STORAGE_SN16 ? sn16_to_f32((int16_t)t[i])
+                                        : (st == TSF_STORAGE_F16 ? f16_to_f32(t[i]) : bf16_to_f32(t[i])) / sc;
     return out;
 }
 
@@ -303,8 +317,10 @@ tsf_storage vectorStorage(const paramList& p) {
     if (s == "f32") return TSF_STORAGE_F32;
     if (s == "f16") return TSF_STORAGE_F16;
     if (s == "bf16") return TSF_STORAGE_BF16;
-    // "auto": the reference's 16-bit mode unless 32-bit vectors are forced
-    return getParamBool(p, "16bit-vectors") && !getParamBool(p, "32bit-vectors") ? TSF_STORAGE_F16 : TSF_STORAGE_F32;
+    if (s == "sn16") return TSF_STORAGE_SN16;
+    // "auto": the reference's own 16-bit format (SNORM16) unless 32-bit
+    // vectors are forced
+    return getParamBool(p, "16bit-vectors") && !getParamBool(p, "32bit-vectors") ? TSF_STORAGE_SN16 : TSF_STORAGE_F32;
 }
 float vectorScale(tsf_storage s) { return s == TSF_STORAGE_F16 ? 16384.0f : 1.0f; }
 
@@ -358,17 +374,25 @@ Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3
     return out;
 }
 
+// The TDF of one field.  With the reference's 16-bit vectors the TDF buffers
+// are 16-bit too (TDF_TYPE ushort, kernels_no_3d_write.cl:19-21): an SN16
+// field gets its TDF rounded to that format.  Synchronises (the workspace is
+// released on return).
+static void circleTDF(OpenCL& ocl, const Image3D& field, float rmin, float rmax, float rstep, float* T, float* R) {
+    tsf_field f = field.field();
+    Buffer ws(tsf_tdf_circle_workspace_size(field.dims(), field.storage()));
+    check(tsf_tdf_circle(&f, field.dims(), rmin, rmax, rstep, T, R, ws.get(), ws.size(), ocl.queue));
+    if (field.storage() == TSF_STORAGE_SN16) check(tsf_tdf_round_unorm16(T, (int64_t)voxels(field.size()), ocl.queue));
+    ocl.finish();
+}
+
 void runCircleFittingTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* TDF, Buffer* radius, float radiusMin,
                          float radiusMax, float radiusStep) {
     if (!vectorField || !TDF || !radius) fail(TSF_ERR_INVALID_ARGUMENT, "runCircleFittingTDF: NULL argument");
     const size_t n = voxels(size);
     if (TDF->size() < n * 4 || radius->size() < n * 4)
         fail(TSF_ERR_INVALID_ARGUMENT, "runCircleFittingTDF: output buffers too small");
-    tsf_field f = vectorField->field();
-    Buffer ws(tsf_tdf_circle_workspace_size(vectorField->dims(), vectorField->storage()));
-    check(tsf_tdf_circle(&f, vectorField->dims(), radiusMin, radiusMax, radiusStep, (float*)TDF->get(),
-                         (float*)radius->get(), ws.get(), ws.size(), ocl.queue));
-    ocl.finish();  // the workspace is released on return
+    circleTDF(ocl, *vectorField, radiusMin, radiusMax, radiusStep, (float*)TDF->get(), (float*)radius->get());
 }
 
 static Image3D blurOrSame(OpenCL& ocl, Image3D& data, float sigma) {
@@ -401,11 +425,7 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
         Image3D Fs = vectorFieldOf(ocl, blurred, p);
         tdfSmall = Image3D(size, 1);
         radSmall = Image3D(size, 1);
-        Buffer ws(tsf_tdf_circle_workspace_size(Fs.dims(), Fs.storage()));
-        tsf_field f = Fs.field();
-        check(tsf_tdf_circle(&f, Fs.dims(), rMin, 3.0f, 0.5f, (float*)tdfSmall.plane(0), (float*)radSmall.plane(0),
-                             ws.get(), ws.size(), ocl.queue));
-        ocl.finish();  // temporaries of this branch are released at its end
+        circleTDF(ocl, Fs, rMin, 3.0f, 0.5f, (float*)tdfSmall.plane(0), (float*)radSmall.plane(0));
         if (rMax < 2.5f) {
             TDF = tdfSmall;
             radiusImage = radSmall;
@@ -423,10 +443,8 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
     else vectorField = runGVF(ocl, &F, p, size, false);
     TDF = Image3D(size, 1);
     radiusImage = Image3D(size, 1);
-    Buffer ws(tsf_tdf_circle_workspace_size(vectorField.dims(), vectorField.storage()));
-    tsf_field f = vectorField.field();
-    check(tsf_tdf_circle(&f, vectorField.dims(), std::max(2.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
-                         (float*)radiusImage.plane(0), ws.get(), ws.size(), ocl.queue));
+    circleTDF(ocl, vectorField, std::max(2.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
+              (float*)radiusImage.plane(0));
     if (rMin < 2.5f)
         check(tsf_combine((const float*)tdfSmall.plane(0), (const float*)radSmall.plane(0), (float*)TDF.plane(0),
                           (float*)radiusImage.plane(0), (int64_t)n, ocl.queue));

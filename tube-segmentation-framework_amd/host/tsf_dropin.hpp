@@ -101,8 +101,9 @@ struct paramList {
 
 // Defaults of every parameter the volume-filter path reads (the reference's
 // parameters/parameters file), plus "3d_write" (set by run() from the device
-// in the reference) and "vector-storage" (f32 | f16 | bf16, new: the storage
-// of the GVF field; "auto" follows 16bit-vectors).
+// in the reference) and "vector-storage" (f32 | f16 | bf16 | sn16, new: the
+// storage of the GVF field; "auto" follows 16bit-vectors and 32bit-vectors,
+// choosing sn16, the reference's 16-bit format, or f32).
 paramList initParameters();
 // Reads a parameters file in the reference's format (name type default ...).
 paramList initParameters(const std::string& parameter_dir);

@@ -7,7 +7,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(os.path.dirname(_HERE), "lib", "libtsf_hip.so")
 
-STORAGE_F32, STORAGE_F16, STORAGE_BF16 = 0, 1, 2
+STORAGE_F32, STORAGE_F16, STORAGE_BF16, STORAGE_SN16 = 0, 1, 2, 3
 _STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "DIMENSIONS", 3: "HIP", 4: "OUT_OF_MEMORY", 5: "CAPACITY",
            6: "RANGE"}
 
@@ -63,6 +63,7 @@ _SIGS = {
     "tsf_tdf_circle_workspace_size": ([Dims, _i32], _sz),
     "tsf_combine": ([_vp, _vp, _vp, _vp, _i64, _vp], _i32),
     "tsf_candidates": ([_vp, _vp, _i64, _f32, _vp], _i32),
+    "tsf_tdf_round_unorm16": ([_vp, _i64, _vp], _i32),
     "tsf_hp_compact": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp, _sz, _vp], _i32),
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
     "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
@@ -124,7 +125,7 @@ def mem_info():
 
 
 def storage_dtype(storage):
-    return {STORAGE_F32: np.float32, STORAGE_F16: np.float16, STORAGE_BF16: np.uint16}[storage]
+    return {STORAGE_F32: np.float32, STORAGE_F16: np.float16, STORAGE_BF16: np.uint16, STORAGE_SN16: np.int16}[storage]
 
 
 def _to_bf16_bits(a):
@@ -269,7 +270,21 @@ class Field:
         return np.stack([_decode(p.download(stream), self.storage, self.scale) for p in self.planes])
 
 
+def sn16_encode(a):
+    """The reference's 16-bit store: convert_short_sat_rte(v * 32767)."""
+    x = np.asarray(a, np.float32) * np.float32(32767.0)
+    r = np.rint(np.nan_to_num(x, nan=0.0))
+    return np.clip(r, -32768, 32767).astype(np.int16)
+
+
+def sn16_decode(s):
+    """The reference's 16-bit load: max(-1, s / 32767)."""
+    return np.maximum(np.float32(-1.0), np.asarray(s, np.int16).astype(np.float32) / np.float32(32767.0))
+
+
 def _encode(a, storage, scale):
+    if storage == STORAGE_SN16:
+        return sn16_encode(a)
     a = np.asarray(a, np.float32) * np.float32(scale)
     if storage == STORAGE_F32:
         return a
@@ -279,6 +294,8 @@ def _encode(a, storage, scale):
 
 
 def _decode(a, storage, scale):
+    if storage == STORAGE_SN16:
+        return sn16_decode(a)
     if storage == STORAGE_F32:
         v = a.astype(np.float32)
     elif storage == STORAGE_F16:
@@ -374,6 +391,11 @@ def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=Non
 def combine(ts, rs, tl, rl, stream=None):
     n = int(np.prod(ts.shape))
     _check(lib().tsf_combine(ts.ptr, rs.ptr, tl.ptr, rl.ptr, n, _sp(stream)))
+
+
+def tdf_round_unorm16(tdf, stream=None):
+    """Rounds TDF values in place to the reference's 16-bit TDF format."""
+    _check(lib().tsf_tdf_round_unorm16(tdf.ptr, int(np.prod(tdf.shape)), _sp(stream)))
 
 
 def candidates(tdf, mask, threshold, stream=None):
