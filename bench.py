@@ -50,7 +50,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--iters", type=int, default=400)
     ap.add_argument("--mu", type=float, default=0.05)
-    ap.add_argument("--storage", choices=["f16", "bf16", "f32"], default="f16")
+    ap.add_argument("--storage", choices=["f16", "bf16", "sn16", "f32"], default="f16",
+                    help="vector-field storage; sn16 is the reference's own 16-bit format")
     ap.add_argument("--fuse", type=int, default=2, help="GVF steps per launch (1 or 2)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
@@ -72,7 +73,8 @@ def main():
 
     if tsf.device_count() < 1:
         raise RuntimeError("no GPU visible")
-    S = {"f32": tsf.STORAGE_F32, "f16": tsf.STORAGE_F16, "bf16": tsf.STORAGE_BF16}[args.storage]
+    S = {"f32": tsf.STORAGE_F32, "f16": tsf.STORAGE_F16, "bf16": tsf.STORAGE_BF16,
+         "sn16": tsf.STORAGE_SN16}[args.storage]
     scale = 2.0 ** 14 if S == tsf.STORAGE_F16 else 1.0
     n = args.size
     shape = (n, n, n)
@@ -205,8 +207,9 @@ def main():
     # rocprofv3 counter passes recorded in profiles/ (null when not measured
     # for this storage type)
     traffic = None
-    gk = "k_gvf_fused2<{}, 1, false>" if args.fuse >= 2 else "k_gvf_march<{}, 2, false>"
-    kname = {"f16": gk.format("F16"), "f32": gk.format("F32")}.get(args.storage)
+    gk = "k_gvf_fused2<{}, false>" if args.fuse >= 2 else "k_gvf_march<{}, 2, false>"
+    kname = {"f16": gk.format("F16"), "f32": gk.format("F32"), "sn16": gk.format("SN16"),
+             "bf16": gk.format("BF16")}.get(args.storage)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_circle_forward_512cubed.json")
     if kname and n == 512 and os.path.exists(tpath):
         with open(tpath) as fh:
@@ -215,7 +218,7 @@ def main():
             traffic = round(rec["traffic_bytes"] / 1e9, 4)
 
     out = {
-        "metric": "GVF Gvoxel-iter/s (FP16-stored field, 512^3, mu 0.05, 400 iterations)",
+        "metric": f"GVF Gvoxel-iter/s ({args.storage.upper()}-stored field, {n}^3, mu {args.mu}, {args.iters} iterations)",
         "value": round(gvox, 3),
         "unit": "Gvoxel-iter/s",
         "n_gpus": 1,
@@ -225,7 +228,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f16": "f16-storage/f32-arith", "bf16": "bf16-storage/f32-arith", "f32": "f32"}[args.storage],
+        "dtype": {"f16": "f16-storage/f32-arith", "bf16": "bf16-storage/f32-arith", "sn16": "snorm16-storage/f32-arith",
+                  "f32": "f32"}[args.storage],
         "data": "synthetic (seeded 512^3 thin-vessel MRA-like tree, Rician noise)",
         "config": {"workload": f"circle-fitting forward at {n}^3: blur, vector field, GVF Euler "
                                f"{args.iters} it mu {args.mu} ({args.storage} storage), TDF small 0.5-3 + large "

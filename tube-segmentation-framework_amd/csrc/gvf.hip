@@ -20,6 +20,9 @@
 // it to plane 0 (q == 2) or plane Z-1 (q == Z-3); boundary columns in x/y copy
 // the result of their mirror column through LDS.  Tiles are placed so that a
 // boundary column and its mirror column always share a tile.
+#include <algorithm>
+#include <type_traits>
+
 #include "tsf_common.hpp"
 
 namespace tsf {
@@ -266,86 +269,69 @@ __global__ void __launch_bounds__(64 * GTY) k_gvf_march(Planes3<S> f, Planes3<S>
 
 // ---------------------------------------------------------------- two steps per launch
 // k_gvf_fused2 runs two Jacobi steps per launch, so v and f cross HBM once
-// per two iterations.  A workgroup of FBX x FBY threads (each VEC adjacent
-// columns) owns an extended column tile E of (FBX*VEC) x FBY columns; it
-// computes v1 = step(v0) on E and v2 = step(v1) on the inner tile O (E less
-// VEC columns at each x edge and one row at each y edge), marching z.  Per
-// step t (one barrier): LDS <- v0 plane t (E plus a one-column ring) and v1
-// plane t-1; v1(t) from LDS x/y neighbours and the thread's register ring
-// of v0 planes; v2(t-1) from LDS v1 plane t-1 and the register ring of v1
-// planes t-2..t.  Loads run three planes ahead into rings of raw stored
-// values that are converted at the top of the step that uses them.  v1 is
-// kept in float.
-// Mirror rule: v1 at a boundary column / plane equals v1 at its mirror
-// (0 -> 2, N-1 -> N-3), so v1 reads there are redirected; v2 computed at q
-// is also written to every w with mirror(w) = q.  Tiles are placed so a
-// boundary column and its mirror lie in the same O (the last tile along x and
-// y is shifted to end at the volume's edge).
+// per two iterations.  A workgroup of FBX x FBY threads (one wave per row)
+// covers FBX x FBY columns, x starting two columns left of its output tile:
+// lanes 1..FBX-2 of every row form the tile E where v1 = step(v0) is
+// computed, lanes 2..FBX-3 of rows 1..FBY-2 the output tile O (FOW x FOH
+// columns) where v2 = step(v1) is computed, and lanes 0 and FBX-1 only supply
+// v0 as the x neighbours of E.  The waves of rows 0 and 1 also load the rows
+// just above and below.  Every load is one whole row of the wave, so no load
+// serves a lone column.  The workgroup marches a chunk of FZC planes in z;
+// per step t (one barrier): LDS <- v0 plane t (its rows plus the two ring
+// rows) and v1 plane t-1; v1(t) from LDS x/y neighbours and the thread's
+// register ring of v0 planes; v2(t-1) from LDS v1 plane t-1 and the register
+// ring of v1 planes t-2..t.  Loads run three planes ahead into rings of raw
+// stored values (buffer loads: a scalar plane offset plus one constant
+// per-thread offset) that are converted at the top of the step that uses
+// them.  The arithmetic runs on stored numbers (value * unit, unit a power of
+// two), so it rounds exactly as on the values; v1 is kept in float.
+// Mirror rule: v1 at a boundary column / row / plane equals v1 at its mirror
+// (0 -> 2, N-1 -> N-3), so v1 reads there are redirected.  v2 is stored at
+// interior voxels only; k_gvf_faces then writes every boundary voxel w from
+// mirror(w).  Tiles are placed so a boundary column and its mirror lie in the
+// same O (the last tile along x and y is shifted to end at the volume's edge).
 constexpr int FBX = 64, FBY = 8;
+constexpr int FOW = FBX - 4, FOH = FBY - 2;
 constexpr int FZC = 64;  // output planes per z chunk, a multiple of 4
 
-// raw stored values of VEC consecutive elements; get() converts element k
-template <class S, int VEC>
-struct RawV;
+// The stored number of one element: getu() reads it as a float in storage
+// units, put() stores a float given in units, bload() is a buffer load.
+template <class S>
+struct Raw;
 template <>
-struct RawV<F32, 1> {
+struct Raw<F32> {
     typedef float T;
-    static __device__ __forceinline__ T load(const float* p, i64 i) { return p[i]; }
-    static __device__ __forceinline__ float get(T r, int, float) { return r; }
+    static __device__ __forceinline__ float getu(T r) { return r; }
+    static __device__ __forceinline__ T put(float u) { return u; }
+    static __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t b, unsigned v, unsigned s) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(b, (int)v, (int)s, 0));
+    }
 };
 template <>
-struct RawV<F32, 2> {
-    typedef float2 T;
-    static __device__ __forceinline__ T load(const float* p, i64 i) { return *reinterpret_cast<const float2*>(p + i); }
-    static __device__ __forceinline__ float get(T r, int k, float) { return k ? r.y : r.x; }
-};
-template <>
-struct RawV<F16, 1> {
+struct Raw<F16> {
     typedef unsigned short T;
-    static __device__ __forceinline__ T load(const __half* p, i64 i) {
-        return reinterpret_cast<const unsigned short*>(p)[i];
-    }
-    static __device__ __forceinline__ float get(T r, int, float inv) { return __half2float(__ushort_as_half(r)) * inv; }
-};
-template <>
-struct RawV<F16, 2> {
-    typedef unsigned T;
-    static __device__ __forceinline__ T load(const __half* p, i64 i) { return *reinterpret_cast<const unsigned*>(p + i); }
-    static __device__ __forceinline__ float get(T r, int k, float inv) {
-        return __half2float(__ushort_as_half((unsigned short)(k ? (r >> 16) : (r & 0xffffu)))) * inv;
+    static __device__ __forceinline__ float getu(T r) { return __half2float(__ushort_as_half(r)); }
+    static __device__ __forceinline__ T put(float u) { return __half_as_ushort(__float2half_rn(u)); }
+    static __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t b, unsigned v, unsigned s) {
+        return __builtin_amdgcn_raw_buffer_load_b16(b, (int)v, (int)s, 0);
     }
 };
 template <>
-struct RawV<BF16, 1> {
+struct Raw<BF16> {
     typedef unsigned short T;
-    static __device__ __forceinline__ T load(const __hip_bfloat16* p, i64 i) {
-        return reinterpret_cast<const unsigned short*>(p)[i];
-    }
-    static __device__ __forceinline__ float get(T r, int, float inv) { return __uint_as_float((unsigned)r << 16) * inv; }
-};
-template <>
-struct RawV<BF16, 2> {
-    typedef unsigned T;
-    static __device__ __forceinline__ T load(const __hip_bfloat16* p, i64 i) {
-        return *reinterpret_cast<const unsigned*>(p + i);
-    }
-    static __device__ __forceinline__ float get(T r, int k, float inv) {
-        return __uint_as_float(k ? (r & 0xffff0000u) : (r << 16)) * inv;
+    static __device__ __forceinline__ float getu(T r) { return __uint_as_float((unsigned)r << 16); }
+    static __device__ __forceinline__ T put(float u) { return __bfloat16_as_ushort(__float2bfloat16(u)); }
+    static __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t b, unsigned v, unsigned s) {
+        return __builtin_amdgcn_raw_buffer_load_b16(b, (int)v, (int)s, 0);
     }
 };
-
 template <>
-struct RawV<SN16, 1> {
+struct Raw<SN16> {
     typedef short T;
-    static __device__ __forceinline__ T load(const short* p, i64 i) { return p[i]; }
-    static __device__ __forceinline__ float get(T r, int, float) { return sn16_to_f(r); }
-};
-template <>
-struct RawV<SN16, 2> {
-    typedef unsigned T;
-    static __device__ __forceinline__ T load(const short* p, i64 i) { return *reinterpret_cast<const unsigned*>(p + i); }
-    static __device__ __forceinline__ float get(T r, int k, float) {
-        return sn16_to_f((short)(k ? (r >> 16) : (r & 0xffffu)));
+    static __device__ __forceinline__ float getu(T r) { return sn16u_to_f(r); }
+    static __device__ __forceinline__ T put(float u) { return f_to_sn16u(u); }
+    static __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t b, unsigned v, unsigned s) {
+        return (short)__builtin_amdgcn_raw_buffer_load_b16(b, (int)v, (int)s, 0);
     }
 };
 
@@ -361,295 +347,314 @@ struct RoundMid<SN16> {
     static constexpr bool value = true;
 };
 
-template <int VEC>
-struct FGeo {
-    static constexpr int EW = FBX * VEC;    // E width (columns)
-    static constexpr int RX = VEC;          // E columns beyond O at each x edge (keeps pairs aligned)
-    static constexpr int OW = EW - 2 * RX;  // O width
-    static constexpr int OH = FBY - 2;      // O height
-    static constexpr int LW = EW + 2 * VEC; // LDS row of v0: own columns at VEC + x, ring at VEC-1 and VEC+EW
+// Whether values in storage units differ from the values (m2 = |f|^2 then
+// needs the factor 1/unit^2): all but F32.
+template <class S>
+struct UnitScaled {
+    static constexpr bool value = true;
+};
+template <>
+struct UnitScaled<F32> {
+    static constexpr bool value = false;
 };
 
-template <class S, int VEC>
+template <class S>
 struct Fused2 {
     Planes3<S> f, vin;
     Out4<S> vout;
     Dims d;
-    float mu, inv_f, inv_v, sc_v;
-    int lane, ty, ex, ey, cy, qs, qe;
-    int cx[VEC];
-    bool act, icol[VEC], inO, hrow, hcol, hok, hsel;
-    i64 P, col, hidx;
-    int lxp[VEC], lxm[VEC], lyp, lym;  // LDS coordinates of v1 neighbours (mirror-redirected)
-    int xw1[VEC], xw2[VEC], yw1, yw2;  // mirror images that receive this column's v2 (-1 = none)
-    int hly, hlx;
+    float mu, m2s, unit;
+    int lane, ty, cy, qs, qe;
+    bool icol, inO, hrow;
+    i64 P;
+    unsigned coff, hoff;  // byte offsets within a plane: this thread's column, its ring-row cell
+    unsigned pb;          // bytes per plane
+    int zb;               // first plane of the chunk's buffer resources
+    __amdgpu_buffer_rsrc_t rv[3], rf[3];  // v0 and f planes zb.. of this chunk
+    int lxp, lxm, lyp, lym;  // LDS coordinates of v1 neighbours (mirror-redirected)
+    int hly;                 // LDS row of this thread's ring-row cell
 };
+
+// Plane z of p (z uniform, so the address stays scalar).
+template <class T>
+__device__ __forceinline__ T* plane_of(T* p, int z, i64 P) {
+    return p + (i64)z * P;
+}
 
 __device__ __forceinline__ int mirror_of(int p, int N) { return p == 0 ? 2 : (p == N - 1 ? N - 3 : p); }
 
-template <class S, int VEC, bool FINISH>
-__device__ __forceinline__ void f2_store(const Fused2<S, VEC>& m, int p, const float (&r)[VEC][3],
-                                         const bool (&ok)[VEC]) {
-    float wv[VEC];
+// Stores v2 at this thread's column of plane p (an interior voxel).
+template <class S, bool FINISH>
+__device__ __forceinline__ void f2_store(const Fused2<S>& m, int p, const float (&r)[3]) {
+    typedef Raw<S> R;
+    typename R::T q[4];
 #pragma unroll
-    for (int k = 0; k < VEC; k++) {
-        wv[k] = 1.0f;
-        if (FINISH) {
-            // magnitude of the stored (rounded) components
-            typename S::T t0, t1, t2;
-            S::st(&t0, 0, r[k][0], m.sc_v);
-            S::st(&t1, 0, r[k][1], m.sc_v);
-            S::st(&t2, 0, r[k][2], m.sc_v);
-            const float ax = S::ld(&t0, 0, m.inv_v), ay = S::ld(&t1, 0, m.inv_v), az = S::ld(&t2, 0, m.inv_v);
-            const float l = sqrtf(ax * ax + ay * ay + az * az);
-            wv[k] = l > 0.0f ? l : 1.0f;
-        }
+    for (int c = 0; c < 3; c++) q[c] = R::put(r[c]);
+    if (FINISH) {
+        // magnitude of the stored (rounded) components, 1 (in units) where 0
+        const float ax = R::getu(q[0]), ay = R::getu(q[1]), az = R::getu(q[2]);
+        const float l = sqrtf(ax * ax + ay * ay + az * az);
+        q[3] = R::put(l > 0.0f ? l : m.unit);
     }
-    typename S::T* OP[3] = {m.vout.x, m.vout.y, m.vout.z};
-    // this position and its mirror images (-1 = none); fully unrolled with
-    // constant indices so the candidates stay in registers
-    const int zw[3] = {p, p == 2 ? 0 : -1, p == m.d.Z - 3 ? m.d.Z - 1 : -1};
-    const int yw[3] = {m.cy, m.yw1, m.yw2};
+    typename S::T* OP[4] = {m.vout.x, m.vout.y, m.vout.z, m.vout.w};
 #pragma unroll
-    for (int k = 0; k < VEC; k++) {
-        if (!ok[k]) continue;
-        const int xw[3] = {m.cx[k], m.xw1[k], m.xw2[k]};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (zw[a] < 0) continue;
-#pragma unroll
-            for (int bb = 0; bb < 3; bb++) {
-                if (yw[bb] < 0) continue;
-#pragma unroll
-                for (int cc = 0; cc < 3; cc++) {
-                    if (xw[cc] < 0) continue;
-                    const i64 i = zw[a] * m.P + (i64)yw[bb] * m.d.X + xw[cc];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) S::st(OP[c], i, r[k][c], m.sc_v);
-                    if (FINISH) S::st(m.vout.w, i, wv[k], m.sc_v);
-                }
-            }
-        }
-    }
+    for (int c = 0; c < (FINISH ? 4 : 3); c++)
+        *reinterpret_cast<typename R::T*>(reinterpret_cast<char*>(plane_of(OP[c], p, m.P)) + m.coff) = q[c];
 }
 
-template <class S, int VEC, bool FINISH, int T4>
-__device__ __forceinline__ void f2_step(const Fused2<S, VEC>& m, int t, typename RawV<S, VEC>::T (&v0)[4][3],
-                                        float (&v1)[4][VEC][3], typename RawV<S, VEC>::T (&fr)[4][3],
-                                        typename RawV<S, VEC>::T (&hr)[2][3],
-                                        float (&L0)[2][3][FBY + 2][FGeo<VEC>::LW],
-                                        float (&L1)[2][3][FBY][FGeo<VEC>::EW]) {
-    typedef RawV<S, VEC> R;
+template <class S, bool FINISH, int T4>
+__device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<S>::T (&v0)[4][3], float (&v1)[4][3],
+                                        typename Raw<S>::T (&fr)[4][3], typename Raw<S>::T (&hr)[2][3],
+                                        float (&L0)[2][3][FBY + 2][FBX + 2], float (&L1)[2][3][FBY][FBX]) {
+    typedef Raw<S> R;
     constexpr int SM1 = (T4 + 3) & 3, S0 = T4, SP1 = (T4 + 1) & 3, SM2 = (T4 + 2) & 3;
     constexpr int B = T4 & 1, BP = (T4 + 1) & 1;
-    const int lane = m.lane, ty = m.ty;
-    const float iv = m.inv_v, ifl = m.inv_f;
+    const int lane = m.lane, ty = m.ty, lx = lane + 1;
     // every conversion in straight-line code, before any branch, so the
     // waits on the loads issued two to four planes ago stay exact
-    float c0[VEC][3], cp[VEC][3], cm[VEC][3], f1[VEC][3], f2[VEC][3], ch[VEC][3];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int k = 0; k < VEC; k++) {
-            c0[k][c] = R::get(v0[S0][c], k, iv);
-            cp[k][c] = R::get(v0[SP1][c], k, iv);
-            cm[k][c] = R::get(v0[SM1][c], k, iv);
-            f1[k][c] = R::get(fr[S0][c], k, ifl);
-            f2[k][c] = R::get(fr[SM1][c], k, ifl);
-            ch[k][c] = R::get(hr[B][c], k, iv);
-            // left alone, the compiler sinks these into the branches that
-            // use them, where the wait becomes a full drain
-            asm volatile("" : "+v"(ch[k][c]), "+v"(f2[k][c]));
-        }
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int k = 0; k < VEC; k++) {
-            L0[B][c][ty + 1][VEC + lane * VEC + k] = c0[k][c];
-            L1[BP][c][ty][lane * VEC + k] = v1[SM1][k][c];
-        }
-    if (m.hrow) {
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int k = 0; k < VEC; k++) L0[B][c][m.hly][VEC + lane * VEC + k] = ch[k][c];
-    } else if (m.hcol) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) L0[B][c][m.hly][m.hlx] = m.hsel ? ch[VEC - 1][c] : ch[0][c];
-    }
-    __syncthreads();
-    // v1(t) at this thread's columns
-    const bool tin = t >= 1 && t <= m.d.Z - 2;
-#pragma unroll
-    for (int k = 0; k < VEC; k++) {
-        const int lx = VEC + lane * VEC + k;
-        const float m2 = f1[k][0] * f1[k][0] + f1[k][1] * f1[k][1] + f1[k][2] * f1[k][2];
-        const bool do1 = m.act && m.icol[k] && tin;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float r = gvf_update(c0[k][c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1],
-                                       L0[B][c][ty + 2][lx], L0[B][c][ty][lx], cp[k][c], cm[k][c], f1[k][c], m2, m.mu);
-            const float rm = RoundMid<S>::value ? sn16_to_f(f_to_sn16(r)) : r;
-            v1[S0][k][c] = do1 ? rm : 0.0f;
-        }
-    }
-    // v2(t - 1) at this thread's columns
-    const int p = t - 1;
-    if (m.inO && p >= m.qs && p < m.qe) {
-        const bool lo = (p - 1) == 0, hi = t == m.d.Z - 1;  // plane 0 / Z-1 alias plane 2 / Z-3
-        float r[VEC][3];
-        bool ok[VEC];
-#pragma unroll
-        for (int k = 0; k < VEC; k++) {
-            ok[k] = m.icol[k];
-            const float m2 = f2[k][0] * f2[k][0] + f2[k][1] * f2[k][1] + f2[k][2] * f2[k][2];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const float zm = lo ? v1[S0][k][c] : v1[SM2][k][c];
-                const float zp = hi ? v1[SM2][k][c] : v1[S0][k][c];
-                r[k][c] = gvf_update(v1[SM1][k][c], L1[BP][c][ty][m.lxp[k]], L1[BP][c][ty][m.lxm[k]],
-                                     L1[BP][c][m.lyp][lane * VEC + k], L1[BP][c][m.lym][lane * VEC + k], zp, zm,
-                                     f2[k][c], m2, m.mu);
-            }
-        }
-        f2_store<S, VEC, FINISH>(m, p, r, ok);
-    }
-    // loads: v0 and f of plane t+3 into the slots of t-1, the ring cells of
-    // plane t+2 into this step's halo slot
-    const int z3 = min(t + 3, m.d.Z - 1), z2 = min(t + 2, m.d.Z - 1);
-    const i64 iv3 = m.act ? z3 * m.P + m.col : 0, ih2 = m.hok ? z2 * m.P + m.hidx : 0;
-    const typename S::T* VP[3] = {m.vin.x, m.vin.y, m.vin.z};
-    const typename S::T* FP[3] = {m.f.x, m.f.y, m.f.z};
+    float c0[3], cp[3], cm[3], f1[3], f2[3], ch[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        v0[SM1][c] = R::load(VP[c], iv3);
-        fr[SM1][c] = R::load(FP[c], iv3);
-        hr[B][c] = R::load(VP[c], ih2);
+        c0[c] = R::getu(v0[S0][c]);
+        cp[c] = R::getu(v0[SP1][c]);
+        cm[c] = R::getu(v0[SM1][c]);
+        f1[c] = R::getu(fr[S0][c]);
+        f2[c] = R::getu(fr[SM1][c]);
+        ch[c] = R::getu(hr[B][c]);
+        // left alone, the compiler sinks these into the branches that use
+        // them, where the wait becomes a full drain
+        asm volatile("" : "+v"(ch[c]), "+v"(f2[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        L0[B][c][ty + 1][lx] = c0[c];
+        L1[BP][c][ty][lane] = v1[SM1][c];
+    }
+    if (m.hrow) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) L0[B][c][m.hly][lx] = ch[c];
+    }
+    __syncthreads();
+    // v1(t) at this thread's column (lanes 1..FBX-2; the others read the
+    // zeroed pad cells and keep 0)
+    const bool do1 = m.icol && t >= 1 && t <= m.d.Z - 2;
+    {
+        float m2 = f1[0] * f1[0] + f1[1] * f1[1] + f1[2] * f1[2];
+        if (UnitScaled<S>::value) m2 = m2 * m.m2s;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float r = gvf_update(c0[c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1], L0[B][c][ty + 2][lx],
+                                       L0[B][c][ty][lx], cp[c], cm[c], f1[c], m2, m.mu);
+            const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
+            v1[S0][c] = do1 ? rm : 0.0f;
+        }
+    }
+    // plane 0 / Z-1 of v1 equal planes 2 / Z-3 (the mirror rule): the ring
+    // slot of plane 0 takes plane 2 when that is computed (t == 2), the slot
+    // of plane Z-1 takes plane Z-3 (t == Z-1); neither slot is read otherwise
+    if (t == 2) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v1[SM2][c] = v1[S0][c];
+    }
+    if (t == m.d.Z - 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v1[S0][c] = v1[SM2][c];
+    }
+    // v2(t - 1) at this thread's column
+    const int p = t - 1;
+    if (m.inO && p >= m.qs && p < m.qe) {
+        float m2 = f2[0] * f2[0] + f2[1] * f2[1] + f2[2] * f2[2];
+        if (UnitScaled<S>::value) m2 = m2 * m.m2s;
+        float r[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            r[c] = gvf_update(v1[SM1][c], L1[BP][c][ty][m.lxp], L1[BP][c][ty][m.lxm], L1[BP][c][m.lyp][lane],
+                              L1[BP][c][m.lym][lane], v1[S0][c], v1[SM2][c], f2[c], m2, m.mu);
+        if (m.icol) f2_store<S, FINISH>(m, p, r);
+    }
+    // loads: v0 and f of plane t+3 into the slots of t-1, the ring-row cells
+    // of plane t+2 into this step's ring slot; issued in a fixed order (the
+    // prologue repeats it), so the wait counts at the loop head are the same
+    // on entry and on the back edge
+    const unsigned s3 = (unsigned)(min(t + 3, m.d.Z - 1) - m.zb) * m.pb;
+    const unsigned s2 = (unsigned)(min(t + 2, m.d.Z - 1) - m.zb) * m.pb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v0[SM1][c] = R::bload(m.rv[c], m.coff, s3);
+        asm volatile("" ::: "memory");
+        fr[SM1][c] = R::bload(m.rf[c], m.coff, s3);
+        asm volatile("" ::: "memory");
+        hr[B][c] = R::bload(m.rv[c], m.hoff, s2);
+        asm volatile("" ::: "memory");
     }
 }
 
-template <class S, int VEC, bool FINISH>
-__global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4) k_gvf_fused2(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d,
-                                                            float mu, float inv_f, float inv_v, float sc_v) {
-    typedef FGeo<VEC> G;
-    typedef RawV<S, VEC> R;
-    __shared__ float L0[2][3][FBY + 2][G::LW];
-    __shared__ float L1[2][3][FBY][G::EW];
-    Fused2<S, VEC> m;
+template <class S, bool FINISH>
+__global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
+    k_gvf_fused2(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d, float mu, float m2s, float unit) {
+    typedef Raw<S> R;
+    typedef typename R::T RT;
+    __shared__ float L0[2][3][FBY + 2][FBX + 2];
+    __shared__ float L1[2][3][FBY][FBX];
+    Fused2<S> m;
     m.f = f;
     m.vin = vin;
     m.vout = vout;
     m.d = d;
     m.mu = mu;
-    m.inv_f = inv_f;
-    m.inv_v = inv_v;
-    m.sc_v = sc_v;
+    m.m2s = m2s;
+    m.unit = unit;
     m.lane = threadIdx.x;
     m.ty = threadIdx.y;
+    // the pad cells left and right of every LDS row of v0 hold 0 (lanes 0
+    // and FBX-1 read them and discard the result)
+    for (int i = m.ty * FBX + m.lane; i < 2 * 3 * (FBY + 2) * 2; i += FBX * FBY) {
+        const int side = i & 1, row = i >> 1;
+        (&L0[0][0][0][0])[row * (FBX + 2) + (side ? FBX + 1 : 0)] = 0.0f;
+    }
     int tbx, tby, tbz;
     xcd_tile(tbx, tby, tbz);
-    const int ox = tile_origin(tbx, G::OW, d.X), oy = tile_origin(tby, G::OH, d.Y);
-    m.ex = ox - G::RX;
-    m.ey = oy - 1;
-    m.cy = m.ey + m.ty;
-    const int x0 = m.ex + m.lane * VEC;  // even when VEC == 2: ox, X and OW are even
-#pragma unroll
-    for (int k = 0; k < VEC; k++) m.cx[k] = x0 + k;
-    // VEC == 2 runs only with X even, so a pair is wholly inside or outside
-    m.act = x0 >= 0 && x0 < d.X && m.cy >= 0 && m.cy < d.Y;
-    const bool yint = m.cy >= 1 && m.cy <= d.Y - 2;
-#pragma unroll
-    for (int k = 0; k < VEC; k++) m.icol[k] = m.act && yint && m.cx[k] >= 1 && m.cx[k] <= d.X - 2;
-    m.inO = m.act && m.lane * VEC >= G::RX && m.lane * VEC < G::RX + G::OW && m.ty >= 1 && m.ty <= G::OH;
+    const int ox = tile_origin(tbx, FOW, d.X), oy = tile_origin(tby, FOH, d.Y);
+    const int ex = ox - 2, ey = oy - 1;
+    const int cx = ex + m.lane;
+    m.cy = ey + m.ty;
+    const bool act = cx >= 0 && cx < d.X && m.cy >= 0 && m.cy < d.Y;
+    m.icol = act && m.lane >= 1 && m.lane <= FBX - 2 && cx >= 1 && cx <= d.X - 2 && m.cy >= 1 && m.cy <= d.Y - 2;
+    m.inO = act && m.lane >= 2 && m.lane <= FBX - 3 && m.ty >= 1 && m.ty <= FBY - 2;
     m.P = (i64)d.X * d.Y;
-    m.col = m.act ? (i64)m.cy * d.X + x0 : 0;
+    // the host launches this kernel only when a chunk's plane bytes fit 31 bits
+    m.coff = act ? (unsigned)(((i64)m.cy * d.X + cx) * (i64)sizeof(typename S::T)) : 0u;
     // v1 neighbours of an O column, redirected at the volume's boundary
-    // columns (their mirror lies inside E for every O column that needs it)
-#pragma unroll
-    for (int k = 0; k < VEC; k++) {
-        m.lxp[k] = clampi(mirror_of(m.cx[k] + 1, d.X) - m.ex, 0, G::EW - 1);
-        m.lxm[k] = clampi(mirror_of(m.cx[k] - 1, d.X) - m.ex, 0, G::EW - 1);
-        m.xw1[k] = m.cx[k] == 2 ? 0 : -1;
-        m.xw2[k] = m.cx[k] == d.X - 3 ? d.X - 1 : -1;
-    }
-    m.lyp = clampi(mirror_of(m.cy + 1, d.Y) - m.ey, 0, FBY - 1);
-    m.lym = clampi(mirror_of(m.cy - 1, d.Y) - m.ey, 0, FBY - 1);
-    m.yw1 = m.cy == 2 ? 0 : -1;
-    m.yw2 = m.cy == d.Y - 3 ? d.Y - 1 : -1;
-    // ring of E: rows ey-1 (ty 0) and ey+FBY (ty 1) at this thread's
-    // columns; columns ex-1 and ex+EW (ty 2, lanes 0..2*FBY-1), read as the
-    // aligned VEC-group holding the cell
+    // columns and rows (their mirror lies inside E for every O column)
+    m.lxp = clampi(mirror_of(cx + 1, d.X) - ex, 0, FBX - 1);
+    m.lxm = clampi(mirror_of(cx - 1, d.X) - ex, 0, FBX - 1);
+    m.lyp = clampi(mirror_of(m.cy + 1, d.Y) - ey, 0, FBY - 1);
+    m.lym = clampi(mirror_of(m.cy - 1, d.Y) - ey, 0, FBY - 1);
+    // ring rows ey-1 (wave 0) and ey+FBY (wave 1) at this thread's column
     m.hrow = m.ty == 0 || m.ty == 1;
-    m.hcol = m.ty == 2 && m.lane < 2 * FBY;
-    m.hsel = false;
-    m.hly = 0;
-    m.hlx = 0;
-    int hx = 0, hy = 0;
-    bool hin = false;
-    if (m.hrow) {
-        hy = m.ty == 0 ? m.ey - 1 : m.ey + FBY;
-        hx = x0;
-        m.hly = m.ty == 0 ? 0 : FBY + 1;
-        hin = x0 >= 0 && x0 < d.X && hy >= 0 && hy < d.Y;
-    } else if (m.hcol) {
-        const bool left = m.lane < FBY;
-        const int cx = left ? m.ex - 1 : m.ex + G::EW;
-        hy = m.ey + (left ? m.lane : m.lane - FBY);
-        hx = VEC == 2 ? (cx & ~1) : cx;
-        m.hsel = VEC == 2 && (cx & 1);
-        m.hly = (left ? m.lane : m.lane - FBY) + 1;
-        m.hlx = left ? VEC - 1 : VEC + G::EW;
-        hin = cx >= 0 && cx < d.X && hy >= 0 && hy < d.Y;
-    }
-    m.hok = hin;
-    m.hidx = hin ? (i64)hy * d.X + hx : 0;
+    const int hy = m.ty == 0 ? ey - 1 : ey + FBY;
+    m.hly = m.ty == 0 ? 0 : FBY + 1;
+    const bool hin = m.hrow && cx >= 0 && cx < d.X && hy >= 0 && hy < d.Y;
+    m.hoff = hin ? (unsigned)(((i64)hy * d.X + cx) * (i64)sizeof(typename S::T)) : 0u;
     m.qs = 1 + tbz * FZC;
     m.qe = min(m.qs + FZC, d.Z - 1);
     const int t0 = m.qs - 1, te = m.qe;
-
-    typename R::T v0[4][3], fr[4][3], hr[2][3];
-    float v1[4][VEC][3];
+    // buffer resources over the planes this chunk reads, zb .. zb + FZC + 8
+    // at most (the loop runs up to three steps past te and loads three
+    // planes ahead)
+    m.pb = (unsigned)(m.P * (i64)sizeof(typename S::T));
+    m.zb = max(t0 - 1, 0);
+    const int zspan = min(d.Z, m.zb + FZC + 9) - m.zb;
     const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
     const typename S::T* FP[3] = {f.x, f.y, f.z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        m.rv[c] = __builtin_amdgcn_make_buffer_rsrc((void*)plane_of(VP[c], m.zb, m.P), 0, (int)(zspan * m.pb),
+                                                    BUFFER_WORD3);
+        m.rf[c] = __builtin_amdgcn_make_buffer_rsrc((void*)plane_of(FP[c], m.zb, m.P), 0, (int)(zspan * m.pb),
+                                                    BUFFER_WORD3);
+    }
+
+    RT v0[4][3], fr[4][3], hr[2][3];
+    float v1[4][3];
     // the loads of the four steps before t0, in the order the steps issue
     // them (plane t0-1, t0, t0+1 with the ring cells of t0, t0+2 with those
     // of t0+1), so the loop is entered with the steady state's load order
 #pragma unroll
     for (int g = 0; g < 4; g++) {
         const int z = min(max(t0 - 1 + g, 0), d.Z - 1), k = (g + 3) & 3;
-        const i64 i = m.act ? z * m.P + m.col : 0;
+        const unsigned sz = (unsigned)(z - m.zb) * m.pb, sh = (unsigned)(min(t0 + g - 2, d.Z - 1) - m.zb) * m.pb;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            v0[k][c] = R::load(VP[c], i);
-            fr[k][c] = R::load(FP[c], i);
-#pragma unroll
-            for (int q = 0; q < VEC; q++) v1[g][q][c] = 0.0f;
-            if (g >= 2) hr[g & 1][c] = R::load(VP[c], m.hok ? min(t0 + g - 2, d.Z - 1) * m.P + m.hidx : 0);
+            v0[k][c] = R::bload(m.rv[c], m.coff, sz);
+            asm volatile("" ::: "memory");
+            fr[k][c] = R::bload(m.rf[c], m.coff, sz);
+            asm volatile("" ::: "memory");
+            v1[g][c] = 0.0f;
+            if (g >= 2) {
+                hr[g & 1][c] = R::bload(m.rv[c], m.hoff, sh);
+                asm volatile("" ::: "memory");
+            }
         }
     }
     // four unconditional steps per iteration (up to three past te, which
     // store nothing: their v2 plane is >= qe and their loads are clamped) so
     // the loop body has no merge points that would blur the load counters
     for (int t = t0; t <= te; t += 4) {
-        f2_step<S, VEC, FINISH, 0>(m, t, v0, v1, fr, hr, L0, L1);
-        f2_step<S, VEC, FINISH, 1>(m, t + 1, v0, v1, fr, hr, L0, L1);
-        f2_step<S, VEC, FINISH, 2>(m, t + 2, v0, v1, fr, hr, L0, L1);
-        f2_step<S, VEC, FINISH, 3>(m, t + 3, v0, v1, fr, hr, L0, L1);
+        f2_step<S, FINISH, 0>(m, t, v0, v1, fr, hr, L0, L1);
+        f2_step<S, FINISH, 1>(m, t + 1, v0, v1, fr, hr, L0, L1);
+        f2_step<S, FINISH, 2>(m, t + 2, v0, v1, fr, hr, L0, L1);
+        f2_step<S, FINISH, 3>(m, t + 3, v0, v1, fr, hr, L0, L1);
     }
 }
 
-template <class S, int VEC>
-static void launch_fused2(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, Dims d, float mu, float inv_f,
-                          float inv_src, float sc_v, hipStream_t stream) {
-    typedef FGeo<VEC> G;
-    dim3 grid((d.X + G::OW - 1) / G::OW, (d.Y + G::OH - 1) / G::OH, (d.Z - 2 + FZC - 1) / FZC);
+// The volume's boundary voxels after a two-step launch: w takes the stored
+// value at mirror(w) (each coordinate 0 -> 2, N-1 -> N-3), which is an
+// interior voxel.  One thread per boundary voxel: both z faces, then the y
+// faces without their z rows, then the x faces without their y and z rows.
+template <class S, bool W>
+__global__ void k_gvf_faces(Out4<S> v, Dims d) {
+    typedef typename S::T T;
+    const i64 XY = (i64)d.X * d.Y, nz = 2 * XY, ny = 2 * (i64)d.X * (d.Z - 2), nx = 2 * (i64)(d.Y - 2) * (d.Z - 2);
+    T* P[4] = {v.x, v.y, v.z, v.w};
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nz + ny + nx; i += (i64)gridDim.x * blockDim.x) {
+        int x, y, z;
+        if (i < nz) {
+            const i64 r = i % XY;
+            x = (int)(r % d.X);
+            y = (int)(r / d.X);
+            z = i < XY ? 0 : d.Z - 1;
+        } else if (i < nz + ny) {
+            const i64 j = i - nz, h = j / 2;
+            x = (int)(h % d.X);
+            z = 1 + (int)(h / d.X);
+            y = (j & 1) ? d.Y - 1 : 0;
+        } else {
+            const i64 j = i - nz - ny, h = j / 2;
+            y = 1 + (int)(h % (d.Y - 2));
+            z = 1 + (int)(h / (d.Y - 2));
+            x = (j & 1) ? d.X - 1 : 0;
+        }
+        const i64 w = d.at(x, y, z), q = d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z));
+#pragma unroll
+        for (int c = 0; c < (W ? 4 : 3); c++) P[c][w] = P[c][q];
+    }
+}
+
+template <class S>
+static void launch_faces(bool with_w, Out4<S> v, Dims d, hipStream_t stream) {
+    const i64 n = 2 * ((i64)d.X * d.Y + (i64)d.X * (d.Z - 2) + (i64)(d.Y - 2) * (d.Z - 2));
+    const unsigned g = (unsigned)std::min<i64>((n + 255) / 256, 4096);
+    if (with_w) hipLaunchKernelGGL((k_gvf_faces<S, true>), g, 256, 0, stream, v, d);
+    else hipLaunchKernelGGL((k_gvf_faces<S, false>), g, 256, 0, stream, v, d);
+}
+
+// Units of the two-step kernel: it computes on stored numbers (value *
+// unit) and converts only at the ends.  unit is a power of two (the field
+// scale; 2^15 for SN16), so every rounding is the one the values would get.
+template <class S>
+static float storage_unit(float scale) {
+    return std::is_same<S, SN16>::value ? 32768.0f : scale;
+}
+
+// Whether the two-step kernel can run: f and v in the same units, and the
+// bytes of the planes one z chunk reads addressable by a buffer resource.
+template <class S>
+static bool fused_ok(float scale_f, float scale_v, Dims d) {
+    const double span = (double)d.X * d.Y * sizeof(typename S::T) * (FZC + 9);
+    return scale_f == scale_v && scale_v >= 0x1p-30f && scale_v <= 0x1p30f && span < 2147483648.0;
+}
+
+template <class S>
+static void launch_fused2(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, Dims d, float mu, float scale,
+                          hipStream_t stream) {
+    dim3 grid(d.X >= FOW ? (d.X + FOW - 1) / FOW : 1, d.Y >= FOH ? (d.Y + FOH - 1) / FOH : 1, (d.Z - 2 + FZC - 1) / FZC);
     dim3 block(FBX, FBY);
+    const float u = storage_unit<S>(scale), m2s = 1.0f / (u * u);
     if (finish)
-        hipLaunchKernelGGL((k_gvf_fused2<S, VEC, true>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src,
-                           sc_v);
+        hipLaunchKernelGGL((k_gvf_fused2<S, true>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
     else
-        hipLaunchKernelGGL((k_gvf_fused2<S, VEC, false>), grid, block, 0, stream, f, src, dst, d, mu, inv_f, inv_src,
-                           sc_v);
+        hipLaunchKernelGGL((k_gvf_fused2<S, false>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
 }
 
 // iterations == 0: v = f, w = |f| or 1.
@@ -702,7 +707,7 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, al));
     // launches of one or two steps: with fuse >= 2 an odd count starts with
     // a single step, then pairs
-    const int pairs = fuse >= 2 ? iters / 2 : 0;
+    const int pairs = fuse >= 2 && fused_ok<S>(f->scale, out->scale, d) ? iters / 2 : 0;
     const int singles = iters - 2 * pairs;
     const int launches = pairs + singles;
     for (int j = 0; j < launches; j++) {
@@ -716,7 +721,8 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
             if (vec2) launch_step<S, 2>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
             else launch_step<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
         } else {
-            launch_fused2<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
+            launch_fused2<S>(fin, F, src, dst, d, mu, sc_v, stream);
+            launch_faces<S>(fin, dst, d, stream);
         }
         TSF_TRY(check_launch("tsf_gvf_euler"));
     }
@@ -740,7 +746,8 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, 8));
     // two steps per launch (the same float operations as single steps), an
     // odd count starting with one single step
-    const int pairs = iters / 2, singles = iters - 2 * pairs, launches = pairs + singles;
+    const int pairs = fused_ok<F32>(1.0f, 1.0f, d) ? iters / 2 : 0;
+    const int singles = iters - 2 * pairs, launches = pairs + singles;
     for (int j = 0; j < launches; j++) {
         const bool to_out = ((launches - 1 - j) % 2) == 0;
         Planes3<F32> src = (j == 0) ? V0 : (to_out ? Wr : Or);
@@ -749,7 +756,8 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
             if (vec2) launch_step<F32, 2>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
             else launch_step<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
         } else {
-            launch_fused2<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
+            launch_fused2<F32>(false, F, src, dst, d, mu, 1.0f, stream);
+            launch_faces<F32>(false, dst, d, stream);
         }
         TSF_TRY(check_launch("gvf_steps_f32"));
     }

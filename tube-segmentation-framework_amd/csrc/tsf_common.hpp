@@ -39,6 +39,11 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
     bz = (int)(t / (gx * gy));
 }
 
+// Word 3 of a raw buffer resource on gfx9-family chips (gfx950 included):
+// 32-bit data format, no swizzle.  With stride 0, num_records counts bytes
+// and a load at or past it returns 0.
+constexpr int BUFFER_WORD3 = 0x00020000;
+
 // Mirror remap of the GVF/FMG kernels: 0 -> 2, >= N-1 -> N-3.
 __host__ __device__ inline int remap(int p, int N) { return p == 0 ? 2 : (p >= N - 1 ? N - 3 : p); }
 
@@ -108,6 +113,19 @@ __device__ __forceinline__ float sn16_to_f(int s) {
 __device__ __forceinline__ short f_to_sn16(float v) {
     const float x = v * 32767.0f;
     // round to nearest even, saturate; NaN converts to 0
+    const float r = rintf(fminf(fmaxf(x, -32768.0f), 32767.0f));
+    return (short)(x == x ? (int)r : 0);
+}
+// The same conversions on values in units of 2^-15 (u = value * 32768), for
+// kernels that compute in storage units: fma(s, 1/32767, s) is the correctly
+// rounded 32768 * s / 32767 for every one of the 65536 codes, and
+// u * (32767/32768) is exactly the product value * 32767.
+__device__ __forceinline__ float sn16u_to_f(int s) {
+    const float sf = (float)s;
+    return fmaxf(-32768.0f, __builtin_fmaf(sf, 1.0f / 32767.0f, sf));
+}
+__device__ __forceinline__ short f_to_sn16u(float u) {
+    const float x = u * (32767.0f / 32768.0f);
     const float r = rintf(fminf(fmaxf(x, -32768.0f), 32767.0f));
     return (short)(x == x ? (int)r : 0);
 }
