@@ -287,9 +287,11 @@ __global__ void __launch_bounds__(64 * GTY) k_gvf_march(Planes3<S> f, Planes3<S>
 // two), so it rounds exactly as on the values; v1 is kept in float.
 // Mirror rule: v1 at a boundary column / row / plane equals v1 at its mirror
 // (0 -> 2, N-1 -> N-3), so v1 reads there are redirected.  v2 is stored at
-// interior voxels only; k_gvf_faces then writes every boundary voxel w from
-// mirror(w).  Tiles are placed so a boundary column and its mirror lie in the
-// same O (the last tile along x and y is shifted to end at the volume's edge).
+// interior voxels and, on interior rows, at columns 0 and X-1 (the value of
+// columns 2 and X-3, taken across lanes); k_gvf_faces then writes the y and
+// z faces from their mirrors.  Tiles are placed so a boundary column and its
+// mirror lie in the same O (the last tile along x and y is shifted to end at
+// the volume's edge).
 constexpr int FBX = 64, FBY = 8;
 constexpr int FOW = FBX - 4, FOH = FBY - 2;
 constexpr int FZC = 64;  // output planes per z chunk, a multiple of 4
@@ -373,6 +375,9 @@ struct Fused2 {
     __amdgpu_buffer_rsrc_t rv[3], rf[3];  // v0 and f planes zb.. of this chunk
     int lxp, lxm, lyp, lym;  // LDS coordinates of v1 neighbours (mirror-redirected)
     int hly;                 // LDS row of this thread's ring-row cell
+    bool xedge;              // the tile holds column 0 or X-1
+    bool xbnd;               // this thread's column is 0 or X-1, on an interior row
+    int xsrc;                // lane holding the mirror column (2 or X-3) of a boundary column, else own lane
 };
 
 // Plane z of p (z uniform, so the address stays scalar).
@@ -470,7 +475,11 @@ __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<
         for (int c = 0; c < 3; c++)
             r[c] = gvf_update(v1[SM1][c], L1[BP][c][ty][m.lxp], L1[BP][c][ty][m.lxm], L1[BP][c][m.lyp][lane],
                               L1[BP][c][m.lym][lane], v1[S0][c], v1[SM2][c], f2[c], m2, m.mu);
-        if (m.icol) f2_store<S, FINISH>(m, p, r);
+        if (m.xedge) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) r[c] = __shfl(r[c], m.xsrc);
+        }
+        if (m.icol || m.xbnd) f2_store<S, FINISH>(m, p, r);
     }
     // loads: v0 and f of plane t+3 into the slots of t-1, the ring-row cells
     // of plane t+2 into this step's ring slot; issued in a fixed order (the
@@ -521,6 +530,12 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
     const bool act = cx >= 0 && cx < d.X && m.cy >= 0 && m.cy < d.Y;
     m.icol = act && m.lane >= 1 && m.lane <= FBX - 2 && cx >= 1 && cx <= d.X - 2 && m.cy >= 1 && m.cy <= d.Y - 2;
     m.inO = act && m.lane >= 2 && m.lane <= FBX - 3 && m.ty >= 1 && m.ty <= FBY - 2;
+    // columns 0 and X-1 of the interior rows take v2 of columns 2 and X-3
+    // (the x mirror) from the lanes that compute it, two lanes over; the y
+    // and z faces are written by k_gvf_faces
+    m.xedge = ex <= 0 || ex + FBX - 1 >= d.X - 1;
+    m.xbnd = m.inO && (cx == 0 || cx == d.X - 1) && m.cy >= 1 && m.cy <= d.Y - 2;
+    m.xsrc = cx == 0 ? m.lane + 2 : (cx == d.X - 1 ? m.lane - 2 : m.lane);
     m.P = (i64)d.X * d.Y;
     // the host launches this kernel only when a chunk's plane bytes fit 31 bits
     m.coff = act ? (unsigned)(((i64)m.cy * d.X + cx) * (i64)sizeof(typename S::T)) : 0u;
@@ -589,31 +604,26 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
 }
 
 // The volume's boundary voxels after a two-step launch: w takes the stored
-// value at mirror(w) (each coordinate 0 -> 2, N-1 -> N-3), which is an
-// interior voxel.  One thread per boundary voxel: both z faces, then the y
-// faces without their z rows, then the x faces without their y and z rows.
+// value at mirror(w) (each coordinate 0 -> 2, N-1 -> N-3), which the
+// two-step kernel computed (an interior voxel, or an x-boundary column of an
+// interior row, which that kernel also writes).  One thread per voxel of
+// both z faces, then of the y faces without their z rows.
 template <class S, bool W>
 __global__ void k_gvf_faces(Out4<S> v, Dims d) {
-    typedef typename S::T T;
-    const i64 XY = (i64)d.X * d.Y, nz = 2 * XY, ny = 2 * (i64)d.X * (d.Z - 2), nx = 2 * (i64)(d.Y - 2) * (d.Z - 2);
-    T* P[4] = {v.x, v.y, v.z, v.w};
-    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nz + ny + nx; i += (i64)gridDim.x * blockDim.x) {
+    const i64 XY = (i64)d.X * d.Y, nz = 2 * XY, ny = 2 * (i64)d.X * (d.Z - 2);
+    typename S::T* P[4] = {v.x, v.y, v.z, v.w};
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < nz + ny; i += (i64)gridDim.x * blockDim.x) {
         int x, y, z;
         if (i < nz) {
             const i64 r = i % XY;
             x = (int)(r % d.X);
             y = (int)(r / d.X);
             z = i < XY ? 0 : d.Z - 1;
-        } else if (i < nz + ny) {
-            const i64 j = i - nz, h = j / 2;
-            x = (int)(h % d.X);
-            z = 1 + (int)(h / d.X);
-            y = (j & 1) ? d.Y - 1 : 0;
         } else {
-            const i64 j = i - nz - ny, h = j / 2;
-            y = 1 + (int)(h % (d.Y - 2));
-            z = 1 + (int)(h / (d.Y - 2));
-            x = (j & 1) ? d.X - 1 : 0;
+            const i64 j = i - nz, row = j / d.X;
+            x = (int)(j % d.X);
+            y = (row & 1) ? d.Y - 1 : 0;
+            z = 1 + (int)(row / 2);
         }
         const i64 w = d.at(x, y, z), q = d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z));
 #pragma unroll
@@ -623,7 +633,7 @@ __global__ void k_gvf_faces(Out4<S> v, Dims d) {
 
 template <class S>
 static void launch_faces(bool with_w, Out4<S> v, Dims d, hipStream_t stream) {
-    const i64 n = 2 * ((i64)d.X * d.Y + (i64)d.X * (d.Z - 2) + (i64)(d.Y - 2) * (d.Z - 2));
+    const i64 n = 2 * ((i64)d.X * d.Y + (i64)d.X * (d.Z - 2));
     const unsigned g = (unsigned)std::min<i64>((n + 255) / 256, 4096);
     if (with_w) hipLaunchKernelGGL((k_gvf_faces<S, true>), g, 256, 0, stream, v, d);
     else hipLaunchKernelGGL((k_gvf_faces<S, false>), g, 256, 0, stream, v, d);
