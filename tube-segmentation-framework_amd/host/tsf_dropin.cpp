@@ -102,7 +102,7 @@ paramList initParameters() {
     B("use-fmg-gvf", false, "gradient-vector-flow");
     B("3d_write", true, "advanced");
     S("vector-storage", "auto", {"auto", "f32", "f16", "bf16", "sn16"}, "advanced");
-    N("gvf-fuse", 1, 1, 16, 1, "gradient-vector-flow");
+    N("gvf-fuse", 2, 1, 16, 1, "gradient-vector-flow");
     return p;
 }
 
