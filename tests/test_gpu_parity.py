@@ -216,6 +216,7 @@ def test_tdf_sn16_field_and_unorm16(tsf, orc):
     tsf.tdf_circle(fld, 1.0, 5.0, 0.5, t, r)
     et, er = orc.tdf_circle(V, 1.0, 5.0, 0.5, "f32")
     nbad, rel = close(t.download(), et)
+    print(f"SN16 TDF exact fraction {np.mean(t.download() == et):.6f}")
     assert nbad == 0, rel
     assert np.array_equal(r.download(), er)
     # UNORM16 rounding of the TDF buffer: bit-exact on the same values
@@ -293,6 +294,7 @@ def test_tdf_reduced_precision_field(tsf, orc, storage):
     Vq = fld.download()
     qt, qr = orc.tdf_circle(Vq, 1.0, 3.0, 0.5, "f32")
     nbad, rel = close(gt, qt)
+    print(f"TDF on the stored field: exact fraction {np.mean(gt == qt):.6f}")
     assert nbad == 0, rel
     assert np.array_equal(gr, qr)
     # quantisation effect (reported): the first-maximum radius search is

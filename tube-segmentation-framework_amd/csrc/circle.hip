@@ -21,12 +21,14 @@ __global__ void k_pack_f32(const float* __restrict__ x, const float* __restrict_
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
         out[i] = make_float4(x[i], y[i], z[i], w ? w[i] : 1.0f);
 }
+// sn16: code -32768 (0x8000) is stored as -32767 (0x8001); both read as -1.
 __global__ void k_pack_16(const uint16_t* __restrict__ x, const uint16_t* __restrict__ y,
-                          const uint16_t* __restrict__ z, const uint16_t* __restrict__ w, uint16_t one,
+                          const uint16_t* __restrict__ z, const uint16_t* __restrict__ w, uint16_t one, bool sn16,
                           uint2* __restrict__ out, i64 n) {
+    auto c = [sn16](unsigned v) { return sn16 && v == 0x8000u ? 0x8001u : v; };
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
         const unsigned ww = w ? w[i] : one;
-        out[i] = make_uint2((unsigned)x[i] | ((unsigned)y[i] << 16), (unsigned)z[i] | (ww << 16));
+        out[i] = make_uint2(c(x[i]) | (c(y[i]) << 16), c(z[i]) | (c(ww) << 16));
     }
 }
 
@@ -286,7 +288,9 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
     const i64 n = d.n();
     Scratch sc;
     TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_circle_workspace_size(dims, field->storage), W));
-    const float inv = 1.0f / field->scale;
+    // texel units: F16/BF16 hold value * scale, SN16 texels are read as
+    // value * 2^15
+    const float inv = field->storage == TSF_STORAGE_SN16 ? 1.0f / 32768.0f : 1.0f / field->scale;
     switch (field->storage) {
         case TSF_STORAGE_F32:
             hipLaunchKernelGGL(k_pack_f32, grid1d(n), 256, 0, stream, (const float*)field->x,
@@ -299,7 +303,7 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
             const uint16_t one = 0x3c00;  // only read when w is absent and the Hessian is not normalised
             hipLaunchKernelGGL(k_pack_16, grid1d(n), 256, 0, stream, (const uint16_t*)field->x,
                                (const uint16_t*)field->y, (const uint16_t*)field->z, (const uint16_t*)field->w, one,
-                               (uint2*)sc.ptr, n);
+                               field->storage == TSF_STORAGE_SN16, (uint2*)sc.ptr, n);
             if (norm) launch_tdf<F16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             else launch_tdf<F16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             break;
@@ -308,7 +312,7 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
             const uint16_t one = 32767;
             hipLaunchKernelGGL(k_pack_16, grid1d(n), 256, 0, stream, (const uint16_t*)field->x,
                                (const uint16_t*)field->y, (const uint16_t*)field->z, (const uint16_t*)field->w, one,
-                               (uint2*)sc.ptr, n);
+                               field->storage == TSF_STORAGE_SN16, (uint2*)sc.ptr, n);
             if (norm) launch_tdf<SN16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             else launch_tdf<SN16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             break;
@@ -317,7 +321,7 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
             const uint16_t one = 0x3f80;
             hipLaunchKernelGGL(k_pack_16, grid1d(n), 256, 0, stream, (const uint16_t*)field->x,
                                (const uint16_t*)field->y, (const uint16_t*)field->z, (const uint16_t*)field->w, one,
-                               (uint2*)sc.ptr, n);
+                               field->storage == TSF_STORAGE_SN16, (uint2*)sc.ptr, n);
             if (norm) launch_tdf<BF16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             else launch_tdf<BF16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
         }

@@ -8,11 +8,16 @@ namespace tsf {
 
 template <class S>
 struct Texel;
+// mulx/muly/mulz(w, p): w times the x / y / z component of texel p, as
+// stored (one rounding, as w * raw(p).x).
 template <>
 struct Texel<F32> {
     typedef float4 P;
     static __device__ __forceinline__ float4 unpack(P p, float) { return p; }
     static __device__ __forceinline__ float4 raw(P p) { return p; }
+    static __device__ __forceinline__ float mulx(float w, P p) { return w * p.x; }
+    static __device__ __forceinline__ float muly(float w, P p) { return w * p.y; }
+    static __device__ __forceinline__ float mulz(float w, P p) { return w * p.z; }
 };
 template <>
 struct Texel<F16> {
@@ -23,6 +28,24 @@ struct Texel<F16> {
     }
     static __device__ __forceinline__ float4 raw(P p) {
         return make_float4(h(p.x & 0xffffu), h(p.x >> 16), h(p.y & 0xffffu), h(p.y >> 16));
+    }
+    // v_fma_mix_f32 with a half operand and addend -0: the product w * h of
+    // the converted half, rounded once (x * y + (-0) == x * y exactly, signed
+    // zeros included), without a separate conversion instruction
+    static __device__ __forceinline__ float mulx(float w, P p) {
+        float r;
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(p.x));
+        return r;
+    }
+    static __device__ __forceinline__ float muly(float w, P p) {
+        float r;
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(p.x));
+        return r;
+    }
+    static __device__ __forceinline__ float mulz(float w, P p) {
+        float r;
+        asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(p.y));
+        return r;
     }
 };
 template <>
@@ -35,16 +58,32 @@ struct Texel<BF16> {
     static __device__ __forceinline__ float4 raw(P p) {
         return make_float4(b(p.x & 0xffffu), b(p.x >> 16), b(p.y & 0xffffu), b(p.y >> 16));
     }
+    static __device__ __forceinline__ float mulx(float w, P p) { return w * b(p.x & 0xffffu); }
+    static __device__ __forceinline__ float muly(float w, P p) { return w * b(p.x >> 16); }
+    static __device__ __forceinline__ float mulz(float w, P p) { return w * b(p.y & 0xffffu); }
 };
 
+// SN16 texels read in units of 2^-15 (32768 * value, exactly); the samplers
+// take inv = 2^-15, so unpack() and the scaled trilinear sums give the
+// decoded values bit for bit.  The packing kernel stores code -32768 as
+// -32767 (both read as -1), so the decode needs no clamp: fma(s, 1/32767, s)
+// is the correctly rounded 32768 * s / 32767.
 template <>
 struct Texel<SN16> {
     typedef uint2 P;
-    static __device__ __forceinline__ float v(unsigned b) { return sn16_to_f((short)(b & 0xffffu)); }
-    static __device__ __forceinline__ float4 unpack(P p, float) {
-        return make_float4(v(p.x), v(p.x >> 16), v(p.y), v(p.y >> 16));
+    static __device__ __forceinline__ float u(unsigned b) {
+        const float sf = (float)(short)(b & 0xffffu);
+        return __builtin_fmaf(sf, 1.0f / 32767.0f, sf);
     }
-    static __device__ __forceinline__ float4 raw(P p) { return unpack(p, 1.0f); }
+    static __device__ __forceinline__ float4 unpack(P p, float inv) {
+        return make_float4(u(p.x) * inv, u(p.x >> 16) * inv, u(p.y) * inv, u(p.y >> 16) * inv);
+    }
+    static __device__ __forceinline__ float4 raw(P p) {
+        return make_float4(u(p.x), u(p.x >> 16), u(p.y), u(p.y >> 16));
+    }
+    static __device__ __forceinline__ float mulx(float w, P p) { return w * u(p.x); }
+    static __device__ __forceinline__ float muly(float w, P p) { return w * u(p.x >> 16); }
+    static __device__ __forceinline__ float mulz(float w, P p) { return w * u(p.y); }
 };
 
 // Texel sources of the sampling functions below: row(d, y, z) locates row
@@ -141,6 +180,42 @@ __device__ __forceinline__ void trilinear_tex(const A& T, const Dims& d, float u
     out[0] = sx * T.inv;
     out[1] = sy * T.inv;
     out[2] = sz * T.inv;
+}
+
+// The same sampling from an LDS tile.  The tile holds the clamp-to-edge
+// texel of every cell and its halo, floor(r + 0.5) + 1 around the brick,
+// keeps both corners of every finite sample inside it; so the global clamps
+// above change no value here and only the lower corner's local cell is
+// computed (and clamped, which matters only for non-finite positions).  The
+// taps and the sums are those of trilinear_tex, in the same order.
+template <class S>
+__device__ __forceinline__ void trilinear_tex(const LdsTex<S>& T, const Dims&, float u, float v, float w,
+                                              float out[3]) {
+    typedef Texel<S> X;
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const int sy = T.tx, sz = __mul24(T.tx, T.ty);
+    const int li = min(max((int)fu - T.ox, 0), T.tx - 2), lj = min(max((int)fv - T.oy, 0), T.ty - 2);
+    const int lk = min(max((int)fw - T.oz, 0), T.tz - 2);
+    const typename X::P* q = T.p + (li + __mul24(lj, sy) + __mul24(lk, sz));
+    const typename X::P p0 = q[0], p1 = q[1], p2 = q[sy], p3 = q[sy + 1];
+    const typename X::P p4 = q[sz], p5 = q[sz + 1], p6 = q[sz + sy], p7 = q[sz + sy + 1];
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    float sx = X::mulx(wt0, p0), sy_ = X::muly(wt0, p0), sz_ = X::mulz(wt0, p0);
+    sx = sx + X::mulx(wt1, p1); sy_ = sy_ + X::muly(wt1, p1); sz_ = sz_ + X::mulz(wt1, p1);
+    sx = sx + X::mulx(wt2, p2); sy_ = sy_ + X::muly(wt2, p2); sz_ = sz_ + X::mulz(wt2, p2);
+    sx = sx + X::mulx(wt3, p3); sy_ = sy_ + X::muly(wt3, p3); sz_ = sz_ + X::mulz(wt3, p3);
+    sx = sx + X::mulx(wt4, p4); sy_ = sy_ + X::muly(wt4, p4); sz_ = sz_ + X::mulz(wt4, p4);
+    sx = sx + X::mulx(wt5, p5); sy_ = sy_ + X::muly(wt5, p5); sz_ = sz_ + X::mulz(wt5, p5);
+    sx = sx + X::mulx(wt6, p6); sy_ = sy_ + X::muly(wt6, p6); sz_ = sz_ + X::mulz(wt6, p6);
+    sx = sx + X::mulx(wt7, p7); sy_ = sy_ + X::muly(wt7, p7); sz_ = sz_ + X::mulz(wt7, p7);
+    out[0] = sx * T.inv;
+    out[1] = sy_ * T.inv;
+    out[2] = sz_ * T.inv;
 }
 
 constexpr int CBX = 16, CBY = 4, CBZ = 4;  // TDF brick
