@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--mu", type=float, default=0.05)
     ap.add_argument("--storage", choices=["f16", "bf16", "sn16", "f32"], default="f16",
                     help="vector-field storage; sn16 is the reference's own 16-bit format")
-    ap.add_argument("--fuse", type=int, default=2, help="GVF steps per launch (1 or 2)")
+    ap.add_argument("--fuse", type=int, default=2, choices=[1, 2, 4], help="GVF steps per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -118,21 +118,24 @@ def main():
         log("pmc workload done")
         return
 
-    def step():
+    def forward(Fs, Fl, Vl, gws, tws, fuse):
         ev[0].record(stream)
         # small-radius branch (radius-min < 2.5)
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
-        tsf.create_vector_field(d_blur, F_small, 0.2, 1, stream)
-        tsf.tdf_circle(F_small, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
+        tsf.create_vector_field(d_blur, Fs, 0.2, 1, stream)
+        tsf.tdf_circle(Fs, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
         # large-radius branch
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
-        tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
+        tsf.create_vector_field(d_blur, Fl, 0.2, 1, stream)
         ev[1].record(stream)
-        tsf.gvf_euler(F, V, args.mu, args.iters, args.fuse, ws, stream)
+        tsf.gvf_euler(Fl, Vl, args.mu, args.iters, fuse, gws, stream)
         ev[2].record(stream)
-        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
+        tsf.tdf_circle(Vl, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
         tsf.combine(t_s, r_s, t_l, r_l, stream)
         ev[3].record(stream)
+
+    def step():
+        forward(F_small, F, V, ws, tws, args.fuse)
 
     for i in range(args.warmup):
         step()
@@ -153,46 +156,47 @@ def main():
     gvf_ms = float(np.median(gvf))
     fwd_ms = float(np.median(fwd))
     gvox = N * args.iters / (gvf_ms * 1e-3) / 1e9
-    pairs = args.iters // 2 if args.fuse >= 2 else 0  # launches of two steps (an odd count adds one single)
-    launches = max(1, pairs + (args.iters - 2 * pairs))
+    # launches: fours, then at most one two-step and one single (as tsf_gvf_euler schedules them)
+    spl = args.fuse
+    quads = args.iters // 4 if spl == 4 else 0
+    pairs = (args.iters - 4 * quads) // 2 if spl >= 2 else 0
+    launches = max(1, quads + pairs + (args.iters - 4 * quads - 2 * pairs))
     bytes_per_voxel = 9 * (4 if S == tsf.STORAGE_F32 else 2)  # read v, f; write v (3 components each)
     launch_ms = gvf_ms / launches
     achieved = bytes_per_voxel * N / (launch_ms * 1e-3)
 
-    # same GVF with float storage, for the speed-up of the reduced-precision form
+    # the same forward with float storage, for the speed-up of the
+    # reduced-precision form: two steps per launch, and the plain FP32 path
+    # (one step per launch)
     f32 = None
     if not args.no_f32_compare and S != tsf.STORAGE_F32:
-        F32 = tsf.Field(shape)
-        V32 = tsf.Field(shape)
+        Fs32, F32, V32 = tsf.Field(shape), tsf.Field(shape), tsf.Field(shape)
         ws32 = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, 0),), np.uint8)
-        tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
-        tsf.create_vector_field(d_blur, F32, 0.2, 1, stream)
-        tsf.gvf_euler(F32, V32, args.mu, args.iters, args.fuse, ws32, stream)  # warm
-        stream.synchronize()
-        ts = []
-        for _ in range(max(1, args.steps)):
-            ev[1].record(stream)
-            tsf.gvf_euler(F32, V32, args.mu, args.iters, args.fuse, ws32, stream)
-            ev[2].record(stream)
+        tws32 = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, tsf.STORAGE_F32),), np.uint8)
+        res = {}
+        f32_fuse = 4  # four steps per launch: the fastest FP32 form
+        for fz in (f32_fuse, 1):
+            forward(Fs32, F32, V32, ws32, tws32, fz)  # warm
             stream.synchronize()
-            ts.append(ev[1].elapsed_ms(ev[2]))
-        f32_ms = float(np.median(ts))
-        # the plain FP32 path: one step per launch
-        ts1 = []
-        for _ in range(max(1, args.steps)):
-            ev[1].record(stream)
-            tsf.gvf_euler(F32, V32, args.mu, args.iters, 1, ws32, stream)
-            ev[2].record(stream)
-            stream.synchronize()
-            ts1.append(ev[1].elapsed_ms(ev[2]))
-        f32_1 = float(np.median(ts1))
-        f32 = {"gvf_ms": round(f32_ms, 3), "gvoxel_iter_per_s": round(N * args.iters / (f32_ms * 1e-3) / 1e9, 2),
+            tf, tg = [], []
+            for _ in range(max(1, args.steps)):
+                forward(Fs32, F32, V32, ws32, tws32, fz)
+                stream.synchronize()
+                tf.append(ev[0].elapsed_ms(ev[3]))
+                tg.append(ev[1].elapsed_ms(ev[2]))
+            res[fz] = (float(np.median(tf)), float(np.median(tg)))
+        (f32_fwd, f32_ms), (f32_fwd1, f32_1) = res[f32_fuse], res[1]
+        f32 = {"steps_per_launch": f32_fuse, "gvf_ms": round(f32_ms, 3),
+               "gvoxel_iter_per_s": round(N * args.iters / (f32_ms * 1e-3) / 1e9, 2),
+               "forward_ms": round(f32_fwd, 3),
                "speedup_reduced_vs_f32": round(f32_ms / gvf_ms, 3),
-               "hbm_frac": round(36 * N / (f32_ms / launches * 1e-3) / PEAK_HBM, 4),
+               "hbm_frac_per_launch": round(36 * N / (f32_ms / (args.iters // f32_fuse) * 1e-3) / PEAK_HBM, 4),
                "single_step_gvf_ms": round(f32_1, 3),
                "single_step_gvoxel_iter_per_s": round(N * args.iters / (f32_1 * 1e-3) / 1e9, 2),
-               "speedup_reduced_vs_f32_single_step": round(f32_1 / gvf_ms, 3)}
-        del F32, V32, ws32
+               "single_step_forward_ms": round(f32_fwd1, 3),
+               "speedup_reduced_vs_f32_single_step": round(f32_1 / gvf_ms, 3),
+               "forward_speedup_reduced_vs_plain_f32": round(f32_fwd1 / fwd_ms, 3)}
+        del Fs32, F32, V32, ws32, tws32
 
     adjoint = None
     if not args.no_adjoint:
@@ -207,7 +211,7 @@ def main():
     # rocprofv3 counter passes recorded in profiles/ (null when not measured
     # for this storage type)
     traffic = None
-    gk = "k_gvf_fused2<{}, false>" if args.fuse >= 2 else "k_gvf_march<{}, 2, false>"
+    gk = {4: "k_gvf_fused4<{}, false>", 2: "k_gvf_fused2<{}, false>"}.get(args.fuse, "k_gvf_march<{}, 2, false>")
     kname = {"f16": gk.format("F16"), "f32": gk.format("F32"), "sn16": gk.format("SN16"),
              "bf16": gk.format("BF16")}.get(args.storage)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_circle_forward_512cubed.json")
@@ -243,7 +247,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
                      "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
-                     "kernel": kname or "k_gvf", "steps_per_launch": 2 if args.fuse >= 2 else 1,
+                     "kernel": kname or "k_gvf", "steps_per_launch": args.fuse,
                      "bytes_per_voxel_per_launch": bytes_per_voxel,
                      "launch_ms": round(launch_ms, 4)},
         "cpu_baseline": cpu,

@@ -90,36 +90,40 @@ def test_gvf_euler_f32(tsf, orc, shape, iters):
     assert nbad == 0, rel
 
 
+@pytest.mark.parametrize("fuse", [2, 4])
 @pytest.mark.parametrize("shape,iters", [((64, 64, 64), 250), ((20, 24, 36), 7), ((16, 17, 18), 2), ((8, 8, 8), 3),
                                          ((5, 13, 130), 4), ((4, 9, 66), 5), ((37, 23, 65), 6), ((70, 6, 4), 2),
-                                         ((4, 4, 4), 4), ((5, 5, 5), 3), ((67, 7, 63), 8), ((130, 20, 125), 10)])
-def test_gvf_euler_fused_two_steps(tsf, orc, shape, iters):
-    # two steps per launch (fuse=2) computes the same float operations in the
-    # same order as two single-step launches: F32 results are identical
+                                         ((4, 4, 4), 4), ((5, 5, 5), 3), ((67, 7, 63), 8), ((130, 20, 125), 10),
+                                         ((9, 31, 57), 12), ((6, 11, 5), 9)])
+def test_gvf_euler_fused_two_steps(tsf, orc, shape, iters, fuse):
+    # two or four steps per launch (fuse 2 / 4) compute the same float
+    # operations in the same order as single-step launches: F32 results are
+    # identical
     F = _field_from(orc, phantom(shape, 4))
     f = tsf.Field.from_host(F)
     a, b = tsf.Field(shape), tsf.Field(shape)
     tsf.gvf_euler(f, a, 0.1, iters, fuse=1)
-    tsf.gvf_euler(f, b, 0.1, iters, fuse=2)
+    tsf.gvf_euler(f, b, 0.1, iters, fuse=fuse)
     ga, gb = a.download(), b.download()
-    print(f"fused {shape} iters={iters}: identical {np.mean(ga == gb):.6f}")
+    print(f"fused{fuse} {shape} iters={iters}: identical {np.mean(ga == gb):.6f}")
     assert np.array_equal(ga, gb)
     nbad, rel = close(gb, orc.gvf_euler(F, 0.1, iters, "f32"))
     assert nbad == 0, rel
 
 
+@pytest.mark.parametrize("fuse", [2, 4])
 @pytest.mark.parametrize("storage,bound", [(1, 2e-3), (2, 2e-2)])
-def test_gvf_euler_fused_reduced_precision(tsf, orc, storage, bound):
+def test_gvf_euler_fused_reduced_precision(tsf, orc, storage, bound, fuse):
     shape = (48, 48, 48)
     F = _field_from(orc, phantom(shape, 6))
     scale = 2.0 ** 14 if storage == 1 else 1.0
     f = tsf.Field.from_host(F, storage, scale)
     out = tsf.Field(shape, storage, scale)
-    tsf.gvf_euler(f, out, 0.05, 100, fuse=2)
+    tsf.gvf_euler(f, out, 0.05, 100, fuse=fuse)
     got = out.download()
     ref = orc.gvf_euler(F.astype(np.float64), 0.05, 100, "f64")
     rel = np.linalg.norm(got[:3] - ref[:3]) / np.linalg.norm(ref[:3])
-    print(f"fused storage {storage}: L2 rel error vs FP64 {rel:.3e}")
+    print(f"fused{fuse} storage {storage}: L2 rel error vs FP64 {rel:.3e}")
     assert rel < bound
     # the magnitude plane is |v| of the stored components, itself stored:
     # one rounding of the storage type (2^-11 FP16, 2^-8 BF16) after the
@@ -179,7 +183,8 @@ def test_vector_field_sn16(tsf, orc, sign, fmax):
 @pytest.mark.parametrize("shape,iters,mu,fuse", [
     ((48, 48, 48), 100, 0.1, 1), ((48, 48, 48), 100, 0.1, 2), ((64, 64, 64), 250, 0.05, 2),
     ((20, 24, 36), 7, 0.05, 1), ((37, 23, 65), 6, 0.1, 2), ((5, 13, 130), 5, 0.1, 2), ((4, 4, 4), 3, 0.05, 2),
-    ((70, 6, 4), 2, 0.1, 1), ((130, 20, 125), 10, 0.05, 2), ((16, 17, 18), 0, 0.05, 2)])
+    ((70, 6, 4), 2, 0.1, 1), ((130, 20, 125), 10, 0.05, 2), ((16, 17, 18), 0, 0.05, 2),
+    ((48, 48, 48), 100, 0.1, 4), ((37, 23, 65), 11, 0.1, 4), ((5, 5, 5), 8, 0.05, 4), ((130, 20, 125), 14, 0.05, 4)])
 def test_gvf_euler_sn16(tsf, orc, shape, iters, mu, fuse):
     F = orc.vector_field_sn16(orc.blur(phantom(shape, 4), 0.5, "f32"), 0.2, 1)
     f = tsf.Field.from_host(F, SN16)

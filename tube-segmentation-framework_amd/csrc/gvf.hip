@@ -603,6 +603,271 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
     }
 }
 
+// ---------------------------------------------------------------- four steps per launch
+// k_gvf_fused4 extends the two-step scheme to four steps: v and f cross HBM
+// once per four iterations.  A workgroup of 64 x F4Y threads (one wave per
+// row) covers 64 x F4Y columns starting four columns left and three rows
+// above its output tile; stage k (k = 1..4) computes v_k = step(v_{k-1}) on
+// lanes k..63-k of rows k-1..F4Y-k, so stage 4 produces the output tile O of
+// (64-8) x (F4Y-6) columns.  Per step t (one barrier): LDS gets v0 plane t
+// (with the ring rows) and v_k plane t-k from each stage k < 4; stage k then
+// computes plane t-k+1 from LDS x/y neighbours and its z neighbours in the
+// register ring of stage k-1 (planes t-k..t-k+2, the last just computed).
+// f and |f|^2 of plane t-k+1 pass from stage to stage in registers.  Loads,
+// units, the mirror rule and the boundary faces are those of k_gvf_fused2.
+constexpr int F4Y = 16;
+constexpr int F4OW = FBX - 8, F4OH = F4Y - 6;
+
+template <class S>
+struct Fused4 {
+    Planes3<S> f, vin;
+    Out4<S> vout;
+    Dims d;
+    float mu, m2s, unit;
+    int lane, ty, cy, qs, qe;
+    bool inner, hrow, xedge, xbnd, inO;
+    i64 P;
+    unsigned coff, hoff, pb;
+    int zb, ex;
+    __amdgpu_buffer_rsrc_t rv[3], rf[3];
+    int lxp, lxm, lyp, lym, hly, xsrc;
+};
+
+template <class S, bool FINISH, int T4>
+__device__ __forceinline__ void f4_step(const Fused4<S>& m, int t, typename Raw<S>::T (&v0)[4][3],
+                                        typename Raw<S>::T (&fr)[4][3], typename Raw<S>::T (&hr)[2][3],
+                                        float (&fk)[4][3], float (&mk)[4], float (&vk)[3][4][3],
+                                        float (&L0)[2][3][F4Y + 2][FBX + 2], float (&LK)[3][2][3][F4Y][FBX]) {
+    typedef Raw<S> R;
+    constexpr int S0 = T4, SP1 = (T4 + 1) & 3, SM1 = (T4 + 3) & 3;
+    constexpr int B = T4 & 1, BP = (T4 + 1) & 1;
+    const int lane = m.lane, ty = m.ty, lx = lane + 1;
+    float c0[3], cp[3], cm[3], fn[3], ch[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        c0[c] = R::getu(v0[S0][c]);
+        cp[c] = R::getu(v0[SP1][c]);
+        cm[c] = R::getu(v0[SM1][c]);
+        fn[c] = R::getu(fr[S0][c]);
+        ch[c] = R::getu(hr[B][c]);
+        asm volatile("" : "+v"(ch[c]));
+    }
+    // f and |f|^2 of plane t-k+1 for stage k
+#pragma unroll
+    for (int k = 3; k >= 1; k--) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) fk[k][c] = fk[k - 1][c];
+        mk[k] = mk[k - 1];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) fk[0][c] = fn[c];
+    {
+        float m2 = fn[0] * fn[0] + fn[1] * fn[1] + fn[2] * fn[2];
+        if (UnitScaled<S>::value) m2 = m2 * m.m2s;
+        mk[0] = m2;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        L0[B][c][ty + 1][lx] = c0[c];
+#pragma unroll
+        for (int k = 1; k <= 3; k++) LK[k - 1][BP][c][ty][lane] = vk[k - 1][(T4 - k) & 3][c];
+    }
+    if (m.hrow) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) L0[B][c][m.hly][lx] = ch[c];
+    }
+    __syncthreads();
+    // stage 1: v1(t)
+    {
+        const bool ok = m.inner && lane >= 1 && lane <= FBX - 2 && t >= 1 && t <= m.d.Z - 2;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float r = gvf_update(c0[c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1], L0[B][c][ty + 2][lx],
+                                       L0[B][c][ty][lx], cp[c], cm[c], fk[0][c], mk[0], m.mu);
+            const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
+            vk[0][S0][c] = ok ? rm : 0.0f;
+        }
+        if (t == 2) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[0][(T4 - 2) & 3][c] = vk[0][S0][c];
+        }
+        if (t == m.d.Z - 1) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[0][S0][c] = vk[0][(T4 - 2) & 3][c];
+        }
+    }
+    // stages 2 and 3: v_k(t-k+1) on rows k-1..F4Y-k
+#pragma unroll
+    for (int k = 2; k <= 3; k++) {
+        const int p = t - k + 1;
+        if (ty >= k - 1 && ty <= F4Y - k) {
+            const int sc = (T4 - k + 1) & 3, sp = (T4 - k + 2) & 3, sm = (T4 - k) & 3;
+            const bool ok = m.inner && lane >= k && lane <= FBX - 1 - k && p >= 1 && p <= m.d.Z - 2;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float r = gvf_update(vk[k - 2][sc][c], LK[k - 2][BP][c][ty][m.lxp], LK[k - 2][BP][c][ty][m.lxm],
+                                           LK[k - 2][BP][c][m.lyp][lane], LK[k - 2][BP][c][m.lym][lane],
+                                           vk[k - 2][sp][c], vk[k - 2][sm][c], fk[k - 1][c], mk[k - 1], m.mu);
+                const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
+                vk[k - 1][sc][c] = ok ? rm : 0.0f;
+            }
+        }
+        if (p == 2) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[k - 1][(T4 - k - 1) & 3][c] = vk[k - 1][(T4 - k + 1) & 3][c];
+        }
+        if (p == m.d.Z - 1) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) vk[k - 1][(T4 - k + 1) & 3][c] = vk[k - 1][(T4 - k - 1) & 3][c];
+        }
+    }
+    // stage 4: the output plane t-3
+    {
+        const int p = t - 3;
+        if (m.inO && p >= m.qs && p < m.qe) {
+            const int sc = (T4 - 3) & 3, sp = (T4 - 2) & 3, sm = (T4 - 4) & 3;
+            float r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                r[c] = gvf_update(vk[2][sc][c], LK[2][BP][c][ty][m.lxp], LK[2][BP][c][ty][m.lxm],
+                                  LK[2][BP][c][m.lyp][lane], LK[2][BP][c][m.lym][lane], vk[2][sp][c], vk[2][sm][c],
+                                  fk[3][c], mk[3], m.mu);
+            if (m.xedge) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) r[c] = __shfl(r[c], m.xsrc);
+            }
+            if ((m.inner && lane >= 4 && lane <= FBX - 5) || m.xbnd) {
+                typename R::T q[4];
+#pragma unroll
+                for (int c = 0; c < 3; c++) q[c] = R::put(r[c]);
+                if (FINISH) {
+                    const float ax = R::getu(q[0]), ay = R::getu(q[1]), az = R::getu(q[2]);
+                    const float l = sqrtf(ax * ax + ay * ay + az * az);
+                    q[3] = R::put(l > 0.0f ? l : m.unit);
+                }
+                typename S::T* OP[4] = {m.vout.x, m.vout.y, m.vout.z, m.vout.w};
+#pragma unroll
+                for (int c = 0; c < (FINISH ? 4 : 3); c++)
+                    *reinterpret_cast<typename R::T*>(reinterpret_cast<char*>(plane_of(OP[c], p, m.P)) + m.coff) =
+                        q[c];
+            }
+        }
+    }
+    // loads of plane t+3 and the ring rows of plane t+2, fixed order
+    const unsigned s3 = (unsigned)(min(t + 3, m.d.Z - 1) - m.zb) * m.pb;
+    const unsigned s2 = (unsigned)(min(t + 2, m.d.Z - 1) - m.zb) * m.pb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v0[SM1][c] = R::bload(m.rv[c], m.coff, s3);
+        asm volatile("" ::: "memory");
+        fr[SM1][c] = R::bload(m.rf[c], m.coff, s3);
+        asm volatile("" ::: "memory");
+        hr[B][c] = R::bload(m.rv[c], m.hoff, s2);
+        asm volatile("" ::: "memory");
+    }
+}
+
+template <class S, bool FINISH>
+__global__ void __launch_bounds__(FBX* F4Y, 4)
+    k_gvf_fused4(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d, float mu, float m2s, float unit) {
+    typedef Raw<S> R;
+    typedef typename R::T RT;
+    __shared__ float L0[2][3][F4Y + 2][FBX + 2];
+    __shared__ float LK[3][2][3][F4Y][FBX];
+    Fused4<S> m;
+    m.f = f;
+    m.vin = vin;
+    m.vout = vout;
+    m.d = d;
+    m.mu = mu;
+    m.m2s = m2s;
+    m.unit = unit;
+    m.lane = threadIdx.x;
+    m.ty = threadIdx.y;
+    for (int i = m.ty * FBX + m.lane; i < 2 * 3 * (F4Y + 2) * 2; i += FBX * F4Y) {
+        const int side = i & 1, row = i >> 1;
+        (&L0[0][0][0][0])[row * (FBX + 2) + (side ? FBX + 1 : 0)] = 0.0f;
+    }
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int ox = tile_origin(tbx, F4OW, d.X), oy = tile_origin(tby, F4OH, d.Y);
+    const int ex = ox - 4, ey = oy - 3;
+    m.ex = ex;
+    const int cx = ex + m.lane;
+    m.cy = ey + m.ty;
+    const bool act = cx >= 0 && cx < d.X && m.cy >= 0 && m.cy < d.Y;
+    // interior column: stage outputs exist there
+    m.inner = act && cx >= 1 && cx <= d.X - 2 && m.cy >= 1 && m.cy <= d.Y - 2;
+    m.inO = act && m.lane >= 4 && m.lane <= FBX - 5 && m.ty >= 3 && m.ty <= F4Y - 4;
+    m.P = (i64)d.X * d.Y;
+    m.coff = act ? (unsigned)(((i64)m.cy * d.X + cx) * (i64)sizeof(typename S::T)) : 0u;
+    m.lxp = clampi(mirror_of(cx + 1, d.X) - ex, 0, FBX - 1);
+    m.lxm = clampi(mirror_of(cx - 1, d.X) - ex, 0, FBX - 1);
+    m.lyp = clampi(mirror_of(m.cy + 1, d.Y) - ey, 0, F4Y - 1);
+    m.lym = clampi(mirror_of(m.cy - 1, d.Y) - ey, 0, F4Y - 1);
+    m.xedge = ex <= 0 || ex + FBX - 1 >= d.X - 1;
+    m.xbnd = m.inO && (cx == 0 || cx == d.X - 1) && m.cy >= 1 && m.cy <= d.Y - 2;
+    m.xsrc = cx == 0 ? m.lane + 2 : (cx == d.X - 1 ? m.lane - 2 : m.lane);
+    m.hrow = m.ty == 0 || m.ty == 1;
+    const int hy = m.ty == 0 ? ey - 1 : ey + F4Y;
+    m.hly = m.ty == 0 ? 0 : F4Y + 1;
+    const bool hin = m.hrow && cx >= 0 && cx < d.X && hy >= 0 && hy < d.Y;
+    m.hoff = hin ? (unsigned)(((i64)hy * d.X + cx) * (i64)sizeof(typename S::T)) : 0u;
+    m.qs = 1 + tbz * FZC;
+    m.qe = min(m.qs + FZC, d.Z - 1);
+    const int t0 = m.qs - 3, te = m.qe + 2;
+    // planes read: max(t0-1, 0) .. te+6 at most
+    m.pb = (unsigned)(m.P * (i64)sizeof(typename S::T));
+    m.zb = max(t0 - 1, 0);
+    const int zspan = min(d.Z, m.zb + FZC + 13) - m.zb;
+    const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
+    const typename S::T* FP[3] = {f.x, f.y, f.z};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        m.rv[c] = __builtin_amdgcn_make_buffer_rsrc((void*)plane_of(VP[c], m.zb, m.P), 0, (int)(zspan * m.pb),
+                                                    BUFFER_WORD3);
+        m.rf[c] = __builtin_amdgcn_make_buffer_rsrc((void*)plane_of(FP[c], m.zb, m.P), 0, (int)(zspan * m.pb),
+                                                    BUFFER_WORD3);
+    }
+    RT v0[4][3], fr[4][3], hr[2][3];
+    float fk[4][3], mk[4], vk[3][4][3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        mk[k] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            fk[k][c] = 0.0f;
+            if (k < 3)
+#pragma unroll
+                for (int s = 0; s < 4; s++) vk[k][s][c] = 0.0f;
+        }
+    }
+    // the loads of the four steps before t0, in the steps' order
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const int z = min(max(t0 - 1 + g, 0), d.Z - 1), k = (g + 3) & 3;
+        const unsigned sz = (unsigned)(z - m.zb) * m.pb;
+        const unsigned sh = (unsigned)(min(max(t0 + g - 2, 0), d.Z - 1) - m.zb) * m.pb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v0[k][c] = R::bload(m.rv[c], m.coff, sz);
+            asm volatile("" ::: "memory");
+            fr[k][c] = R::bload(m.rf[c], m.coff, sz);
+            asm volatile("" ::: "memory");
+            if (g >= 2) {
+                hr[g & 1][c] = R::bload(m.rv[c], m.hoff, sh);
+                asm volatile("" ::: "memory");
+            }
+        }
+    }
+    for (int t = t0; t <= te; t += 4) {
+        f4_step<S, FINISH, 0>(m, t, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, 1>(m, t + 1, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, 2>(m, t + 2, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, 3>(m, t + 3, v0, fr, hr, fk, mk, vk, L0, LK);
+    }
+}
+
 // The volume's boundary voxels after a two-step launch: w takes the stored
 // value at mirror(w) (each coordinate 0 -> 2, N-1 -> N-3), which the
 // two-step kernel computed (an interior voxel, or an x-boundary column of an
@@ -650,8 +915,8 @@ static float storage_unit(float scale) {
 // Whether the two-step kernel can run: f and v in the same units, and the
 // bytes of the planes one z chunk reads addressable by a buffer resource.
 template <class S>
-static bool fused_ok(float scale_f, float scale_v, Dims d) {
-    const double span = (double)d.X * d.Y * sizeof(typename S::T) * (FZC + 9);
+static bool fused_ok(float scale_f, float scale_v, Dims d, int steps = 2) {
+    const double span = (double)d.X * d.Y * sizeof(typename S::T) * (FZC + (steps == 4 ? 13 : 9));
     return scale_f == scale_v && scale_v >= 0x1p-30f && scale_v <= 0x1p30f && span < 2147483648.0;
 }
 
@@ -665,6 +930,19 @@ static void launch_fused2(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst
         hipLaunchKernelGGL((k_gvf_fused2<S, true>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
     else
         hipLaunchKernelGGL((k_gvf_fused2<S, false>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
+}
+
+template <class S>
+static void launch_fused4(bool finish, Planes3<S> f, Planes3<S> src, Out4<S> dst, Dims d, float mu, float scale,
+                          hipStream_t stream) {
+    dim3 grid(d.X >= F4OW ? (d.X + F4OW - 1) / F4OW : 1, d.Y >= F4OH ? (d.Y + F4OH - 1) / F4OH : 1,
+              (d.Z - 2 + FZC - 1) / FZC);
+    dim3 block(FBX, F4Y);
+    const float u = storage_unit<S>(scale), m2s = 1.0f / (u * u);
+    if (finish)
+        hipLaunchKernelGGL((k_gvf_fused4<S, true>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
+    else
+        hipLaunchKernelGGL((k_gvf_fused4<S, false>), grid, block, 0, stream, f, src, dst, d, mu, m2s, u);
 }
 
 // iterations == 0: v = f, w = |f| or 1.
@@ -715,11 +993,15 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
     bool vec2 = (d.X % 2) == 0;
     const void* ptrs[8] = {f->x, f->y, f->z, out->x, out->y, out->z, out->w, A};
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, al));
-    // launches of one or two steps: with fuse >= 2 an odd count starts with
-    // a single step, then pairs
-    const int pairs = fuse >= 2 && fused_ok<S>(f->scale, out->scale, d) ? iters / 2 : 0;
-    const int singles = iters - 2 * pairs;
-    const int launches = pairs + singles;
+    // launches of one, two or four steps: with fuse >= 4 the count is
+    // covered by fours, then at most one two-step and one single launch (run
+    // first); with fuse 2 or 3 by pairs and at most one single
+    const bool f2 = fuse >= 2 && fused_ok<S>(f->scale, out->scale, d);
+    const bool f4 = fuse >= 4 && fused_ok<S>(f->scale, out->scale, d, 4);
+    const int quads = f4 ? iters / 4 : 0;
+    const int pairs = f2 ? (iters - 4 * quads) / 2 : 0;
+    const int singles = iters - 4 * quads - 2 * pairs;
+    const int launches = quads + pairs + singles;
     for (int j = 0; j < launches; j++) {
         // ping-pong so that the last launch lands in `out`
         const bool to_out = ((launches - 1 - j) % 2) == 0;
@@ -730,8 +1012,11 @@ static tsf_status run_euler(const tsf_field* f, const tsf_field* out, Dims d, fl
         if (j < singles) {
             if (vec2) launch_step<S, 2>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
             else launch_step<S, 1>(fin, F, src, dst, d, mu, inv_f, inv_src, sc_v, stream);
-        } else {
+        } else if (j < singles + pairs) {
             launch_fused2<S>(fin, F, src, dst, d, mu, sc_v, stream);
+            launch_faces<S>(fin, dst, d, stream);
+        } else {
+            launch_fused4<S>(fin, F, src, dst, d, mu, sc_v, stream);
             launch_faces<S>(fin, dst, d, stream);
         }
         TSF_TRY(check_launch("tsf_gvf_euler"));
@@ -754,10 +1039,12 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
     bool vec2 = (d.X % 2) == 0;
     const void* ptrs[10] = {f[0], f[1], f[2], v0[0], v0[1], v0[2], out[0], out[1], out[2], ws};
     for (const void* p : ptrs) vec2 = vec2 && (p == nullptr || aligned(p, 8));
-    // two steps per launch (the same float operations as single steps), an
-    // odd count starting with one single step
-    const int pairs = fused_ok<F32>(1.0f, 1.0f, d) ? iters / 2 : 0;
-    const int singles = iters - 2 * pairs, launches = pairs + singles;
+    // four steps per launch (the fastest form for float planes; the same
+    // float operations as single steps), then at most one two-step and one
+    // single launch, run first
+    const int quads = fused_ok<F32>(1.0f, 1.0f, d, 4) ? iters / 4 : 0;
+    const int pairs = fused_ok<F32>(1.0f, 1.0f, d) ? (iters - 4 * quads) / 2 : 0;
+    const int singles = iters - 4 * quads - 2 * pairs, launches = quads + pairs + singles;
     for (int j = 0; j < launches; j++) {
         const bool to_out = ((launches - 1 - j) % 2) == 0;
         Planes3<F32> src = (j == 0) ? V0 : (to_out ? Wr : Or);
@@ -765,8 +1052,11 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
         if (j < singles) {
             if (vec2) launch_step<F32, 2>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
             else launch_step<F32, 1>(false, F, src, dst, d, mu, 1.0f, 1.0f, 1.0f, stream);
-        } else {
+        } else if (j < singles + pairs) {
             launch_fused2<F32>(false, F, src, dst, d, mu, 1.0f, stream);
+            launch_faces<F32>(false, dst, d, stream);
+        } else {
+            launch_fused4<F32>(false, F, src, dst, d, mu, 1.0f, stream);
             launch_faces<F32>(false, dst, d, stream);
         }
         TSF_TRY(check_launch("gvf_steps_f32"));
