@@ -86,6 +86,31 @@ def test_gvf_adjoint_checkpointed(tsf, orc, iters, every, sh):
     assert abs(dmu - odmu) <= 1e-4 * abs(odmu)
 
 
+@pytest.mark.parametrize("sh", [(18, 20, 22), (70, 19, 130), (9, 6, 5)])
+def test_gvf_adjoint_recompute_bit_exact(tsf, sh):
+    # every state stored (checkpoint every iteration, nothing recomputed)
+    # against segments of 11 recomputed by four-step launches and single
+    # steps: the recomputed states are the stored ones bit for bit, so dF
+    # and dmu are identical
+    g = rng(5)
+    F = np.zeros((4,) + sh, np.float32)
+    F[:3] = g.uniform(-1, 1, (3,) + sh) / np.sqrt(3)
+    F[3] = 1
+    dV = tsf.DeviceArray.from_host(g.standard_normal((3,) + sh).astype(np.float32))
+    f = tsf.Field.from_host(F)
+    iters, mu = 23, 0.07
+    out = []
+    for k in (1, 11):
+        V = tsf.Field(sh)
+        cps = tsf.DeviceArray((tsf.gvf_checkpoint_count(iters, k), 3) + sh, np.float32)
+        tsf.gvf_euler_checkpointed(f, V, mu, iters, k, cps)
+        dF = tsf.DeviceArray((3,) + sh, np.float32)
+        dmu = tsf.gvf_euler_adjoint(f, mu, iters, k, cps, dV, dF)
+        out.append((dF.download(), dmu))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+
+
 def test_gvf_finish_adjoint(tsf):
     g = rng(7)
     sh = (6, 7, 8)
