@@ -288,14 +288,16 @@ tsf_status tsf_gvf_finish_adjoint(const tsf_field* v, const tsf_field* dv, tsf_d
  * (+=) dL/dV into dfield.xyz and, when rmax >= 4 (normalised Hessian),
  * dL/dW into dfield.w.  Includes the sampled values, the sample positions
  * and the eigenvector directions (first-order perturbation of e2, e3).
- * Voxels with T = 0 carry no gradient.  The radius search is the forward
- * filter's own code on the same packed texels, so r* is the forward's.
- * Contributions are summed in an LDS tile per 16x4x4 brick and then added
- * with float atomics, so the summation order (last bits) is not
+ * Voxels with T = 0 carry no gradient.  radius: the forward's radius output
+ * (tsf_tdf_circle on the same field and radius range), from which r* is
+ * read and the response at r* recomputed (the forward's own code on the same
+ * packed texels, so bit-identical); NULL repeats the forward's radius search
+ * instead.  Contributions are summed in an LDS tile per 16x4x4 brick and
+ * then added with float atomics, so the summation order (last bits) is not
  * reproducible between runs.  Workspace: the packed field (16 B/voxel). */
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
-                           const float* tdf_grad, const tsf_field* dfield, void* workspace, size_t workspace_bytes,
-                           hipStream_t stream);
+                           const float* tdf_grad, const float* radius, const tsf_field* dfield, void* workspace,
+                           size_t workspace_bytes, hipStream_t stream);
 size_t tsf_tdf_adjoint_workspace_size(tsf_dims dims);
 
 /* Mean squared error of t against target over n floats: *loss (host) =

@@ -154,6 +154,20 @@ def test_tdf_adjoint(tsf, orc, rmin, rmax, rstep):
     print(f"TDF adjoint vs FP64: dV {e:.2e}, dW {ew:.2e}")
     assert e < 1e-3
     assert ew < 1e-3
+    # r* read from the forward's radius output instead of a repeated search:
+    # the same contributions (only the order of the float atomics differs),
+    # also with gradient on voxels without response (T = 0, no r*)
+    for gg in (gT, np.ones_like(gT)):
+        outs = []
+        for rad in (None, R):
+            d2 = tsf.Field(V.shape[1:])
+            for p in d2.planes:
+                p.zero()
+            tsf.tdf_adjoint(fld, rmin, rmax, rstep, tsf.DeviceArray.from_host(gg), d2, radius=rad)
+            outs.append(d2.download())
+        er = rel(outs[1], outs[0])
+        print(f"TDF adjoint with the forward's radius vs repeated search: {er:.2e}")
+        assert er < 1e-6
 
 
 def _forward(tsf, I, **kw):

@@ -512,6 +512,7 @@ struct AdjTile {
 template <bool NORM>
 __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims d, float rmin, float rmax, float rstep,
                                                                CircleTables tab, const float* __restrict__ gT,
+                                                               const float* __restrict__ rad,
                                                                float* __restrict__ dVx, float* __restrict__ dVy,
                                                                float* __restrict__ dVz, float* __restrict__ dW,
                                                                int halo) {
@@ -542,10 +543,27 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
         eigen3(H, E, lam);
         const float e2x = E[0][1], e2y = E[1][1], e2z = E[2][1];
         const float e3x = E[0][2], e3y = E[1][2], e3z = E[2][2];
-        // the forward's radius search
         float maxSum = 0.0f, rstar = 0.0f;
         const float fx = (float)x, fy = (float)y, fz = (float)z;
-        for (float radius = rmin; radius <= rmax; radius += rstep) {
+        if (rad) {
+            // the forward's r*: a radius the search can return (>= rmin) is
+            // one where the forward's response was its sum there; the sum is
+            // recomputed by the forward's code, so bit-identical
+            rstar = rad[d.at(x, y, z)];
+            if (rstar >= rmin && rstar <= rmax) {
+                float sum = 0.0f;
+                for (int j = 0; j < 32; j++) {
+                    const float cj = tab.c[j], sj = tab.s[j];
+                    const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
+                    float s[3];
+                    trilinear_tex(T, d, fx + rstar * vx, fy + rstar * vy, fz + rstar * vz, s);
+                    sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
+                }
+                maxSum = sum / 32.0f;
+            }
+        }
+        // the forward's radius search
+        for (float radius = rmin; !rad && radius <= rmax; radius += rstep) {
             float sum = 0.0f;
             for (int j = 0; j < 32; j++) {
                 const float cj = tab.c[j], sj = tab.s[j];
@@ -944,8 +962,8 @@ tsf_status tsf_mse(const float* t, const float* target, int64_t n, float* grad, 
 size_t tsf_tdf_adjoint_workspace_size(tsf_dims dims) { return (size_t)D(dims).n() * sizeof(float4); }
 
 tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
-                           const float* tdf_grad, const tsf_field* dfield, void* workspace, size_t workspace_bytes,
-                           hipStream_t stream) {
+                           const float* tdf_grad, const float* radius, const tsf_field* dfield, void* workspace,
+                           size_t workspace_bytes, hipStream_t stream) {
     const char* W = "tsf_tdf_adjoint";
     const bool norm = !(rmax < 4.0f);
     TSF_TRY(check_field(field, norm, W));
@@ -980,11 +998,11 @@ tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, fl
     float* dw = (float*)dfield->w;
     dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
     if (norm)
-        hipLaunchKernelGGL(k_tdf_adjoint<true>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad, dx,
-                           dy, dz, dw, halo);
+        hipLaunchKernelGGL(k_tdf_adjoint<true>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
+                           radius, dx, dy, dz, dw, halo);
     else
         hipLaunchKernelGGL(k_tdf_adjoint<false>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
-                           dx, dy, dz, dw, halo);
+                           radius, dx, dy, dz, dw, halo);
     TSF_TRY(check_launch(W));
     if (sc.owned) {
         hipError_t e = hipStreamSynchronize(stream);

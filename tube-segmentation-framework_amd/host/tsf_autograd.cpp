@@ -61,16 +61,16 @@ std::vector<Image3D> TubeFilterFunction::forward(AutogradContext* ctx, OpenCL& o
     ctx->options.checkpoint_every = k;
     ctx->saved_data["mu"] = mu;
     ctx->saved_data["sigma"] = sigma;
-    ctx->save_for_backward({I, B, F, V});
+    ctx->save_for_backward({I, B, F, V, R});
     return {T, R};
 }
 
 TubeFilterGrads TubeFilterFunction::backward(AutogradContext* ctx, OpenCL& ocl,
                                              const std::vector<Image3D>& grad_outputs) {
-    if (!ctx || ctx->get_saved_variables().size() != 4)
+    if (!ctx || ctx->get_saved_variables().size() != 5)
         throw Error(TSF_ERR_INVALID_ARGUMENT, "TubeFilterFunction::backward: context holds no forward");
     const auto& sv = ctx->get_saved_variables();
-    const Image3D &I = sv[0], &B = sv[1], &F = sv[2], &V = sv[3];
+    const Image3D &I = sv[0], &B = sv[1], &F = sv[2], &V = sv[3], &R = sv[4];
     if (grad_outputs.empty() || !grad_outputs[0].valid() || grad_outputs[0].channels() != 1 ||
         grad_outputs[0].storage() != TSF_STORAGE_F32 || grad_outputs[0].size().x != I.size().x ||
         grad_outputs[0].size().y != I.size().y || grad_outputs[0].size().z != I.size().z)
@@ -87,8 +87,8 @@ TubeFilterGrads TubeFilterFunction::backward(AutogradContext* ctx, OpenCL& ocl,
     check(tsf_memset(dV.get(), 0, dV.size(), ocl.queue));
     const tsf_field dv = planes_field(dV.get(), N, true);
     const tsf_field v = V.field();
-    check(tsf_tdf_adjoint(&v, d, o.radiusMin, o.radiusMax, o.radiusStep, (const float*)grad_outputs[0].plane(0), &dv,
-                          ws, wsb, ocl.queue));
+    check(tsf_tdf_adjoint(&v, d, o.radiusMin, o.radiusMax, o.radiusStep, (const float*)grad_outputs[0].plane(0),
+                          (const float*)R.plane(0), &dv, ws, wsb, ocl.queue));
     check(tsf_gvf_finish_adjoint(&v, &dv, d, ocl.queue));
     Buffer dF(3 * N * sizeof(float));
     const tsf_field f = F.field();

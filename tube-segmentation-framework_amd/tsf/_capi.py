@@ -80,7 +80,7 @@ _SIGS = {
     "tsf_gvf_finish_adjoint": ([_PF, _PF, Dims, _vp], _i32),
     "tsf_mse": ([_vp, _vp, ctypes.c_int64, _vp, ctypes.POINTER(ctypes.c_double), _vp, ctypes.c_size_t, _vp], _i32),
     "tsf_mse_workspace_size": ([], ctypes.c_size_t),
-    "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _PF, _vp, ctypes.c_size_t, _vp], _i32),
+    "tsf_tdf_adjoint": ([_PF, Dims, _f32, _f32, _f32, _vp, _vp, _PF, _vp, ctypes.c_size_t, _vp], _i32),
     "tsf_tdf_adjoint_workspace_size": ([Dims], ctypes.c_size_t),
 }
 
@@ -475,11 +475,13 @@ def tdf_adjoint_workspace_size(shape):
     return lib().tsf_tdf_adjoint_workspace_size(_dims(shape))
 
 
-def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None, workspace=None):
+def tdf_adjoint(field, rmin, rmax, rstep, gT, dfield, stream=None, workspace=None, radius=None):
+    """radius: the forward's radius output (r* is read from it), or None to
+    repeat the forward's radius search."""
     a, b = field.desc(), dfield.desc()
     w, wb = _ws(workspace)
-    _check(lib().tsf_tdf_adjoint(ctypes.byref(a), _dims(field.shape), rmin, rmax, rstep, gT.ptr, ctypes.byref(b),
-                                 w, wb, _sp(stream)))
+    _check(lib().tsf_tdf_adjoint(ctypes.byref(a), _dims(field.shape), rmin, rmax, rstep, gT.ptr,
+                                 radius.ptr if radius is not None else None, ctypes.byref(b), w, wb, _sp(stream)))
 
 
 def mse(t, target, grad, workspace=None, stream=None):

@@ -36,7 +36,7 @@ class TubeFilterFunction:
         T = C.DeviceArray(shape, np.float32)
         R = C.DeviceArray(shape, np.float32)
         C.tdf_circle(V, rmin, rmax, rstep, T, R, None, stream)
-        ctx.saved = dict(I=I, B=B, F=F, V=V, cps=cps, k=k, mu=mu, sigma=sigma, iterations=iterations, rmin=rmin,
+        ctx.saved = dict(I=I, B=B, F=F, V=V, R=R, cps=cps, k=k, mu=mu, sigma=sigma, iterations=iterations, rmin=rmin,
                          rmax=rmax, rstep=rstep, fmax=fmax, sign=sign, shape=shape)
         return T, R
 
@@ -52,7 +52,7 @@ class TubeFilterFunction:
         dVf = C.Field.over(dV)
         ws = C.DeviceArray((max(C.gvf_euler_adjoint_workspace_size(shape, s["k"]),
                                 C.tdf_adjoint_workspace_size(shape)),), np.uint8)
-        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dVf, stream, ws)
+        C.tdf_adjoint(s["V"], s["rmin"], s["rmax"], s["rstep"], gT, dVf, stream, ws, radius=s["R"])
         C.gvf_finish_adjoint(s["V"], dVf, stream)
         dF = C.DeviceArray((3,) + shape, np.float32)
         dmu = C.gvf_euler_adjoint(s["F"], s["mu"], s["iterations"], s["k"], s["cps"], dV, dF, ws, stream)
