@@ -292,7 +292,7 @@ __global__ void __launch_bounds__(64 * GTY) k_gvf_march(Planes3<S> f, Planes3<S>
 // z faces from their mirrors.  Tiles are placed so a boundary column and its
 // mirror lie in the same O (the last tile along x and y is shifted to end at
 // the volume's edge).
-constexpr int FBX = 64, FBY = 8;
+constexpr int FBX = 64, FBY = 12;
 constexpr int FOW = FBX - 4, FOH = FBY - 2;
 constexpr int FZC = 64;  // output planes per z chunk, a multiple of 4
 
