@@ -407,9 +407,13 @@ __device__ __forceinline__ void f2_store(const Fused2<S>& m, int p, const float 
         *reinterpret_cast<typename R::T*>(reinterpret_cast<char*>(plane_of(OP[c], p, m.P)) + m.coff) = q[c];
 }
 
+// cv: v0 of planes t-1 and t, fv / mv: f and |f|^2 of plane t-1, all
+// converted at an earlier step and carried in registers (each stored value
+// is converted once).
 template <class S, bool FINISH, int T4>
 __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<S>::T (&v0)[4][3], float (&v1)[4][3],
                                         typename Raw<S>::T (&fr)[4][3], typename Raw<S>::T (&hr)[2][3],
+                                        float (&cv)[2][3], float (&fv)[3], float& mv,
                                         float (&L0)[2][3][FBY + 2][FBX + 2], float (&L1)[2][3][FBY][FBX]) {
     typedef Raw<S> R;
     constexpr int SM1 = (T4 + 3) & 3, S0 = T4, SP1 = (T4 + 1) & 3, SM2 = (T4 + 2) & 3;
@@ -420,15 +424,15 @@ __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<
     float c0[3], cp[3], cm[3], f1[3], f2[3], ch[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        c0[c] = R::getu(v0[S0][c]);
+        cm[c] = cv[0][c];
+        c0[c] = cv[1][c];
         cp[c] = R::getu(v0[SP1][c]);
-        cm[c] = R::getu(v0[SM1][c]);
         f1[c] = R::getu(fr[S0][c]);
-        f2[c] = R::getu(fr[SM1][c]);
+        f2[c] = fv[c];
         ch[c] = R::getu(hr[B][c]);
-        // left alone, the compiler sinks these into the branches that use
-        // them, where the wait becomes a full drain
-        asm volatile("" : "+v"(ch[c]), "+v"(f2[c]));
+        // left alone, the compiler sinks this into the branch that uses it,
+        // where the wait becomes a full drain
+        asm volatile("" : "+v"(ch[c]));
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -443,16 +447,14 @@ __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<
     // v1(t) at this thread's column (lanes 1..FBX-2; the others read the
     // zeroed pad cells and keep 0)
     const bool do1 = m.icol && t >= 1 && t <= m.d.Z - 2;
-    {
-        float m2 = f1[0] * f1[0] + f1[1] * f1[1] + f1[2] * f1[2];
-        if (UnitScaled<S>::value) m2 = m2 * m.m2s;
+    float m1 = f1[0] * f1[0] + f1[1] * f1[1] + f1[2] * f1[2];
+    if (UnitScaled<S>::value) m1 = m1 * m.m2s;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float r = gvf_update(c0[c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1], L0[B][c][ty + 2][lx],
-                                       L0[B][c][ty][lx], cp[c], cm[c], f1[c], m2, m.mu);
-            const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
-            v1[S0][c] = do1 ? rm : 0.0f;
-        }
+    for (int c = 0; c < 3; c++) {
+        const float r = gvf_update(c0[c], L0[B][c][ty + 1][lx + 1], L0[B][c][ty + 1][lx - 1], L0[B][c][ty + 2][lx],
+                                   L0[B][c][ty][lx], cp[c], cm[c], f1[c], m1, m.mu);
+        const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
+        v1[S0][c] = do1 ? rm : 0.0f;
     }
     // plane 0 / Z-1 of v1 equal planes 2 / Z-3 (the mirror rule): the ring
     // slot of plane 0 takes plane 2 when that is computed (t == 2), the slot
@@ -468,8 +470,7 @@ __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<
     // v2(t - 1) at this thread's column
     const int p = t - 1;
     if (m.inO && p >= m.qs && p < m.qe) {
-        float m2 = f2[0] * f2[0] + f2[1] * f2[1] + f2[2] * f2[2];
-        if (UnitScaled<S>::value) m2 = m2 * m.m2s;
+        const float m2 = mv;
         float r[3];
 #pragma unroll
         for (int c = 0; c < 3; c++)
@@ -481,6 +482,13 @@ __device__ __forceinline__ void f2_step(const Fused2<S>& m, int t, typename Raw<
         }
         if (m.icol || m.xbnd) f2_store<S, FINISH>(m, p, r);
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cv[0][c] = c0[c];
+        cv[1][c] = cp[c];
+        fv[c] = f1[c];
+    }
+    mv = m1;
     // loads: v0 and f of plane t+3 into the slots of t-1, the ring-row cells
     // of plane t+2 into this step's ring slot; issued in a fixed order (the
     // prologue repeats it), so the wait counts at the loop head are the same
@@ -592,14 +600,24 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
             }
         }
     }
+    // planes t0-1 and t0 of v0 and plane t0-1 of f, converted once here
+    float cv[2][3], fv[3], mv;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cv[0][c] = R::getu(v0[3][c]);
+        cv[1][c] = R::getu(v0[0][c]);
+        fv[c] = R::getu(fr[3][c]);
+    }
+    mv = fv[0] * fv[0] + fv[1] * fv[1] + fv[2] * fv[2];
+    if (UnitScaled<S>::value) mv = mv * m2s;
     // four unconditional steps per iteration (up to three past te, which
     // store nothing: their v2 plane is >= qe and their loads are clamped) so
     // the loop body has no merge points that would blur the load counters
     for (int t = t0; t <= te; t += 4) {
-        f2_step<S, FINISH, 0>(m, t, v0, v1, fr, hr, L0, L1);
-        f2_step<S, FINISH, 1>(m, t + 1, v0, v1, fr, hr, L0, L1);
-        f2_step<S, FINISH, 2>(m, t + 2, v0, v1, fr, hr, L0, L1);
-        f2_step<S, FINISH, 3>(m, t + 3, v0, v1, fr, hr, L0, L1);
+        f2_step<S, FINISH, 0>(m, t, v0, v1, fr, hr, cv, fv, mv, L0, L1);
+        f2_step<S, FINISH, 1>(m, t + 1, v0, v1, fr, hr, cv, fv, mv, L0, L1);
+        f2_step<S, FINISH, 2>(m, t + 2, v0, v1, fr, hr, cv, fv, mv, L0, L1);
+        f2_step<S, FINISH, 3>(m, t + 3, v0, v1, fr, hr, cv, fv, mv, L0, L1);
     }
 }
 
