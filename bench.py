@@ -102,7 +102,8 @@ def main():
     if args.pmc_workload:
         # Fixed launches for counter passes: k_gvf_copy_finish on F32 planes is
         # the calibration copy (reads 3 x 4 B, writes 4 x 4 B per voxel), then
-        # GVF steps in the benchmark storage and both TDF passes.
+        # GVF steps in the benchmark storage (and FP32 with one, two and four
+        # steps per launch) and both TDF passes.
         F32c, V32c = tsf.Field(shape), tsf.Field(shape)
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, F32c, 0.2, 1, stream)
@@ -112,6 +113,7 @@ def main():
         tsf.gvf_euler(F, V, args.mu, 8, 2, ws, stream)  # two steps per launch
         tsf.gvf_euler(F32c, V32c, args.mu, 2, 1, None, stream)
         tsf.gvf_euler(F32c, V32c, args.mu, 4, 2, None, stream)
+        tsf.gvf_euler(F32c, V32c, args.mu, 8, 4, None, stream)  # four steps per launch
         tsf.tdf_circle(F, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
         tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
         stream.synchronize()

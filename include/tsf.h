@@ -126,10 +126,11 @@ tsf_status tsf_create_vector_field(const float* volume, const tsf_field* out, ts
  *   v(w) <- v(q) + mu*lap(v)(q) - (v(q) - f(q))*|f(q)|^2,  q = mirror(w)
  * with the reference's boundary rule (0 -> 2, N-1 -> N-3); then
  * w = |v| (or 1 where |v| == 0).  f needs planes x, y, z; out needs all four.
- * f and out may use different storage types.  `fuse` iterations are executed
- * per launch (1 = one launch per iteration; the result does not depend on it
- * for F32 storage; for F16/BF16 storage intermediate iterates of a fused
- * launch stay in float).  Note: the reference returns iterate
+ * f and out use the same storage type.  `fuse`: iterations per launch, 1, 2
+ * or 4 (>= 4); a remainder runs first as one two-step and one single launch.
+ * The result does not depend on it for F32 and SN16 storage (bit for bit);
+ * for F16/BF16 the intermediate iterates of a launch stay in float.  Two per
+ * launch is the fastest for 16-bit storage, four for F32.  Note: the reference returns iterate
  * 2*floor(iterations/2) (its finish kernel reads the first ping-pong buffer);
  * this entry point returns iterate `iterations`, the C++ runGVF adapter
  * reproduces the reference's rounding.  Dimensions must be >= 4. */
