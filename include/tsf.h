@@ -86,6 +86,9 @@ tsf_status tsf_event_create(hipEvent_t* ev);
 tsf_status tsf_event_destroy(hipEvent_t ev);
 tsf_status tsf_event_record(hipEvent_t ev, hipStream_t stream);
 tsf_status tsf_event_elapsed_ms(hipEvent_t start, hipEvent_t stop, float* ms);
+/* Work queued on `stream` after this call waits for `ev` (recorded on another
+ * stream): independent stages of the path can run concurrently. */
+tsf_status tsf_stream_wait_event(hipStream_t stream, hipEvent_t ev);
 tsf_status tsf_device_count(int* count);
 /* Free and total device memory of the current device (hipMemGetInfo). */
 tsf_status tsf_mem_info(size_t* free_bytes, size_t* total_bytes);

@@ -152,6 +152,10 @@ tsf_status tsf_event_elapsed_ms(hipEvent_t a, hipEvent_t b, float* ms) {
     HIPCALL(hipEventElapsedTime(ms, a, b), "tsf_event_elapsed_ms");
     return TSF_OK;
 }
+tsf_status tsf_stream_wait_event(hipStream_t s, hipEvent_t ev) {
+    HIPCALL(hipStreamWaitEvent(s, ev, 0), "tsf_stream_wait_event");
+    return TSF_OK;
+}
 tsf_status tsf_device_count(int* count) {
     if (!count) return set_error(TSF_ERR_INVALID_ARGUMENT, "tsf_device_count: NULL");
     hipError_t e = hipGetDeviceCount(count);

@@ -47,6 +47,7 @@ _SIGS = {
     "tsf_event_destroy": ([_vp], _i32),
     "tsf_event_record": ([_vp, _vp], _i32),
     "tsf_event_elapsed_ms": ([_vp, _vp, ctypes.POINTER(_f32)], _i32),
+    "tsf_stream_wait_event": ([_vp, _vp], _i32),
     "tsf_device_count": ([ctypes.POINTER(ctypes.c_int)], _i32),
     "tsf_mem_info": ([ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)], _i32),
     "tsf_create_blur_mask": ([_f32, _vp, ctypes.POINTER(_i32)], _i32),
@@ -191,6 +192,10 @@ class Stream:
 
     def synchronize(self):
         _check(lib().tsf_stream_synchronize(self.handle))
+
+    def wait(self, event):
+        """Work queued after this waits for `event`."""
+        _check(lib().tsf_stream_wait_event(self.handle, event.handle))
 
     def __del__(self):
         if getattr(self, "handle", None) and _lib is not None:
