@@ -88,10 +88,13 @@ def test_gvf_adjoint_checkpointed(tsf, orc, iters, every, sh):
 
 @pytest.mark.parametrize("sh", [(18, 20, 22), (70, 19, 130), (9, 6, 5)])
 def test_gvf_adjoint_recompute_bit_exact(tsf, sh):
-    # every state stored (checkpoint every iteration, nothing recomputed)
-    # against segments of 11 recomputed by four-step launches and single
-    # steps: the recomputed states are the stored ones bit for bit, so dF
-    # and dmu are identical
+    # every state stored (checkpoint every iteration, nothing recomputed,
+    # every reverse step a single launch) against segments of 2 and 11
+    # recomputed by four-step launches and single steps and reversed two
+    # steps per launch (the last pair of segment 0 ends at v^0 = f): the
+    # recomputed states are the stored ones and a two-step launch is two
+    # single ones bit for bit, so dF is identical; dmu sums the same terms
+    # in double over other blocks, so it agrees to double rounding
     g = rng(5)
     F = np.zeros((4,) + sh, np.float32)
     F[:3] = g.uniform(-1, 1, (3,) + sh) / np.sqrt(3)
@@ -100,15 +103,17 @@ def test_gvf_adjoint_recompute_bit_exact(tsf, sh):
     f = tsf.Field.from_host(F)
     iters, mu = 23, 0.07
     out = []
-    for k in (1, 11):
+    for k in (1, 2, 11):
         V = tsf.Field(sh)
         cps = tsf.DeviceArray((tsf.gvf_checkpoint_count(iters, k), 3) + sh, np.float32)
         tsf.gvf_euler_checkpointed(f, V, mu, iters, k, cps)
         dF = tsf.DeviceArray((3,) + sh, np.float32)
         dmu = tsf.gvf_euler_adjoint(f, mu, iters, k, cps, dV, dF)
         out.append((dF.download(), dmu))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
+    for k, (dF, dmu) in zip((2, 11), out[1:]):
+        print(f"k {k}: dF identical {np.mean(dF == out[0][0]):.6f}, dmu {dmu!r} vs {out[0][1]!r}")
+        assert np.array_equal(dF, out[0][0])
+        assert abs(dmu - out[0][1]) <= 1e-12 * abs(out[0][1])
 
 
 def test_gvf_finish_adjoint(tsf):

@@ -18,6 +18,8 @@
 // The GVF backward runs over checkpoints of the forward taken every k
 // iterations; each segment is recomputed into a k-state buffer.
 #include <math.h>
+
+#include <algorithm>
 #include <vector>
 
 #include "eigen3.hpp"
@@ -223,12 +225,15 @@ __global__ void k_fold3(P3 lam, Dims d, M3 out) {
 // boundary plane), v = v^n, F.  lam^n(y) = (1 - 6 mu - |f(y)|^2) Lf(y) + mu sum_e
 // Lf(y+e).  FOLD: out = fold(lam^n), the input of the next reverse step;
 // otherwise (the step from v^0 = f) dF += lam^n.  Always: dF += data-term
-// gradient Lf m2 + 2 f sum_c Lf_c (f_c - v_c); part[block] = sum Lf . lap(v).
-// Structure: a workgroup of ABX x
-// ABY threads owns that tile of columns and a chunk of AZC planes; per plane
-// (one barrier) Lf and v of the plane, with a one-column ring, go to LDS for
-// the x/y neighbours, and each thread keeps its column's planes in register
-// rings loaded three planes ahead.
+// gradient Lf m2 + 2 f sum_c Lf_c (f_c - v_c); part[block] = the block's
+// share of d/dmu = sum over interior q of Lf(q) . lap(v)(q).  Lf is zero off
+// the interior and the 7-point Laplacian is symmetric, so that sum equals
+// sum over every voxel w of v(w) . (S(w) - 6 Lf(w)), S(w) the sum of Lf over
+// w's in-volume neighbours, which the update computes anyway.
+// Structure: a workgroup of ABX x ABY threads owns that tile of columns and a
+// chunk of AZC planes; per plane (one barrier) Lf of the plane, with a
+// one-column ring, goes to LDS for the x/y neighbours, and each thread keeps
+// its column's planes in register rings loaded three planes ahead.
 // The fold of the output needs no gather: Lf is zero on every boundary
 // plane, so lam at a boundary position w is mu * Lf at w's single interior
 // neighbour when w has one boundary coordinate, and zero when it has more.
@@ -249,23 +254,16 @@ struct AdjCtx {
 
 template <bool FOLD, int T4>
 __device__ __forceinline__ void adj_step(const AdjCtx& m, int t, float (&lf)[4][3], float (&vv)[4][3],
-                                         float (&fr)[4][3], float (&dfr)[4][3], float (&hl)[2][3], float (&hv)[2][3],
-                                         float (&LL)[2][3][ABY + 2][ABX + 2], float (&LV)[2][3][ABY + 2][ABX + 2],
-                                         double& gm) {
+                                         float (&fr)[4][3], float (&dfr)[4][3], float (&hl)[2][3],
+                                         float (&LL)[2][3][ABY + 2][ABX + 2], double& gm) {
     constexpr int SM1 = (T4 + 3) & 3, S0 = T4, SP1 = (T4 + 1) & 3, B = T4 & 1;
     const int lane = m.lane, ty = m.ty;
     // out-of-volume cells read as zero
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        LL[B][c][ty + 1][lane + 1] = m.act ? lf[S0][c] : 0.0f;
-        LV[B][c][ty + 1][lane + 1] = m.act ? vv[S0][c] : 0.0f;
-    }
+    for (int c = 0; c < 3; c++) LL[B][c][ty + 1][lane + 1] = m.act ? lf[S0][c] : 0.0f;
     if (m.hrole) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            LL[B][c][m.hly][m.hlx] = m.hok ? hl[B][c] : 0.0f;
-            LV[B][c][m.hly][m.hlx] = m.hok ? hv[B][c] : 0.0f;
-        }
+        for (int c = 0; c < 3; c++) LL[B][c][m.hly][m.hlx] = m.hok ? hl[B][c] : 0.0f;
     }
     __syncthreads();
     if (m.act && t < m.z1) {
@@ -289,11 +287,7 @@ __device__ __forceinline__ void adj_step(const AdjCtx& m, int t, float (&lf)[4][
             lam[c] = diag * l0 + m.mu * nb;
             const float v0 = vv[S0][c];
             s += l0 * (fc[c] - v0);
-            if (interior) {
-                const float lap = LV[B][c][ty + 1][lane + 2] + LV[B][c][ty + 1][lane] + LV[B][c][ty + 2][lane + 1] +
-                                  LV[B][c][ty][lane + 1] + vv[SP1][c] + vv[SM1][c] - 6.0f * v0;
-                gm += (double)l0 * (double)lap;
-            }
+            gm += (double)v0 * (double)(nb - 6.0f * l0);
             const float e = (m.fx2 ? lxm : 0.0f) + (m.fxN ? lxp : 0.0f) + (m.fy2 ? lym : 0.0f) +
                             (m.fyN ? lyp : 0.0f) + (fz2 ? lzm : 0.0f) + (fzN ? lzp : 0.0f);
             fold[c] = interior ? lam[c] + m.mu * e : 0.0f;
@@ -318,7 +312,6 @@ __device__ __forceinline__ void adj_step(const AdjCtx& m, int t, float (&lf)[4][
         fr[SM1][c] = m.F.c[c][i3];
         dfr[SM1][c] = m.dF.c[c][i3];
         hl[B][c] = m.Lf.c[c][ih];
-        hv[B][c] = m.v.c[c][ih];
     }
 }
 
@@ -326,7 +319,6 @@ template <bool FOLD>
 __global__ void __launch_bounds__(ABX* ABY, 4) k_gvf_adj_march(P3 Lf, P3 v, P3 F, Dims d, float mu, M3 out, M3 dF,
                                                             double* __restrict__ part) {
     __shared__ float LL[2][3][ABY + 2][ABX + 2];
-    __shared__ float LV[2][3][ABY + 2][ABX + 2];
     int tbx, tby, tbz;
     xcd_tile(tbx, tby, tbz);
     AdjCtx m;
@@ -374,7 +366,7 @@ __global__ void __launch_bounds__(ABX* ABY, 4) k_gvf_adj_march(P3 Lf, P3 v, P3 F
     m.hidx = m.hok ? (i64)hy * d.X + hx : 0;
     const int z0 = tbz * AZC;
     m.z1 = min(z0 + AZC, d.Z);
-    float lf[4][3], vv[4][3], fr[4][3], dfr[4][3], hl[2][3], hv[2][3];
+    float lf[4][3], vv[4][3], fr[4][3], dfr[4][3], hl[2][3];
     // the loads of the four steps before z0 in the steps' order
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -389,16 +381,15 @@ __global__ void __launch_bounds__(ABX* ABY, 4) k_gvf_adj_march(P3 Lf, P3 v, P3 F
             if (g >= 2) {
                 const i64 ih = m.hok ? min(z0 + g - 2, d.Z - 1) * m.P + m.hidx : 0;
                 hl[g & 1][c] = Lf.c[c][ih];
-                hv[g & 1][c] = v.c[c][ih];
             }
         }
     }
     double gm = 0.0;
     for (int t = z0; t < m.z1; t += 4) {
-        adj_step<FOLD, 0>(m, t, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
-        adj_step<FOLD, 1>(m, t + 1, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
-        adj_step<FOLD, 2>(m, t + 2, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
-        adj_step<FOLD, 3>(m, t + 3, lf, vv, fr, dfr, hl, hv, LL, LV, gm);
+        adj_step<FOLD, 0>(m, t, lf, vv, fr, dfr, hl, LL, gm);
+        adj_step<FOLD, 1>(m, t + 1, lf, vv, fr, dfr, hl, LL, gm);
+        adj_step<FOLD, 2>(m, t + 2, lf, vv, fr, dfr, hl, LL, gm);
+        adj_step<FOLD, 3>(m, t + 3, lf, vv, fr, dfr, hl, LL, gm);
     }
     __shared__ double sh[ABX * ABY];
     const int tid = m.ty * ABX + m.lane;
@@ -409,6 +400,266 @@ __global__ void __launch_bounds__(ABX* ABY, 4) k_gvf_adj_march(P3 Lf, P3 v, P3 F
         __syncthreads();
     }
     if (tid == 0) part[blockIdx.x + (i64)gridDim.x * (blockIdx.y + (i64)gridDim.y * blockIdx.z)] = sh[0];
+}
+
+// ---------------------------------------------------------------- two reverse steps per launch
+// k_gvf_adj2 runs reverse steps n+1 and n in one launch, so the adjoint, the
+// states, f and dF cross HBM once per two steps:
+//   in   Lf = fold(lam^{n+2}) (zero on every boundary plane), va = v^{n+1},
+//        vb = v^n, F, dF
+//   Lf1 = fold(lam^{n+1}) is computed on the tile E and kept on chip
+//   out  fold(lam^n) (FOLD), or dF += lam^n (the step from v^0 = f)
+//   dF  += the data-term gradients of both steps
+//   part[b], part[nblk + b]: block b's share of d/dmu of step n+1 and of
+//        step n, in the form of k_gvf_adj_march (v . (S - 6 Lf) per voxel)
+// A workgroup of 64 x A2Y threads (one wave per row) covers 64 x A2Y columns
+// starting two columns left of and one row above its output tile O (A2OW x
+// A2OH columns); Lf1 is computed on every row at lanes 1..62 (the tile E),
+// the second step on O.  Waves 0 and 1 also load the Lf rows just above and
+// below the workgroup.  Tiles do not overlap, since dF is read and written:
+// the last one along x and y ends past the volume's edge.  A chunk of A2ZC
+// output planes is marched in z; per step t (one barrier): LDS <- Lf plane t
+// (with the ring rows) and Lf1 plane t-1; Lf1(t) on E from the LDS x/y
+// neighbours and the register ring of Lf planes t-1..t+1 (with step n+1's
+// d/dmu term at plane t on O); then, at plane t-1 on O, lam^n from Lf1
+// planes t-2..t, the data terms of both steps, step n's d/dmu term, the dF
+// update and the output.  Every value comes from the expressions of
+// k_gvf_adj_march in the same order (the build does not contract), so one
+// launch of this kernel and two of that one give the same adjoint and dF
+// bit for bit.  Loads run ahead into register rings: Lf and f three planes,
+// va and the ring rows two, vb and dF one, issued in that order so every
+// value is waited for a step after its load.
+constexpr int A2X = 64, A2Y = 12;
+constexpr int A2OW = A2X - 4, A2OH = A2Y - 2;
+constexpr int A2ZC = 64;  // output planes per z chunk
+
+// The arrays are read and written through buffer resources: L, va, vb, dF
+// and out each hold three component volumes back to back (one resource each,
+// the component and plane offsets in a scalar register), f's components are
+// three resources.  The host launches the kernel only when 3 volumes' bytes
+// fit 32 bits.
+struct Adj2Ctx {
+    __amdgpu_buffer_rsrc_t rL, ra, rb, rd, ro, rf[3];
+    unsigned cb, pb;     // bytes per component volume, per plane
+    unsigned col, hcol;  // byte offsets of this thread's cell and its ring-row cell within a plane
+    unsigned ocol;       // col on the rows of O, else past every resource's range (the load reads 0, no memory)
+    Dims d;
+    float mu;
+    int lane, ty, q0, q1, hly;
+    bool act, icol, own, hrow, hin, fx2, fxN, fy2, fyN;
+};
+
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    // keeps the loads in program order, so the waits counted at the loop
+    // head are the same on entry and on the back edge
+    asm volatile("" ::: "memory");
+    return v;
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+
+template <bool FOLD, int K>
+__device__ __forceinline__ void adj2_step(const Adj2Ctx& m, int t, float (&lr)[4][3], float (&ar)[4][3],
+                                          float (&br)[4][3], float (&fr)[4][3], float (&dr)[4][3],
+                                          float (&l1)[4][3], float (&hl)[2][3], float& mq,
+                                          float (&LL)[2][3][A2Y + 2][A2X + 2], float (&L1)[2][3][A2Y][A2X],
+                                          double& gmA, double& gmB) {
+    constexpr int S0 = K, SM1 = (K + 3) & 3, SM2 = (K + 2) & 3, SP1 = (K + 1) & 3;
+    constexpr int B = K & 1, BP = (K + 1) & 1;
+    const int lane = m.lane, ty = m.ty, Z = m.d.Z;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        LL[B][c][ty + 1][lane + 1] = m.act ? lr[S0][c] : 0.0f;
+        L1[BP][c][ty][lane] = l1[SM1][c];
+    }
+    if (m.hrow) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) LL[B][c][m.hly][lane + 1] = m.hin ? hl[B][c] : 0.0f;
+    }
+    __syncthreads();
+    // Lf1 = fold(lam^{n+1}) at plane t (zero off the interior), and step
+    // n+1's d/dmu term at plane t on O
+    const bool own_t = m.own && t >= m.q0 && t < m.q1;
+    const float f0 = fr[S0][0], f1 = fr[S0][1], f2 = fr[S0][2];
+    const float m2 = f0 * f0 + f1 * f1 + f2 * f2;
+    {
+        const bool in = m.icol && t >= 1 && t <= Z - 2;
+        const float diag = 1.0f - 6.0f * m.mu - m2;
+        const bool fz2 = t == 2, fzN = t == Z - 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float l0 = lr[S0][c];
+            const float lxp = LL[B][c][ty + 1][lane + 2], lxm = LL[B][c][ty + 1][lane];
+            const float lyp = LL[B][c][ty + 2][lane + 1], lym = LL[B][c][ty][lane + 1];
+            const float lzp = t < Z - 1 ? lr[SP1][c] : 0.0f, lzm = t > 0 ? lr[SM1][c] : 0.0f;
+            const float nb = lxp + lxm + lyp + lym + lzp + lzm;
+            const float lam = diag * l0 + m.mu * nb;
+            const double q = (double)ar[S0][c] * (double)(nb - 6.0f * l0);
+            gmA += own_t ? q : 0.0;
+            const float e = (m.fx2 ? lxm : 0.0f) + (m.fxN ? lxp : 0.0f) + (m.fy2 ? lym : 0.0f) +
+                            (m.fyN ? lyp : 0.0f) + (fz2 ? lzm : 0.0f) + (fzN ? lzp : 0.0f);
+            l1[S0][c] = in ? lam + m.mu * e : 0.0f;
+        }
+    }
+    // the second step at plane p = t - 1 (|f|^2 of plane p carried from the
+    // last step)
+    const int p = t - 1;
+    if (m.own && p >= m.q0 && p < m.q1) {
+        const float m2p = mq;
+        const bool pin = m.icol && p >= 1 && p <= Z - 2;
+        const bool pz2 = p == 2, pzN = p == Z - 3;
+        const float diagp = 1.0f - 6.0f * m.mu - m2p;
+        const float fc[3] = {fr[SM1][0], fr[SM1][1], fr[SM1][2]};
+        float lam0[3], fold0[3];
+        float sA = 0.0f, sB = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float k0 = l1[SM1][c];
+            const float kxp = L1[BP][c][ty][lane + 1], kxm = L1[BP][c][ty][lane - 1];
+            const float kyp = L1[BP][c][ty + 1][lane], kym = L1[BP][c][ty - 1][lane];
+            const float kzp = l1[S0][c], kzm = l1[SM2][c];
+            const float nb = kxp + kxm + kyp + kym + kzp + kzm;
+            lam0[c] = diagp * k0 + m.mu * nb;
+            sA += lr[SM1][c] * (fc[c] - ar[SM1][c]);
+            sB += k0 * (fc[c] - br[SM1][c]);
+            gmB += (double)br[SM1][c] * (double)(nb - 6.0f * k0);
+            const float e = (m.fx2 ? kxm : 0.0f) + (m.fxN ? kxp : 0.0f) + (m.fy2 ? kym : 0.0f) +
+                            (m.fyN ? kyp : 0.0f) + (pz2 ? kzm : 0.0f) + (pzN ? kzp : 0.0f);
+            fold0[c] = pin ? lam0[c] + m.mu * e : 0.0f;
+        }
+        const unsigned sp = (unsigned)p * m.pb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float g = dr[SM1][c] + (lr[SM1][c] * m2p + 2.0f * fc[c] * sA);
+            g = g + (l1[SM1][c] * m2p + 2.0f * fc[c] * sB);
+            if (!FOLD) g += lam0[c];
+            bst(m.rd, m.col, c * m.cb + sp, g);
+            if (FOLD) bst(m.ro, m.col, c * m.cb + sp, fold0[c]);
+        }
+    }
+    mq = m2;
+    // loads: Lf and f of plane t+3, va and the Lf ring-row cells of plane
+    // t+2, vb and dF of plane t+1
+    const unsigned s3 = (unsigned)min(t + 3, Z - 1) * m.pb, s2 = (unsigned)min(t + 2, Z - 1) * m.pb;
+    const unsigned s1 = (unsigned)min(t + 1, Z - 1) * m.pb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        lr[SM1][c] = bld(m.rL, m.col, c * m.cb + s3);
+        fr[SM1][c] = bld(m.rf[c], m.col, s3);
+        ar[SM2][c] = bld(m.ra, m.ocol, c * m.cb + s2);
+        hl[B][c] = bld(m.rL, m.hcol, c * m.cb + s2);
+        br[SP1][c] = bld(m.rb, m.ocol, c * m.cb + s1);
+        dr[SP1][c] = bld(m.rd, m.ocol, c * m.cb + s1);
+    }
+}
+
+template <bool FOLD>
+__global__ void __launch_bounds__(A2X* A2Y, 4) k_gvf_adj2(const float* L, const float* va, const float* vb, P3 F,
+                                                         Dims d, float mu, float* out, float* dF,
+                                                         double* __restrict__ part) {
+    __shared__ float LL[2][3][A2Y + 2][A2X + 2];
+    __shared__ float L1[2][3][A2Y][A2X];
+    __shared__ double red[2][A2Y];
+    Adj2Ctx m;
+    m.d = d;
+    m.mu = mu;
+    m.lane = threadIdx.x;
+    m.ty = threadIdx.y;
+    m.pb = (unsigned)d.X * (unsigned)d.Y * 4u;
+    m.cb = m.pb * (unsigned)d.Z;
+    const int nr = (int)(3u * m.cb), nf = (int)m.cb;
+    m.rL = __builtin_amdgcn_make_buffer_rsrc((void*)L, 0, nr, BUFFER_WORD3);
+    m.ra = __builtin_amdgcn_make_buffer_rsrc((void*)va, 0, nr, BUFFER_WORD3);
+    m.rb = __builtin_amdgcn_make_buffer_rsrc((void*)vb, 0, nr, BUFFER_WORD3);
+    m.rd = __builtin_amdgcn_make_buffer_rsrc((void*)dF, 0, nr, BUFFER_WORD3);
+    m.ro = __builtin_amdgcn_make_buffer_rsrc((void*)out, 0, nr, BUFFER_WORD3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) m.rf[c] = __builtin_amdgcn_make_buffer_rsrc((void*)F.c[c], 0, nf, BUFFER_WORD3);
+    // the pad cells left and right of every Lf row hold 0
+    for (int i = m.ty * A2X + m.lane; i < 2 * 3 * (A2Y + 2) * 2; i += A2X * A2Y) {
+        const int side = i & 1, row = i >> 1;
+        (&LL[0][0][0][0])[row * (A2X + 2) + (side ? A2X + 1 : 0)] = 0.0f;
+    }
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int ex = tbx * A2OW - 2, ey = tby * A2OH - 1;
+    const int cx = ex + m.lane, cy = ey + m.ty;
+    m.act = cx >= 0 && cx < d.X && cy >= 0 && cy < d.Y;
+    m.icol = m.act && cx >= 1 && cx <= d.X - 2 && cy >= 1 && cy <= d.Y - 2;
+    m.own = m.act && m.lane >= 2 && m.lane <= A2X - 3 && m.ty >= 1 && m.ty <= A2Y - 2;
+    m.fx2 = cx == 2;
+    m.fxN = cx == d.X - 3;
+    m.fy2 = cy == 2;
+    m.fyN = cy == d.Y - 3;
+    m.col = m.act ? (unsigned)(cy * d.X + cx) * 4u : 0u;
+    // va, vb and dF are used on O only: the waves of rows 0 and A2Y-1 read
+    // none of them (a buffer load at an offset of 2^31 or more is past the
+    // range, 3 volumes < 2^31 bytes, and returns 0 without a memory access)
+    m.ocol = m.ty >= 1 && m.ty <= A2Y - 2 ? m.col : 0x80000000u;
+    m.hrow = m.ty == 0 || m.ty == 1;
+    const int hy = m.ty == 0 ? ey - 1 : ey + A2Y;
+    m.hly = m.ty == 0 ? 0 : A2Y + 1;
+    m.hin = m.hrow && cx >= 0 && cx < d.X && hy >= 0 && hy < d.Y;
+    m.hcol = m.hin ? (unsigned)(hy * d.X + cx) * 4u : 0u;
+    m.q0 = tbz * A2ZC;
+    m.q1 = min(m.q0 + A2ZC, d.Z);
+    const int t0 = m.q0 - 1, te = m.q1;
+    float lr[4][3], ar[4][3], br[4][3], fr[4][3], dr[4][3], l1[4][3], hl[2][3];
+    // the loads of the four steps before t0, in the order the steps issue
+    // them; ring slot k holds a plane congruent to t0 + k (mod 4)
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const int s = t0 - 4 + g, k3 = (g + 3) & 3, k2 = (g + 2) & 3, k1 = (g + 1) & 3;
+        const unsigned s3 = (unsigned)min(max(s + 3, 0), d.Z - 1) * m.pb;
+        const unsigned s2 = (unsigned)min(max(s + 2, 0), d.Z - 1) * m.pb;
+        const unsigned s1 = (unsigned)min(max(s + 1, 0), d.Z - 1) * m.pb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            lr[k3][c] = bld(m.rL, m.col, c * m.cb + s3);
+            fr[k3][c] = bld(m.rf[c], m.col, s3);
+            ar[k2][c] = bld(m.ra, m.ocol, c * m.cb + s2);
+            hl[g & 1][c] = bld(m.rL, m.hcol, c * m.cb + s2);
+            br[k1][c] = bld(m.rb, m.ocol, c * m.cb + s1);
+            dr[k1][c] = bld(m.rd, m.ocol, c * m.cb + s1);
+            l1[g][c] = 0.0f;
+        }
+    }
+    float mq = 0.0f;
+    double gmA = 0.0, gmB = 0.0;
+    // four unconditional steps per iteration (up to three past te: they
+    // store nothing, and their loads are clamped into the volume)
+    for (int t = t0; t <= te; t += 4) {
+        adj2_step<FOLD, 0>(m, t, lr, ar, br, fr, dr, l1, hl, mq, LL, L1, gmA, gmB);
+        adj2_step<FOLD, 1>(m, t + 1, lr, ar, br, fr, dr, l1, hl, mq, LL, L1, gmA, gmB);
+        adj2_step<FOLD, 2>(m, t + 2, lr, ar, br, fr, dr, l1, hl, mq, LL, L1, gmA, gmB);
+        adj2_step<FOLD, 3>(m, t + 3, lr, ar, br, fr, dr, l1, hl, mq, LL, L1, gmA, gmB);
+    }
+    // block sums in a fixed order: within each wave by xor shuffles, then
+    // the waves in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        gmA += __shfl_xor(gmA, o);
+        gmB += __shfl_xor(gmB, o);
+    }
+    if (m.lane == 0) {
+        red[0][m.ty] = gmA;
+        red[1][m.ty] = gmB;
+    }
+    __syncthreads();
+    if (m.lane == 0 && m.ty == 0) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int w = 0; w < A2Y; w++) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        const i64 nblk = (i64)gridDim.x * gridDim.y * gridDim.z;
+        const i64 blk = blockIdx.x + (i64)gridDim.x * (blockIdx.y + (i64)gridDim.y * blockIdx.z);
+        part[blk] = a;
+        part[nblk + blk] = b;
+    }
 }
 
 // part2[b] = sum of a contiguous chunk of n partials (fixed order)
@@ -842,6 +1093,15 @@ static size_t adj_blocks(const Dims& d) {
     const dim3 g = adj_grid(d);
     return (size_t)g.x * g.y * g.z;
 }
+static dim3 adj2_grid(const Dims& d) {
+    return dim3((d.X + A2OW - 1) / A2OW, (d.Y + A2OH - 1) / A2OH, (d.Z + A2ZC - 1) / A2ZC);
+}
+static size_t adj2_blocks(const Dims& d) {
+    const dim3 g = adj2_grid(d);
+    return (size_t)g.x * g.y * g.z;
+}
+// d/dmu block partials: one set per single reverse step, two per pair
+static size_t adj_parts(const Dims& d) { return std::max(adj_blocks(d), 2 * adj2_blocks(d)); }
 
 // float states first, then the double partial sums at a 256-byte boundary
 static size_t adj_float_bytes(const Dims& d, int k) {
@@ -851,7 +1111,7 @@ static size_t adj_float_bytes(const Dims& d, int k) {
 size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every) {
     const Dims d = D(dims);
     if (checkpoint_every < 1) return 0;
-    return adj_float_bytes(d, checkpoint_every) + (NPART + 1 + adj_blocks(d)) * sizeof(double);
+    return adj_float_bytes(d, checkpoint_every) + (NPART + 1 + adj_parts(d)) * sizeof(double);
 }
 
 tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
@@ -877,7 +1137,10 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
     double* part2 = (double*)((char*)sc.ptr + adj_float_bytes(d, k));  // NPART
     double* acc = part2 + NPART;
     double* part = acc + 1;                   // one per stencil block
-    const i64 nblk = (i64)adj_blocks(d);
+    const i64 nblk = (i64)adj_blocks(d), nblk2 = (i64)adj2_blocks(d);
+    // two reverse steps per launch address three volumes with 32-bit byte
+    // offsets
+    const bool pairs = 3 * N * (i64)sizeof(float) < ((i64)1 << 31);
     const float* fp[3] = {(const float*)f->x, (const float*)f->y, (const float*)f->z};
     hipError_t e = hipMemsetAsync(dF, 0, 3 * N * sizeof(float), stream);
     if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
@@ -902,7 +1165,26 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
             float* nvv[3] = {nv, nv + N, nv + 2 * N};
             TSF_TRY(gvf_steps_f32(fp, pvv, nvv, d, mu, 1, tmp, stream));
         }
-        for (int n = end - 1; n >= start; n--) {
+        // steps n and n-1 in one launch while both lie in the segment; the
+        // adjoint, dF and d/dmu come out as from two single steps
+        int n = end - 1;
+        for (; pairs && n - 1 >= start; n -= 2) {
+            const float* va = seg + (i64)(n - start) * 3 * N;
+            const float* vb = va - 3 * N;
+            if (n - 1 > 0)
+                hipLaunchKernelGGL(k_gvf_adj2<true>, adj2_grid(d), dim3(A2X, A2Y), 0, stream, lin, va, vb, Fp, d, mu,
+                                   lout, dF, part);
+            else
+                hipLaunchKernelGGL(k_gvf_adj2<false>, adj2_grid(d), dim3(A2X, A2Y), 0, stream, lin, va, vb, Fp, d,
+                                   mu, lout, dF, part);
+            for (int h = 0; h < 2; h++) {
+                hipLaunchKernelGGL(k_reduce_parts, NPART, RB, 0, stream, part + h * nblk2, nblk2, part2);
+                hipLaunchKernelGGL(k_fold_partials, 1, RB, 0, stream, part2, NPART, acc);
+            }
+            TSF_TRY(check_launch(W));
+            std::swap(lin, lout);
+        }
+        for (; n >= start; n--) {
             const float* vn = seg + (i64)(n - start) * 3 * N;
             P3 L{{lin, lin + N, lin + 2 * N}};
             P3 Vn{{vn, vn + N, vn + 2 * N}};
