@@ -106,32 +106,18 @@ __device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx,
 // e3, best sum) are compacted to the front of the workgroup through LDS, so
 // a wave's lanes stay busy and waves with no survivor retire.  The radius of
 // round k is the same float accumulation for every voxel, so each voxel's
-// arithmetic is that of the sequential loop.
-template <class S, bool NORM, int BX, int BY, int BZ>
-__global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>::P* __restrict__ tex, float inv,
-                                                        Dims d, float rmin, float rmax, float rstep, CircleTables tab,
-                                                        float* __restrict__ tdf, float* __restrict__ rad, int h) {
+// arithmetic is that of the sequential loop.  T is the staged tile (its
+// fill is the caller's, followed by a barrier).
+template <bool NORM, int BX, int BY, int BZ, class A>
+__device__ __forceinline__ void tdf_brick(const A& T, Dims d, int bx0, int by0, int bz0, float rmin, float rmax,
+                                          float rstep, const CircleTables& tab, float* __restrict__ tdf,
+                                          float* __restrict__ rad) {
     constexpr int NT = BX * BY * BZ, NW = NT / 64;
-    typedef typename Texel<S>::P P;
-    extern __shared__ float4 lds_f4[];
     __shared__ float st_e[6][NT];
     __shared__ float st_m[NT];
     __shared__ int st_v[NT];
     __shared__ int wcount[NW];
-    P* tile = reinterpret_cast<P*>(lds_f4);
-    int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz);
-    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
-    LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
-    const int cells = T.tx * T.ty * T.tz;
-    for (int c = tid; c < cells; c += NT) {
-        const int lx = c % T.tx, rr = c / T.tx, ly = rr % T.ty, lz = rr / T.ty;
-        const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
-        const int gz = clampi(T.oz + lz, 0, d.Z - 1);
-        tile[c] = tex[d.at(gx, gy, gz)];
-    }
-    __syncthreads();
     // round 0 at the thread's own voxel
     int v = tid;  // voxel of the brick this thread works on
     float e2[3], e3[3], best = 0.0f;
@@ -224,6 +210,61 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>:
     }
 }
 
+// Packed texels (8 bytes per cell for the 16-bit storage types).
+template <class S, bool NORM, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>::P* __restrict__ tex, float inv,
+                                                        Dims d, float rmin, float rmax, float rstep, CircleTables tab,
+                                                        float* __restrict__ tdf, float* __restrict__ rad, int h) {
+    constexpr int NT = BX * BY * BZ;
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    const int cells = T.tx * T.ty * T.tz;
+    for (int c = tid; c < cells; c += NT) {
+        const int lx = c % T.tx, rr = c / T.tx, ly = rr % T.ty, lz = rr / T.ty;
+        const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
+        const int gz = clampi(T.oz + lz, 0, d.Z - 1);
+        tile[c] = tex[d.at(gx, gy, gz)];
+    }
+    __syncthreads();
+    tdf_brick<NORM, BX, BY, BZ>(T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
+// F32 fields straight from their planes: the tile holds x, y and z as three
+// float planes (12 bytes per cell); w, read only by the normalised Hessian,
+// comes from its plane in device memory.
+template <bool NORM, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_f32(const float* __restrict__ fx, const float* __restrict__ fy, const float* __restrict__ fz,
+                  const float* __restrict__ fw, Dims d, float rmin, float rmax, float rstep, CircleTables tab,
+                  float* __restrict__ tdf, float* __restrict__ rad, int h) {
+    constexpr int NT = BX * BY * BZ;
+    extern __shared__ float4 lds_f4[];
+    float* tile = reinterpret_cast<float*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    const int tx = BX + 2 * h, ty = BY + 2 * h, tz = BZ + 2 * h, cells = tx * ty * tz;
+    LdsSoaTex T{tile, fw, cells, bx0 - h, by0 - h, bz0 - h, tx, ty, tz};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    for (int c = tid; c < cells; c += NT) {
+        const int lx = c % tx, rr = c / tx, ly = rr % ty, lz = rr / ty;
+        const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
+        const int gz = clampi(T.oz + lz, 0, d.Z - 1);
+        const i64 g = d.at(gx, gy, gz);
+        tile[c] = fx[g];
+        tile[cells + c] = fy[g];
+        tile[2 * cells + c] = fz[g];
+    }
+    __syncthreads();
+    tdf_brick<NORM, BX, BY, BZ>(T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
 static unsigned grid1d(i64 n) {
     i64 g = (n + 255) / 256;
     return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -242,6 +283,26 @@ static int tdf_halo(float rmin, float rmax) {
     return (int)floorf(r + 0.5f) + 1;
 }
 
+// F32 through the three-plane tile (bricks of 8 x 8 x 4 voxels): whether
+// it fits LDS, and its size.
+constexpr int F3X = 8, F3Y = 8, F3Z = 4;
+static bool tdf_f32_lds(float rmin, float rmax, size_t& lds) {
+    const int h = tdf_halo(rmin, rmax);
+    lds = (size_t)(F3X + 2 * h) * (F3Y + 2 * h) * (F3Z + 2 * h) * 3 * sizeof(float);
+    const size_t state = (size_t)F3X * F3Y * F3Z * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
+    return h <= 64 && lds + state <= 65536;
+}
+
+template <bool NORM>
+static void launch_tdf_f32(const tsf_field* f, Dims d, float rmin, float rmax, float rstep, float* tdf,
+                           float* radius, size_t lds, hipStream_t stream) {
+    static const CircleTables tab = make_circle_tables();
+    dim3 grid((d.X + F3X - 1) / F3X, (d.Y + F3Y - 1) / F3Y, (d.Z + F3Z - 1) / F3Z), block(F3X, F3Y, F3Z);
+    hipLaunchKernelGGL((k_tdf_lds_f32<NORM, F3X, F3Y, F3Z>), grid, block, lds, stream, (const float*)f->x,
+                       (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr, d, rmin, rmax,
+                       rstep, tab, tdf, radius, tdf_halo(rmin, rmax));
+}
+
 template <class S, bool NORM>
 static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rmax, float rstep, float* tdf,
                        float* radius, hipStream_t stream) {
@@ -250,8 +311,8 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     constexpr int BX = sizeof(P) == 16 ? 8 : 16, BY = 4, BZ = 4;
     const int h = tdf_halo(rmin, rmax);
     const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
-    const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // k_tdf_lds's static LDS
-    if (h <= 64 && lds + state <= 65536) {
+    const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
+    if (sizeof(P) == 8 && h <= 64 && lds + state <= 65536) {
         dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
         hipLaunchKernelGGL((k_tdf_lds<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, d, rmin,
                            rmax, rstep, tab, tdf, radius, h);
@@ -286,13 +347,23 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
         return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: more than 1e5 radii", W);
     Dims d = D(dims);
     const i64 n = d.n();
+    // F32 fields whose tile fits LDS are read from their planes: no packed
+    // texels, no workspace
+    size_t lds_f32 = 0;
+    const bool planes = field->storage == TSF_STORAGE_F32 && tdf_f32_lds(rmin, rmax, lds_f32);
     Scratch sc;
-    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_circle_workspace_size(dims, field->storage), W));
+    if (!planes)
+        TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_circle_workspace_size(dims, field->storage), W));
     // texel units: F16/BF16 hold value * scale, SN16 texels are read as
     // value * 2^15
     const float inv = field->storage == TSF_STORAGE_SN16 ? 1.0f / 32768.0f : 1.0f / field->scale;
     switch (field->storage) {
         case TSF_STORAGE_F32:
+            if (planes) {
+                if (norm) launch_tdf_f32<true>(field, d, rmin, rmax, rstep, tdf, radius, lds_f32, stream);
+                else launch_tdf_f32<false>(field, d, rmin, rmax, rstep, tdf, radius, lds_f32, stream);
+                break;
+            }
             hipLaunchKernelGGL(k_pack_f32, grid1d(n), 256, 0, stream, (const float*)field->x,
                                (const float*)field->y, (const float*)field->z, (const float*)field->w,
                                (float4*)sc.ptr, n);

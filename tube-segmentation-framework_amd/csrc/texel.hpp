@@ -121,6 +121,27 @@ struct LdsTex {
     }
 };
 
+// LdsSoaTex: an F32 field's x, y and z staged in LDS as three float planes
+// of `cells` floats over the same kind of tile; w (1 when absent) is read
+// from its plane in device memory, at the global cell.
+struct LdsSoaTex {
+    struct Row {
+        int l;  // local row offset in the tile
+        i64 g;  // global row index
+    };
+    const float* p;
+    const float* __restrict__ w;
+    int cells, ox, oy, oz, tx, ty, tz;
+    static constexpr float inv = 1.0f;
+    __device__ __forceinline__ Row row(const Dims& d, int y, int z) const {
+        return {tx * (clampi(y - oy, 0, ty - 1) + ty * clampi(z - oz, 0, tz - 1)), (i64)d.X * ((i64)y + (i64)d.Y * z)};
+    }
+    __device__ __forceinline__ float4 at(Row r, int x) const {
+        const int l = r.l + clampi(x - ox, 0, tx - 1);
+        return make_float4(p[l], p[cells + l], p[2 * cells + l], w ? w[r.g + x] : 1.0f);
+    }
+};
+
 template <bool NORM, class A>
 __device__ __forceinline__ void hessian_tex(const A& T, const Dims& d, int x, int y, int z, float H[3][3]) {
     const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1);
@@ -216,6 +237,38 @@ __device__ __forceinline__ void trilinear_tex(const LdsTex<S>& T, const Dims&, f
     out[0] = sx * T.inv;
     out[1] = sy_ * T.inv;
     out[2] = sz_ * T.inv;
+}
+
+// The same sampling from the three-plane F32 tile (same taps, terms and
+// order; the planes hold the values, so no scale is applied).
+__device__ __forceinline__ void trilinear_tex(const LdsSoaTex& T, const Dims&, float u, float v, float w,
+                                              float out[3]) {
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const int sy = T.tx, sz = __mul24(T.tx, T.ty);
+    const int li = min(max((int)fu - T.ox, 0), T.tx - 2), lj = min(max((int)fv - T.oy, 0), T.ty - 2);
+    const int lk = min(max((int)fw - T.oz, 0), T.tz - 2);
+    const int q = li + __mul24(lj, sy) + __mul24(lk, sz);
+    const int o[8] = {q, q + 1, q + sy, q + sy + 1, q + sz, q + sz + 1, q + sz + sy, q + sz + sy + 1};
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    const float wt[8] = {wt0, wt1, wt2, wt3, wt4, wt5, wt6, wt7};
+    const float* px = T.p;
+    const float* py = T.p + T.cells;
+    const float* pz = T.p + 2 * T.cells;
+    float sx = wt[0] * px[o[0]], sy_ = wt[0] * py[o[0]], sz_ = wt[0] * pz[o[0]];
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        sx = sx + wt[k] * px[o[k]];
+        sy_ = sy_ + wt[k] * py[o[k]];
+        sz_ = sz_ + wt[k] * pz[o[k]];
+    }
+    out[0] = sx;
+    out[1] = sy_;
+    out[2] = sz_;
 }
 
 constexpr int CBX = 16, CBY = 4, CBZ = 4;  // TDF brick
