@@ -690,40 +690,63 @@ __global__ void k_finish_adjoint(const float* __restrict__ vx, const float* __re
 // ---------------------------------------------------------------- TDF
 // The filter's trilinear sample (trilinear_tex: identical terms and order,
 // so the recomputed radius is the forward's) together with its spatial
-// Jacobian and the eight corner indices and weights.
-__device__ __forceinline__ void tri_grad(const Tex<F32>& T, const Dims& d, float u, float v, float w, float s[3],
-                                         float J[3][3], int ci[8], int cj[8], int ck[8], float wt[8]) {
+// Jacobian and the eight corner indices and weights, in two parts:
+// tri_fetch issues the eight texel loads, tri_eval uses them, so a caller
+// can have the next sample's loads in flight while it evaluates this one.
+struct TriTaps {
+    float4 t[8];
+    float a, b, c;
+    int i0, i1, j0, j1, k0, k1;
+};
+__device__ __forceinline__ void tri_fetch(const Tex<F32>& T, const Dims& d, float u, float v, float w, TriTaps& q) {
     const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
     const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
-    const float a = uu - fu, b = vv - fv, c = ww - fw;
-    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
-    const int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
-    i0 = clampi(i0, 0, d.X - 1);
-    j0 = clampi(j0, 0, d.Y - 1);
-    k0 = clampi(k0, 0, d.Z - 1);
-    const int I[8] = {i0, i1, i0, i1, i0, i1, i0, i1}, Jj[8] = {j0, j0, j1, j1, j0, j0, j1, j1};
-    const int K[8] = {k0, k0, k0, k0, k1, k1, k1, k1};
+    q.a = uu - fu;
+    q.b = vv - fv;
+    q.c = ww - fw;
+    const int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    q.i1 = clampi(i0 + 1, 0, d.X - 1);
+    q.j1 = clampi(j0 + 1, 0, d.Y - 1);
+    q.k1 = clampi(k0 + 1, 0, d.Z - 1);
+    q.i0 = clampi(i0, 0, d.X - 1);
+    q.j0 = clampi(j0, 0, d.Y - 1);
+    q.k0 = clampi(k0, 0, d.Z - 1);
+    const auto r00 = T.row(d, q.j0, q.k0), r10 = T.row(d, q.j1, q.k0);
+    const auto r01 = T.row(d, q.j0, q.k1), r11 = T.row(d, q.j1, q.k1);
+    q.t[0] = T.at(r00, q.i0);
+    q.t[1] = T.at(r00, q.i1);
+    q.t[2] = T.at(r10, q.i0);
+    q.t[3] = T.at(r10, q.i1);
+    q.t[4] = T.at(r01, q.i0);
+    q.t[5] = T.at(r01, q.i1);
+    q.t[6] = T.at(r11, q.i0);
+    q.t[7] = T.at(r11, q.i1);
+}
+__device__ __forceinline__ void tri_eval(const TriTaps& q, float s[3], float J[3][3], int ci[8], int cj[8], int ck[8],
+                                         float wt[8]) {
+    const float a = q.a, b = q.b, c = q.c;
+    const int I[8] = {q.i0, q.i1, q.i0, q.i1, q.i0, q.i1, q.i0, q.i1};
+    const int Jj[8] = {q.j0, q.j0, q.j1, q.j1, q.j0, q.j0, q.j1, q.j1};
+    const int K[8] = {q.k0, q.k0, q.k0, q.k0, q.k1, q.k1, q.k1, q.k1};
     const float A[8] = {1 - a, a, 1 - a, a, 1 - a, a, 1 - a, a};
     const float Bv[8] = {1 - b, 1 - b, b, b, 1 - b, 1 - b, b, b};
     const float C[8] = {1 - c, 1 - c, 1 - c, 1 - c, c, c, c, c};
     const float dA[8] = {-1, 1, -1, 1, -1, 1, -1, 1}, dB[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
     const float dC[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-    float4 t[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        t[k] = T.at(T.row(d, Jj[k], K[k]), I[k]);
         ci[k] = I[k];
         cj[k] = Jj[k];
         ck[k] = K[k];
         wt[k] = A[k] * Bv[k] * C[k];
     }
     // the filter's sum, term by term
-    float sx = wt[0] * t[0].x, sy = wt[0] * t[0].y, sz = wt[0] * t[0].z;
+    float sx = wt[0] * q.t[0].x, sy = wt[0] * q.t[0].y, sz = wt[0] * q.t[0].z;
 #pragma unroll
     for (int k = 1; k < 8; k++) {
-        sx = sx + wt[k] * t[k].x;
-        sy = sy + wt[k] * t[k].y;
-        sz = sz + wt[k] * t[k].z;
+        sx = sx + wt[k] * q.t[k].x;
+        sy = sy + wt[k] * q.t[k].y;
+        sz = sz + wt[k] * q.t[k].z;
     }
     s[0] = sx;
     s[1] = sy;
@@ -733,7 +756,7 @@ __device__ __forceinline__ void tri_grad(const Tex<F32>& T, const Dims& d, float
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const float gx = dA[k] * Bv[k] * C[k], gy = A[k] * dB[k] * C[k], gz = A[k] * Bv[k] * dC[k];
-        const float tv[3] = {t[k].x, t[k].y, t[k].z};
+        const float tv[3] = {q.t[k].x, q.t[k].y, q.t[k].z};
 #pragma unroll
         for (int m = 0; m < 3; m++) {
             J[m][0] += gx * tv[m];
@@ -745,49 +768,92 @@ __device__ __forceinline__ void tri_grad(const Tex<F32>& T, const Dims& d, float
 
 // Accumulation tile of dV (3 planes) over one brick plus a halo of `h`
 // voxels, in LDS; contributions outside it go to global atomics.
+// The sample scatter accumulates in an LDS tile over one brick plus a halo
+// of `halo` voxels, as 64-bit fixed point: a contribution v adds
+// rint(v * S) to its cell, S = 2^(30 - e) for the brick's largest |dL/dT|
+// < 2^e.  Every sample contribution is at most |dL/dT| / 32 in magnitude
+// (times a unit direction component and a trilinear weight), so a rounded
+// contribution fits 32 bits (larger ones, never expected, take the global
+// path) and a cell's sum (at most 8 * 32 contributions from each of the
+// brick's voxels) fits 64 with room to spare; the quantum 2^(e-31) is below a float's rounding of the values
+// added.  Integer adds are exact and order-free, so the tile's sums are
+// deterministic, and on this chip an integer LDS atomic costs a third of a
+// float one.  Contributions that are not finite, or fall outside the tile,
+// or arrive when the tile is off, go to global float atomics.  The tile's
+// atomics name the LDS address space explicitly (through a generic pointer
+// they would become flat atomics).
+constexpr int TAX = 16, TAY = 8, TAZ = 4;  // TDF adjoint brick
+constexpr int TA_HALO = 5;                 // the largest halo the tile covers (r_max < 4.5)
+constexpr int TA_CELLS = (TAX + 2 * TA_HALO) * (TAY + 2 * TA_HALO) * (TAZ + 2 * TA_HALO);
+typedef __attribute__((address_space(3))) long long lds_i64;
 struct AdjTile {
-    float* a;  // 3 * cells floats
+    long long* a;  // 3 * cells fixed-point sums in LDS
+    float S;       // scale (a power of two)
+    bool on;
     int ox, oy, oz, tx, ty, tz, cells;
     __device__ __forceinline__ void add(float* const dV[3], const Dims& d, int c, int x, int y, int z, float v) const {
         const int lx = x - ox, ly = y - oy, lz = z - oz;
-        if (a && (unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty && (unsigned)lz < (unsigned)tz)
-            atomicAdd(&a[c * cells + lx + tx * (ly + ty * lz)], v);
+        const float q = v * S;  // |q| < 2^25 for a sample's contribution; NaN fails the test
+        if (on && fabsf(q) < 1073741824.0f && (unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty &&
+            (unsigned)lz < (unsigned)tz)
+            __hip_atomic_fetch_add((lds_i64*)(a + (c * cells + lx + tx * (ly + ty * lz))),
+                                   (long long)(int)__builtin_rintf(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         else
             unsafeAtomicAdd(dV[c] + d.at(x, y, z), v);
     }
 };
 
-// Adjoint of circleFittingTDF at the forward's radius.  Block = one 16x4x4
-// brick; halo > 0 selects the LDS tile (dynamic shared memory of
-// 3 * (16 + 2 halo)(4 + 2 halo)^2 floats), halo == 0 global atomics only.
+// Adjoint of circleFittingTDF at the forward's radius.  Block = one 16x8x4
+// brick; halo > 0 (at most TA_HALO) turns the fixed-point tile on, halo == 0
+// leaves global atomics only.  The Hessian's terms (12 per voxel, scaled by
+// 1 / eigenvalue gaps, so unbounded) always go to global atomics.
 template <bool NORM>
-__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims d, float rmin, float rmax, float rstep,
-                                                               CircleTables tab, const float* __restrict__ gT,
-                                                               const float* __restrict__ rad,
-                                                               float* __restrict__ dVx, float* __restrict__ dVy,
-                                                               float* __restrict__ dVz, float* __restrict__ dW,
-                                                               int halo) {
-    extern __shared__ float lds[];
-    const int tid = threadIdx.x + CBX * (threadIdx.y + CBY * threadIdx.z);
-    AdjTile tile;
-    tile.a = halo > 0 ? lds : nullptr;
-    tile.tx = CBX + 2 * halo;
-    tile.ty = CBY + 2 * halo;
-    tile.tz = CBZ + 2 * halo;
-    tile.cells = tile.tx * tile.ty * tile.tz;
+__global__ void __launch_bounds__(TAX* TAY* TAZ) k_tdf_adjoint(Tex<F32> T, Dims d, float rmin, float rmax, float rstep,
+                                                              CircleTables tab, const float* __restrict__ gT,
+                                                              const float* __restrict__ rad,
+                                                              float* __restrict__ dVx, float* __restrict__ dVy,
+                                                              float* __restrict__ dVz, float* __restrict__ dW,
+                                                              int halo) {
+    constexpr int NT = TAX * TAY * TAZ;
+    __shared__ long long acc[3 * TA_CELLS];
+    // the circle's direction tables, indexed per lane below
+    __shared__ float tc[32], ts[32];
+    __shared__ float wmax[NT / 64];
+    const int tid = threadIdx.x + TAX * (threadIdx.y + TAY * threadIdx.z);
+    if (tid < 32) {
+        tc[tid] = tab.c[tid];
+        ts[tid] = tab.s[tid];
+    }
     int tbx, tby, tbz;
     xcd_tile(tbx, tby, tbz);
-    tile.ox = tbx * CBX - halo;
-    tile.oy = tby * CBY - halo;
-    tile.oz = tbz * CBZ - halo;
-    if (halo > 0) {
-        for (int k = tid; k < 3 * tile.cells; k += CBX * CBY * CBZ) lds[k] = 0.0f;
-        __syncthreads();
-    }
-    float* const dV[3] = {dVx, dVy, dVz};
-    const int x = tbx * CBX + threadIdx.x, y = tby * CBY + threadIdx.y;
-    const int z = tbz * CBZ + threadIdx.z;
+    const int x = tbx * TAX + threadIdx.x, y = tby * TAY + threadIdx.y;
+    const int z = tbz * TAZ + threadIdx.z;
     const float g = (x < d.X && y < d.Y && z < d.Z) ? gT[d.at(x, y, z)] : 0.0f;
+    // the brick's largest |dL/dT| (NaN or infinity turn the tile off)
+    float m = __builtin_isfinite(g) ? fabsf(g) : __builtin_inff();
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((tid & 63) == 0) wmax[tid >> 6] = m;
+    AdjTile tile;
+    tile.a = acc;
+    tile.tx = TAX + 2 * halo;
+    tile.ty = TAY + 2 * halo;
+    tile.tz = TAZ + 2 * halo;
+    tile.cells = tile.tx * tile.ty * tile.tz;
+    tile.ox = tbx * TAX - halo;
+    tile.oy = tby * TAY - halo;
+    tile.oz = tbz * TAZ - halo;
+    const bool lds_on = halo > 0 && halo <= TA_HALO;
+    if (lds_on)
+        for (int k = tid; k < 3 * tile.cells; k += NT) acc[k] = 0;
+    __syncthreads();
+    float gmax = 0.0f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) gmax = fmaxf(gmax, wmax[w]);
+    tile.on = lds_on && gmax > 0.0f && __builtin_isfinite(gmax);
+    int e = 0;
+    if (tile.on) frexpf(gmax, &e);  // gmax < 2^e
+    tile.S = tile.on ? ldexpf(1.0f, 30 - e) : 0.0f;
+    float* const dV[3] = {dVx, dVy, dVz};
     if (g != 0.0f) {
         float H[3][3], E[3][3], lam[3];
         hessian_tex<NORM>(T, d, x, y, z, H);
@@ -798,10 +864,15 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
         const float fx = (float)x, fy = (float)y, fz = (float)z;
         if (rad) {
             // the forward's r*: a radius the search can return (>= rmin) is
-            // one where the forward's response was its sum there; the sum is
-            // recomputed by the forward's code, so bit-identical
+            // one where the forward's response was its sum there.  With
+            // rmin > 0 the forward records such a radius only where that sum
+            // is positive (elsewhere it records 0), which is all the
+            // gradient needs to know; otherwise the sum is recomputed by the
+            // forward's code, so bit-identical
             rstar = rad[d.at(x, y, z)];
-            if (rstar >= rmin && rstar <= rmax) {
+            if (rmin > 0.0f && rstar >= rmin && rstar <= rmax) {
+                maxSum = 1.0f;
+            } else if (rstar >= rmin && rstar <= rmax) {
                 float sum = 0.0f;
                 for (int j = 0; j < 32; j++) {
                     const float cj = tab.c[j], sj = tab.s[j];
@@ -834,12 +905,31 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
         if (maxSum > 0.0f) {
             const float cf = -g / 32.0f;
             float A2[3] = {0, 0, 0}, A3[3] = {0, 0, 0};
-            for (int j = 0; j < 32; j++) {
-                const float dj[3] = {tab.c[j] * e3x + tab.s[j] * e2x, tab.c[j] * e3y + tab.s[j] * e2y,
-                                     tab.c[j] * e3z + tab.s[j] * e2z};
+            // the lanes of a wave take the 32 directions in rotated orders:
+            // neighbouring voxels then sample far-apart points in the same
+            // iteration, and their LDS atomics rarely meet on one cell
+            // (the taps of direction jr + 1 are loaded while jr is evaluated)
+            const int rot = tid & 31;
+            TriTaps cur, nxt;
+            {
+                const float cc = tc[rot], ss = ts[rot];
+                tri_fetch(T, d, fx + rstar * (cc * e3x + ss * e2x), fy + rstar * (cc * e3y + ss * e2y),
+                          fz + rstar * (cc * e3z + ss * e2z), cur);
+            }
+            for (int jr = 0; jr < 32; jr++) {
+                const int j = (jr + rot) & 31;
+                const float cc = tc[j], ss = ts[j];
+                const float dj[3] = {cc * e3x + ss * e2x, cc * e3y + ss * e2y, cc * e3z + ss * e2z};
+                if (jr < 31) {
+                    const int jn = (j + 1) & 31;
+                    const float cn = tc[jn], sn = ts[jn];
+                    tri_fetch(T, d, fx + rstar * (cn * e3x + sn * e2x), fy + rstar * (cn * e3y + sn * e2y),
+                              fz + rstar * (cn * e3z + sn * e2z), nxt);
+                }
                 float s[3], J[3][3], wt[8];
                 int ci[8], cj[8], ck[8];
-                tri_grad(T, d, fx + rstar * dj[0], fy + rstar * dj[1], fz + rstar * dj[2], s, J, ci, cj, ck, wt);
+                tri_eval(cur, s, J, ci, cj, ck, wt);
+                cur = nxt;
 #pragma unroll
                 for (int k = 0; k < 8; k++)
 #pragma unroll
@@ -847,8 +937,8 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
                     const float u = s[m] + rstar * (dj[0] * J[0][m] + dj[1] * J[1][m] + dj[2] * J[2][m]);
-                    A3[m] += cf * tab.c[j] * u;
-                    A2[m] += cf * tab.s[j] * u;
+                    A3[m] += cf * cc * u;
+                    A2[m] += cf * ss * u;
                 }
             }
             // eigenvector perturbation: de_k = sum_{i != k} e_i (e_i^T dH e_k) / (lam_k - lam_i)
@@ -875,13 +965,14 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
                 xx = clampi(xx, 0, d.X - 1);
                 yy = clampi(yy, 0, d.Y - 1);
                 zz = clampi(zz, 0, d.Z - 1);
+                const i64 q = d.at(xx, yy, zz);
                 if (NORM) {
                     const float4 tq = T.at(T.row(d, yy, zz), xx);
                     const float vq = comp == 0 ? tq.x : (comp == 1 ? tq.y : tq.z);
-                    tile.add(dV, d, comp, xx, yy, zz, cv / tq.w);
-                    unsafeAtomicAdd(dW + d.at(xx, yy, zz), -cv * vq / (tq.w * tq.w));
+                    unsafeAtomicAdd(dV[comp] + q, cv / tq.w);
+                    unsafeAtomicAdd(dW + q, -cv * vq / (tq.w * tq.w));
                 } else {
-                    tile.add(dV, d, comp, xx, yy, zz, cv);
+                    unsafeAtomicAdd(dV[comp] + q, cv);
                 }
             };
             put(0, x + 1, y, z, 0.5f * gxx);
@@ -898,18 +989,19 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_adjoint(Tex<F32> T, Dims 
             put(2, x, y, z - 1, -0.5f * gzz);
         }
     }
-    if (halo > 0) {
+    if (tile.on) {
         // one global atomic per touched tile cell inside the volume
         __syncthreads();
-        for (int k = tid; k < tile.cells; k += CBX * CBY * CBZ) {
+        const double inv = 1.0 / (double)tile.S;
+        for (int k = tid; k < tile.cells; k += NT) {
             const int lx = k % tile.tx, ly = (k / tile.tx) % tile.ty, lz = k / (tile.tx * tile.ty);
             const int gx = tile.ox + lx, gy = tile.oy + ly, gz = tile.oz + lz;
             if (gx < 0 || gy < 0 || gz < 0 || gx >= d.X || gy >= d.Y || gz >= d.Z) continue;
             const i64 q = d.at(gx, gy, gz);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                const float v = lds[c * tile.cells + k];
-                if (v != 0.0f) unsafeAtomicAdd(dV[c] + q, v);
+                const long long v = acc[c * tile.cells + k];
+                if (v != 0) unsafeAtomicAdd(dV[c] + q, (float)((double)v * inv));
             }
         }
     }
@@ -1265,25 +1357,21 @@ tsf_status tsf_tdf_adjoint(const tsf_field* field, tsf_dims dims, float rmin, fl
     pack_texels_f32(field, (float4*)sc.ptr, N, stream);
     Tex<F32> T{(const float4*)sc.ptr, 1.0f};
     // LDS tile: samples reach ceil(rmax + 0.5) voxels from the centre (the
-    // trilinear corner below u - 0.5), the Hessian one; at most 64 KiB so
-    // two bricks fit a compute unit, else global atomics only
+    // trilinear corner below u - 0.5); wider halos than the tile's leave
+    // global atomics only
     int halo = (int)ceilf(fmaxf(fabsf(rmin), fabsf(rmax)) + 0.5f);
     if (halo < 1) halo = 1;
-    size_t lds = (size_t)3 * (CBX + 2 * halo) * (CBY + 2 * halo) * (CBZ + 2 * halo) * sizeof(float);
-    if (lds > 65536) {
-        halo = 0;
-        lds = 0;
-    }
+    if (halo > TA_HALO) halo = 0;
     float* dx = (float*)dfield->x;
     float* dy = (float*)dfield->y;
     float* dz = (float*)dfield->z;
     float* dw = (float*)dfield->w;
-    dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
+    dim3 grid((d.X + TAX - 1) / TAX, (d.Y + TAY - 1) / TAY, (d.Z + TAZ - 1) / TAZ), block(TAX, TAY, TAZ);
     if (norm)
-        hipLaunchKernelGGL(k_tdf_adjoint<true>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
+        hipLaunchKernelGGL(k_tdf_adjoint<true>, grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
                            radius, dx, dy, dz, dw, halo);
     else
-        hipLaunchKernelGGL(k_tdf_adjoint<false>, grid, block, lds, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
+        hipLaunchKernelGGL(k_tdf_adjoint<false>, grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf_grad,
                            radius, dx, dy, dz, dw, halo);
     TSF_TRY(check_launch(W));
     if (sc.owned) {
