@@ -281,7 +281,7 @@ tsf_status tsf_gvf_euler_checkpointed(const tsf_field* f, const tsf_field* out, 
 /* Adjoint of the GVF iterations: from dV (3 float planes, dL/dv^N) returns
  * dF (3 planes: through v^0 = f and the data term of every step; mirror
  * boundary transposed) and dmu (host).  Each checkpoint segment is
- * recomputed once; memory (checkpoint_every + 3) states + the checkpoints. */
+ * recomputed once; memory (checkpoint_every + 2) states + the checkpoints. */
 tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, int32_t iterations,
                                  int32_t checkpoint_every, const float* checkpoints, const float* dV, float* dF,
                                  double* dmu, void* workspace, size_t workspace_bytes, hipStream_t stream);

@@ -175,6 +175,36 @@ def test_tdf_adjoint(tsf, orc, rmin, rmax, rstep):
         assert er < 1e-6
 
 
+def test_tdf_adjoint_linear_in_gradient_scale(tsf, orc):
+    # the scatter tile accumulates in fixed point scaled by each brick's
+    # largest |dL/dT|: the gradient must still be linear in dL/dT, for a
+    # scale that moves the fixed-point scale (2^20: the same integers) and
+    # one below the tile's range (2^-110: the float atomics)
+    from tsf import synth
+    vol, _ = synth.tube_volume((24, 26, 28), 5, 1.5, 3.5, 3, noise=0.05)
+    V = orc.gvf_euler(orc.vector_field(orc.blur(vol, 0.8, "f32"), 0.2, 1, "f32"), 0.05, 30, "f32")
+    fld = tsf.Field.from_host(V)
+    rmin, rmax, rstep = 1.0, 3.5, 0.5
+    T = tsf.DeviceArray(vol.shape, np.float32)
+    R = tsf.DeviceArray(vol.shape, np.float32)
+    tsf.tdf_circle(fld, rmin, rmax, rstep, T, R)
+    g = rng(9)
+    gT = np.where(T.download() > 0, g.uniform(-1.5, 1.5, vol.shape), 0.0).astype(np.float32)
+    out = {}
+    for scale in (1.0, 2.0 ** 20, 2.0 ** -110):
+        d = tsf.Field(V.shape[1:])
+        for p in d.planes:
+            p.zero()
+        tsf.tdf_adjoint(fld, rmin, rmax, rstep, tsf.DeviceArray.from_host((gT * scale).astype(np.float32)), d,
+                        radius=R)
+        out[scale] = d.download().astype(np.float64) / scale
+    for scale in (2.0 ** 20, 2.0 ** -110):
+        e = rel(out[scale], out[1.0])
+        print(f"TDF adjoint at dL/dT x {scale:g}, rescaled: {e:.2e}")
+        assert np.all(np.isfinite(out[scale]))
+        assert e < 1e-6
+
+
 def _forward(tsf, I, **kw):
     from tsf.chain import Ctx, TubeFilterFunction
     ctx = Ctx()

@@ -30,6 +30,8 @@ namespace tsf {
 
 tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], float* const out[3], Dims d, float mu,
                          int iters, float* ws, hipStream_t stream);
+tsf_status gvf_states_f32(const float* const f[3], const float* v0, float* const* states, int count, Dims d,
+                          float mu, hipStream_t stream);
 
 constexpr int RB = 256;     // reduction block
 constexpr int NPART = 1024; // partial sums per launch
@@ -852,6 +854,9 @@ __global__ void __launch_bounds__(TAX* TAY* TAZ) k_tdf_adjoint(Tex<F32> T, Dims 
     tile.on = lds_on && gmax > 0.0f && __builtin_isfinite(gmax);
     int e = 0;
     if (tile.on) frexpf(gmax, &e);  // gmax < 2^e
+    // S = 2^(30 - e) must be a finite float: bricks whose gradients are all
+    // below 2^-97 use the global float atomics
+    tile.on = tile.on && e >= -96;
     tile.S = tile.on ? ldexpf(1.0f, 30 - e) : 0.0f;
     float* const dV[3] = {dVx, dVy, dVz};
     if (g != 0.0f) {
@@ -1197,7 +1202,7 @@ static size_t adj_parts(const Dims& d) { return std::max(adj_blocks(d), 2 * adj2
 
 // float states first, then the double partial sums at a 256-byte boundary
 static size_t adj_float_bytes(const Dims& d, int k) {
-    return (((size_t)k + 3) * 3 * (size_t)d.n() * sizeof(float) + 255) & ~(size_t)255;
+    return (((size_t)k + 2) * 3 * (size_t)d.n() * sizeof(float) + 255) & ~(size_t)255;
 }
 
 size_t tsf_gvf_euler_adjoint_workspace_size(tsf_dims dims, int32_t checkpoint_every) {
@@ -1225,7 +1230,6 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
     float* seg = (float*)sc.ptr;              // k states of 3 planes
     float* lamA = seg + (i64)k * 3 * N;       // folded adjoint, ping-pong
     float* lamB = lamA + 3 * N;
-    float* tmp = lamB + 3 * N;                // ping-pong of the recomputation
     double* part2 = (double*)((char*)sc.ptr + adj_float_bytes(d, k));  // NPART
     double* acc = part2 + NPART;
     double* part = acc + 1;                   // one per stencil block
@@ -1250,12 +1254,10 @@ tsf_status tsf_gvf_euler_adjoint(const tsf_field* f, tsf_dims dims, float mu, in
         const float* cp = checkpoints + s * 3 * N;
         e = hipMemcpyAsync(seg, cp, 3 * N * sizeof(float), hipMemcpyDeviceToDevice, stream);
         if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
-        for (int t = 1; t < end - start; t++) {
-            const float* pv = seg + (i64)(t - 1) * 3 * N;
-            float* nv = seg + (i64)t * 3 * N;
-            const float* pvv[3] = {pv, pv + N, pv + 2 * N};
-            float* nvv[3] = {nv, nv + N, nv + 2 * N};
-            TSF_TRY(gvf_steps_f32(fp, pvv, nvv, d, mu, 1, tmp, stream));
+        if (end - start > 1) {
+            std::vector<float*> st(end - start - 1);
+            for (int t = 1; t < end - start; t++) st[t - 1] = seg + (i64)t * 3 * N;
+            TSF_TRY(gvf_states_f32(fp, seg, st.data(), end - start - 1, d, mu, stream));
         }
         // steps n and n-1 in one launch while both lie in the segment; the
         // adjoint, dF and d/dmu come out as from two single steps

@@ -636,10 +636,20 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
 constexpr int F4Y = 16;
 constexpr int F4OW = FBX - 8, F4OH = F4Y - 6;
 
+// The three intermediate iterates of a four-step launch, stored when the
+// caller needs every state (the adjoint's segment recomputation): each one
+// block of three component volumes of n elements.
+template <class S>
+struct Mids3 {
+    typename S::T* s[3];
+    i64 n;
+};
+
 template <class S>
 struct Fused4 {
     Planes3<S> f, vin;
     Out4<S> vout;
+    Mids3<S> mids;
     Dims d;
     float mu, m2s, unit;
     int lane, ty, cy, qs, qe;
@@ -651,7 +661,26 @@ struct Fused4 {
     int lxp, lxm, lyp, lym, hly, xsrc;
 };
 
-template <class S, bool FINISH, int T4>
+// Stage k's iterate (k = 1..3) at this thread's column of plane p, for the
+// threads of the output tile: interior voxels, and the x-boundary columns
+// of interior rows from their mirror lanes (k_gvf_faces writes the rest).
+template <class S>
+__device__ __forceinline__ void f4_store_mid(const Fused4<S>& m, int k, int p, const float (&v)[3]) {
+    typedef Raw<S> R;
+    float r[3] = {v[0], v[1], v[2]};
+    if (m.xedge) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) r[c] = __shfl(r[c], m.xsrc);
+    }
+    if ((m.inner && m.lane >= 4 && m.lane <= FBX - 5) || m.xbnd) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            *reinterpret_cast<typename R::T*>(reinterpret_cast<char*>(plane_of(m.mids.s[k - 1], p, m.P) + c * m.mids.n) +
+                                              m.coff) = R::put(r[c]);
+    }
+}
+
+template <class S, bool FINISH, bool MIDS, int T4>
 __device__ __forceinline__ void f4_step(const Fused4<S>& m, int t, typename Raw<S>::T (&v0)[4][3],
                                         typename Raw<S>::T (&fr)[4][3], typename Raw<S>::T (&hr)[2][3],
                                         float (&fk)[4][3], float (&mk)[4], float (&vk)[3][4][3],
@@ -705,6 +734,7 @@ __device__ __forceinline__ void f4_step(const Fused4<S>& m, int t, typename Raw<
             const float rm = RoundMid<S>::value ? R::getu(R::put(r)) : r;
             vk[0][S0][c] = ok ? rm : 0.0f;
         }
+        if (MIDS && m.inO && t >= m.qs && t < m.qe) f4_store_mid(m, 1, t, vk[0][S0]);
         if (t == 2) {
 #pragma unroll
             for (int c = 0; c < 3; c++) vk[0][(T4 - 2) & 3][c] = vk[0][S0][c];
@@ -730,6 +760,7 @@ __device__ __forceinline__ void f4_step(const Fused4<S>& m, int t, typename Raw<
                 vk[k - 1][sc][c] = ok ? rm : 0.0f;
             }
         }
+        if (MIDS && m.inO && p >= m.qs && p < m.qe) f4_store_mid(m, k, p, vk[k - 1][(T4 - k + 1) & 3]);
         if (p == 2) {
 #pragma unroll
             for (int c = 0; c < 3; c++) vk[k - 1][(T4 - k - 1) & 3][c] = vk[k - 1][(T4 - k + 1) & 3][c];
@@ -785,9 +816,10 @@ __device__ __forceinline__ void f4_step(const Fused4<S>& m, int t, typename Raw<
     }
 }
 
-template <class S, bool FINISH>
+template <class S, bool FINISH, bool MIDS = false>
 __global__ void __launch_bounds__(FBX* F4Y, 4)
-    k_gvf_fused4(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d, float mu, float m2s, float unit) {
+    k_gvf_fused4(Planes3<S> f, Planes3<S> vin, Out4<S> vout, Dims d, float mu, float m2s, float unit,
+                 Mids3<S> mids = Mids3<S>{}) {
     typedef Raw<S> R;
     typedef typename R::T RT;
     __shared__ float L0[2][3][F4Y + 2][FBX + 2];
@@ -796,6 +828,7 @@ __global__ void __launch_bounds__(FBX* F4Y, 4)
     m.f = f;
     m.vin = vin;
     m.vout = vout;
+    m.mids = mids;
     m.d = d;
     m.mu = mu;
     m.m2s = m2s;
@@ -879,10 +912,10 @@ __global__ void __launch_bounds__(FBX* F4Y, 4)
         }
     }
     for (int t = t0; t <= te; t += 4) {
-        f4_step<S, FINISH, 0>(m, t, v0, fr, hr, fk, mk, vk, L0, LK);
-        f4_step<S, FINISH, 1>(m, t + 1, v0, fr, hr, fk, mk, vk, L0, LK);
-        f4_step<S, FINISH, 2>(m, t + 2, v0, fr, hr, fk, mk, vk, L0, LK);
-        f4_step<S, FINISH, 3>(m, t + 3, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, MIDS, 0>(m, t, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, MIDS, 1>(m, t + 1, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, MIDS, 2>(m, t + 2, v0, fr, hr, fk, mk, vk, L0, LK);
+        f4_step<S, FINISH, MIDS, 3>(m, t + 3, v0, fr, hr, fk, mk, vk, L0, LK);
     }
 }
 
@@ -1078,6 +1111,43 @@ tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], floa
             launch_faces<F32>(false, dst, d, stream);
         }
         TSF_TRY(check_launch("gvf_steps_f32"));
+    }
+    return TSF_OK;
+}
+
+// v^1 .. v^count from v^0 into states[0 .. count-1] (each three planes),
+// every state stored with its faces: four steps per launch storing all four
+// iterates (bit-identical to single steps), the remainder as single steps.
+tsf_status gvf_states_f32(const float* const f[3], const float* v0, float* const* states, int count, Dims d,
+                          float mu, hipStream_t stream) {
+    const i64 N = d.n();
+    Planes3<F32> F{f[0], f[1], f[2]};
+    auto planes = [N](const float* p) { return Planes3<F32>{p, p + N, p + 2 * N}; };
+    auto out = [N](float* p) { return Out4<F32>{p, p + N, p + 2 * N, nullptr}; };
+    bool vec2 = (d.X % 2) == 0;
+    for (const void* p : {(const void*)f[0], (const void*)f[1], (const void*)f[2], (const void*)v0})
+        vec2 = vec2 && aligned(p, 8);
+    for (int i = 0; i < count; i++) vec2 = vec2 && aligned(states[i], 8);
+    const bool quads = fused_ok<F32>(1.0f, 1.0f, d, 4);
+    const float* src = v0;
+    int i = 0;
+    while (i < count) {
+        if (quads && count - i >= 4) {
+            dim3 grid(d.X >= F4OW ? (d.X + F4OW - 1) / F4OW : 1, d.Y >= F4OH ? (d.Y + F4OH - 1) / F4OH : 1,
+                      (d.Z - 2 + FZC - 1) / FZC);
+            Mids3<F32> mids{{states[i], states[i + 1], states[i + 2]}, N};
+            hipLaunchKernelGGL((k_gvf_fused4<F32, false, true>), grid, dim3(FBX, F4Y), 0, stream, F, planes(src),
+                               out(states[i + 3]), d, mu, 1.0f, 1.0f, mids);
+            for (int k = 0; k < 4; k++) launch_faces<F32>(false, out(states[i + k]), d, stream);
+            src = states[i + 3];
+            i += 4;
+        } else {
+            if (vec2) launch_step<F32, 2>(false, F, planes(src), out(states[i]), d, mu, 1.0f, 1.0f, 1.0f, stream);
+            else launch_step<F32, 1>(false, F, planes(src), out(states[i]), d, mu, 1.0f, 1.0f, 1.0f, stream);
+            src = states[i];
+            i += 1;
+        }
+        TSF_TRY(check_launch("gvf_states_f32"));
     }
     return TSF_OK;
 }
