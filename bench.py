@@ -332,18 +332,22 @@ def fit(args, vol, d_vol, shape, stream):
 def cpu_baseline(vol):
     """Oracle FP32 GVF (OpenMP, all host threads) on a bounded sample: the
     central 256^3 block of the same volume after the same blur and vector
-    field, timed over enough iterations for ~10 s."""
+    field, timed over enough iterations for ~12 s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     c = min(256, vol.shape[0])
     o = (vol.shape[0] - c) // 2
     sub = np.ascontiguousarray(vol[o:o + c, o:o + c, o:o + c])
     F = oracle.vector_field(oracle.blur(sub, 0.5, "f32"), 0.2, 1, "f32")
-    oracle.gvf_euler(F, 0.05, 2, "f32")  # thread pool and page warm-up
-    t0 = time.time()
-    oracle.gvf_euler(F, 0.05, 4, "f32")
-    t1 = time.time() - t0
-    iters = max(4, int(12.0 / max(t1 / 4, 1e-3)))
+    oracle.gvf_euler(F, 0.05, 4, "f32")  # thread pool and page warm-up
+    # the cost per iteration from the difference of 48 and 16 iterations (a
+    # call also has a fixed cost: copies in and out), then ~12 s timed
+    t = []
+    for n in (16, 48):
+        t0 = time.time()
+        oracle.gvf_euler(F, 0.05, n, "f32")
+        t.append(time.time() - t0)
+    iters = max(16, int(12.0 / max((t[1] - t[0]) / 32, 1e-3)))
     t0 = time.time()
     oracle.gvf_euler(F, 0.05, iters, "f32")
     dt = time.time() - t0
