@@ -247,6 +247,49 @@ tsf_status tsf_candidate_nms(const float* tdf, const float* radius, const tsf_fi
 tsf_status tsf_cube_max(const float* tdf, const uint8_t* candidates, tsf_dims dims, int32_t cube, uint8_t* out,
                         hipStream_t stream);
 
+/* ------------------------------------------------------------ centerline graph */
+
+/* Centerpoint linking: linkLengths, compact, linkCenterpoints and
+ * removeDuplicateEdges (kernels_no_3d_write.cl:727-882, 947-968) and the 2D
+ * pyramid compaction of the edge image (createPositions2D/traverseHP2D,
+ * 349-390, 624-725), as runNewCenterlineAlg runs them
+ * (parallelCenterlineExtraction.cpp:912-1060).  vertices: int32 (x, y, z)
+ * triples, `count` of them (< 65536).  For each vertex a, among its
+ * candidates b (0 < |xb - xa| < max_distance, in ascending index) and pairs
+ * (b, c) with c listed before b, the first of smallest round|ab| + round|ac|
+ * with an angle at a of at least 2 rad (acos of the normalised dot product)
+ * and a mean TDF (linearly interpolated, clamp-to-edge, at xa + ab k/n,
+ * k = 0..n, n = round|ab|) of at least min_mean_tdf on both segments; then
+ * edges (a, b) and (a, c), where (x, y), x > y, is dropped when (y, x) is an
+ * edge.  The edges (int32 pairs) are written in the order of the pyramid's
+ * traversal, key sum_k (2 x_k + y_k) 4^k of the bits of x and y.  *n_edges
+ * = their number (the stream is synchronised); TSF_ERR_CAPACITY if it
+ * exceeds `capacity` (edges == NULL, capacity 0: count only).  Workspace:
+ * a row of `count` candidates per vertex (count^2 int32, as the reference's
+ * count x count images). */
+tsf_status tsf_link_centerpoints(const int32_t* vertices, int32_t count, const float* tdf, tsf_dims dims,
+                                 float max_distance, float min_mean_tdf, int32_t* edges, int32_t capacity,
+                                 int32_t* n_edges, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_link_centerpoints_workspace_size(int32_t count);
+
+/* graphComponentLabeling + calculateTreeLength (kernels_no_3d_write.cl:
+ * 884-918; parallelCenterlineExtraction.cpp:1062-1160): labels[v] = the
+ * smallest vertex index of v's connected component (the fixed point the
+ * reference's minimum propagation reaches), sizes[l] = the number of vertices
+ * labelled l (0 for indices that are no label).  labels, sizes: `count`
+ * int32 each.  Synchronises the stream. */
+tsf_status tsf_graph_components(const int32_t* edges, int32_t n_edges, int32_t count, int32_t* labels,
+                                int32_t* sizes, hipStream_t stream);
+
+/* removeSmallTrees (kernels_no_3d_write.cl:920-945), the rasterisation used
+ * when loop removal is off: for every edge (a, b) with
+ * sizes[labels[a]] >= min_tree_length, sets centerlines (one byte per voxel)
+ * to 1 at round(xa + (xb - xa) i / l), i = 0..l-1, l = round|xb - xa|.
+ * Other voxels are untouched. */
+tsf_status tsf_rasterize_tree_edges(const int32_t* edges, int32_t n_edges, const int32_t* vertices,
+                                    const int32_t* labels, const int32_t* sizes, int32_t min_tree_length,
+                                    uint8_t* centerlines, tsf_dims dims, hipStream_t stream);
+
 /* ------------------------------------------------------------ adjoints
  * Reverse-mode derivatives of the chain
  *   B = blur_sigma(I), F = createVectorField(B), V = GVF_N(F; mu), W = |V|,

@@ -165,6 +165,38 @@ def hp_compact(mask, method="sort"):
     return pos[:n]
 
 
+def link_centerpoints(vertices, tdf, max_distance, min_mean_tdf):
+    """The edge list (n, 2) int32 of runNewCenterlineAlg's linking, in the 2D
+    pyramid's order (tsf_oracle.cpp orc_link_centerpoints)."""
+    v = np.ascontiguousarray(vertices, np.int32)
+    T = np.ascontiguousarray(tdf, np.float32)
+    X, Y, Z = _dims(T)
+    cap = 2 * len(v)
+    out = np.zeros((max(cap, 1), 2), np.int32)
+    n = _fn("link_centerpoints")(_p(v), len(v), _p(T), X, Y, Z, ctypes.c_float(max_distance),
+                                 ctypes.c_float(min_mean_tdf), _p(out), cap)
+    return out[:n]
+
+
+def graph_components(edges, count):
+    """(labels, sizes) int32 of the fixed point of graphComponentLabeling."""
+    e = np.ascontiguousarray(edges, np.int32).reshape(-1, 2)
+    C = np.zeros(count, np.int32)
+    S = np.zeros(count, np.int32)
+    _fn("graph_components")(_p(e), len(e), int(count), _p(C), _p(S))
+    return C, S
+
+
+def rasterize_tree_edges(edges, vertices, labels, sizes, min_tree_length, shape, out=None):
+    e = np.ascontiguousarray(edges, np.int32).reshape(-1, 2)
+    v = np.ascontiguousarray(vertices, np.int32)
+    Z, Y, X = shape
+    o = np.zeros(shape, np.uint8) if out is None else out
+    _fn("rasterize_tree_edges")(_p(e), len(e), _p(v), _p(np.ascontiguousarray(labels, np.int32)),
+                                _p(np.ascontiguousarray(sizes, np.int32)), int(min_tree_length), _p(o), X, Y, Z)
+    return o
+
+
 def hp_key(x, y, z):
     f = _fn("hp_key")
     f.restype = ctypes.c_ulonglong

@@ -1427,3 +1427,196 @@ extern "C" long long orc_hp_compact_pyramid(const uint8_t* mask, int X, int Y, i
     return hp_compact_pyramid(mask, DIMS, pos, cap);
 }
 extern "C" unsigned long long orc_hp_key(unsigned x, unsigned y, unsigned z) { return hp_key(x, y, z); }
+
+// ---------------------------------------------------------------- centerline graph
+// runNewCenterlineAlg after the vertex list (parallelCenterlineExtraction.cpp:
+// 912-1160, 1270-1316), restated with the reference's own data layout: the
+// count x count length image, per-vertex compacted rows of (length,
+// neighbour) -- filled in ascending neighbour index, the order an in-order
+// atomic_inc would give -- a count x count edge image, removeDuplicateEdges,
+// and a 2D histogram pyramid (constructHPLevel2D sums, scanHPLevel2D descent
+// with the children (0,0), (0,1), (1,0), (1,1)) that lists the edges.
+
+// float distance of two integer positions (OpenCL distance / length)
+static float odist(const int32_t* v, int a, int b) {
+    const float dx = (float)v[3 * a] - (float)v[3 * b], dy = (float)v[3 * a + 1] - (float)v[3 * b + 1];
+    const float dz = (float)v[3 * a + 2] - (float)v[3 * b + 2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+// CLK_FILTER_LINEAR, unnormalised, clamp-to-edge (the TDF's sampler) on a
+// scalar float volume
+static float olinear(const float* T, int X, int Y, int Z, float u, float v, float w) {
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = std::floor(uu), fv = std::floor(vv), fw = std::floor(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    auto cl = [](int q, int n) { return q < 0 ? 0 : (q > n - 1 ? n - 1 : q); };
+    const int i0 = cl((int)fu, X), i1 = cl((int)fu + 1, X), j0 = cl((int)fv, Y), j1 = cl((int)fv + 1, Y);
+    const int k0 = cl((int)fw, Z), k1 = cl((int)fw + 1, Z);
+    auto at = [&](int i, int j, int k) { return T[(size_t)i + (size_t)X * ((size_t)j + (size_t)Y * k)]; };
+    float s = (1.0f - a) * (1.0f - b) * (1.0f - c) * at(i0, j0, k0);
+    s = s + a * (1.0f - b) * (1.0f - c) * at(i1, j0, k0);
+    s = s + (1.0f - a) * b * (1.0f - c) * at(i0, j1, k0);
+    s = s + a * b * (1.0f - c) * at(i1, j1, k0);
+    s = s + (1.0f - a) * (1.0f - b) * c * at(i0, j0, k1);
+    s = s + a * (1.0f - b) * c * at(i1, j0, k1);
+    s = s + (1.0f - a) * b * c * at(i0, j1, k1);
+    s = s + a * b * c * at(i1, j1, k1);
+    return s;
+}
+
+extern "C" int orc_link_centerpoints(const int32_t* v, int sum, const float* T, int X, int Y, int Z, float maxDistance,
+                                     float minAvgTDF, int32_t* edgesOut, int cap) {
+    // linkLengths and compact
+    std::vector<float> lengths((size_t)sum * sum);
+    for (int j = 0; j < sum; j++)
+        for (int i = 0; i < sum; i++) lengths[(size_t)j * sum + i] = odist(v, i, j);
+    std::vector<std::vector<std::pair<float, int>>> rows(sum);
+    for (int i = 0; i < sum; i++)
+        for (int j = 0; j < sum; j++) {
+            const float l = lengths[(size_t)j * sum + i];
+            if (l < maxDistance && l > 0.0f) rows[i].push_back({l, j});
+        }
+    // linkCenterpoints, kernels_no_3d_write.cl:753-882
+    std::vector<uint8_t> E((size_t)sum * sum, 0);  // E[y * sum + x]: image (x, y)
+    for (int id = 0; id < sum; id++) {
+        const float xa[3] = {(float)v[3 * id], (float)v[3 * id + 1], (float)v[3 * id + 2]};
+        int bx = 0, by = 0;
+        float shortestDistance = maxDistance * 2;
+        bool valid = false;
+        const auto& R = rows[id];
+        for (size_t i = 0; i < R.size(); i++) {
+            const float clx = R[i].first;
+            const int cly = R[i].second;
+            const int db = (int)std::round(clx);
+            if (db >= shortestDistance) continue;
+            for (size_t j = 0; j < i; j++) {
+                if (R[j].second == cly) continue;
+                const int dc = (int)std::round(R[j].first);
+                if (db + dc < shortestDistance) {
+                    float ab[3], ac[3];
+                    for (int q = 0; q < 3; q++) {
+                        ab[q] = (float)v[3 * cly + q] - xa[q];
+                        ac[q] = (float)v[3 * R[j].second + q] - xa[q];
+                    }
+                    const float nb = std::sqrt(ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2]);
+                    const float nc = std::sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+                    const float d = (ab[0] / nb) * (ac[0] / nc) + (ab[1] / nb) * (ac[1] / nc) + (ab[2] / nb) * (ac[2] / nc);
+                    if (std::acos(d) < 2.0f) continue;
+                    float avg = 0.0f;
+                    for (int k = 0; k <= db; k++) {
+                        const float alpha = (float)k / db;
+                        avg += olinear(T, X, Y, Z, xa[0] + ab[0] * alpha, xa[1] + ab[1] * alpha, xa[2] + ab[2] * alpha);
+                    }
+                    avg /= db + 1;
+                    if (avg < minAvgTDF) continue;
+                    avg = 0.0f;
+                    for (int k = 0; k <= dc; k++) {
+                        const float alpha = (float)k / dc;
+                        avg += olinear(T, X, Y, Z, xa[0] + ac[0] * alpha, xa[1] + ac[1] * alpha, xa[2] + ac[2] * alpha);
+                    }
+                    avg /= dc + 1;
+                    if (avg < minAvgTDF) continue;
+                    valid = true;
+                    bx = cly;
+                    by = R[j].second;
+                    shortestDistance = db + dc;
+                }
+            }
+        }
+        if (valid) {
+            E[(size_t)bx * sum + id] = 1;
+            E[(size_t)by * sum + id] = 1;
+        }
+    }
+    // removeDuplicateEdges, kernels_no_3d_write.cl:947-968
+    std::vector<uint8_t> E2((size_t)sum * sum, 0);
+    for (int y = 0; y < sum; y++)
+        for (int x = 0; x < sum; x++) {
+            if (E[(size_t)y * sum + x] == 0) continue;
+            E2[(size_t)y * sum + x] = (x > y && E[(size_t)x * sum + y] == 1) ? 0 : 1;
+        }
+    // 2D histogram pyramid: level 0 = the edge image padded to a power of two
+    int size = 1;
+    while (size < sum) size *= 2;
+    std::vector<std::vector<int>> lv;
+    lv.push_back(std::vector<int>((size_t)size * size, 0));
+    for (int y = 0; y < sum; y++)
+        for (int x = 0; x < sum; x++) lv[0][(size_t)y * size + x] = E2[(size_t)y * sum + x];
+    for (int s = size; s > 1; s /= 2) {
+        const int h = s / 2;
+        std::vector<int> up((size_t)h * h);
+        const auto& lo = lv.back();
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < h; x++)
+                up[(size_t)y * h + x] = lo[(size_t)(2 * y) * s + 2 * x] + lo[(size_t)(2 * y) * s + 2 * x + 1] +
+                                        lo[(size_t)(2 * y + 1) * s + 2 * x] + lo[(size_t)(2 * y + 1) * s + 2 * x + 1];
+        lv.push_back(up);
+    }
+    const int total = lv.back()[0];
+    // descent (scanHPLevel2D): children (0,0), (0,1), (1,0), (1,1)
+    const int off[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+    for (int target = 0; target < total && target < cap; target++) {
+        int cx = 0, cy = 0, acc0 = 0;
+        for (int l = (int)lv.size() - 2; l >= 0; l--) {
+            const int s = size >> l;
+            const auto& L = lv[l];
+            cx *= 2;
+            cy *= 2;
+            int acc = acc0;
+            int k = 0;
+            for (; k < 3; k++) {
+                const int n = L[(size_t)(cy + off[k][1]) * s + cx + off[k][0]];
+                if (acc + n <= target) acc += n;
+                else break;
+            }
+            cx += off[k][0];
+            cy += off[k][1];
+            acc0 = acc;
+        }
+        edgesOut[2 * target] = cx;
+        edgesOut[2 * target + 1] = cy;
+    }
+    return total;
+}
+
+// graphComponentLabeling to its fixed point, then calculateTreeLength
+extern "C" void orc_graph_components(const int32_t* edges, int nEdges, int sum, int32_t* C, int32_t* S) {
+    for (int i = 0; i < sum; i++) {
+        C[i] = i;
+        S[i] = 0;
+    }
+    bool m = true;
+    while (m) {
+        m = false;
+        for (int e = 0; e < nEdges; e++) {
+            const int ca = C[edges[2 * e]], cb = C[edges[2 * e + 1]];
+            if (ca == cb) continue;
+            if (ca < cb) C[edges[2 * e + 1]] = std::min(C[edges[2 * e + 1]], ca);
+            else C[edges[2 * e]] = std::min(C[edges[2 * e]], cb);
+            m = true;
+        }
+    }
+    for (int i = 0; i < sum; i++) S[C[i]]++;
+}
+
+// removeSmallTrees, kernels_no_3d_write.cl:920-945
+extern "C" void orc_rasterize_tree_edges(const int32_t* edges, int nEdges, const int32_t* v, const int32_t* C,
+                                         const int32_t* S, int minTreeLength, uint8_t* out, int X, int Y, int Z) {
+    for (int e = 0; e < nEdges; e++) {
+        const int a = edges[2 * e], b = edges[2 * e + 1];
+        if (S[C[a]] < minTreeLength) continue;
+        float xa[3], d[3];
+        for (int q = 0; q < 3; q++) {
+            xa[q] = (float)v[3 * a + q];
+            d[q] = (float)v[3 * b + q] - xa[q];
+        }
+        const int l = (int)std::round(std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+        for (int i = 0; i < l; i++) {
+            const float alpha = (float)i / l;
+            const int px = (int)std::round(xa[0] + d[0] * alpha), py = (int)std::round(xa[1] + d[1] * alpha);
+            const int pz = (int)std::round(xa[2] + d[2] * alpha);
+            if (px >= 0 && py >= 0 && pz >= 0 && px < X && py < Y && pz < Z)
+                out[(size_t)px + (size_t)X * ((size_t)py + (size_t)Y * pz)] = 1;
+        }
+    }
+}

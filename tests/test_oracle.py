@@ -38,6 +38,15 @@ Pinning status per oracle function:
   gvf_euler_sn16 .......... equals the pinned float step with every iterate
                             rounded (and w = round(|v|)); constant fixed
                             point; bounded where the float scheme diverges
+  link_centerpoints ....... known answers: a straight chain links each inner
+                            vertex to its two neighbours (duplicates dropped),
+                            a right angle and a zero TDF link nothing; the
+                            literal 2D pyramid descent lists the edges sorted
+                            by the key sum (2 x_b + y_b) 4^b
+  graph_components ........ scipy connected_components (labels = smallest
+                            index of the component), sizes by bincount
+  rasterize_tree_edges .... known voxels of axis and diagonal edges, the tree
+                            size threshold
 """
 import numpy as np
 import pytest
@@ -574,3 +583,85 @@ def test_gvf_euler_sn16_fixed_point_and_saturation(orc):
     s16 = orc.gvf_euler_sn16(F2, 0.1, 300)
     assert np.abs(f32[:3]).max() > 10
     assert np.abs(s16[:3]).max() <= 1.0 and np.all(np.isfinite(s16))
+
+
+# ---------------------------------------------------------------- centerline graph
+def _key2d(x, y):
+    k = 0
+    for b in range(16):
+        k |= (((x >> b) & 1) * 2 + ((y >> b) & 1)) << (2 * b)
+    return k
+
+
+def test_link_straight_chain_known_answer(orc):
+    # five vertices 3 voxels apart on a line in a uniform TDF: each inner
+    # vertex links its two neighbours (180 degrees), the ends link nothing
+    # (all their candidates lie in one direction); (x, y) with x > y is
+    # dropped when (y, x) exists
+    T = np.ones((12, 12, 20), np.float32)
+    v = np.array([[2 + 3 * i, 5, 6] for i in range(5)], np.int32)
+    e = orc.link_centerpoints(v, T, 25.0, 0.5)
+    got = {tuple(map(int, r)) for r in e}
+    assert got == {(1, 0), (1, 2), (2, 3), (3, 4)}
+    keys = [_key2d(int(x), int(y)) for x, y in e]
+    assert keys == sorted(keys)
+    # a TDF below the minimum mean links nothing; so does a right angle
+    assert len(orc.link_centerpoints(v, np.zeros_like(T), 25.0, 0.5)) == 0
+    L = np.array([[5, 5, 5], [9, 5, 5], [5, 9, 5]], np.int32)
+    assert len(orc.link_centerpoints(L, T, 25.0, 0.5)) == 0
+
+
+def test_link_pyramid_order_is_key_order(orc):
+    # many vertices on a few random polylines: the literal pyramid descent
+    # lists the edges in increasing key, with no duplicate pair
+    g = np.random.default_rng(11)
+    pts = []
+    for _ in range(6):
+        p = g.uniform(8, 40, 3)
+        d = g.normal(size=3)
+        d /= np.linalg.norm(d)
+        for k in range(12):
+            p = p + 2.5 * d
+            d = d + 0.2 * g.normal(size=3)
+            d /= np.linalg.norm(d)
+            pts.append(np.clip(np.round(p), 0, 47))
+    v = np.unique(np.array(pts, np.int32), axis=0)
+    T = np.ones((48, 48, 48), np.float32)
+    e = orc.link_centerpoints(v, T, 25.0, 0.5)
+    assert len(e) > 20
+    keys = [_key2d(int(x), int(y)) for x, y in e]
+    assert keys == sorted(keys) and len(set(keys)) == len(keys)
+    pairs = {tuple(sorted(map(int, r))) for r in e}
+    assert len(pairs) == len(e)
+
+
+def test_graph_components_vs_scipy(orc):
+    from scipy.sparse.csgraph import connected_components
+    g = np.random.default_rng(12)
+    n = 300
+    e = g.integers(0, n, (220, 2)).astype(np.int32)
+    C, S = orc.graph_components(e, n)
+    A = sp.coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n))
+    _, lab = connected_components(A, directed=False)
+    mins = {}
+    for i, l in enumerate(lab):
+        mins.setdefault(l, i)
+    assert np.array_equal(C, np.array([mins[l] for l in lab]))
+    assert np.array_equal(S, np.bincount(C, minlength=n))
+
+
+def test_rasterize_tree_edges_known_voxels(orc):
+    v = np.array([[1, 2, 3], [5, 2, 3], [1, 6, 7]], np.int32)
+    e = np.array([[0, 1], [0, 2]], np.int32)
+    C = np.zeros(3, np.int32)
+    S = np.array([3, 0, 0], np.int32)
+    out = orc.rasterize_tree_edges(e, v, C, S, 3, (10, 10, 10))
+    # axis edge: x = 1..4 (the end voxel is not set); diagonal (0, 4, 4):
+    # l = round(5.657) = 6, round(4 i / 6) = 0, 1, 1, 2, 3, 3
+    exp = np.zeros((10, 10, 10), np.uint8)
+    exp[3, 2, 1:5] = 1
+    for i in range(6):
+        q = int(np.floor(4 * i / 6 + 0.5))
+        exp[3 + q, 2 + q, 1] = 1
+    assert np.array_equal(out, exp)
+    assert not orc.rasterize_tree_edges(e, v, C, S, 4, (10, 10, 10)).any()

@@ -69,6 +69,11 @@ _SIGS = {
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
     "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
     "tsf_cube_max": ([_vp, _vp, Dims, _i32, _vp, _vp], _i32),
+    "tsf_link_centerpoints": ([_vp, _i32, _vp, Dims, _f32, _f32, _vp, _i32, ctypes.POINTER(_i32), _vp, _sz, _vp],
+                              _i32),
+    "tsf_link_centerpoints_workspace_size": ([_i32], _sz),
+    "tsf_graph_components": ([_vp, _i32, _i32, _vp, _vp, _vp], _i32),
+    "tsf_rasterize_tree_edges": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, Dims, _vp], _i32),
     "tsf_blur_adjoint": ([_vp, _vp, _vp, Dims, _f32, ctypes.POINTER(ctypes.c_double), _vp, _sz, _vp], _i32),
     "tsf_blur_adjoint_workspace_size": ([], _sz),
     "tsf_vector_field_adjoint": ([_vp, _PF, Dims, _f32, _i32, _vp, _vp, _sz, _vp], _i32),
@@ -430,6 +435,34 @@ def candidate_nms(tdf, radius, field, positions, count, maxd_cap, out, stream=No
 
 def cube_max(tdf, cand, cube, out, stream=None):
     _check(lib().tsf_cube_max(tdf.ptr, cand.ptr, _dims(tdf.shape), cube, out.ptr, _sp(stream)))
+
+
+# ------------------------------------------------------------------- centerline graph
+def link_centerpoints_workspace_size(count):
+    return lib().tsf_link_centerpoints_workspace_size(int(count))
+
+
+def link_centerpoints(vertices, count, tdf, max_distance, min_mean_tdf, edges=None, workspace=None, stream=None):
+    """vertices: DeviceArray (>= count, 3) int32; tdf: DeviceArray (Z, Y, X) float32.
+    Returns the edge count; edges (DeviceArray (capacity, 2) int32) receives them."""
+    n = _i32(0)
+    cap = edges.shape[0] if edges is not None else 0
+    w, wb = _ws(workspace)
+    _check(lib().tsf_link_centerpoints(vertices.ptr, int(count), tdf.ptr, _dims(tdf.shape), max_distance,
+                                       min_mean_tdf, edges.ptr if edges is not None else None, cap, ctypes.byref(n),
+                                       w, wb, _sp(stream)))
+    return n.value
+
+
+def graph_components(edges, n_edges, count, labels, sizes, stream=None):
+    _check(lib().tsf_graph_components(edges.ptr if edges is not None else None, int(n_edges), int(count), labels.ptr,
+                                      sizes.ptr, _sp(stream)))
+
+
+def rasterize_tree_edges(edges, n_edges, vertices, labels, sizes, min_tree_length, centerlines, stream=None):
+    _check(lib().tsf_rasterize_tree_edges(edges.ptr, int(n_edges), vertices.ptr, labels.ptr, sizes.ptr,
+                                          int(min_tree_length), centerlines.ptr, _dims(centerlines.shape),
+                                          _sp(stream)))
 
 
 # ------------------------------------------------------------------- adjoints
