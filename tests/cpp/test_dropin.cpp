@@ -12,6 +12,7 @@
 //                                 the oracle's FP64 chain gradient
 // Prints one "PASS name" / "FAIL name: reason" line per case; exit status is
 // the number of failures.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,6 +29,7 @@ using namespace tsf;
 // SIPL::int3/float3 roles are taken by tsf::int3/float3
 using I3 = tsf::int3;
 using F3 = tsf::float3;
+using I2 = tsf::int2;
 
 // ----------------------------------------------------------------- oracle (extern "C" in libtsf_oracle.so)
 extern "C" {
@@ -49,6 +51,11 @@ void orc_do_eigen_f32(const float* Fx, const float* Fy, const float* Fz, int X, 
 void orc_vector_field_sn16(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign);
 void orc_gvf_euler_sn16(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters);
 void orc_unorm16_round(float* a, long long n);
+int orc_link_centerpoints(const int32_t* v, int sum, const float* T, int X, int Y, int Z, float maxDistance,
+                          float minAvgTDF, int32_t* edgesOut, int cap);
+void orc_graph_components(const int32_t* edges, int nEdges, int sum, int32_t* C, int32_t* S);
+void orc_rasterize_tree_edges(const int32_t* edges, int nEdges, const int32_t* v, const int32_t* C, const int32_t* S,
+                              int minTreeLength, uint8_t* out, int X, int Y, int Z);
 void orc_chain_forward_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
                            int iters, float rmin, float rmax, float rstep, double* tdf, double* rad);
 void orc_chain_gradient_f64(const double* I, int X, int Y, int Z, double sigma, float Fmax, int sign, double mu,
@@ -120,6 +127,56 @@ static void param_tests() {
         setParameter(p, "parameters", "Missing");
         if (!throws([&] { loadParameterPreset(p, dir); })) return std::string("missing preset accepted");
         return std::string();
+    });
+    // loop removal leaves a forest (no cycle; V - E components) whose
+    // vertices are input vertices, keeping every vertex whose degree is not
+    // two (nodes of degree two on edges left out of the spanning tree go,
+    // which is how the loops are removed)
+    run("removeLoops_forest", [] {
+        std::vector<I3> v;
+        std::vector<I2> e;
+        // a ring of 12 with two chords and a tail; a triangle with a tail
+        for (int i = 0; i < 12; i++)
+            v.push_back(I3(50 + (int)std::lround(20 * std::cos(i * 0.5236)), 50 + (int)std::lround(20 * std::sin(i * 0.5236)), 50));
+        for (int i = 0; i < 12; i++) e.push_back(I2(i, (i + 1) % 12));
+        e.push_back(I2(0, 6));
+        e.push_back(I2(3, 9));
+        v.push_back(I3(75, 50, 50));
+        e.push_back(I2(0, 12));
+        v.push_back(I3(10, 10, 10));
+        v.push_back(I3(13, 10, 10));
+        v.push_back(I3(10, 14, 10));
+        v.push_back(I3(6, 10, 10));
+        e.push_back(I2(13, 14));
+        e.push_back(I2(14, 15));
+        e.push_back(I2(15, 13));
+        e.push_back(I2(13, 16));
+        std::vector<int> deg(v.size(), 0);
+        for (auto& ed : e) deg[ed.x]++, deg[ed.y]++;
+        const std::vector<I3> vin = v;
+        I3 sz(100, 100, 100);
+        removeLoops(v, e, sz);
+        auto key = [](const I3& a) { return a.x + 100 * (a.y + 100 * a.z); };
+        std::vector<int> in, out;
+        for (auto& a : vin) in.push_back(key(a));
+        for (auto& a : v) out.push_back(key(a));
+        for (int k : out)
+            if (std::find(in.begin(), in.end(), k) == in.end()) return std::string("a vertex not in the input");
+        for (size_t i = 0; i < vin.size(); i++)
+            if (deg[i] != 2 && std::find(out.begin(), out.end(), key(vin[i])) == out.end())
+                return std::string("a vertex of degree != 2 was dropped");
+        std::vector<int> par(v.size());
+        for (size_t i = 0; i < par.size(); i++) par[i] = (int)i;
+        std::function<int(int)> f = [&](int a) { return par[a] == a ? a : par[a] = f(par[a]); };
+        for (auto& ed : e) {
+            int a = f(ed.x), b = f(ed.y);
+            if (a == b) return std::string("the output has a cycle");
+            par[a] = b;
+        }
+        int comps = 0;
+        for (size_t i = 0; i < par.size(); i++) comps += f((int)i) == (int)i;
+        std::printf("  %zu of %zu vertices kept, %zu edges, %d trees\n", v.size(), vin.size(), e.size(), comps);
+        return comps == 2 && e.size() == v.size() - 2 ? std::string() : std::string("not a two-tree forest");
     });
 }
 
@@ -231,6 +288,81 @@ static void gpu_tests(const std::string& dsdir) {
             if (gv[i] != pos[i]) return std::string("vertex list differs (order or position)");
         std::printf("  candidates %d, vertices %d\n", cv.candidates, cv.count);
         return std::string();
+    });
+    // the centerline graph: device rasterisation (loop removal off) against
+    // the oracle on the device's vertices and TDF
+    run("runNewCenterlineAlg_device_rasterisation", [&] {
+        paramList q = p;
+        setParameter(q, "loop-removal", "false");
+        Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
+        CenterpointVertices cv = runNewCenterlineAlgCandidates(ocl, size, q, vf2, tdf2, rad2);
+        Buffer cl = runNewCenterlineAlg(ocl, size, q, vf2, tdf2, rad2);
+        std::vector<uint8_t> got(n);
+        cl.download(ocl, got.data(), n);
+        std::vector<int32_t> v(3 * cv.count), e(4 * cv.count), C(cv.count), S(cv.count);
+        cv.vertices.download(ocl, v.data(), v.size() * 4);
+        int ne = orc_link_centerpoints(v.data(), cv.count, gT.data(), X, Y, Z, 10.0f, 0.3f, e.data(), 2 * cv.count);
+        orc_graph_components(e.data(), ne, cv.count, C.data(), S.data());
+        std::vector<uint8_t> exp(n, 0);
+        orc_rasterize_tree_edges(e.data(), ne, v.data(), C.data(), S.data(), 3, exp.data(), X, Y, Z);
+        size_t on = 0;
+        for (uint8_t b : exp) on += b;
+        std::printf("  %d vertices, %d edges, %zu centerline voxels\n", cv.count, ne, on);
+        return got == exp && on > 0 ? std::string() : std::string("centerline voxels differ from the oracle");
+    });
+    // the default path (loop removal on the host) against the reference's
+    // own system-test bar on dataset_1 (tests/tubeSegmentationTests.cpp:62-65:
+    // average distance from the real centerline < 1.5 voxels, more than 75 %
+    // of it extracted; measured as tests/tubeValidation.cpp:22-105 does)
+    run("runNewCenterlineAlg_dataset_1_centerline_quality", [&] {
+        Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
+        std::vector<float> r0 = rad.downloadPlane(ocl, 0);
+        Buffer cl = runNewCenterlineAlg(ocl, size, p, vf2, tdf2, rad2);
+        std::vector<uint8_t> e(n);
+        cl.download(ocl, e.data(), n);
+        std::vector<uint8_t> real = read_raw(dsdir + "/real_centerline.raw", n), detected(n, 0);
+        const int R = 4;
+        double sumd = 0;
+        long found = 0, incorrect = 0, extracted = 0;
+        for (int z = 0; z < Z; z++)
+            for (int y = 0; y < Y; y++)
+                for (int x = 0; x < X; x++) {
+                    if (e[x + X * (y + Y * z)] != 1) continue;
+                    extracted++;
+                    float best = 9999.0f;
+                    bool ok = false;
+                    for (int c = -R; c <= R; c++)
+                        for (int b = -R; b <= R; b++)
+                            for (int a = -R; a <= R; a++) {
+                                const int qx = x + a, qy = y + b, qz = z + c;
+                                if (qx < 0 || qy < 0 || qz < 0 || qx >= X || qy >= Y || qz >= Z) continue;
+                                const float dd = std::sqrt((float)(a * a + b * b + c * c));
+                                if (!(dd < R) || real[qx + X * (qy + Y * qz)] != 1) continue;
+                                ok = true;
+                                best = std::min(best, dd);
+                                detected[qx + X * (qy + Y * qz)] = 1;
+                            }
+                    if (ok) {
+                        sumd += best;
+                        found++;
+                    } else {
+                        incorrect++;
+                    }
+                }
+        long det = 0, undet = 0;
+        for (size_t i = 0; i < n; i++) {
+            det += detected[i] == 1;
+            undet += real[i] == 1 && detected[i] == 0;
+        }
+        const double avg = sumd / found, pct = 100.0 * det / (det + undet);
+        // the radius is rewritten only on centerline voxels
+        std::vector<float> r1 = rad2.downloadPlane(ocl, 0);
+        size_t moved = 0;
+        for (size_t i = 0; i < n; i++) moved += r1[i] != r0[i] && e[i] != 1;
+        std::printf("  %ld centerline voxels: average distance %.3f, %.1f %% extracted, %ld off the real one\n",
+                    extracted, avg, pct, incorrect);
+        if (moved) return std::string("radius changed off the centerline");
+        return avg < 1.5 && pct > 75.0 ? std::string() : std::string("below the reference's system-test bar");
     });
     run("candidate_count_range_error", [&] {
         paramList q = p;

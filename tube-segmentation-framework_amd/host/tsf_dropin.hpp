@@ -35,6 +35,11 @@ struct float3 {
     float3() = default;
     float3(float a, float b, float c) : x(a), y(b), z(c) {}
 };
+struct int2 {
+    int x = 0, y = 0;
+    int2() = default;
+    int2(int a, int b) : x(a), y(b) {}
+};
 
 class Error : public std::runtime_error {
    public:
@@ -208,6 +213,28 @@ struct CenterpointVertices {
 };
 CenterpointVertices runNewCenterlineAlgCandidates(OpenCL& ocl, int3 size, paramList& parameters,
                                                   Image3D& vectorField, Image3D& TDF, Image3D& radius);
+
+// runNewCenterlineAlg (parallelCenterlineExtraction.hpp:7;
+// parallelCenterlineExtraction.cpp:667-1332): the candidates and vertices
+// above, then centerpoint linking, component labelling and the centerline
+// voxels.  Returns the centerline mask, one byte per voxel (the reference's
+// CL_SIGNED_INT8 image), as a device Buffer; with "centerpoints-only" the
+// suppressed candidates instead.  With "loop-removal" (the default) or a
+// "centerline-vtk-file" the large trees go through removeLoops and
+// createCenterlineVoxels on the host, which also rewrites `radius` along the
+// centerline (and the VTK file is written); otherwise the device rasterises
+// the large trees' edges.  Throws tsf::Error (TSF_ERR_RANGE) where the
+// reference throws: no edges, more than 10 million edges, an empty graph.
+Buffer runNewCenterlineAlg(OpenCL& ocl, int3 size, paramList& parameters, Image3D& vectorField, Image3D& TDF,
+                           Image3D& radius);
+// The host steps it uses (file-local in the reference,
+// parallelCenterlineExtraction.cpp:129-289, 291-322; inputOutput.cpp:48-64):
+// loop removal by a spanning tree over the graph with its degree-two nodes
+// fused; the voxels (and averaged radius) along each edge, ceil|ab| samples;
+// the VTK polydata file.  createCenterlineVoxels returns new[]'d memory.
+void removeLoops(std::vector<int3>& vertices, std::vector<int2>& edges, int3& size);
+char* createCenterlineVoxels(std::vector<int3>& vertices, std::vector<int2>& edges, float* radius, int3& size);
+void writeToVtkFile(paramList& parameters, std::vector<int3> vertices, std::vector<int2> edges);
 
 // eigenanalysisOfHessian.hpp:7-9 on the host arrays of TubeSegmentation
 // (tube-segmentation.hpp:22-30).  The arrays are mirrored to the device on
