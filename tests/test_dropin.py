@@ -24,7 +24,7 @@ def _run(*args, timeout=300):
 def test_dropin_parameters():
     r = _run("params")
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("PASS") == 3
+    assert r.stdout.count("PASS") == 4
 
 
 @pytest.mark.gpu
@@ -32,7 +32,7 @@ def test_dropin_pipeline_on_dataset_1(tsf):
     r = _run("gpu", os.path.join(ROOT, "tests", "golden", "dataset_1"), timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 8
+    assert r.stdout.count("PASS") == 10
 
 
 @pytest.mark.gpu
