@@ -290,6 +290,35 @@ tsf_status tsf_rasterize_tree_edges(const int32_t* edges, int32_t n_edges, const
                                     const int32_t* labels, const int32_t* sizes, int32_t min_tree_length,
                                     uint8_t* centerlines, tsf_dims dims, hipStream_t stream);
 
+/* ------------------------------------------------------------ segmentation */
+
+/* runInverseGradientSegmentation (segmentation.cpp:5-221; kernels
+ * initGrowing, grow, dilate, erode, kernels_no_3d_write.cl:1100-1207,
+ * 1370-1412).  centerlines: one byte per voxel (1 = centerline); radius:
+ * float per voxel (the averaged radius along the centerline); field: the
+ * GVF field with w = |v|.  Seeds: a cube of half size
+ * clamp(round(radius/2), 1, 4) around each centerline voxel.  A pass
+ * accepts a candidate X when a neighbour Y not yet accepted, with
+ * w(Y) > w(X), has its normalised field pointing most nearly at X among
+ * Y's 26 neighbour directions (the first of equal ones in x, y, z loop
+ * order); Y becomes a candidate; candidates without such a Y are dropped.
+ * Where a candidate's verdict and a nomination of it meet, accepted >
+ * nominated > dropped (the reference leaves this to the write order).
+ * At least two passes, then until a pass accepts nothing; then a 3x3x3
+ * dilation and erosion.  segmentation: one byte per voxel (0/1).
+ * *iterations (if not NULL): the number of passes.  Synchronises once per
+ * pass (the reference polls its stop flag likewise). */
+tsf_status tsf_segment_inverse_gradient(const uint8_t* centerlines, const float* radius, const tsf_field* field,
+                                        tsf_dims dims, uint8_t* segmentation, int32_t* iterations, void* workspace,
+                                        size_t workspace_bytes, hipStream_t stream);
+size_t tsf_segment_inverse_gradient_workspace_size(tsf_dims dims);
+
+/* runSphereSegmentation / sphereSegmentation (segmentation.cpp:223-307;
+ * kernels_no_3d_write.cl:1209-1234): 1 within |offset| < radius of every
+ * centerline voxel, 0 elsewhere. */
+tsf_status tsf_segment_spheres(const uint8_t* centerlines, const float* radius, tsf_dims dims, uint8_t* segmentation,
+                               hipStream_t stream);
+
 /* ------------------------------------------------------------ adjoints
  * Reverse-mode derivatives of the chain
  *   B = blur_sigma(I), F = createVectorField(B), V = GVF_N(F; mu), W = |V|,

@@ -197,6 +197,27 @@ def rasterize_tree_edges(edges, vertices, labels, sizes, min_tree_length, shape,
     return o
 
 
+def segment_inverse_gradient(centerlines, radius, F):
+    """(segmentation uint8, passes) of runInverseGradientSegmentation;
+    F: (4, Z, Y, X) float32 field with w."""
+    cl = np.ascontiguousarray(centerlines, np.uint8)
+    r = np.ascontiguousarray(radius, np.float32)
+    F = np.ascontiguousarray(F, np.float32)
+    X, Y, Z = _dims(cl)
+    seg = np.zeros(cl.shape, np.uint8)
+    passes = _fn("segment_inverse_gradient")(_p(cl), _p(r), _p(F), X, Y, Z, _p(seg))
+    return seg, passes
+
+
+def segment_spheres(centerlines, radius):
+    cl = np.ascontiguousarray(centerlines, np.uint8)
+    r = np.ascontiguousarray(radius, np.float32)
+    X, Y, Z = _dims(cl)
+    seg = np.zeros(cl.shape, np.uint8)
+    _fn("segment_spheres")(_p(cl), _p(r), X, Y, Z, _p(seg))
+    return seg
+
+
 def hp_key(x, y, z):
     f = _fn("hp_key")
     f.restype = ctypes.c_ulonglong

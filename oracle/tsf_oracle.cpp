@@ -1620,3 +1620,128 @@ extern "C" void orc_rasterize_tree_edges(const int32_t* edges, int nEdges, const
         }
     }
 }
+
+// ---------------------------------------------------------------- segmentation
+// runInverseGradientSegmentation (segmentation.cpp:5-221) with the kernels
+// initGrowing / grow / dilate / erode (kernels_no_3d_write.cl:1100-1207,
+// 1370-1412) on a float field (x, y, z, w planes).  The reference's next
+// state is written by many threads in an unspecified order; this
+// restatement evaluates every candidate on the current state, then merges
+// with the precedence accepted > nominated > dropped.
+static int oclamp(int q, int n) { return q < 0 ? 0 : (q > n - 1 ? n - 1 : q); }
+
+extern "C" int orc_segment_inverse_gradient(const uint8_t* cl, const float* radius, const float* F, int X, int Y,
+                                            int Z, uint8_t* seg) {
+    const size_t n = (size_t)X * Y * Z;
+    auto at = [&](int x, int y, int z) { return (size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z); };
+    auto in = [&](int x, int y, int z) { return x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z; };
+    auto cat = [&](int x, int y, int z) { return at(oclamp(x, X), oclamp(y, Y), oclamp(z, Z)); };
+    const float* Fx = F;
+    const float* Fy = F + n;
+    const float* Fz = F + 2 * n;
+    const float* Fw = F + 3 * n;
+    std::vector<uint8_t> cur(cl, cl + n);
+    // initGrowing
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                if (cl[at(x, y, z)] != 1) continue;
+                const int N = std::min(std::max(1, (int)std::round(radius[at(x, y, z)] / 2.0f)), 4);
+                for (int a = -N; a < N + 1; a++)
+                    for (int b = -N; b < N + 1; b++)
+                        for (int c = -N; c < N + 1; c++)
+                            if (cl[cat(x + a, y + b, z + c)] == 0 && in(x + a, y + b, z + c))
+                                cur[at(x + a, y + b, z + c)] = 2;
+            }
+    int passes = 0;
+    for (;;) {
+        std::vector<uint8_t> accepted(n, 0), nominated(n, 0);
+        bool grew = false;
+        for (int z = 0; z < Z; z++)
+            for (int y = 0; y < Y; y++)
+                for (int x = 0; x < X; x++) {
+                    if (cur[at(x, y, z)] != 2) continue;
+                    const float FNXw = Fw[at(x, y, z)];
+                    bool cont = false;
+                    for (int a = -1; a < 2; a++)
+                        for (int b = -1; b < 2; b++)
+                            for (int c = -1; c < 2; c++) {
+                                if (a == 0 && b == 0 && c == 0) continue;
+                                const int yx = x + a, yy = y + b, yz = z + c;
+                                const size_t q = cat(yx, yy, yz);
+                                if (cur[q] == 1) continue;
+                                const float w = Fw[q], fx = Fx[q] / w, fy = Fy[q] / w, fz = Fz[q] / w;
+                                if (!(w > FNXw)) continue;
+                                int zx = 0, zy = 0, zz = 0;
+                                float maxDot = -2.0f;
+                                for (int a2 = -1; a2 < 2; a2++)
+                                    for (int b2 = -1; b2 < 2; b2++)
+                                        for (int c2 = -1; c2 < 2; c2++) {
+                                            if (a2 == 0 && b2 == 0 && c2 == 0) continue;
+                                            const float l = std::sqrt((float)(a2 * a2 + b2 * b2 + c2 * c2));
+                                            const float v = fx * ((float)a2 / l) + fy * ((float)b2 / l) + fz * ((float)c2 / l);
+                                            if (v > maxDot) {
+                                                maxDot = v;
+                                                zx = yx + a2;
+                                                zy = yy + b2;
+                                                zz = yz + c2;
+                                            }
+                                        }
+                                if (zx == x && zy == y && zz == z && in(yx, yy, yz)) {
+                                    nominated[at(yx, yy, yz)] = 1;
+                                    cont = true;
+                                }
+                            }
+                    if (cont) {
+                        accepted[at(x, y, z)] = 1;
+                        grew = true;
+                    }
+                }
+        for (size_t i = 0; i < n; i++) cur[i] = (cur[i] == 1 || accepted[i]) ? 1 : (nominated[i] ? 2 : 0);
+        passes++;
+        if (passes >= 2 && !grew) break;
+    }
+    // dilate into an empty volume, then erode (clamp-to-edge reads)
+    std::vector<uint8_t> dil(n, 0);
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                if (cur[at(x, y, z)] != 1) continue;
+                for (int a = -1; a < 2; a++)
+                    for (int b = -1; b < 2; b++)
+                        for (int c = -1; c < 2; c++)
+                            if (in(x + a, y + b, z + c)) dil[at(x + a, y + b, z + c)] = 1;
+            }
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                bool keep = dil[at(x, y, z)] == 1;
+                for (int a = -1; a < 2; a++)
+                    for (int b = -1; b < 2; b++)
+                        for (int c = -1; c < 2; c++) keep = keep && dil[cat(x + a, y + b, z + c)] == 1;
+                seg[at(x, y, z)] = keep ? 1 : 0;
+            }
+    return passes;
+}
+
+// sphereSegmentation (kernels_no_3d_write.cl:1209-1234)
+extern "C" void orc_segment_spheres(const uint8_t* cl, const float* radius, int X, int Y, int Z, uint8_t* seg) {
+    const size_t n = (size_t)X * Y * Z;
+    std::fill(seg, seg + n, 0);
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                const size_t i = (size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z);
+                if (cl[i] == 0) continue;
+                const float r = radius[i];
+                const int N = (int)std::ceil(r);
+                for (int a = -N; a <= N; a++)
+                    for (int b = -N; b <= N; b++)
+                        for (int c = -N; c <= N; c++) {
+                            if (!(std::sqrt((float)a * a + (float)b * b + (float)c * c) < r)) continue;
+                            const int px = x + a, py = y + b, pz = z + c;
+                            if (px >= 0 && py >= 0 && pz >= 0 && px < X && py < Y && pz < Z)
+                                seg[(size_t)px + (size_t)X * ((size_t)py + (size_t)Y * pz)] = 1;
+                        }
+            }
+}

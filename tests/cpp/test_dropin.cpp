@@ -314,7 +314,7 @@ static void gpu_tests(const std::string& dsdir) {
     // own system-test bar on dataset_1 (tests/tubeSegmentationTests.cpp:62-65:
     // average distance from the real centerline < 1.5 voxels, more than 75 %
     // of it extracted; measured as tests/tubeValidation.cpp:22-105 does)
-    run("runNewCenterlineAlg_dataset_1_centerline_quality", [&] {
+    run("runNewCenterlineAlg_dataset_1_centerline_and_segmentation", [&] {
         Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
         std::vector<float> r0 = rad.downloadPlane(ocl, 0);
         Buffer cl = runNewCenterlineAlg(ocl, size, p, vf2, tdf2, rad2);
@@ -362,7 +362,23 @@ static void gpu_tests(const std::string& dsdir) {
         std::printf("  %ld centerline voxels: average distance %.3f, %.1f %% extracted, %ld off the real one\n",
                     extracted, avg, pct, incorrect);
         if (moved) return std::string("radius changed off the centerline");
-        return avg < 1.5 && pct > 75.0 ? std::string() : std::string("below the reference's system-test bar");
+        if (!(avg < 1.5 && pct > 75.0)) return std::string("below the reference's system-test bar");
+        // the segmentation from this centerline: voxel agreement with the
+        // ground truth, as tests/tubeValidation.cpp:107-137 measures it (the
+        // reference's bar: precision and recall > 0.7)
+        Buffer seg = runInverseGradientSegmentation(ocl, cl, vf2, rad2, size, p);
+        std::vector<uint8_t> s(n), orig = read_raw(dsdir + "/original.raw", n);
+        seg.download(ocl, s.data(), n);
+        long tp = 0, fp = 0, fn = 0;
+        for (size_t i = 0; i < n; i++) {
+            const bool t = orig[i] == 1, g = s[i] == 1;
+            tp += t && g;
+            fp += !t && g;
+            fn += t && !g;
+        }
+        const double recall = (double)tp / (tp + fn), precision = (double)tp / (tp + fp);
+        std::printf("  segmentation: %ld voxels, precision %.3f, recall %.3f\n", tp + fp, precision, recall);
+        return precision > 0.7 && recall > 0.7 ? std::string() : std::string("segmentation below the bar");
     });
     run("candidate_count_range_error", [&] {
         paramList q = p;

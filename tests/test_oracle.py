@@ -47,6 +47,12 @@ Pinning status per oracle function:
                             index of the component), sizes by bincount
   rasterize_tree_edges .... known voxels of axis and diagonal edges, the tree
                             size threshold
+  segment_inverse_gradient  with no growth (a uniform field) the closing of
+                            the centerline equals scipy's maximum then
+                            minimum filter (zero / nearest borders), two
+                            passes; an inward field rising to a tube wall
+                            grows the tube's cylinder and no further
+  segment_spheres ......... the lattice points of the open ball, clipped
 """
 import numpy as np
 import pytest
@@ -665,3 +671,55 @@ def test_rasterize_tree_edges_known_voxels(orc):
         exp[3 + q, 2 + q, 1] = 1
     assert np.array_equal(out, exp)
     assert not orc.rasterize_tree_edges(e, v, C, S, 4, (10, 10, 10)).any()
+
+
+# ---------------------------------------------------------------- segmentation
+def test_segment_closing_of_centerline_without_growth(orc):
+    # w equal everywhere: no neighbour has a larger magnitude, so nothing is
+    # accepted and the result is the 3x3x3 closing of the centerline
+    g = np.random.default_rng(13)
+    shape = (14, 16, 18)
+    cl = (g.random(shape) < 0.04).astype(np.uint8)
+    F = np.zeros((4,) + shape, np.float32)
+    F[3] = 1.0
+    seg, passes = orc.segment_inverse_gradient(cl, np.full(shape, 3.0, np.float32), F)
+    exp = ndi.minimum_filter(ndi.maximum_filter(cl, size=3, mode="constant", cval=0), size=3, mode="nearest")
+    assert passes == 2
+    assert np.array_equal(seg, exp)
+
+
+def test_segment_tube_grows_to_its_wall(orc):
+    # a tube along x: the field points at the axis with magnitude rising to
+    # rho = 6 and falling beyond; seeds of half size 1 on the axis grow out
+    # over several passes while the magnitude rises, and never past it
+    shape, P = (28, 28, 32), 6.0
+    z, y, x = np.mgrid[:shape[0], :shape[1], :shape[2]].astype(np.float32)
+    dy, dz = y - 13.5, z - 13.5
+    rho = np.sqrt(dy * dy + dz * dz)
+    m = np.where(rho <= P, rho, np.maximum(0.0, 2 * P - rho))
+    F = np.zeros((4,) + shape, np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        F[1] = np.where(rho > 0, -m * dy / rho, 0.0)
+        F[2] = np.where(rho > 0, -m * dz / rho, 0.0)
+    w = np.sqrt(F[1] ** 2 + F[2] ** 2)
+    F[3] = np.where(w > 0, w, 1.0)
+    cl = np.zeros(shape, np.uint8)
+    cl[13:15, 13:15, 2:30] = 1
+    seg, passes = orc.segment_inverse_gradient(cl, np.full(shape, 2.0, np.float32), F)
+    core = (rho <= 4.0) & (x >= 3) & (x < 29)
+    cover = (seg.astype(bool) & core).sum() / core.sum()
+    print(f"tube: {seg.sum()} voxels in {passes} passes, core covered {cover:.3f}")
+    assert passes >= 4 and cover > 0.95
+    assert not (seg.astype(bool) & (rho > P + 1.0)).any()
+
+
+def test_segment_spheres_ball(orc):
+    shape = (11, 12, 13)
+    cl = np.zeros(shape, np.uint8)
+    cl[5, 6, 6] = 1
+    cl[0, 0, 0] = 1
+    r = np.full(shape, 2.5, np.float32)
+    seg = orc.segment_spheres(cl, r)
+    z, y, x = np.mgrid[:shape[0], :shape[1], :shape[2]]
+    ball = lambda cz, cy, cx: (z - cz) ** 2 + (y - cy) ** 2 + (x - cx) ** 2 < 6.25
+    assert np.array_equal(seg.astype(bool), ball(5, 6, 6) | ball(0, 0, 0))

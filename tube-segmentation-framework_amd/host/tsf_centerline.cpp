@@ -1,7 +1,8 @@
 // The host side of the centerline graph: runNewCenterlineAlg after the
 // device stages (parallelCenterlineExtraction.cpp:1160-1316), loop removal
 // and the centerline voxels (parallelCenterlineExtraction.cpp:27-322), the
-// VTK writer (inputOutput.cpp:48-64).
+// VTK writer (inputOutput.cpp:48-64); and the segmentation drivers
+// (segmentation.cpp:5-307), each one device call.
 //
 // Loop removal is a sequential graph algorithm on a few thousand vertices,
 // host code in the reference as here.  Its result depends on the order of
@@ -300,6 +301,29 @@ Buffer runNewCenterlineAlg(OpenCL& ocl, int3 size, paramList& parameters, Image3
     if (vtk) writeToVtkFile(parameters, vertices, graphEdges);
     ocl.finish();
     return centerlines;
+}
+
+Buffer runInverseGradientSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& vectorField, Image3D& radius,
+                                      int3 size, paramList parameters) {
+    (void)parameters;
+    const tsf_dims d{size.x, size.y, size.z};
+    const size_t total = (size_t)size.x * size.y * size.z;
+    Buffer seg(total), ws(tsf_segment_inverse_gradient_workspace_size(d));
+    tsf_field f = vectorField.field();
+    int32_t passes = 0;
+    check(tsf_segment_inverse_gradient((const uint8_t*)centerline.get(), (const float*)radius.plane(0), &f, d,
+                                       (uint8_t*)seg.get(), &passes, ws.get(), ws.size(), ocl.queue));
+    ocl.finish();
+    return seg;
+}
+
+Buffer runSphereSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& radius, int3 size, paramList parameters) {
+    (void)parameters;
+    const tsf_dims d{size.x, size.y, size.z};
+    Buffer seg((size_t)size.x * size.y * size.z);
+    check(tsf_segment_spheres((const uint8_t*)centerline.get(), (const float*)radius.plane(0), d,
+                              (uint8_t*)seg.get(), ocl.queue));
+    return seg;
 }
 
 }  // namespace tsf

@@ -236,6 +236,16 @@ void removeLoops(std::vector<int3>& vertices, std::vector<int2>& edges, int3& si
 char* createCenterlineVoxels(std::vector<int3>& vertices, std::vector<int2>& edges, float* radius, int3& size);
 void writeToVtkFile(paramList& parameters, std::vector<int3> vertices, std::vector<int2> edges);
 
+// segmentation.hpp:6-7: the segmentation from the centerline mask (one byte
+// per voxel, as runNewCenterlineAlg returns it), the GVF field (with w) and
+// the radius volume; returns the mask (one byte per voxel, 0/1) as a device
+// Buffer.  Inverse-gradient region growing with a 3x3x3 closing (the
+// default), or balls of the radius around every centerline voxel
+// ("sphere-segmentation").
+Buffer runInverseGradientSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& vectorField, Image3D& radius,
+                                      int3 size, paramList parameters);
+Buffer runSphereSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& radius, int3 size, paramList parameters);
+
 // eigenanalysisOfHessian.hpp:7-9 on the host arrays of TubeSegmentation
 // (tube-segmentation.hpp:22-30).  The arrays are mirrored to the device on
 // first use; pos must lie one voxel inside every border.

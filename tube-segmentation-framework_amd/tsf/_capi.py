@@ -74,6 +74,9 @@ _SIGS = {
     "tsf_link_centerpoints_workspace_size": ([_i32], _sz),
     "tsf_graph_components": ([_vp, _i32, _i32, _vp, _vp, _vp], _i32),
     "tsf_rasterize_tree_edges": ([_vp, _i32, _vp, _vp, _vp, _i32, _vp, Dims, _vp], _i32),
+    "tsf_segment_inverse_gradient": ([_vp, _vp, _PF, Dims, _vp, ctypes.POINTER(_i32), _vp, _sz, _vp], _i32),
+    "tsf_segment_inverse_gradient_workspace_size": ([Dims], _sz),
+    "tsf_segment_spheres": ([_vp, _vp, Dims, _vp, _vp], _i32),
     "tsf_blur_adjoint": ([_vp, _vp, _vp, Dims, _f32, ctypes.POINTER(ctypes.c_double), _vp, _sz, _vp], _i32),
     "tsf_blur_adjoint_workspace_size": ([], _sz),
     "tsf_vector_field_adjoint": ([_vp, _PF, Dims, _f32, _i32, _vp, _vp, _sz, _vp], _i32),
@@ -463,6 +466,21 @@ def rasterize_tree_edges(edges, n_edges, vertices, labels, sizes, min_tree_lengt
     _check(lib().tsf_rasterize_tree_edges(edges.ptr, int(n_edges), vertices.ptr, labels.ptr, sizes.ptr,
                                           int(min_tree_length), centerlines.ptr, _dims(centerlines.shape),
                                           _sp(stream)))
+
+
+# ------------------------------------------------------------------- segmentation
+def segment_inverse_gradient(centerlines, radius, field, out, workspace=None, stream=None):
+    """Returns the number of growing passes."""
+    it = _i32(0)
+    d = field.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_segment_inverse_gradient(centerlines.ptr, radius.ptr, ctypes.byref(d), _dims(out.shape), out.ptr,
+                                              ctypes.byref(it), w, wb, _sp(stream)))
+    return it.value
+
+
+def segment_spheres(centerlines, radius, out, stream=None):
+    _check(lib().tsf_segment_spheres(centerlines.ptr, radius.ptr, _dims(out.shape), out.ptr, _sp(stream)))
 
 
 # ------------------------------------------------------------------- adjoints
