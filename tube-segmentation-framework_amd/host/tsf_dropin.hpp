@@ -236,7 +236,7 @@ void removeLoops(std::vector<int3>& vertices, std::vector<int2>& edges, int3& si
 char* createCenterlineVoxels(std::vector<int3>& vertices, std::vector<int2>& edges, float* radius, int3& size);
 void writeToVtkFile(paramList& parameters, std::vector<int3> vertices, std::vector<int2> edges);
 
-// segmentation.hpp:6-7: the segmentation from the centerline mask (one byte
+// segmentation.hpp:8-10: the segmentation from the centerline mask (one byte
 // per voxel, as runNewCenterlineAlg returns it), the GVF field (with w) and
 // the radius volume; returns the mask (one byte per voxel, 0/1) as a device
 // Buffer.  Inverse-gradient region growing with a 3x3x3 closing (the
