@@ -114,6 +114,49 @@ tsf_status tsf_blur_gaussian(const float* in, float* out, tsf_dims dims, float s
 tsf_status tsf_to_float(const void* in, int32_t in_type, float* out, tsf_dims dims, float minimum,
                         float maximum, hipStream_t stream);
 
+/* ------------------------------------------------------------ ingest */
+
+/* getLimits with "minimum"/"maximum" off (getMinimum / getMaximum,
+ * tube-segmentation.cpp:1356-1374, over the raw volume as read from the file):
+ * std::min / std::max in index order from numeric_limits<T>::max() / ::min()
+ * (for float ::min() = FLT_MIN), NaN ignored, of equal values the first.
+ * in_type as tsf_to_float.  The results are host floats; the stream is
+ * synchronised. */
+tsf_status tsf_volume_limits(const void* in, int32_t in_type, tsf_dims dims, float* minimum, float* maximum,
+                             hipStream_t stream);
+
+enum { TSF_CROP_THRESHOLD = 0, TSF_CROP_LUNG = 1 };
+
+/* The scan-line counts of cropping (cropDatasetThreshold / cropDatasetLung,
+ * kernels_no_3d_write.cl:1236-1368, enqueued at tube-segmentation.cpp:
+ * 1577-1610), on the raw volume (in_type as tsf_to_float):
+ *   lines_x[x] = number of y whose z-line (x, y, .) is inside,
+ *   lines_y[y] = number of x whose z-line (x, y, .) is inside,
+ *   lines_z[z] = number of y whose x-line (., y, z) is inside.
+ * TSF_CROP_THRESHOLD: a line is inside when some value > threshold.
+ * TSF_CROP_LUNG: the reference's run-length test on HU = (short)value against
+ * -150 (+1024 for unsigned types); integer volumes only.  Counts are int32
+ * (the reference's are short, equal below 32768). */
+tsf_status tsf_crop_scan_lines(const void* in, int32_t in_type, tsf_dims dims, int32_t method, float threshold,
+                               int32_t* lines_x, int32_t* lines_y, int32_t* lines_z, hipStream_t stream);
+
+/* Host only: the crop region from the scan-line counts (host arrays), as
+ * readDatasetAndTransfer computes it (tube-segmentation.cpp:1612-1745): the
+ * first slices with more than min_scan_lines lines from either end (z: from
+ * the middle outwards to the first slice with fewer, when start_z_middle,
+ * as lung cropping and "cropping-start-z middle" do), then each extent grown
+ * to a multiple of 4 alternately at its two ends (or trimmed once it spans
+ * the volume).  TSF_ERR_RANGE when an extent is empty or negative (the
+ * reference throws "Invalid cropping" for an empty one). */
+tsf_status tsf_crop_region(const int32_t* lines_x, const int32_t* lines_y, const int32_t* lines_z, tsf_dims dims,
+                           int32_t min_scan_lines, int32_t start_z_middle, tsf_dims* offset, tsf_dims* size);
+
+/* The copy of a region (enqueueCopyImage, tube-segmentation.cpp:1747-1764)
+ * fused with toFloat (as tsf_to_float): out(x, y, z) = toFloat(in(x + offset.x,
+ * y + offset.y, z + offset.z)). */
+tsf_status tsf_to_float_region(const void* in, int32_t in_type, tsf_dims in_dims, tsf_dims offset, float* out,
+                               tsf_dims out_dims, float minimum, float maximum, hipStream_t stream);
+
 /* createVectorField (kernels_no_3d_write.cl:1470-1498): F = sign * central
  * difference of I (clamp-to-edge); F /= Fmax if |F| < Fmax else F /= |F|;
  * w = 1.  `out` must have all four planes.  sign = -1 for mode "black"

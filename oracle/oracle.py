@@ -218,6 +218,37 @@ def segment_spheres(centerlines, radius):
     return seg
 
 
+_TYPES = {np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16): 2, np.dtype(np.int16): 3,
+          np.dtype(np.float32): 4}
+
+
+def volume_limits(raw):
+    """getMinimum / getMaximum of a raw volume (its dtype picks T)."""
+    v = np.ascontiguousarray(raw, np.float64)
+    mn, mx = ctypes.c_float(0), ctypes.c_float(0)
+    _fn("volume_limits")(_p(v), ctypes.c_longlong(v.size), _TYPES[raw.dtype], ctypes.byref(mn), ctypes.byref(mx))
+    return mn.value, mx.value
+
+
+def crop_scan_lines(raw, method, threshold=0.0):
+    """(lines_x, lines_y, lines_z) of cropDatasetThreshold (0) / cropDatasetLung (1)."""
+    v = np.ascontiguousarray(raw, np.float64)
+    X, Y, Z = _dims(raw)
+    lx, ly, lz = np.zeros(X, np.int32), np.zeros(Y, np.int32), np.zeros(Z, np.int32)
+    _fn("crop_scan_lines")(_p(v), X, Y, Z, _TYPES[raw.dtype], int(method), ctypes.c_float(threshold), _p(lx),
+                           _p(ly), _p(lz))
+    return lx, ly, lz
+
+
+def crop_region(lines_x, lines_y, lines_z, min_scan_lines, start_z_middle):
+    """((ox, oy, oz), (sx, sy, sz)) or None where the reference cannot crop."""
+    lx, ly, lz = (np.ascontiguousarray(a, np.int32) for a in (lines_x, lines_y, lines_z))
+    off, size = np.zeros(3, np.int32), np.zeros(3, np.int32)
+    r = _fn("crop_region")(_p(lx), _p(ly), _p(lz), lx.size, ly.size, lz.size, int(min_scan_lines),
+                           int(bool(start_z_middle)), _p(off), _p(size))
+    return None if r != 0 else (tuple(int(a) for a in off), tuple(int(a) for a in size))
+
+
 def hp_key(x, y, z):
     f = _fn("hp_key")
     f.restype = ctypes.c_ulonglong

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <limits>
 
 #define TSF_QL_MAX_ITER 100
 
@@ -1744,4 +1745,154 @@ extern "C" void orc_segment_spheres(const uint8_t* cl, const float* radius, int 
                                 seg[(size_t)px + (size_t)X * ((size_t)py + (size_t)Y * pz)] = 1;
                         }
             }
+}
+
+// ---------------------------------------------------------------- ingest
+// getMinimum / getMaximum (tube-segmentation.cpp:1356-1374): std::min /
+// std::max from numeric_limits<T>::max() / ::min() in index order.  The
+// values arrive as doubles (exact for every element type); `type` 0..4 =
+// uint8, int8, uint16, int16, float picks T.
+template <class T>
+static void limits_t(const double* v, long long n, float* mn, float* mx) {
+    T lo = std::numeric_limits<T>::max(), hi = std::numeric_limits<T>::min();
+    for (long long i = 0; i < n; i++) {
+        const T t = (T)v[i];
+        lo = std::min(lo, t);
+        hi = std::max(hi, t);
+    }
+    *mn = (float)lo;
+    *mx = (float)hi;
+}
+extern "C" void orc_volume_limits(const double* v, long long n, int type, float* mn, float* mx) {
+    switch (type) {
+        case 0: limits_t<uint8_t>(v, n, mn, mx); break;
+        case 1: limits_t<int8_t>(v, n, mn, mx); break;
+        case 2: limits_t<uint16_t>(v, n, mn, mx); break;
+        case 3: limits_t<int16_t>(v, n, mn, mx); break;
+        default: limits_t<float>(v, n, mn, mx); break;
+    }
+}
+
+// cropDatasetThreshold / cropDatasetLung (kernels_no_3d_write.cl:1236-1368):
+// one work-item per slice of direction dir (0: x, 1: y, 2: z), scan lines and
+// elements in the kernel's order.  type as above; method 0 threshold, 1 lung.
+extern "C" void orc_crop_scan_lines(const double* v, int X, int Y, int Z, int type, int method, float threshold,
+                                    int32_t* lines_x, int32_t* lines_y, int32_t* lines_z) {
+    const int hu_limit = -150 + ((type == 0 || type == 2) ? 1024 : 0);
+    for (int dir = 0; dir < 3; dir++) {
+        const int slices = dir == 0 ? X : (dir == 1 ? Y : Z);
+        const int lines = dir == 1 ? X : Y;
+        const int elems = dir == 2 ? X : Z;
+        int32_t* out = dir == 0 ? lines_x : (dir == 1 ? lines_y : lines_z);
+        for (int s = 0; s < slices; s++) {
+            int count = 0;
+            for (int l = 0; l < lines; l++) {
+                bool found = false;
+                int wc = 0, bc = 0, dw = 0, db = 0;
+                for (int e = 0; e < elems; e++) {
+                    int x, y, z;
+                    if (dir == 0) {
+                        x = s; y = l; z = e;
+                    } else if (dir == 1) {
+                        x = l; y = s; z = e;
+                    } else {
+                        x = e; y = l; z = s;
+                    }
+                    const double val = v[(size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z)];
+                    if (method == 0) {
+                        if ((float)val > threshold) found = true;
+                    } else {
+                        const short hu = (short)(int)val;
+                        if (hu > hu_limit) {
+                            if (wc == 30) {
+                                dw++;
+                                bc = 0;
+                            }
+                            wc++;
+                        } else {
+                            if (bc == 30) {
+                                db++;
+                                wc = 0;
+                            }
+                            bc++;
+                        }
+                    }
+                }
+                if (method == 0 ? found : ((dw == 2 && db == 1) || (db > 1 && dw > 1))) count++;
+            }
+            out[s] = count;
+        }
+    }
+}
+
+// The crop region of readDatasetAndTransfer (tube-segmentation.cpp:1612-1745)
+// from the scan-line counts.  Returns 0, or -1 where an extent is empty or
+// negative.  offset and size are 3 ints each.
+extern "C" int orc_crop_region(const int32_t* lx, const int32_t* ly, const int32_t* lz, int X, int Y, int Z,
+                               int min_lines, int start_z_middle, int32_t* offset, int32_t* size) {
+    int x1 = 0, x2 = X, y1 = 0, y2 = Y, z1 = 0, z2 = Z;
+    const int start = start_z_middle ? Z / 2 : 0, a = start_z_middle ? -1 : 1;
+    for (int s = 0; s < X; s++)
+        if (lx[s] > min_lines) {
+            x1 = s;
+            break;
+        }
+    for (int s = X - 1; s > 0; s--)
+        if (lx[s] > min_lines) {
+            x2 = s;
+            break;
+        }
+    for (int s = 0; s < Y; s++)
+        if (ly[s] > min_lines) {
+            y1 = s;
+            break;
+        }
+    for (int s = Y - 1; s > 0; s--)
+        if (ly[s] > min_lines) {
+            y2 = s;
+            break;
+        }
+    for (int s = start; s < Z; s++)
+        if (a * lz[s] > a * min_lines) {
+            z2 = s;
+            break;
+        }
+    for (int s = Z - start - 1; s > 0; s--)
+        if (a * lz[s] > a * min_lines) {
+            z1 = s;
+            break;
+        }
+    if (!start_z_middle) std::swap(z1, z2);
+    int SX = x2 - x1, SY = y2 - y1, SZ = z2 - z1;
+    if (SX <= 0 || SY <= 0 || SZ <= 0) return -1;
+    bool lower = false;
+    while (SX % 4 != 0 && SX < X) {
+        if (lower && x1 > 0) x1--;
+        else if (x2 < X) x2++;
+        lower = !lower;
+        SX = x2 - x1;
+    }
+    while (SX % 4 != 0) SX--;
+    while (SY % 4 != 0 && SY < Y) {
+        if (lower && y1 > 0) y1--;
+        else if (y2 < Y) y2++;
+        lower = !lower;
+        SY = y2 - y1;
+    }
+    while (SY % 4 != 0) SY--;
+    while (SZ % 4 != 0 && SZ < Z) {
+        if (lower && z1 > 0) z1--;
+        else if (z2 < Z) z2++;
+        lower = !lower;
+        SZ = z2 - z1;
+    }
+    while (SZ % 4 != 0) SZ--;
+    if (SX <= 0 || SY <= 0 || SZ <= 0) return -1;
+    offset[0] = x1;
+    offset[1] = y1;
+    offset[2] = z1;
+    size[0] = SX;
+    size[1] = SY;
+    size[2] = SZ;
+    return 0;
 }

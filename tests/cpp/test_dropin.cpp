@@ -15,7 +15,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <fstream>
 #include <functional>
 #include <string>
@@ -188,6 +190,82 @@ static std::vector<uint8_t> read_raw(const std::string& fn, size_t n) {
     return v;
 }
 
+// The reference's validation measures (tests/tubeValidation.cpp:17-137):
+// centerline voxels within a ball of radius 4 of a real one, their average
+// distance to the nearest, the share of real centerline voxels found, and
+// the segmentation's voxel precision and recall against the ground truth.
+struct TubeValidation {
+    double averageDistance = 0, percentageExtracted = 0, precision = 0, recall = 0;
+    long extracted = 0, incorrect = 0, segmented = 0;
+};
+static TubeValidation validate_tube(const char* centerline, const char* segmentation, const std::string& dsdir,
+                                    int X, int Y, int Z) {
+    const size_t n = (size_t)X * Y * Z;
+    std::vector<uint8_t> real = read_raw(dsdir + "/real_centerline.raw", n), detected(n, 0);
+    TubeValidation r;
+    const int R = 4;
+    double sumd = 0;
+    long found = 0;
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                if (centerline[x + X * (y + Y * z)] != 1) continue;
+                r.extracted++;
+                float best = 9999.0f;
+                bool ok = false;
+                for (int c = -R; c <= R; c++)
+                    for (int b = -R; b <= R; b++)
+                        for (int a = -R; a <= R; a++) {
+                            const int qx = x + a, qy = y + b, qz = z + c;
+                            if (qx < 0 || qy < 0 || qz < 0 || qx >= X || qy >= Y || qz >= Z) continue;
+                            const float dd = std::sqrt((float)(a * a + b * b + c * c));
+                            if (!(dd < R) || real[qx + X * (qy + Y * qz)] != 1) continue;
+                            ok = true;
+                            best = std::min(best, dd);
+                            detected[qx + X * (qy + Y * qz)] = 1;
+                        }
+                if (ok) {
+                    sumd += best;
+                    found++;
+                } else {
+                    r.incorrect++;
+                }
+            }
+    long det = 0, undet = 0;
+    for (size_t i = 0; i < n; i++) {
+        det += detected[i] == 1;
+        undet += real[i] == 1 && detected[i] == 0;
+    }
+    r.averageDistance = sumd / found;
+    r.percentageExtracted = 100.0 * det / (det + undet);
+    if (segmentation) {
+        std::vector<uint8_t> orig = read_raw(dsdir + "/original.raw", n);
+        long tp = 0, fp = 0, fn = 0;
+        for (size_t i = 0; i < n; i++) {
+            const bool t = orig[i] == 1, g = segmentation[i] == 1;
+            tp += t && g;
+            fp += !t && g;
+            fn += t && !g;
+        }
+        r.recall = (double)tp / (tp + fn);
+        r.precision = (double)tp / (tp + fp);
+        r.segmented = tp + fp;
+    }
+    return r;
+}
+
+// The reference's system-test bars (tests/clinicalTests.cpp:58-61,
+// tests/tubeSegmentationTests.cpp:57-101).
+static std::string tube_bars(const TubeValidation& v, bool with_segmentation) {
+    std::printf("  %ld centerline voxels: average distance %.3f, %.1f %% extracted, %ld off the real one\n",
+                v.extracted, v.averageDistance, v.percentageExtracted, v.incorrect);
+    if (with_segmentation)
+        std::printf("  segmentation: %ld voxels, precision %.3f, recall %.3f\n", v.segmented, v.precision, v.recall);
+    if (!(v.averageDistance < 1.5 && v.percentageExtracted > 75.0)) return "centerline below the bar";
+    if (with_segmentation && !(v.precision > 0.7 && v.recall > 0.7)) return "segmentation below the bar";
+    return std::string();
+}
+
 static std::string compare(const std::vector<float>& a, const std::vector<float>& b, float rtol, float atol,
                            const char* what) {
     size_t bad = 0, worst = 0;
@@ -318,67 +396,17 @@ static void gpu_tests(const std::string& dsdir) {
         Image3D vf2 = vf, tdf2 = tdf, rad2 = rad;
         std::vector<float> r0 = rad.downloadPlane(ocl, 0);
         Buffer cl = runNewCenterlineAlg(ocl, size, p, vf2, tdf2, rad2);
-        std::vector<uint8_t> e(n);
+        std::vector<char> e(n), s(n);
         cl.download(ocl, e.data(), n);
-        std::vector<uint8_t> real = read_raw(dsdir + "/real_centerline.raw", n), detected(n, 0);
-        const int R = 4;
-        double sumd = 0;
-        long found = 0, incorrect = 0, extracted = 0;
-        for (int z = 0; z < Z; z++)
-            for (int y = 0; y < Y; y++)
-                for (int x = 0; x < X; x++) {
-                    if (e[x + X * (y + Y * z)] != 1) continue;
-                    extracted++;
-                    float best = 9999.0f;
-                    bool ok = false;
-                    for (int c = -R; c <= R; c++)
-                        for (int b = -R; b <= R; b++)
-                            for (int a = -R; a <= R; a++) {
-                                const int qx = x + a, qy = y + b, qz = z + c;
-                                if (qx < 0 || qy < 0 || qz < 0 || qx >= X || qy >= Y || qz >= Z) continue;
-                                const float dd = std::sqrt((float)(a * a + b * b + c * c));
-                                if (!(dd < R) || real[qx + X * (qy + Y * qz)] != 1) continue;
-                                ok = true;
-                                best = std::min(best, dd);
-                                detected[qx + X * (qy + Y * qz)] = 1;
-                            }
-                    if (ok) {
-                        sumd += best;
-                        found++;
-                    } else {
-                        incorrect++;
-                    }
-                }
-        long det = 0, undet = 0;
-        for (size_t i = 0; i < n; i++) {
-            det += detected[i] == 1;
-            undet += real[i] == 1 && detected[i] == 0;
-        }
-        const double avg = sumd / found, pct = 100.0 * det / (det + undet);
         // the radius is rewritten only on centerline voxels
         std::vector<float> r1 = rad2.downloadPlane(ocl, 0);
         size_t moved = 0;
         for (size_t i = 0; i < n; i++) moved += r1[i] != r0[i] && e[i] != 1;
-        std::printf("  %ld centerline voxels: average distance %.3f, %.1f %% extracted, %ld off the real one\n",
-                    extracted, avg, pct, incorrect);
         if (moved) return std::string("radius changed off the centerline");
-        if (!(avg < 1.5 && pct > 75.0)) return std::string("below the reference's system-test bar");
-        // the segmentation from this centerline: voxel agreement with the
-        // ground truth, as tests/tubeValidation.cpp:107-137 measures it (the
-        // reference's bar: precision and recall > 0.7)
+        // the segmentation from this centerline
         Buffer seg = runInverseGradientSegmentation(ocl, cl, vf2, rad2, size, p);
-        std::vector<uint8_t> s(n), orig = read_raw(dsdir + "/original.raw", n);
         seg.download(ocl, s.data(), n);
-        long tp = 0, fp = 0, fn = 0;
-        for (size_t i = 0; i < n; i++) {
-            const bool t = orig[i] == 1, g = s[i] == 1;
-            tp += t && g;
-            fp += !t && g;
-            fn += t && !g;
-        }
-        const double recall = (double)tp / (tp + fn), precision = (double)tp / (tp + fp);
-        std::printf("  segmentation: %ld voxels, precision %.3f, recall %.3f\n", tp + fp, precision, recall);
-        return precision > 0.7 && recall > 0.7 ? std::string() : std::string("segmentation below the bar");
+        return tube_bars(validate_tube(e.data(), s.data(), dsdir, X, Y, Z), true);
     });
     run("candidate_count_range_error", [&] {
         paramList q = p;
@@ -574,13 +602,146 @@ static void autograd_tests() {
     });
 }
 
+// ----------------------------------------------------------------- run(), ingest and output
+// The Synthetic-Vascusynth preset of the gpu method (parameters/centerline-gpu/
+// Synthetic-Vascusynth), set parameter by parameter.
+static paramList vascusynth() {
+    paramList p = initParameters();
+    setParameter(p, "parameters", "Synthetic-Vascusynth");
+    setParameter(p, "centerline-method", "gpu");
+    for (auto kv : std::vector<std::pair<const char*, const char*>>{
+             {"mode", "white"}, {"small-blur", "1.0"}, {"fmax", "0.1"}, {"minimum", "0"}, {"maximum", "75"},
+             {"min-mean-tdf", "0.3"}, {"tdf-high", "0.35"}, {"cube-size", "3"}, {"min-tree-length", "3"},
+             {"max-distance", "10.0"}, {"radius-max", "4.0"}})
+        setParameter(p, kv.first, kv.second);
+    return p;
+}
+
+static std::string make_temp_dir() {
+    char tmpl[] = "/tmp/tsf_io_XXXXXX";
+    const char* d = mkdtemp(tmpl);
+    return d ? std::string(d) + "/" : std::string();
+}
+
+static void write_file(const std::string& fn, const void* data, size_t n) {
+    std::ofstream f(fn, std::ios::binary);
+    f.write((const char*)data, (std::streamsize)n);
+}
+
+static void run_tests(const std::string& dsdir) {
+    const int X = 100, Y = 100, Z = 100;
+    // tests/tubeSegmentationTests.cpp:4-7
+    run("run_WrongFilenameException", [&] {
+        paramList p = initParameters();
+        try {
+            delete run("somefilethatdoesntexist.mhd", p, "");
+        } catch (const IOException&) {
+            return std::string();
+        }
+        return std::string("no IOException");
+    });
+    // tests/tubeSegmentationTests.cpp:57-101: the four storage / buffer
+    // variants of the gpu method on dataset_1, the reference's bars
+    for (const char* buffers : {"false", "true"})
+        for (const char* bits32 : {"false", "true"}) {
+            const std::string name = std::string("run_SystemTestWithSyntheticData") +
+                                     (bits32[0] == 't' ? "32bit" : "16bit") + (buffers[0] == 't' ? "Buffers" : "");
+            run(name.c_str(), [&] {
+                paramList p = vascusynth();
+                setParameter(p, "buffers-only", buffers);
+                setParameter(p, "32bit-vectors", bits32);
+                setParameter(p, "16bit-vectors", bits32[0] == 't' ? "false" : "true");
+                TSFOutput* out = run(dsdir + "/noisy.mhd", p, "");
+                const I3 s = *out->getSize();
+                std::string err;
+                if (s.x != X || s.y != Y || s.z != Z) err = "size changed";
+                else err = tube_bars(validate_tube(out->getCenterlineVoxels(), out->getSegmentation(), dsdir, X, Y, Z),
+                                     true);
+                delete out;
+                return err;
+            });
+        }
+    // threshold cropping on a box: x [10, 31) -> [10, 30) (the last slice is
+    // the exclusive end), size 20; y [5, 27) -> [5, 26) grown alternately to
+    // [4, 28), size 24; z [7, 20) from the end -> [7, 19), size 12.  Without
+    // cropping, each extent shrinks to a multiple of 4.
+    run("readDatasetAndTransfer_threshold_cropping", [&] {
+        const std::string dir = make_temp_dir();
+        if (dir.empty()) return std::string("no temp dir");
+        const int VX = 50, VY = 41, VZ = 30;
+        std::vector<uint16_t> v((size_t)VX * VY * VZ, 0);
+        for (int z = 7; z < 20; z++)
+            for (int y = 5; y < 27; y++)
+                for (int x = 10; x < 31; x++) v[x + VX * (y + VY * z)] = 500;
+        write_file(dir + "box.raw", v.data(), v.size() * 2);
+        std::ofstream(dir + "box.mhd") << "NDims = 3\nDimSize = 50 41 30\nElementSpacing = 0.5 0.75 2\n"
+                                          "ElementType = MET_USHORT\nElementDataFile = box.raw\n";
+        OpenCL ocl;
+        paramList p = initParameters();
+        setParameter(p, "cropping", "threshold");
+        setParameter(p, "cropping-threshold", "100");
+        setParameter(p, "min-scan-lines-threshold", "0");
+        TSFOutput out(new I3());
+        Image3D vol = readDatasetAndTransfer(ocl, dir + "box.mhd", p, out.getSize(), &out);
+        const I3 s = *out.getSize(), sh = out.getShiftVector();
+        const F3 sp = out.getSpacing();
+        if (s.x != 20 || s.y != 24 || s.z != 12 || sh.x != 10 || sh.y != 4 || sh.z != 7)
+            return std::string("crop region ") + std::to_string(s.x) + "," + std::to_string(s.y) + "," +
+                   std::to_string(s.z) + " at " + std::to_string(sh.x) + "," + std::to_string(sh.y) + "," +
+                   std::to_string(sh.z);
+        if (sp.x != 0.5f || sp.y != 0.75f || sp.z != 2.0f) return std::string("spacing");
+        std::vector<float> f = vol.downloadPlane(ocl, 0);
+        for (int z = 0; z < s.z; z++)
+            for (int y = 0; y < s.y; y++)
+                for (int x = 0; x < s.x; x++) {
+                    const float want = v[(x + 10) + VX * ((y + 4) + VY * (z + 7))] == 500 ? 1.0f : 0.0f;
+                    if (f[x + s.x * (y + s.y * z)] != want) return std::string("converted value");
+                }
+        paramList q = initParameters();
+        TSFOutput out2(new I3());
+        Image3D vol2 = readDatasetAndTransfer(ocl, dir + "box.mhd", q, out2.getSize(), &out2);
+        const I3 s2 = *out2.getSize();
+        if (s2.x != 48 || s2.y != 40 || s2.z != 28) return std::string("multiple-of-4 shrink");
+        std::remove((dir + "box.raw").c_str());
+        std::remove((dir + "box.mhd").c_str());
+        return std::string();
+    });
+    // storage-dir: the .mhd/.raw pairs of inputOutput.cpp:14-47
+    run("writeDataToDisk_round_trip", [&] {
+        const std::string dir = make_temp_dir();
+        if (dir.empty()) return std::string("no temp dir");
+        paramList p = vascusynth();
+        setParameter(p, "storage-dir", dir);
+        setParameter(p, "storage-name", "d1");
+        TSFOutput* out = run(dsdir + "/noisy.mhd", p, "");
+        const size_t n = (size_t)X * Y * Z;
+        std::string err;
+        for (const char* kind : {"centerline", "segmentation"}) {
+            std::ifstream mhd(dir + "d1." + kind + ".mhd");
+            std::string text((std::istreambuf_iterator<char>(mhd)), std::istreambuf_iterator<char>());
+            const std::string want = std::string("ObjectType = Image\nNDims = 3\nDimSize = 100 100 100\n") +
+                                     "ElementSpacing = 1 1 1\nElementType = MET_CHAR\nElementDataFile = d1." + kind +
+                                     ".raw\n";
+            if (text != want) err = std::string("header of ") + kind;
+            std::vector<uint8_t> raw = read_raw(dir + "d1." + kind + ".raw", n);
+            const char* mem = kind[0] == 'c' ? out->getCenterlineVoxels() : out->getSegmentation();
+            if (std::memcmp(raw.data(), mem, n) != 0) err = std::string("voxels of ") + kind;
+            std::remove((dir + "d1." + kind + ".mhd").c_str());
+            std::remove((dir + "d1." + kind + ".raw").c_str());
+        }
+        delete out;
+        return err;
+    });
+}
+
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "params";
     if (mode == "params") param_tests();
     else if (mode == "gpu" && argc > 2) gpu_tests(argv[2]);
     else if (mode == "autograd") autograd_tests();
+    else if (mode == "run" && argc > 2) run_tests(argv[2]);
     else {
-        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir> | autograd\n");
+        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir> | run <dataset_dir> | autograd\n");
         return 2;
     }
     std::printf("%d failure(s)\n", failures);

@@ -36,6 +36,14 @@ def test_dropin_pipeline_on_dataset_1(tsf):
 
 
 @pytest.mark.gpu
+def test_dropin_run_ingest_and_output(tsf):
+    r = _run("run", os.path.join(ROOT, "tests", "golden", "dataset_1"), timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 7
+
+
+@pytest.mark.gpu
 def test_dropin_autograd_function(tsf):
     r = _run("autograd", timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr

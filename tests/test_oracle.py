@@ -723,3 +723,104 @@ def test_segment_spheres_ball(orc):
     z, y, x = np.mgrid[:shape[0], :shape[1], :shape[2]]
     ball = lambda cz, cy, cx: (z - cz) ** 2 + (y - cy) ** 2 + (x - cx) ** 2 < 6.25
     assert np.array_equal(seg.astype(bool), ball(5, 6, 6) | ball(0, 0, 0))
+
+
+# ----------------------------------------------------------------- ingest
+def test_volume_limits_known_answers(orc):
+    """getMinimum / getMaximum: numpy's min/max on integer volumes; the
+    initial values numeric_limits<T>::max() / ::min() (FLT_MIN for float);
+    NaN never wins; of +0 and -0 the first stays."""
+    rng = np.random.default_rng(5)
+    for dt in (np.uint8, np.int8, np.uint16, np.int16):
+        info = np.iinfo(dt)
+        v = rng.integers(info.min, info.max, size=(5, 6, 7), endpoint=True).astype(dt)
+        assert orc.volume_limits(v) == (float(v.min()), float(v.max()))
+        assert orc.volume_limits(np.full((2, 2, 2), info.max, dt)) == (float(info.max), float(info.max))
+    flt_min, flt_max = float(np.finfo(np.float32).tiny), float(np.finfo(np.float32).max)
+    neg = -rng.random((3, 4, 5)).astype(np.float32) - 1.0
+    assert orc.volume_limits(neg) == (float(neg.min()), flt_min)
+    nan = np.full((2, 2, 2), np.nan, np.float32)
+    assert orc.volume_limits(nan) == (flt_max, flt_min)
+    nan[0, 0, 1] = 3.0
+    assert orc.volume_limits(nan) == (3.0, 3.0)
+    z = np.array([-0.0, 0.0, 1.0], np.float32).reshape(1, 1, 3)
+    assert np.signbit(orc.volume_limits(z)[0])
+    assert not np.signbit(orc.volume_limits(z[..., [1, 0, 2]])[0])
+
+
+def test_crop_scan_lines_threshold_box(orc):
+    """A bright box: lines_x[x] = its y extent for x inside, lines_y[y] = its x
+    extent, lines_z[z] = its y extent; nothing outside; strict '>'."""
+    v = np.zeros((10, 12, 14), np.uint8)
+    v[2:7, 3:9, 4:11] = 200
+    lx, ly, lz = orc.crop_scan_lines(v, 0, 100.0)
+    ex = np.zeros(14, np.int32); ex[4:11] = 6
+    ey = np.zeros(12, np.int32); ey[3:9] = 7
+    ez = np.zeros(10, np.int32); ez[2:7] = 6
+    assert (lx == ex).all() and (ly == ey).all() and (lz == ez).all()
+    assert all((a == 0).all() for a in orc.crop_scan_lines(v, 0, 200.0))
+
+
+def _lung_pattern(runs, white, black):
+    return np.concatenate([np.full(n, white if w else black) for w, n in runs])
+
+
+def test_crop_scan_lines_lung_automaton(orc):
+    """cropDatasetLung's run-length test: white 31, black 31, white 31 counts
+    (two white areas, one black); a black run of 30 counts no area; five runs
+    of 31 count (more than one of each); an all-white line does not.  Lines
+    along z: X, Y < 31 keep every x-line empty."""
+    W, B = 0, -1000
+    cases = {
+        (0, 0): ([(1, 31), (0, 31), (1, 31)], True),
+        (1, 0): ([(1, 31), (0, 30), (1, 32)], False),
+        (2, 1): ([(1, 31), (0, 31), (1, 31), (0, 31), (1, 31)], True),
+        (3, 2): ([(1, 155)], False),
+    }
+    Z = 155
+    v = np.full((Z, 4, 5), B, np.int16)
+    for (x, y), (runs, _) in cases.items():
+        p = _lung_pattern(runs, W, B)
+        v[:len(p), y, x] = p
+    lx, ly, lz = orc.crop_scan_lines(v, 1)
+    ex, ey = np.zeros(5, np.int32), np.zeros(4, np.int32)
+    for (x, y), (_, inside) in cases.items():
+        ex[x] += inside
+        ey[y] += inside
+    assert (lx == ex).all() and (ly == ey).all() and (lz == 0).all()
+    # the same lines along x (Z, Y < 31): only lines_z counts
+    t = np.ascontiguousarray(np.transpose(v, (2, 1, 0)))   # (x, y, z) -> shape (5, 4, 155) = (Z', Y', X')
+    lx2, ly2, lz2 = orc.crop_scan_lines(t, 1)
+    assert (lx2 == 0).all() and (ly2 == 0).all()
+    ez = np.zeros(5, np.int32)
+    for (x, y), (_, inside) in cases.items():
+        ez[x] += inside
+    assert (lz2 == ez).all()
+
+
+def test_crop_scan_lines_lung_unsigned_limit_and_short_wrap(orc):
+    """Unsigned volumes shift the limit by 1024 (white > 874); a value read
+    into the kernel's short wraps (40000 -> -25536: black)."""
+    runs = [(1, 31), (0, 31), (1, 31)]
+    for white, inside in ((1024, 1), (875, 1), (874, 0), (30000, 1), (40000, 0)):
+        v = np.zeros((93, 1, 1), np.uint16)
+        v[:, 0, 0] = _lung_pattern(runs, white, 0)
+        assert orc.crop_scan_lines(v, 1)[0][0] == inside, white
+
+
+def test_crop_region_known_answers(orc):
+    """The box: x [4, 11) grows high then low to [3, 11); y [3, 8] (the last
+    qualifying slice is the exclusive end) grows to [2, 10); z [2, 6) stays.
+    Middle start: z from the first slices at or below the minimum on either
+    side of Z/2.  All slices qualifying: the top slice is excluded, then the
+    extent grows back to the volume and is trimmed to a multiple of 4.  No
+    qualifying z slice with the end start: an empty extent."""
+    v = np.zeros((10, 12, 14), np.uint8)
+    v[2:7, 3:9, 4:11] = 200
+    assert orc.crop_region(*orc.crop_scan_lines(v, 0, 100.0), 0, False) == ((3, 2, 2), (8, 8, 4))
+    lz = np.array([0] * 3 + [300] * 14 + [0] * 3, np.int32)
+    full = np.full(16, 300, np.int32)
+    assert orc.crop_region(full, full, lz, 200, True) == ((0, 0, 2), (16, 16, 16))
+    ten = np.full(10, 300, np.int32)
+    assert orc.crop_region(ten, full, full, 200, False) == ((0, 0, 0), (8, 16, 16))
+    assert orc.crop_region(full, full, np.zeros(16, np.int32), 200, False) is None

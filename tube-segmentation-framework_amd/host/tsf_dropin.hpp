@@ -47,6 +47,13 @@ class Error : public std::runtime_error {
     tsf_status status;
 };
 
+// SIPL::IOException (a file that cannot be opened or read).
+class IOException : public Error {
+   public:
+    explicit IOException(const std::string& filename);
+    std::string filename;
+};
+
 // Throws tsf::Error for a failed tsf_* call (the reference's calls throw
 // cl::Error from the OpenCL C++ bindings).
 void check(tsf_status st);
@@ -245,6 +252,79 @@ void writeToVtkFile(paramList& parameters, std::vector<int3> vertices, std::vect
 Buffer runInverseGradientSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& vectorField, Image3D& radius,
                                       int3 size, paramList parameters);
 Buffer runSphereSegmentation(OpenCL& ocl, Buffer& centerline, Image3D& radius, int3 size, paramList parameters);
+
+// ----------------------------------------------------------------- entry point, ingest and output
+// TSFOutput (inputOutput.hpp:10-52): the results of run().  It owns its
+// context (one stream), the size, and every volume set on it; the device
+// volumes are copied to the host on the first get*() (new[]'d arrays that
+// the output frees).  Centerline and segmentation are one byte per voxel.
+class TSFOutput {
+   public:
+    explicit TSFOutput(int3* size, bool TDFis16bit = false);
+    ~TSFOutput();
+    TSFOutput(const TSFOutput&) = delete;
+    TSFOutput& operator=(const TSFOutput&) = delete;
+    bool hasSegmentation() const { return oclSegmentation || segmentation; }
+    bool hasCenterlineVoxels() const { return oclCenterlineVoxels || centerlineVoxels; }
+    bool hasTDF() const { return oclTDF || TDF; }
+    void setTDF(Image3D* image);
+    void setSegmentation(Buffer* mask);
+    void setCenterlineVoxels(Buffer* mask);
+    void setTDF(float* data);
+    void setSegmentation(char* data);
+    void setCenterlineVoxels(char* data);
+    void setSize(int3* size);
+    char* getSegmentation();
+    char* getCenterlineVoxels();
+    float* getTDF();
+    int3* getSize() { return size; }
+    int3 getShiftVector() const { return shiftVector; }
+    void setShiftVector(int3 v) { shiftVector = v; }
+    float3 getSpacing() const { return spacing; }
+    void setSpacing(float3 v) { spacing = v; }
+    OpenCL& getContext() { return ocl; }
+    bool isTDF16bit() const { return TDFis16bit; }
+
+   private:
+    OpenCL ocl;
+    int3* size;
+    bool TDFis16bit;
+    int3 shiftVector;
+    float3 spacing{1.0f, 1.0f, 1.0f};
+    Image3D* oclTDF = nullptr;
+    Buffer* oclSegmentation = nullptr;
+    Buffer* oclCenterlineVoxels = nullptr;
+    float* TDF = nullptr;
+    char* segmentation = nullptr;
+    char* centerlineVoxels = nullptr;
+};
+
+// tube-segmentation.hpp / tube-segmentation.cpp:53-163: reads the .mhd
+// volume, runs the centerline method ("gpu": circle fitting, the centerline
+// graph and the segmentation) and returns the results.  kernel_dir is
+// accepted for the reference's signature; the kernels are in the library.
+// Throws IOException for a missing file, tsf::Error otherwise (the "ridge"
+// and "test" methods are not provided).
+TSFOutput* run(std::string filename, paramList& parameters, std::string kernel_dir);
+
+// tube-segmentation.cpp:1399-1905: parses the .mhd header (ElementType,
+// ElementDataFile, DimSize, ElementSpacing), uploads the raw volume, finds
+// the intensity limits ("minimum" / "maximum", or the volume's own), crops
+// ("cropping" lung / threshold by scan lines, the AAA preset's 15 % margin,
+// else each extent shrunk to a multiple of 4) and converts the region to a
+// float volume in [0, 1].  *size becomes the cropped size; the shift vector
+// and spacing are set on output (when not NULL).
+Image3D readDatasetAndTransfer(OpenCL& ocl, std::string filename, paramList& parameters, int3* size,
+                               TSFOutput* output);
+
+// tube-segmentation.cpp:943-986: runCircleFittingMethod, runNewCenterlineAlg
+// and the segmentation, results set on output; storage-dir writes them.
+void runCircleFittingAndNewCenterlineAlg(OpenCL& ocl, Image3D* dataset, int3* size, paramList& parameters,
+                                         TSFOutput* output);
+
+// inputOutput.cpp:14-47: <dir><name>.centerline.mhd/.raw and
+// <dir><name>.segmentation.mhd/.raw (MET_CHAR) for the results present.
+void writeDataToDisk(TSFOutput* output, std::string storageDirectory, std::string name);
 
 // eigenanalysisOfHessian.hpp:7-9 on the host arrays of TubeSegmentation
 // (tube-segmentation.hpp:22-30).  The arrays are mirrored to the device on

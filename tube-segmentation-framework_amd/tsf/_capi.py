@@ -77,6 +77,10 @@ _SIGS = {
     "tsf_segment_inverse_gradient": ([_vp, _vp, _PF, Dims, _vp, ctypes.POINTER(_i32), _vp, _sz, _vp], _i32),
     "tsf_segment_inverse_gradient_workspace_size": ([Dims], _sz),
     "tsf_segment_spheres": ([_vp, _vp, Dims, _vp, _vp], _i32),
+    "tsf_volume_limits": ([_vp, _i32, Dims, ctypes.POINTER(_f32), ctypes.POINTER(_f32), _vp], _i32),
+    "tsf_crop_scan_lines": ([_vp, _i32, Dims, _i32, _f32, _vp, _vp, _vp, _vp], _i32),
+    "tsf_crop_region": ([_vp, _vp, _vp, Dims, _i32, _i32, ctypes.POINTER(Dims), ctypes.POINTER(Dims)], _i32),
+    "tsf_to_float_region": ([_vp, _i32, Dims, Dims, _vp, Dims, _f32, _f32, _vp], _i32),
     "tsf_blur_adjoint": ([_vp, _vp, _vp, Dims, _f32, ctypes.POINTER(ctypes.c_double), _vp, _sz, _vp], _i32),
     "tsf_blur_adjoint_workspace_size": ([], _sz),
     "tsf_vector_field_adjoint": ([_vp, _PF, Dims, _f32, _i32, _vp, _vp, _sz, _vp], _i32),
@@ -347,6 +351,42 @@ _TYPE_CODES = {np.dtype(np.uint8): 0, np.dtype(np.int8): 1, np.dtype(np.uint16):
 def to_float(src, dst, minimum, maximum, stream=None):
     _check(lib().tsf_to_float(src.ptr, _TYPE_CODES[src.dtype], dst.ptr, _dims(src.shape), minimum, maximum,
                               _sp(stream)))
+
+
+def volume_limits(src, stream=None):
+    """(minimum, maximum) of a raw volume as getMinimum / getMaximum find them."""
+    mn, mx = _f32(0.0), _f32(0.0)
+    _check(lib().tsf_volume_limits(src.ptr, _TYPE_CODES[src.dtype], _dims(src.shape), ctypes.byref(mn),
+                                   ctypes.byref(mx), _sp(stream)))
+    return mn.value, mx.value
+
+
+CROP_THRESHOLD, CROP_LUNG = 0, 1
+
+
+def crop_scan_lines(src, method, threshold=0.0, stream=None):
+    """Scan-line counts (lines_x, lines_y, lines_z) as int32 numpy arrays."""
+    Z, Y, X = src.shape
+    lx, ly, lz = DeviceArray((X,), np.int32), DeviceArray((Y,), np.int32), DeviceArray((Z,), np.int32)
+    _check(lib().tsf_crop_scan_lines(src.ptr, _TYPE_CODES[src.dtype], _dims(src.shape), int(method), threshold,
+                                     lx.ptr, ly.ptr, lz.ptr, _sp(stream)))
+    return lx.download(stream), ly.download(stream), lz.download(stream)
+
+
+def crop_region(lines_x, lines_y, lines_z, min_scan_lines, start_z_middle):
+    """Host only: ((ox, oy, oz), (sx, sy, sz)) of the crop."""
+    lx, ly, lz = (np.ascontiguousarray(a, np.int32) for a in (lines_x, lines_y, lines_z))
+    off, size = Dims(0, 0, 0), Dims(0, 0, 0)
+    _check(lib().tsf_crop_region(lx.ctypes.data, ly.ctypes.data, lz.ctypes.data, Dims(lx.size, ly.size, lz.size),
+                                 int(min_scan_lines), int(bool(start_z_middle)), ctypes.byref(off),
+                                 ctypes.byref(size)))
+    return (off.x, off.y, off.z), (size.x, size.y, size.z)
+
+
+def to_float_region(src, offset, dst, minimum, maximum, stream=None):
+    """dst (float32, shape (sz, sy, sx)) = toFloat of src from offset (ox, oy, oz)."""
+    _check(lib().tsf_to_float_region(src.ptr, _TYPE_CODES[src.dtype], _dims(src.shape), Dims(*offset), dst.ptr,
+                                     _dims(dst.shape), minimum, maximum, _sp(stream)))
 
 
 def create_vector_field(vol, field, fmax, sign=1, stream=None):
