@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-downstream", action="store_true",
+                    help="skip timing the stages around the filters (ingest, centerline graph, segmentation)")
     ap.add_argument("--checkpoint-every", type=int, default=40)
     ap.add_argument("--fit-steps", type=int, default=0,
                     help="run this many mu/sigma fitting steps instead of the forward benchmark")
@@ -167,6 +169,12 @@ def main():
     launch_ms = gvf_ms / launches
     achieved = bytes_per_voxel * N / (launch_ms * 1e-3)
 
+    # the stages around the filters, on this forward's TDF, radius and field
+    # (before the FP32 comparison below reuses the TDF and radius buffers)
+    downstream = None
+    if not args.no_downstream:
+        downstream = downstream_stages(vol, shape, t_s, r_s, V, stream)
+
     # the same forward with float storage, for the speed-up of the
     # reduced-precision form: two steps per launch, and the plain FP32 path
     # (one step per launch)
@@ -246,6 +254,7 @@ def main():
         "wall_s_timed": round(wall, 3),
         "f32_gvf": f32,
         "adjoint": adjoint,
+        "downstream": downstream,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
                      "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
@@ -255,6 +264,101 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def downstream_stages(vol, shape, tdf, radius, field, stream):
+    """Times the stages either side of the filters on the same volume, each
+    from device-resident inputs: ingest of the raw volume as 16-bit MRA
+    intensities (limits, threshold-cropping scan lines, toFloat of the
+    region), the centerline graph from the forward's TDF, radius and GVF
+    field (candidates, compaction, suppression, vertices, linking, components,
+    rasterisation; the Neuro-Vessels-MRA preset's thresholds) and the
+    inverse-gradient segmentation of that centerline."""
+    import tsf
+    N = int(np.prod(shape))
+    ev = [tsf.Event() for _ in range(2)]
+
+    def timed(fn, reps=3):
+        fn()
+        stream.synchronize()
+        ts = []
+        for _ in range(reps):
+            ev[0].record(stream)
+            r = fn()
+            ev[1].record(stream)
+            stream.synchronize()
+            ts.append(ev[0].elapsed_ms(ev[1]))
+        return float(np.median(ts)), r
+
+    out = {}
+    # ingest: 16-bit intensities 100..300 (the MRA preset's window)
+    raw = tsf.DeviceArray.from_host(np.round(vol * 200.0 + 100.0).astype(np.uint16), stream=stream)
+    ms_lim, lim = timed(lambda: tsf.volume_limits(raw, stream))
+    ms_scan, lines = timed(lambda: tsf.crop_scan_lines(raw, tsf.CROP_THRESHOLD, 150.0, stream))
+    off, size = tsf.crop_region(*lines, 10, False)
+    dst = tsf.DeviceArray((size[2], size[1], size[0]), np.float32)
+    ms_tf, _ = timed(lambda: tsf.to_float_region(raw, off, dst, 100.0, 300.0, stream))
+    Nc = size[0] * size[1] * size[2]
+    out["ingest"] = {
+        "raw": "uint16", "limits": [lim[0], lim[1]], "crop_offset": off, "crop_size": size,
+        "limits_ms": round(ms_lim, 3), "limits_gb_s": round(2 * N / (ms_lim * 1e-3) / 1e9, 1),
+        "scan_lines_ms": round(ms_scan, 3), "scan_lines_gb_s": round(4 * N / (ms_scan * 1e-3) / 1e9, 1),
+        "to_float_region_ms": round(ms_tf, 3), "to_float_region_gb_s": round(6 * Nc / (ms_tf * 1e-3) / 1e9, 1),
+        "bytes": "limits 2 B/voxel read; scan lines 2 x 2 B/voxel (z-lines and x-lines); toFloat 2 B read + "
+                 "4 B written per cropped voxel"}
+    del raw, dst
+
+    # centerline graph
+    d = {}
+    mask = tsf.DeviceArray(shape, np.uint8)
+    hws = tsf.DeviceArray((tsf.hp_compact_workspace_size(shape),), np.uint8)
+
+    def candidates():
+        tsf.candidates(tdf, mask, 0.5, stream)
+        c = tsf.hp_compact(mask, None, hws, stream)
+        pos = tsf.DeviceArray((max(c, 1), 3), np.int32)
+        tsf.hp_compact(mask, pos, hws, stream)
+        cp = tsf.DeviceArray(shape, np.uint8)
+        cp.zero(stream)
+        tsf.candidate_nms(tdf, radius, field, pos, c, 8, cp, stream)
+        cells = tsf.DeviceArray(shape, np.uint8)
+        cells.zero(stream)
+        tsf.cube_max(tdf, cp, 5, cells, stream)
+        nv = tsf.hp_compact(cells, None, hws, stream)
+        verts = tsf.DeviceArray((max(nv, 1), 3), np.int32)
+        tsf.hp_compact(cells, verts, hws, stream)
+        return c, nv, verts
+
+    ms_cand, (ncand, nv, verts) = timed(candidates, reps=2)
+    lws = tsf.DeviceArray((tsf.link_centerpoints_workspace_size(nv),), np.uint8)
+    cl = tsf.DeviceArray(shape, np.uint8)
+
+    def graph():
+        ne = tsf.link_centerpoints(verts, nv, tdf, 10.0, 0.5, None, lws, stream)
+        edges = tsf.DeviceArray((max(ne, 1), 2), np.int32)
+        tsf.link_centerpoints(verts, nv, tdf, 10.0, 0.5, edges, lws, stream)
+        labels, sizes = tsf.DeviceArray((nv,), np.int32), tsf.DeviceArray((nv,), np.int32)
+        tsf.graph_components(edges, ne, nv, labels, sizes, stream)
+        cl.zero(stream)
+        tsf.rasterize_tree_edges(edges, ne, verts, labels, sizes, 5, cl, stream)
+        return ne
+
+    ms_graph, ne = timed(graph, reps=2)
+    d.update({"candidates": ncand, "vertices": nv, "edges": ne, "candidates_to_vertices_ms": round(ms_cand, 3),
+              "link_components_rasterise_ms": round(ms_graph, 3),
+              "centerline_voxels": int(cl.download(stream).sum())})
+    del mask, hws, lws
+    out["centerline"] = d
+
+    # segmentation of that centerline
+    seg = tsf.DeviceArray(shape, np.uint8)
+    sws = tsf.DeviceArray((tsf.segment_inverse_gradient_workspace_size(shape),), np.uint8)
+    ms_seg, passes = timed(lambda: tsf.segment_inverse_gradient(cl, radius, field, seg, sws, stream), reps=2)
+    out["segmentation"] = {"ms": round(ms_seg, 3), "growing_passes": passes,
+                           "ms_per_pass": round(ms_seg / max(passes, 1), 3),
+                           "voxels": int(seg.download(stream).sum())}
+    log(f"downstream: {out}")
+    return out
 
 
 CHAIN = dict(sigma=0.5, rmin=0.5, rmax=4.0, rstep=0.5)

@@ -95,34 +95,50 @@ __global__ void k_seg_decide(const uint8_t* __restrict__ cur, Field<S> F, Dims d
     }
 }
 
-// grow, second half: accepted > nominated > dropped; clears the marks
-__global__ void k_seg_merge(const uint8_t* __restrict__ cur, uint8_t* __restrict__ acc, uint8_t* __restrict__ nom,
-                            uint8_t* __restrict__ next, Dims d) {
-    FOR_VOXEL(d)
-    next[i] = cur[i] == 1 || acc[i] ? 1 : (nom[i] ? 2 : 0);
-    acc[i] = 0;
-    nom[i] = 0;
+// grow, second half: accepted > nominated > dropped; clears the marks.
+// Elementwise over the 256-byte-aligned workspace arrays (their lengths are
+// multiples of 256), 16 voxels per thread; the padding past the last voxel
+// is merged too and never read as a voxel.
+__device__ __forceinline__ unsigned merge_word(unsigned c, unsigned a, unsigned m) {
+    unsigned o = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) {
+        const unsigned cb = (c >> k) & 0xffu, ab = (a >> k) & 0xffu, mb = (m >> k) & 0xffu;
+        o |= (cb == 1u || ab ? 1u : (mb ? 2u : 0u)) << k;
+    }
+    return o;
+}
+__global__ void __launch_bounds__(256) k_seg_merge(const uint4* __restrict__ cur, uint4* __restrict__ acc,
+                                                   uint4* __restrict__ nom, uint4* __restrict__ next, i64 n16) {
+    const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n16) return;
+    const uint4 c = cur[t], a = acc[t], m = nom[t];
+    next[t] = make_uint4(merge_word(c.x, a.x, m.x), merge_word(c.y, a.y, m.y), merge_word(c.z, a.z, m.z),
+                         merge_word(c.w, a.w, m.w));
+    acc[t] = make_uint4(0, 0, 0, 0);
+    nom[t] = make_uint4(0, 0, 0, 0);
 }
 
-// dilate: 1 where a voxel of the 3x3x3 neighbourhood inside the volume is 1
-__global__ void k_seg_dilate(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Dims d) {
+// The 3x3x3 closing, one axis per pass (a box is the product of its three
+// axis ranges, and both the inside-the-volume test of the dilation and the
+// clamp-to-edge of the erosion act per axis):
+// dilate: 1 where a voxel of the axis neighbourhood inside the volume is 1;
+template <int AXIS>
+__global__ void k_seg_dilate_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Dims d) {
     FOR_VOXEL(d)
-    uint8_t v = 0;
-    for (int a = -1; a < 2; a++)
-        for (int b = -1; b < 2; b++)
-            for (int c = -1; c < 2; c++)
-                if (inside(d, x + a, y + b, z + c) && in[d.at(x + a, y + b, z + c)] == 1) v = 1;
-    out[i] = v;
+    const int p = AXIS == 0 ? x : (AXIS == 1 ? y : z), e = AXIS == 0 ? d.X : (AXIS == 1 ? d.Y : d.Z);
+    const i64 s = AXIS == 0 ? 1 : (AXIS == 1 ? (i64)d.X : (i64)d.X * d.Y);
+    const bool v = in[i] == 1 || (p > 0 && in[i - s] == 1) || (p + 1 < e && in[i + s] == 1);
+    out[i] = v ? 1 : 0;
 }
-
-// erode: 1 where the 3x3x3 neighbourhood (clamp-to-edge) is all 1
-__global__ void k_seg_erode(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Dims d) {
+// erode: 1 where the axis neighbourhood (clamp-to-edge) is all 1.
+template <int AXIS>
+__global__ void k_seg_erode_axis(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, Dims d) {
     FOR_VOXEL(d)
-    bool keep = in[i] == 1;
-    for (int a = -1; a < 2; a++)
-        for (int b = -1; b < 2; b++)
-            for (int c = -1; c < 2; c++) keep = keep && in[clamped(d, x + a, y + b, z + c)] == 1;
-    out[i] = keep ? 1 : 0;
+    const int p = AXIS == 0 ? x : (AXIS == 1 ? y : z), e = AXIS == 0 ? d.X : (AXIS == 1 ? d.Y : d.Z);
+    const i64 s = AXIS == 0 ? 1 : (AXIS == 1 ? (i64)d.X : (i64)d.X * d.Y);
+    const bool v = in[i] == 1 && in[p > 0 ? i - s : i] == 1 && in[p + 1 < e ? i + s : i] == 1;
+    out[i] = v ? 1 : 0;
 }
 
 // sphereSegmentation: a ball of the voxel's radius (|offset| < r) around
@@ -196,7 +212,8 @@ tsf_status tsf_segment_inverse_gradient(const uint8_t* centerlines, const float*
             case TSF_STORAGE_SN16: launch_decide<SN16>(field, A, d, acc, nom, grew, stream); break;
             default: launch_decide<BF16>(field, A, d, acc, nom, grew, stream);
         }
-        hipLaunchKernelGGL(k_seg_merge, grid, block, 0, stream, A, acc, nom, B, d);
+        hipLaunchKernelGGL(k_seg_merge, (unsigned)((nb / 16 + 255) / 256), 256, 0, stream, (const uint4*)A,
+                           (uint4*)acc, (uint4*)nom, (uint4*)B, (i64)(nb / 16));
         TSF_TRY(check_launch(W));
         std::swap(A, B);
         if (pass == 0) continue;
@@ -206,8 +223,13 @@ tsf_status tsf_segment_inverse_gradient(const uint8_t* centerlines, const float*
         if (g == 0 || pass > n + 2) break;
     }
     if (iterations) *iterations = (int32_t)(pass + 1);
-    hipLaunchKernelGGL(k_seg_dilate, grid, block, 0, stream, A, B, d);
-    hipLaunchKernelGGL(k_seg_erode, grid, block, 0, stream, B, segmentation, d);
+    // closing: A -> acc -> nom -> B (dilation), B -> acc -> nom -> segmentation (erosion)
+    hipLaunchKernelGGL(k_seg_dilate_axis<0>, grid, block, 0, stream, A, acc, d);
+    hipLaunchKernelGGL(k_seg_dilate_axis<1>, grid, block, 0, stream, acc, nom, d);
+    hipLaunchKernelGGL(k_seg_dilate_axis<2>, grid, block, 0, stream, nom, B, d);
+    hipLaunchKernelGGL(k_seg_erode_axis<0>, grid, block, 0, stream, B, acc, d);
+    hipLaunchKernelGGL(k_seg_erode_axis<1>, grid, block, 0, stream, acc, nom, d);
+    hipLaunchKernelGGL(k_seg_erode_axis<2>, grid, block, 0, stream, nom, segmentation, d);
     TSF_TRY(check_launch(W));
     if (sc.owned) {
         e = hipStreamSynchronize(stream);

@@ -17,6 +17,6 @@ from ._capi import (  # noqa: F401
     blur_adjoint, vector_field_adjoint, gvf_checkpoint_count, gvf_euler_checkpointed, gvf_euler_adjoint,
     gvf_euler_adjoint_workspace_size, gvf_finish_adjoint, tdf_adjoint, tdf_adjoint_workspace_size, mem_info, mse,
     link_centerpoints, link_centerpoints_workspace_size, graph_components, rasterize_tree_edges,
-    segment_inverse_gradient, segment_spheres, volume_limits, crop_scan_lines, crop_region, to_float_region,
+    segment_inverse_gradient, segment_inverse_gradient_workspace_size, segment_spheres, volume_limits, crop_scan_lines, crop_region, to_float_region,
     CROP_THRESHOLD, CROP_LUNG,
 )

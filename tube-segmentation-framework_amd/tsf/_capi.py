@@ -509,6 +509,10 @@ def rasterize_tree_edges(edges, n_edges, vertices, labels, sizes, min_tree_lengt
 
 
 # ------------------------------------------------------------------- segmentation
+def segment_inverse_gradient_workspace_size(shape):
+    return lib().tsf_segment_inverse_gradient_workspace_size(_dims(shape))
+
+
 def segment_inverse_gradient(centerlines, radius, field, out, workspace=None, stream=None):
     """Returns the number of growing passes."""
     it = _i32(0)
