@@ -237,6 +237,23 @@ tsf_status tsf_do_eigen_points(const float* fx, const float* fy, const float* fz
  * workspace, if given, is unused). */
 tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep, float* tdf,
                           float* radius, void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* splineTDF (kernels_no_3d_write.cl:1574-1667) as runSplineTDF enqueues it
+ * (tubeDetectionFilters.cpp:4-55: 12 arms; the caller passes
+ * max(1, radiusMin) and min_average_mag 0.1), used for the large-radius TDF
+ * when "use-spline-tdf" is set (tube-segmentation.cpp:785-786).  Per voxel:
+ * normalised Hessian (w required), eigenvectors e1..e3; along 12 arms
+ * cos(a) e3 + sin(a) e2, a = 2 pi j / 12, the first radius from rmin + rstep
+ * (step rstep, up to rmax) where the trilinear field magnitude, after rising,
+ * falls with a two-sample mean above the voxel's own |F| by min_average_mag.
+ * radius = sum of the borders found / 12; tdf = 0 when an arm finds none or
+ * its border field points outward, else min(1, mean(1 - |n . e1|) *
+ * symmetry + 0.2).  Workspace: tsf_tdf_spline_workspace_size (packed
+ * texels). */
+size_t tsf_tdf_spline_workspace_size(tsf_dims dims, int32_t storage);
+tsf_status tsf_tdf_spline(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
+                          float min_average_mag, float* tdf, float* radius, void* workspace, size_t workspace_bytes,
+                          hipStream_t stream);
 size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage);
 
 /* combine (kernels_no_3d_write.cl:1087-1098): where tdf_large < tdf_small,

@@ -142,6 +142,17 @@ def tdf_circle(F, rmin, rmax, rstep, prec="f32"):
     return tdf, rad
 
 
+def tdf_spline(F, rmin, rmax, rstep, arms=12, min_average_mag=0.1, prec="f32"):
+    """splineTDF: (tdf, radius) of a 4-plane field (w read by the normalised Hessian)."""
+    F = np.ascontiguousarray(F, _dt(prec))
+    X, Y, Z = _dims(F)
+    tdf = np.empty(F.shape[1:], _dt(prec))
+    rad = np.empty(F.shape[1:], _dt(prec))
+    _fn("tdf_spline", prec)(_p(F), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep),
+                            int(arms), ctypes.c_float(min_average_mag), _p(tdf), _p(rad))
+    return tdf, rad
+
+
 def combine(ts, rs, tl, rl, prec="f32"):
     ts, rs = np.ascontiguousarray(ts, _dt(prec)), np.ascontiguousarray(rs, _dt(prec))
     tl, rl = np.array(tl, _dt(prec)), np.array(rl, _dt(prec))

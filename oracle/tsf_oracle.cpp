@@ -743,6 +743,96 @@ static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, 
             }
 }
 
+// splineTDF, kernels_no_3d_write.cl:1574-1667 (enqueued by runSplineTDF,
+// tubeDetectionFilters.cpp:4-55, with 12 arms and minAverageMag 0.1): along
+// `arms` rays in the plane of e2, e3 (directions alpha_j = 2 pi j / arms,
+// M_PI_F in float), the first radius where the field magnitude, after
+// rising, falls and its two-sample mean exceeds the voxel's own by
+// minAverageMag.  Radius = sum of the border radii found / arms; TDF = 0
+// when an arm finds none or its border field points outward, else
+// min(1, mean(1 - |n . e1|) * symmetry + 0.2), symmetry the mean of
+// min/max over opposite arms.  The direction tables are cos and sin of the
+// float angle, correctly rounded.
+static void spline_tables(int arms, float* c, float* s) {
+    for (int j = 0; j < arms; j++) {
+        const float alpha = 2.0f * 3.14159274f * (float)j / (float)arms;
+        c[j] = (float)std::cos((double)alpha);
+        s[j] = (float)std::sin((double)alpha);
+    }
+}
+template <typename T>
+static void tdf_spline(const T* F, Dims d, float rMin, float rMax, float rStep, int arms, float minAvg, T* tdf,
+                       T* rad) {
+    float ct[12], st[12];
+    spline_tables(arms, ct, st);
+    const i64 N = d.n();
+    auto len = [](const T v[3]) { return (T)std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int z = 0; z < d.Z; z++)
+        for (int y = 0; y < d.Y; y++)
+            for (int x = 0; x < d.X; x++) {
+                T H[3][3], V[3][3], ev[3];
+                hessian_at(F, d, x, y, z, true, H);
+                eigen3(H, V, ev);
+                const T e1[3] = {V[0][0], V[1][0], V[2][0]};
+                const T e2[3] = {V[0][1], V[1][1], V[2][1]};
+                const T e3[3] = {V[0][2], V[1][2], V[2][2]};
+                const i64 i = d.at(x, y, z);
+                const T f0[3] = {F[i], F[N + i], F[2 * N + i]};
+                const T cur = len(f0);
+                T maxR[12], avgR = 0, sum = 0;
+                bool invalid = false;
+                for (int j = 0; j < arms; j++) {
+                    maxR[j] = 999;
+                    T va[3];
+                    for (int k = 0; k < 3; k++) va[k] = (T)ct[j] * e3[k] + (T)st[j] * e2[k];
+                    const T prev2 = cur;
+                    T s[3];
+                    trilinear3(F, d, (T)x + (T)rMin * va[0], (T)y + (T)rMin * va[1], (T)z + (T)rMin * va[2], s);
+                    T prev = len(s);
+                    int up = prev2 > prev ? 0 : 1;
+                    for (float radius = rMin + rStep; radius <= rMax; radius += rStep) {
+                        trilinear3(F, d, (T)x + (T)radius * va[0], (T)y + (T)radius * va[1],
+                                   (T)z + (T)radius * va[2], s);
+                        const T mag = len(s);
+                        if (up == 1 && mag < prev && (prev + mag) / (T)2 - cur > (T)minAvg) {
+                            maxR[j] = (T)radius;
+                            avgR += (T)radius;
+                            const T lv = len(va);
+                            T nv[3], nva[3];
+                            for (int k = 0; k < 3; k++) {
+                                nv[k] = s[k] / mag;
+                                nva[k] = -(va[k] / lv);
+                            }
+                            if (nv[0] * nva[0] + nv[1] * nva[1] + nv[2] * nva[2] < (T)0) {
+                                invalid = true;
+                                sum = 0;
+                            }
+                            sum += (T)1 - std::fabs(nv[0] * e1[0] + nv[1] * e1[1] + nv[2] * e1[2]);
+                            break;
+                        }
+                        if (mag > prev) up = 1;
+                        prev = mag;
+                    }
+                    if (maxR[j] == (T)999 || invalid) {
+                        invalid = true;
+                        break;
+                    }
+                }
+                avgR = avgR / (T)arms;
+                rad[i] = avgR;
+                if (!invalid) {
+                    T sym = 0;
+                    for (int j = 0; j < arms / 2; j++)
+                        sym += std::min(maxR[j], maxR[arms / 2 + j]) / std::max(maxR[j], maxR[arms / 2 + j]);
+                    sym /= (T)(arms / 2);
+                    tdf[i] = std::fmin((T)1, (sum / (T)arms) * sym + (T)0.2f);
+                } else {
+                    tdf[i] = 0;
+                }
+            }
+}
+
 // ---------------------------------------------------------------- combine / candidates (A12, A13)
 // combine, kernels_no_3d_write.cl:1087-1098.
 template <typename T>
@@ -1317,6 +1407,10 @@ static void chain_gradient(const T* I, Dims d, T sigma, float Fmax, int sign, T 
     extern "C" void orc_tdf_circle_##S(const T* F, int X, int Y, int Z, float rmin, float rmax, float rstep, \
                                        T* tdf, T* rad) {                                                     \
         tdf_circle<T>(F, DIMS, rmin, rmax, rstep, tdf, rad);                                                 \
+    }                                                                                                        \
+    extern "C" void orc_tdf_spline_##S(const T* F, int X, int Y, int Z, float rmin, float rmax, float rstep, \
+                                       int arms, float min_avg, T* tdf, T* rad) {                            \
+        tdf_spline<T>(F, DIMS, rmin, rmax, rstep, arms, min_avg, tdf, rad);                                  \
     }                                                                                                        \
     extern "C" void orc_combine_##S(const T* ts, const T* rs, T* tl, T* rl, long long n) {                  \
         combine<T>(ts, rs, tl, rl, n);                                                                       \

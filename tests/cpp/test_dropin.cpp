@@ -40,6 +40,8 @@ void orc_blur_f32(const float* in, float* out, int X, int Y, int Z, int m, const
 void orc_to_float_f32(const double* in, float* out, long long n, float mn, float mx);
 void orc_vector_field_f32(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign);
 void orc_gvf_euler_f32(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters);
+void orc_tdf_spline_f32(const float* F, int X, int Y, int Z, float rmin, float rmax, float rstep, int arms,
+                        float min_avg, float* tdf, float* rad);
 void orc_tdf_circle_f32(const float* F, int X, int Y, int Z, float rmin, float rmax, float rstep, float* tdf,
                         float* rad);
 void orc_combine_f32(const float* ts, const float* rs, float* tl, float* rl, long long n);
@@ -407,6 +409,25 @@ static void gpu_tests(const std::string& dsdir) {
         Buffer seg = runInverseGradientSegmentation(ocl, cl, vf2, rad2, size, p);
         seg.download(ocl, s.data(), n);
         return tube_bars(validate_tube(e.data(), s.data(), dsdir, X, Y, Z), true);
+    });
+    // use-spline-tdf: the large-radius TDF is the spline TDF from 1.5
+    // (tube-segmentation.cpp:785-786), combined with the small circle TDF
+    run("runCircleFittingMethod_spline_tdf", [&] {
+        paramList q = p;
+        setParameter(q, "use-spline-tdf", "true");
+        Image3D vf3, tdf3, rad3;
+        Image3D ds3(size, 1);
+        check(tsf_to_float(draw.get(), 0, (float*)ds3.plane(0), ds3.dims(), 0.0f, 75.0f, ocl.queue));
+        runCircleFittingMethod(ocl, &ds3, size, q, vf3, tdf3, rad3);
+        std::vector<float> t2(n), r2(n), tsc = ts, rsc = rs;
+        orc_tdf_spline_f32(V.data(), X, Y, Z, 1.5f, 4.0f, getParam(q, "radius-step"), 12, 0.1f, t2.data(), r2.data());
+        orc_combine_f32(tsc.data(), rsc.data(), t2.data(), r2.data(), (long long)n);
+        std::string r = compare(tdf3.downloadPlane(ocl, 0), t2, 1e-4f, 1e-7f, "spline TDF");
+        if (r.empty()) r = compare(rad3.downloadPlane(ocl, 0), r2, 0.0f, 0.0f, "spline radius");
+        size_t spline_wins = 0;
+        for (size_t i = 0; i < n; i++) spline_wins += t2[i] > ts[i];
+        std::printf("  voxels where the spline TDF exceeds the small circle TDF: %zu\n", spline_wins);
+        return r;
     });
     run("candidate_count_range_error", [&] {
         paramList q = p;

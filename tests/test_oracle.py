@@ -824,3 +824,58 @@ def test_crop_region_known_answers(orc):
     ten = np.full(10, 300, np.int32)
     assert orc.crop_region(ten, full, full, 200, False) == ((0, 0, 0), (8, 16, 16))
     assert orc.crop_region(full, full, np.zeros(16, np.int32), 200, False) is None
+
+
+# ----------------------------------------------------------------- spline TDF
+def _spline_tube(shape, peak, axis, sign=-1.0):
+    """A tube along `axis` (0 = z, 1 = y, 2 = x) centred in the volume: the
+    field points inward (sign -1) with magnitude rho up to `peak`, then falls."""
+    Z, Y, X = shape
+    zz, yy, xx = np.mgrid[:Z, :Y, :X].astype(np.float64)
+    dd = [zz - (Z - 1) / 2, yy - (Y - 1) / 2, xx - (X - 1) / 2]
+    a, b = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[axis]
+    rho = np.sqrt(dd[a] ** 2 + dd[b] ** 2)
+    m = np.where(rho <= peak, rho, np.maximum(0.0, 2 * peak - rho))
+    F = np.zeros((4,) + shape)
+    plane = {0: 2, 1: 1, 2: 0}
+    with np.errstate(invalid="ignore", divide="ignore"):
+        F[plane[a]] = np.where(rho > 0, sign * m * dd[a] / rho, 0)
+        F[plane[b]] = np.where(rho > 0, sign * m * dd[b] / rho, 0)
+    w = np.sqrt(F[0] ** 2 + F[1] ** 2 + F[2] ** 2)
+    F[3] = np.where(w > 0, w, 1.0)
+    return F.astype(np.float32)
+
+
+def test_spline_tdf_tube_axis_known_answer(orc):
+    """On the axis of an ideal tube of wall radius 6 the 12 arms lie in the
+    cross-section (e2, e3 perpendicular to the axis), every border field is
+    perpendicular to the axis and opposite arms are equal: T = 1, and each
+    border radius is the first sample past the wall (6.0 or 6.5 with step
+    0.5).  The same in all three orientations."""
+    for axis in (0, 1, 2):
+        shape = [33, 33, 33]
+        shape[axis] = 12
+        t, r = orc.tdf_spline(_spline_tube(tuple(shape), 6.0, axis), 1.0, 10.0, 0.5)
+        c = [16, 16, 16]
+        c[axis] = 6
+        assert t[tuple(c)] == 1.0, axis
+        assert 6.0 <= r[tuple(c)] <= 6.5, axis
+
+
+def test_spline_tdf_off_axis_and_invalid_cases(orc):
+    """Off the axis the opposite arms differ, so T falls with the offset; a
+    field pointing outward fails the border-direction test on the first arm
+    (T = 0, radius = that arm's border, a sample just past the wall, / 12: the
+    reference's partial mean); a
+    uniform field has no border (T = 0, radius 0)."""
+    F = _spline_tube((33, 33, 12), 6.0, 2)
+    t, r = orc.tdf_spline(F, 1.0, 10.0, 0.5)
+    ts = [float(t[16 + k, 16, 6]) for k in range(4)]
+    assert all(a >= b for a, b in zip(ts, ts[1:])) and ts[3] < 1.0 and ts[3] > 0.2
+    t, r = orc.tdf_spline(_spline_tube((33, 33, 12), 6.0, 2, sign=1.0), 1.0, 10.0, 0.5)
+    assert t[16, 16, 6] == 0.0 and 6.0 / 12 <= r[16, 16, 6] <= 7.5 / 12
+    u = np.zeros((4, 10, 10, 10), np.float32)
+    u[0] = 0.3
+    u[3] = 0.3
+    t, r = orc.tdf_spline(u, 1.0, 5.0, 0.5)
+    assert (t == 0).all() and (r == 0).all()

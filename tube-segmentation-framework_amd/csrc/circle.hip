@@ -82,6 +82,90 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float r
     rad[i] = r;
 }
 
+// splineTDF (kernels_no_3d_write.cl:1574-1667) at one voxel, with the 12
+// arms runSplineTDF enqueues (tubeDetectionFilters.cpp:46): normalised
+// Hessian, eigenvectors e1..e3, and along each arm cos(a_j) e3 + sin(a_j) e2
+// the first radius where the sampled field magnitude, after rising, falls
+// with a two-sample mean above the voxel's own by min_avg.  The arms loop is
+// unrolled so the border radii stay in registers.
+struct SplineTables {
+    float c[12], s[12];
+};
+inline SplineTables make_spline_tables() {
+    SplineTables t;
+    for (int j = 0; j < 12; j++) {
+        const float alpha = 2.0f * 3.14159274f * (float)j / 12.0f;  // 2 * M_PI_F * j / arms, in float
+        t.c[j] = (float)cos((double)alpha);
+        t.s[j] = (float)sin((double)alpha);
+    }
+    return t;
+}
+
+__device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+template <class S>
+__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_spline(Tex<S> T, Dims d, float rmin, float rmax, float rstep,
+                                                              float min_avg, SplineTables tab,
+                                                              float* __restrict__ tdf, float* __restrict__ rad) {
+    constexpr int ARMS = 12;
+    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
+    const int z = blockIdx.z * CBZ + threadIdx.z;
+    if (x >= d.X || y >= d.Y || z >= d.Z) return;
+    float H[3][3], V[3][3], ev[3];
+    hessian_tex<true>(T, d, x, y, z, H);
+    eigen3(H, V, ev);
+    const float e1x = V[0][0], e1y = V[1][0], e1z = V[2][0];
+    const float e2x = V[0][1], e2y = V[1][1], e2z = V[2][1];
+    const float e3x = V[0][2], e3y = V[1][2], e3z = V[2][2];
+    const float4 f0 = T.at(T.row(d, y, z), x);
+    const float cur = len3(f0.x, f0.y, f0.z);
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    float maxR[ARMS];
+    float avgR = 0.0f, sum = 0.0f;
+    bool invalid = false;
+#pragma unroll
+    for (int j = 0; j < ARMS; j++) {
+        maxR[j] = 999.0f;
+        if (invalid) continue;  // the reference leaves the arms loop here
+        const float vx = tab.c[j] * e3x + tab.s[j] * e2x, vy = tab.c[j] * e3y + tab.s[j] * e2y;
+        const float vz = tab.c[j] * e3z + tab.s[j] * e2z;
+        float s[3];
+        trilinear_tex(T, d, fx + rmin * vx, fy + rmin * vy, fz + rmin * vz, s);
+        float prev = len3(s[0], s[1], s[2]);
+        int up = cur > prev ? 0 : 1;
+        for (float radius = rmin + rstep; radius <= rmax; radius += rstep) {
+            trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+            const float mag = len3(s[0], s[1], s[2]);
+            if (up == 1 && mag < prev && (prev + mag) / 2.0f - cur > min_avg) {
+                maxR[j] = radius;
+                avgR += radius;
+                const float lv = len3(vx, vy, vz);
+                const float nx = s[0] / mag, ny = s[1] / mag, nz = s[2] / mag;
+                if (nx * (-(vx / lv)) + ny * (-(vy / lv)) + nz * (-(vz / lv)) < 0.0f) {
+                    invalid = true;
+                    sum = 0.0f;
+                }
+                sum += 1.0f - fabsf(nx * e1x + ny * e1y + nz * e1z);
+                break;
+            }
+            if (mag > prev) up = 1;
+            prev = mag;
+        }
+        if (maxR[j] == 999.0f) invalid = true;
+    }
+    const i64 i = d.at(x, y, z);
+    rad[i] = avgR / (float)ARMS;
+    if (invalid) {
+        tdf[i] = 0.0f;
+        return;
+    }
+    float sym = 0.0f;
+#pragma unroll
+    for (int j = 0; j < ARMS / 2; j++) sym += fminf(maxR[j], maxR[ARMS / 2 + j]) / fmaxf(maxR[j], maxR[ARMS / 2 + j]);
+    sym /= (float)(ARMS / 2);
+    tdf[i] = fminf(1.0f, (sum / (float)ARMS) * sym + 0.2f);
+}
+
 // The mean of dot(-F(p + r V_j), V_j) over the 32 circle directions.
 template <class A>
 __device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx, float fy, float fz, float radius,
@@ -396,6 +480,58 @@ tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, flo
             if (norm) launch_tdf<BF16, true>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
             else launch_tdf<BF16, false>(sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream);
         }
+    }
+    TSF_TRY(check_launch(W));
+    if (sc.owned) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    }
+    return TSF_OK;
+}
+
+size_t tsf_tdf_spline_workspace_size(tsf_dims dims, int32_t storage) {
+    return tsf_tdf_circle_workspace_size(dims, storage);
+}
+
+tsf_status tsf_tdf_spline(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep,
+                          float min_average_mag, float* tdf, float* radius, void* workspace, size_t workspace_bytes,
+                          hipStream_t stream) {
+    const char* W = "tsf_tdf_spline";
+    TSF_TRY(check_field(field, true, W));
+    TSF_TRY(check_dims(dims, 1, W));
+    if (!tdf || !radius) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL output", W);
+    if (!(rstep > 0.0f) || !__builtin_isfinite(rmin) || !__builtin_isfinite(rmax) ||
+        !__builtin_isfinite(min_average_mag))
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: radius range must be finite with rstep > 0", W);
+    if ((double)(rmax - rmin) / rstep > 100000.0)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: more than 1e5 radii", W);
+    const Dims d = D(dims);
+    const i64 n = d.n();
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_spline_workspace_size(dims, field->storage), W));
+    static const SplineTables tab = make_spline_tables();
+    const float inv = field->storage == TSF_STORAGE_SN16 ? 1.0f / 32768.0f : 1.0f / field->scale;
+    const dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
+    if (field->storage == TSF_STORAGE_F32) {
+        hipLaunchKernelGGL(k_pack_f32, grid1d(n), 256, 0, stream, (const float*)field->x, (const float*)field->y,
+                           (const float*)field->z, (const float*)field->w, (float4*)sc.ptr, n);
+        Tex<F32> T{(const float4*)sc.ptr, inv};
+        hipLaunchKernelGGL(k_tdf_spline<F32>, grid, block, 0, stream, T, d, rmin, rmax, rstep, min_average_mag, tab,
+                           tdf, radius);
+    } else {
+        const bool sn16 = field->storage == TSF_STORAGE_SN16;
+        hipLaunchKernelGGL(k_pack_16, grid1d(n), 256, 0, stream, (const uint16_t*)field->x, (const uint16_t*)field->y,
+                           (const uint16_t*)field->z, (const uint16_t*)field->w, (uint16_t)0, sn16, (uint2*)sc.ptr, n);
+        const uint2* p = (const uint2*)sc.ptr;
+        if (sn16)
+            hipLaunchKernelGGL(k_tdf_spline<SN16>, grid, block, 0, stream, Tex<SN16>{p, inv}, d, rmin, rmax, rstep,
+                               min_average_mag, tab, tdf, radius);
+        else if (field->storage == TSF_STORAGE_F16)
+            hipLaunchKernelGGL(k_tdf_spline<F16>, grid, block, 0, stream, Tex<F16>{p, inv}, d, rmin, rmax, rstep,
+                               min_average_mag, tab, tdf, radius);
+        else
+            hipLaunchKernelGGL(k_tdf_spline<BF16>, grid, block, 0, stream, Tex<BF16>{p, inv}, d, rmin, rmax, rstep,
+                               min_average_mag, tab, tdf, radius);
     }
     TSF_TRY(check_launch(W));
     if (sc.owned) {

@@ -395,6 +395,26 @@ void runCircleFittingTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* 
     circleTDF(ocl, *vectorField, radiusMin, radiusMax, radiusStep, (float*)TDF->get(), (float*)radius->get());
 }
 
+// runSplineTDF (tubeDetectionFilters.cpp:4-55): rMin raised to 1, 12 arms,
+// minimum average magnitude 0.1; a 16-bit run stores the TDF as UNORM16.
+static void splineTDF(OpenCL& ocl, const Image3D& field, float rmin, float rmax, float rstep, float* T, float* R) {
+    tsf_field f = field.field();
+    Buffer ws(tsf_tdf_spline_workspace_size(field.dims(), field.storage()));
+    check(tsf_tdf_spline(&f, field.dims(), std::max(1.0f, rmin), rmax, rstep, 0.1f, T, R, ws.get(), ws.size(),
+                         ocl.queue));
+    if (field.storage() == TSF_STORAGE_SN16) check(tsf_tdf_round_unorm16(T, (int64_t)voxels(field.size()), ocl.queue));
+    ocl.finish();
+}
+
+void runSplineTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* TDF, Buffer* radius, float radiusMin,
+                  float radiusMax, float radiusStep) {
+    if (!vectorField || !TDF || !radius) fail(TSF_ERR_INVALID_ARGUMENT, "runSplineTDF: NULL argument");
+    const size_t n = voxels(size);
+    if (TDF->size() < n * 4 || radius->size() < n * 4)
+        fail(TSF_ERR_INVALID_ARGUMENT, "runSplineTDF: output buffers too small");
+    splineTDF(ocl, *vectorField, radiusMin, radiusMax, radiusStep, (float*)TDF->get(), (float*)radius->get());
+}
+
 static Image3D blurOrSame(OpenCL& ocl, Image3D& data, float sigma) {
     if (!(sigma > 0)) return data;
     Image3D out(data.size(), 1);
@@ -415,8 +435,6 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
                             Image3D& TDF, Image3D& radiusImage) {
     if (!dataset || !dataset->valid() || dataset->channels() != 1 || dataset->storage() != TSF_STORAGE_F32)
         fail(TSF_ERR_INVALID_ARGUMENT, "runCircleFittingMethod: dataset must be a float scalar volume");
-    if (getParamBool(p, "use-spline-tdf"))
-        fail(TSF_ERR_INVALID_ARGUMENT, "runCircleFittingMethod: the spline TDF is not part of this library");
     const float rMin = getParam(p, "radius-min"), rMax = getParam(p, "radius-max"), rStep = getParam(p, "radius-step");
     const size_t n = voxels(size);
     Image3D tdfSmall, radSmall;
@@ -443,8 +461,14 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
     else vectorField = runGVF(ocl, &F, p, size, false);
     TDF = Image3D(size, 1);
     radiusImage = Image3D(size, 1);
-    circleTDF(ocl, vectorField, std::max(2.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
-              (float*)radiusImage.plane(0));
+    // the large-radius TDF: spline (from 1.5) or circle fitting (from 2.5),
+    // tube-segmentation.cpp:785-789
+    if (getParamBool(p, "use-spline-tdf"))
+        splineTDF(ocl, vectorField, std::max(1.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
+                  (float*)radiusImage.plane(0));
+    else
+        circleTDF(ocl, vectorField, std::max(2.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
+                  (float*)radiusImage.plane(0));
     if (rMin < 2.5f)
         check(tsf_combine((const float*)tdfSmall.plane(0), (const float*)radSmall.plane(0), (float*)TDF.plane(0),
                           (float*)radiusImage.plane(0), (int64_t)n, ocl.queue));

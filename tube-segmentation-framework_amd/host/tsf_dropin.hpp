@@ -199,8 +199,14 @@ Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3
 void runCircleFittingTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* TDF, Buffer* radius,
                          float radiusMin, float radiusMax, float radiusStep);
 
+// tubeDetectionFilters.hpp: the spline TDF (12 arms, rMin raised to 1),
+// same buffers as runCircleFittingTDF.
+void runSplineTDF(OpenCL& ocl, int3& size, Image3D* vectorField, Buffer* TDF, Buffer* radius, float radiusMin,
+                  float radiusMax, float radiusStep);
+
 // tube-segmentation.cpp:212-939: blur -> vector field -> (small TDF) -> blur
-// -> vector field -> GVF (Euler or FMG) -> large TDF -> combine.  dataset is a
+// -> vector field -> GVF (Euler or FMG) -> large TDF (circle fitting, or the
+// spline TDF with "use-spline-tdf") -> combine.  dataset is a
 // float scalar volume in [0, 1] (toFloat output); it is consumed.  Outputs
 // are set by reference as in the reference.
 void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList& parameters, Image3D& vectorField,

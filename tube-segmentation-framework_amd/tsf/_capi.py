@@ -77,6 +77,8 @@ _SIGS = {
     "tsf_segment_inverse_gradient": ([_vp, _vp, _PF, Dims, _vp, ctypes.POINTER(_i32), _vp, _sz, _vp], _i32),
     "tsf_segment_inverse_gradient_workspace_size": ([Dims], _sz),
     "tsf_segment_spheres": ([_vp, _vp, Dims, _vp, _vp], _i32),
+    "tsf_tdf_spline_workspace_size": ([Dims, _i32], _sz),
+    "tsf_tdf_spline": ([_PF, Dims, _f32, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
     "tsf_volume_limits": ([_vp, _i32, Dims, ctypes.POINTER(_f32), ctypes.POINTER(_f32), _vp], _i32),
     "tsf_crop_scan_lines": ([_vp, _i32, Dims, _i32, _f32, _vp, _vp, _vp, _vp], _i32),
     "tsf_crop_region": ([_vp, _vp, _vp, Dims, _i32, _i32, ctypes.POINTER(Dims), ctypes.POINTER(Dims)], _i32),
@@ -439,6 +441,13 @@ def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=Non
     w, wb = _ws(workspace)
     _check(lib().tsf_tdf_circle(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, tdf.ptr, radius.ptr, w, wb,
                                 _sp(stream)))
+
+
+def tdf_spline(field, rmin, rmax, rstep, tdf, radius, min_average_mag=0.1, workspace=None, stream=None):
+    d = field.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_tdf_spline(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, min_average_mag, tdf.ptr,
+                                radius.ptr, w, wb, _sp(stream)))
 
 
 def combine(ts, rs, tl, rl, stream=None):
