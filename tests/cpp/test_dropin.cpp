@@ -682,6 +682,37 @@ static void run_tests(const std::string& dsdir) {
                 return err;
             });
         }
+    // tests/tubeSegmentationTests.cpp:24-37, 103-146: the ridge method on
+    // dataset_1 (its Synthetic-Vascusynth preset), the reference's bars
+    for (const char* buffers : {"false", "true"})
+        for (const char* bits32 : {"false", "true"}) {
+            const std::string name = std::string("run_RidgeSystemTestWithSyntheticData") +
+                                     (bits32[0] == 't' ? "32bit" : "16bit") + (buffers[0] == 't' ? "Buffers" : "");
+            run(name.c_str(), [&] {
+                paramList p = initParameters();
+                setParameter(p, "parameters", "Synthetic-Vascusynth");
+                setParameter(p, "centerline-method", "ridge");
+                for (auto kv : std::vector<std::pair<const char*, const char*>>{
+                         {"mode", "white"}, {"small-blur", "1.0"}, {"fmax", "0.1"}, {"minimum", "0"},
+                         {"maximum", "75"}, {"m-low", "0.05"}, {"min-tree-length", "5"}, {"max-below-tdf-low", "0"},
+                         {"tdf-high", "0.5"}, {"tdf-low", "0.5"}, {"min-mean-tdf", "0.5"}})
+                    setParameter(p, kv.first, kv.second);
+                setParameter(p, "buffers-only", buffers);
+                setParameter(p, "32bit-vectors", bits32);
+                setParameter(p, "16bit-vectors", bits32[0] == 't' ? "false" : "true");
+                TSFOutput* out = run(dsdir + "/noisy.mhd", p, "");
+                const TubeValidation v =
+                    validate_tube(out->getCenterlineVoxels(), out->getSegmentation(), dsdir, X, Y, Z);
+                delete out;
+                std::printf("  ridge: %ld centerline voxels: average distance %.3f, %.1f %% extracted, precision "
+                            "%.3f, recall %.3f\n", v.extracted, v.averageDistance, v.percentageExtracted,
+                            v.precision, v.recall);
+                if (!(v.averageDistance < 0.5 && v.percentageExtracted > 75.0 && v.precision > 0.7 &&
+                      v.recall > 0.6))
+                    return std::string("below the reference's ridge bars");
+                return std::string();
+            });
+        }
     // threshold cropping on a box: x [10, 31) -> [10, 30) (the last slice is
     // the exclusive end), size 20; y [5, 27) -> [5, 26) grown alternately to
     // [4, 28), size 24; z [7, 20) from the end -> [7, 19), size 12.  Without

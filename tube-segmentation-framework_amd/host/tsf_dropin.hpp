@@ -17,6 +17,7 @@
 
 #include <map>
 #include <memory>
+#include <stack>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -307,10 +308,11 @@ class TSFOutput {
 
 // tube-segmentation.hpp / tube-segmentation.cpp:53-163: reads the .mhd
 // volume, runs the centerline method ("gpu": circle fitting, the centerline
-// graph and the segmentation) and returns the results.  kernel_dir is
-// accepted for the reference's signature; the kernels are in the library.
-// Throws IOException for a missing file, tsf::Error otherwise (the "ridge"
-// and "test" methods are not provided).
+// graph and the segmentation; "ridge": circle fitting, the ridge traversal
+// and the segmentation) and returns the results.  kernel_dir is accepted for
+// the reference's signature; the kernels are in the library.  Throws
+// IOException for a missing file, tsf::Error otherwise (the experimental
+// "test" method is not provided).
 TSFOutput* run(std::string filename, paramList& parameters, std::string kernel_dir);
 
 // tube-segmentation.cpp:1399-1905: parses the .mhd header (ElementType,
@@ -350,5 +352,24 @@ float3 getTubeDirection(TubeSegmentation& T, int3 pos, int3 size);
 void doEigenBatch(TubeSegmentation& T, int3 size, const std::vector<int3>& pos, std::vector<float>& out12);
 // Drops the device mirror of T's arrays (call when the host arrays change).
 void releaseEigenMirror(TubeSegmentation& T);
+
+// ridgeTraversalCenterlineExtraction.hpp:9-15: the ridge-traversal method.
+// Returns the centerline mask (new[]'d, one byte per voxel): the longest
+// traversed tree and every tree longer than min-tree-length.  It sets voxel
+// 0 of T.TDF / T.Fx / T.Fy / T.Fz to (0, 1, 0, 0), as the reference does,
+// and reads the Hessian through doEigen.  centerlineStack is taken by value,
+// as in the reference.  Throws tsf::Error (TSF_ERR_RANGE) when no start point
+// exists.
+struct CenterlinePoint {
+    int3 pos;
+    bool large = false;
+    CenterlinePoint* next = nullptr;
+};
+char* runRidgeTraversal(TubeSegmentation& T, int3 size, paramList& parameters,
+                        std::stack<CenterlinePoint> centerlineStack);
+// tube-segmentation.cpp:1260-1350: runCircleFittingMethod, the traversal on
+// the host copies of the field, TDF and radius, then the segmentation.
+void runCircleFittingAndRidgeTraversal(OpenCL& ocl, Image3D* dataset, int3* size, paramList& parameters,
+                                       TSFOutput* output);
 
 }  // namespace tsf

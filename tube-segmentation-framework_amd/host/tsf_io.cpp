@@ -273,6 +273,8 @@ TSFOutput* run(std::string filename, paramList& parameters, std::string kernel_d
         if (method == "gpu")
             runCircleFittingAndNewCenterlineAlg(output->getContext(), &dataset, output->getSize(), parameters,
                                                 output);
+        else if (method == "ridge")
+            runCircleFittingAndRidgeTraversal(output->getContext(), &dataset, output->getSize(), parameters, output);
         else
             throw Error(TSF_ERR_INVALID_ARGUMENT, "centerline-method " + method + " is not provided by this library");
         output->getContext().finish();
