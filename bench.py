@@ -270,7 +270,7 @@ def downstream_stages(vol, shape, tdf, radius, field, stream):
     """Times the stages either side of the filters on the same volume, each
     from device-resident inputs: ingest of the raw volume as 16-bit MRA
     intensities (limits, threshold-cropping scan lines, toFloat of the
-    region), the centerline graph from the forward's TDF, radius and GVF
+    region), the spline TDF on the GVF field, the centerline graph from the forward's TDF, radius and GVF
     field (candidates, compaction, suppression, vertices, linking, components,
     rasterisation; the Neuro-Vessels-MRA preset's thresholds) and the
     inverse-gradient segmentation of that centerline."""
@@ -307,6 +307,15 @@ def downstream_stages(vol, shape, tdf, radius, field, stream):
         "bytes": "limits 2 B/voxel read; scan lines 2 x 2 B/voxel (z-lines and x-lines); toFloat 2 B read + "
                  "4 B written per cropped voxel"}
     del raw, dst
+
+    # the spline TDF in place of the large circle TDF (use-spline-tdf: radii
+    # from 1.5), on the forward's GVF field
+    st, sr = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    sws = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, field.storage),), np.uint8)
+    ms_spl, _ = timed(lambda: tsf.tdf_spline(field, 1.5, 4.0, 0.5, st, sr, 0.1, sws, stream), reps=2)
+    out["spline_tdf"] = {"ms": round(ms_spl, 3), "radii": "1.5-4.0 step 0.5, 12 arms",
+                         "voxels_tdf_positive": int((st.download(stream) > 0).sum())}
+    del st, sr, sws
 
     # centerline graph
     d = {}
