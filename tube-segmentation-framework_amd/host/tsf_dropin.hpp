@@ -306,7 +306,7 @@ class TSFOutput {
     char* centerlineVoxels = nullptr;
 };
 
-// tube-segmentation.hpp / tube-segmentation.cpp:53-163: reads the .mhd
+// tube-segmentation.hpp:47 (tube-segmentation.cpp:53-163): reads the .mhd
 // volume, runs the centerline method ("gpu": circle fitting, the centerline
 // graph and the segmentation; "ridge": circle fitting, the ridge traversal
 // and the segmentation) and returns the results.  kernel_dir is accepted for
