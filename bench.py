@@ -67,7 +67,7 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the volume-filter path runs on a single device "
-              "(multi-GPU z-slab sharding is a design in DESIGN.md, not built); use --gpus 1", file=sys.stderr)
+              "(multi-GPU z-slab sharding is designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import tsf

@@ -4,7 +4,7 @@ seeded inputs.
 Tolerances (stated per test):
   * integer / mask / index outputs: bit-exact;
   * FP32 field, eigenvalue and TDF outputs: per voxel |a - b| <= 1e-4 |b| + 1e-7
-    (the north-star FP32 bar), and the L2-relative error is reported;
+    (the stated FP32 bar), and the L2-relative error is reported;
   * reduced-precision storage forms: error against the FP64 oracle is
     measured and bounded, not parity.
 """
