@@ -302,10 +302,10 @@ def downstream_stages(vol, shape, tdf, radius, field, stream):
     out["ingest"] = {
         "raw": "uint16", "limits": [lim[0], lim[1]], "crop_offset": off, "crop_size": size,
         "limits_ms": round(ms_lim, 3), "limits_gb_s": round(2 * N / (ms_lim * 1e-3) / 1e9, 1),
-        "scan_lines_ms": round(ms_scan, 3), "scan_lines_gb_s": round(4 * N / (ms_scan * 1e-3) / 1e9, 1),
+        "scan_lines_ms": round(ms_scan, 3), "scan_lines_gb_s": round(2 * N / (ms_scan * 1e-3) / 1e9, 1),
         "to_float_region_ms": round(ms_tf, 3), "to_float_region_gb_s": round(6 * Nc / (ms_tf * 1e-3) / 1e9, 1),
-        "bytes": "limits 2 B/voxel read; scan lines 2 x 2 B/voxel (z-lines and x-lines); toFloat 2 B read + "
-                 "4 B written per cropped voxel"}
+        "bytes": "limits 2 B/voxel read; threshold scan lines 2 B/voxel read (z-lines and x-lines in one pass); "
+                 "toFloat 2 B read + 4 B written per cropped voxel; times include the host synchronisation"}
     del raw, dst
 
     # the spline TDF in place of the large circle TDF (use-spline-tdf: radii

@@ -8,11 +8,12 @@
 // The reference reads the raw file on the host, finds the limits there,
 // uploads the whole volume, counts scan lines with one work-item per slice
 // (each walking a whole slice: X, Y or Z work-items in all), copies the
-// region to a new image and converts it.  Here the raw volume is read once
-// per kernel by many waves: one kernel walks every z-line (the counts of the
-// x and y slices come from the same lines), one walks every x-line through an
-// LDS transpose (the counts of the z slices), and the conversion reads the
-// region straight from the uploaded volume.
+// region to a new image and converts it.  Here the raw volume is read by
+// many waves: one kernel walks every z-line (the counts of the x and y
+// slices come from the same lines, and for threshold cropping also the
+// x-line verdicts, by wave ballots); lung cropping walks every x-line in a
+// second kernel through an LDS transpose; the conversion reads the region
+// straight from the uploaded volume.
 #include <cfloat>
 #include <cmath>
 
@@ -144,24 +145,32 @@ __global__ void k_zero_i32(int32_t* __restrict__ a, int n) {
 // z-lines: one thread per (x, y) walks z, 16 planes of loads in flight.
 // lines_x[x] counts the y with an inside z-line (slices of constant x,
 // scan lines along y, elements along z); lines_y[y] counts the x (slices of
-// constant y, scan lines along x).
+// constant y, scan lines along x).  With ROWS (threshold cropping, whose
+// x-line verdict is an OR along x) each wave also marks row (y, z) in
+// `rows` when one of its 64 values passes, so the x-lines need no second
+// read of the volume.
 constexpr int ZB = 16;
-template <class T, class L>
+template <class T, class L, bool ROWS>
 __global__ void __launch_bounds__(256) k_scan_z(const T* __restrict__ in, Dims d, float thr, int hu_limit,
-                                                int32_t* __restrict__ lines_x, int32_t* __restrict__ lines_y) {
+                                                int32_t* __restrict__ lines_x, int32_t* __restrict__ lines_y,
+                                                uint8_t* __restrict__ rows) {
     const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     const bool on = x < d.X && y < d.Y;
     L line;
-    if (on) {
-        const i64 plane = (i64)d.X * d.Y;
-        const T* p = in + (i64)y * d.X + x;
-        for (int z0 = 0; z0 < d.Z; z0 += ZB) {
-            float v[ZB];
+    const i64 plane = (i64)d.X * d.Y;
+    const T* p = in + (on ? (i64)y * d.X + x : 0);
+    for (int z0 = 0; z0 < d.Z; z0 += ZB) {
+        float v[ZB];
 #pragma unroll
-            for (int j = 0; j < ZB; j++) v[j] = z0 + j < d.Z ? (float)p[(i64)(z0 + j) * plane] : 0.0f;
+        for (int j = 0; j < ZB; j++) v[j] = on && z0 + j < d.Z ? (float)p[(i64)(z0 + j) * plane] : 0.0f;
 #pragma unroll
-            for (int j = 0; j < ZB; j++)
-                if (z0 + j < d.Z) line.step(v[j], thr, hu_limit);
+        for (int j = 0; j < ZB; j++) {
+            if (z0 + j >= d.Z) break;
+            if (on) line.step(v[j], thr, hu_limit);
+            if (ROWS) {
+                const unsigned long long hit = __ballot(on && v[j] > thr);
+                if (threadIdx.x == 0 && hit) rows[(i64)(z0 + j) * d.Y + y] = 1;
+            }
         }
     }
     const bool ins = on && line.inside();
@@ -174,6 +183,23 @@ __global__ void __launch_bounds__(256) k_scan_z(const T* __restrict__ in, Dims d
         const int s = cnt[0][threadIdx.x] + cnt[1][threadIdx.x] + cnt[2][threadIdx.x] + cnt[3][threadIdx.x];
         if (s) atomicAdd(lines_x + x, s);
     }
+}
+
+__global__ void k_zero_u8(uint8_t* __restrict__ a, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) a[i] = 0;
+}
+
+// lines_z[z] = number of marked rows of plane z (one workgroup per plane)
+__global__ void __launch_bounds__(256) k_count_rows(const uint8_t* __restrict__ rows, Dims d,
+                                                    int32_t* __restrict__ lines_z) {
+    const int z = blockIdx.x;
+    int c = 0;
+    for (int y = threadIdx.x; y < d.Y; y += 256) c += rows[(i64)z * d.Y + y];
+    for (int m = 32; m >= 1; m >>= 1) c += __shfl_xor(c, m);
+    __shared__ int part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) lines_z[z] = part[0] + part[1] + part[2] + part[3];
 }
 
 // x-lines: one wave per 64 rows of one z plane; each 64-column chunk of the
@@ -203,9 +229,19 @@ __global__ void __launch_bounds__(64) k_scan_x(const T* __restrict__ in, Dims d,
 }
 
 template <class T, class L>
-void launch_scan(const void* in, Dims d, float thr, int hu, int32_t* lx, int32_t* ly, int32_t* lz, hipStream_t s) {
-    hipLaunchKernelGGL((k_scan_z<T, L>), dim3((d.X + 63) / 64, (d.Y + 3) / 4), dim3(64, 4), 0, s, (const T*)in, d,
-                       thr, hu, lx, ly);
+void launch_scan(const void* in, Dims d, float thr, int hu, int32_t* lx, int32_t* ly, int32_t* lz, uint8_t* rows,
+                 hipStream_t s) {
+    const dim3 gz((d.X + 63) / 64, (d.Y + 3) / 4), bz(64, 4);
+    if (rows) {
+        // threshold: the x-line verdicts come from the z-line pass
+        const i64 nr = (i64)d.Y * d.Z;
+        hipLaunchKernelGGL(k_zero_u8, (unsigned)((nr + 255) / 256 < 65536 ? (nr + 255) / 256 : 65536), 256, 0, s,
+                           rows, nr);
+        hipLaunchKernelGGL((k_scan_z<T, L, true>), gz, bz, 0, s, (const T*)in, d, thr, hu, lx, ly, rows);
+        hipLaunchKernelGGL(k_count_rows, d.Z, 256, 0, s, (const uint8_t*)rows, d, lz);
+        return;
+    }
+    hipLaunchKernelGGL((k_scan_z<T, L, false>), gz, bz, 0, s, (const T*)in, d, thr, hu, lx, ly, nullptr);
     hipLaunchKernelGGL((k_scan_x<T, L>), dim3((d.Y + 63) / 64, d.Z), dim3(64), 0, s, (const T*)in, d, thr, hu, lz);
 }
 
@@ -314,20 +350,27 @@ tsf_status tsf_crop_scan_lines(const void* in, int32_t in_type, tsf_dims dims, i
     // HU limit of the lung automaton: -150, +1024 for unsigned volumes
     const int hu = (in_type == 0 || in_type == 2) ? -150 + 1024 : -150;
     if (method == TSF_CROP_THRESHOLD) {
+        // one byte per (y, z) row for the x-line verdicts
+        Scratch sc;
+        TSF_TRY(sc.get(nullptr, 0, (size_t)d.Y * d.Z, W));
+        uint8_t* rows = (uint8_t*)sc.ptr;
         switch (in_type) {
-            case 0: launch_scan<uint8_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            case 1: launch_scan<int8_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            case 2: launch_scan<uint16_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            case 3: launch_scan<int16_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            default: launch_scan<float, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
+            case 0: launch_scan<uint8_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, rows, stream); break;
+            case 1: launch_scan<int8_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, rows, stream); break;
+            case 2: launch_scan<uint16_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, rows, stream); break;
+            case 3: launch_scan<int16_t, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, rows, stream); break;
+            default: launch_scan<float, ThresholdLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, rows, stream); break;
         }
-    } else {
-        switch (in_type) {
-            case 0: launch_scan<uint8_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            case 1: launch_scan<int8_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            case 2: launch_scan<uint16_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-            default: launch_scan<int16_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, stream); break;
-        }
+        TSF_TRY(check_launch(W));
+        const hipError_t e = hipStreamSynchronize(stream);  // the rows scratch is freed on return
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+        return TSF_OK;
+    }
+    switch (in_type) {
+        case 0: launch_scan<uint8_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, nullptr, stream); break;
+        case 1: launch_scan<int8_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, nullptr, stream); break;
+        case 2: launch_scan<uint16_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, nullptr, stream); break;
+        default: launch_scan<int16_t, LungLine>(in, d, threshold, hu, lines_x, lines_y, lines_z, nullptr, stream); break;
     }
     return check_launch(W);
 }
