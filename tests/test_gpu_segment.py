@@ -29,8 +29,10 @@ def run_device(tsf, cl, radius, F, storage=None):
     return out.download(), passes
 
 
-def test_segment_tube_vs_oracle(tsf, orc):
-    shape = (28, 28, 32)
+@pytest.mark.parametrize("nx", [32, 31], ids=["words", "bytes"])
+def test_segment_tube_vs_oracle(tsf, orc, nx):
+    """X = 32 takes the closing on 4-voxel words, X = 31 the byte passes."""
+    shape = (28, 28, nx)
     F = tube_field(shape, 6.0)
     cl = np.zeros(shape, np.uint8)
     cl[13:15, 13:15, 2:30] = 1

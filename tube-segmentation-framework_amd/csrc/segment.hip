@@ -141,6 +141,41 @@ __global__ void k_seg_erode_axis(const uint8_t* __restrict__ in, uint8_t* __rest
     out[i] = v ? 1 : 0;
 }
 
+// The same closing passes on 32-bit words (4 voxels along x) when X is a
+// multiple of 4, so that no word crosses a row.  Each byte of `is1` is 1
+// where the input byte is 1; along x the neighbour bytes come from the
+// adjacent words (little-endian: byte k is x0 + k), zero (dilation) or the
+// edge byte itself (erosion) past the row ends.
+__device__ __forceinline__ unsigned is1(unsigned w) {
+    unsigned o = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k += 8) o |= (((w >> k) & 0xffu) == 1u ? 1u : 0u) << k;
+    return o;
+}
+template <int AXIS, bool DILATE>
+__global__ void __launch_bounds__(256) k_seg_close_words(const unsigned* __restrict__ in, unsigned* __restrict__ out,
+                                                         Dims d) {
+    const int XW = d.X >> 2;
+    const int xw = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    if (xw >= XW || y >= d.Y) return;
+    const i64 i = (i64)xw + (i64)XW * ((i64)y + (i64)d.Y * z);
+    const unsigned c = is1(in[i]);
+    unsigned r;
+    if (AXIS == 0) {
+        unsigned l = xw > 0 ? is1(in[i - 1]) >> 24 : (DILATE ? 0u : c & 0xffu);
+        unsigned h = xw + 1 < XW ? (is1(in[i + 1]) & 0xffu) << 24 : (DILATE ? 0u : c & 0xff000000u);
+        const unsigned left = (c << 8) | l, right = (c >> 8) | h;
+        r = DILATE ? (c | left | right) : (c & left & right);
+    } else {
+        const int p = AXIS == 1 ? y : z, e = AXIS == 1 ? d.Y : d.Z;
+        const i64 st = AXIS == 1 ? (i64)XW : (i64)XW * d.Y;
+        const unsigned lo = p > 0 ? is1(in[i - st]) : (DILATE ? 0u : c);
+        const unsigned hi = p + 1 < e ? is1(in[i + st]) : (DILATE ? 0u : c);
+        r = DILATE ? (c | lo | hi) : (c & lo & hi);
+    }
+    out[i] = r;
+}
+
 // sphereSegmentation: a ball of the voxel's radius (|offset| < r) around
 // every centerline voxel
 __global__ void k_seg_spheres(const uint8_t* __restrict__ cl, const float* __restrict__ radius, uint8_t* out, Dims d) {
@@ -224,12 +259,24 @@ tsf_status tsf_segment_inverse_gradient(const uint8_t* centerlines, const float*
     }
     if (iterations) *iterations = (int32_t)(pass + 1);
     // closing: A -> acc -> nom -> B (dilation), B -> acc -> nom -> segmentation (erosion)
-    hipLaunchKernelGGL(k_seg_dilate_axis<0>, grid, block, 0, stream, A, acc, d);
-    hipLaunchKernelGGL(k_seg_dilate_axis<1>, grid, block, 0, stream, acc, nom, d);
-    hipLaunchKernelGGL(k_seg_dilate_axis<2>, grid, block, 0, stream, nom, B, d);
-    hipLaunchKernelGGL(k_seg_erode_axis<0>, grid, block, 0, stream, B, acc, d);
-    hipLaunchKernelGGL(k_seg_erode_axis<1>, grid, block, 0, stream, acc, nom, d);
-    hipLaunchKernelGGL(k_seg_erode_axis<2>, grid, block, 0, stream, nom, segmentation, d);
+    if (d.X % 4 == 0 && ((uintptr_t)segmentation & 3u) == 0) {
+        typedef unsigned* W;
+        const dim3 gw((d.X / 4 + 63) / 64, (d.Y + 3) / 4, d.Z);
+        hipLaunchKernelGGL((k_seg_close_words<0, true>), gw, block, 0, stream, (const unsigned*)A, (W)acc, d);
+        hipLaunchKernelGGL((k_seg_close_words<1, true>), gw, block, 0, stream, (const unsigned*)acc, (W)nom, d);
+        hipLaunchKernelGGL((k_seg_close_words<2, true>), gw, block, 0, stream, (const unsigned*)nom, (W)B, d);
+        hipLaunchKernelGGL((k_seg_close_words<0, false>), gw, block, 0, stream, (const unsigned*)B, (W)acc, d);
+        hipLaunchKernelGGL((k_seg_close_words<1, false>), gw, block, 0, stream, (const unsigned*)acc, (W)nom, d);
+        hipLaunchKernelGGL((k_seg_close_words<2, false>), gw, block, 0, stream, (const unsigned*)nom,
+                           (W)segmentation, d);
+    } else {
+        hipLaunchKernelGGL(k_seg_dilate_axis<0>, grid, block, 0, stream, A, acc, d);
+        hipLaunchKernelGGL(k_seg_dilate_axis<1>, grid, block, 0, stream, acc, nom, d);
+        hipLaunchKernelGGL(k_seg_dilate_axis<2>, grid, block, 0, stream, nom, B, d);
+        hipLaunchKernelGGL(k_seg_erode_axis<0>, grid, block, 0, stream, B, acc, d);
+        hipLaunchKernelGGL(k_seg_erode_axis<1>, grid, block, 0, stream, acc, nom, d);
+        hipLaunchKernelGGL(k_seg_erode_axis<2>, grid, block, 0, stream, nom, segmentation, d);
+    }
     TSF_TRY(check_launch(W));
     if (sc.owned) {
         e = hipStreamSynchronize(stream);
