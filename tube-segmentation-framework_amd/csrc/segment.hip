@@ -52,10 +52,9 @@ __global__ void k_seg_seed(const uint8_t* __restrict__ cl, const float* __restri
 
 // grow, first half: the verdicts of the candidates and their nominations
 template <class S>
-__global__ void k_seg_decide(const uint8_t* __restrict__ cur, Field<S> F, Dims d, uint8_t* __restrict__ acc,
-                             uint8_t* nom, int32_t* grew) {
-    FOR_VOXEL(d)
-    if (cur[i] != 2) return;
+__device__ __forceinline__ void decide_voxel(const uint8_t* __restrict__ cur, const Field<S>& F, const Dims& d, int x,
+                                             int y, int z, i64 i, uint8_t* __restrict__ acc, uint8_t* nom,
+                                             int32_t* grew) {
     const float wx = F.mag(i);
     bool cont = false;
     for (int a = -1; a < 2; a++)
@@ -93,6 +92,13 @@ __global__ void k_seg_decide(const uint8_t* __restrict__ cur, Field<S> F, Dims d
         acc[i] = 1;
         *grew = 1;
     }
+}
+
+template <class S>
+__global__ void k_seg_decide(const uint8_t* __restrict__ cur, Field<S> F, Dims d, uint8_t* __restrict__ acc,
+                             uint8_t* nom, int32_t* grew) {
+    FOR_VOXEL(d)
+    if (cur[i] == 2) decide_voxel<S>(cur, F, d, x, y, z, i, acc, nom, grew);
 }
 
 // grow, second half: accepted > nominated > dropped; clears the marks.
