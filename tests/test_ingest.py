@@ -110,6 +110,15 @@ def test_to_float_region_vs_oracle(tsf, orc, dt):
         tsf.to_float_region(tsf.DeviceArray.from_host(v), off, out, mn * 0.5, mx * 0.5)
         exp = orc.to_float(v[oz:oz + sz, oy:oy + sy, ox:ox + sx], mn * 0.5, mx * 0.5)
         assert np.array_equal(out.download().view(np.uint32), exp.view(np.uint32)), off
+    # X and the x offset multiples of 4: four voxels per thread
+    v4 = _random(rng, (21, 19, 72), dt)
+    for off, size in [((0, 0, 0), (72, 19, 21)), ((4, 3, 2), (64, 16, 19)), ((1, 0, 0), (68, 19, 21))]:
+        ox, oy, oz = off
+        sx, sy, sz = size
+        out = tsf.DeviceArray((sz, sy, sx), np.float32)
+        tsf.to_float_region(tsf.DeviceArray.from_host(v4), off, out, mn * 0.5, mx * 0.5)
+        exp = orc.to_float(v4[oz:oz + sz, oy:oy + sy, ox:ox + sx], mn * 0.5, mx * 0.5)
+        assert np.array_equal(out.download().view(np.uint32), exp.view(np.uint32)), off
     out = tsf.DeviceArray((37, 45, 70), np.float32)
     with pytest.raises(tsf.TSFError):
         tsf.to_float_region(tsf.DeviceArray.from_host(v), (1, 0, 0), out, 0.0, 1.0)

@@ -257,6 +257,49 @@ __global__ void __launch_bounds__(256) k_to_float_region(const T* __restrict__ i
     out[dout.at(x, y, z)] = (v - mn) / (mx - mn);
 }
 
+// Four voxels per thread (one 4-element load, one float4 store) when the
+// region's x offset, both row lengths and both pointers are 4-element
+// aligned.
+template <class T> struct Vec4;
+template <> struct Vec4<uint8_t> { typedef uchar4 V; };
+template <> struct Vec4<int8_t> { typedef char4 V; };
+template <> struct Vec4<uint16_t> { typedef ushort4 V; };
+template <> struct Vec4<int16_t> { typedef short4 V; };
+template <> struct Vec4<float> { typedef float4 V; };
+
+__device__ __forceinline__ float to_unit(float v, float mn, float mx) {
+    v = v > mx ? mx : v;
+    v = v < mn ? mn : v;
+    return (v - mn) / (mx - mn);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_to_float_region4(const T* __restrict__ in, Dims din, int ox, int oy, int oz,
+                                                          float* __restrict__ out, Dims dout, float mn, float mx) {
+    const int x = 4 * (blockIdx.x * 256 + threadIdx.x), y = blockIdx.y, z = blockIdx.z;
+    if (x >= dout.X) return;
+    const typename Vec4<T>::V v = *reinterpret_cast<const typename Vec4<T>::V*>(in + din.at(x + ox, y + oy, z + oz));
+    *reinterpret_cast<float4*>(out + dout.at(x, y, z)) =
+        make_float4(to_unit((float)v.x, mn, mx), to_unit((float)v.y, mn, mx), to_unit((float)v.z, mn, mx),
+                    to_unit((float)v.w, mn, mx));
+}
+
+template <class T>
+void launch_to_float_region(const void* in, Dims di, tsf_dims off, float* out, Dims dout, float mn, float mx,
+                            hipStream_t s) {
+    const bool vec = off.x % 4 == 0 && di.X % 4 == 0 && dout.X % 4 == 0 &&
+                     (uintptr_t)in % (4 * sizeof(T)) == 0 && (uintptr_t)out % 16 == 0;
+    if (vec) {
+        const dim3 g((dout.X / 4 + 255) / 256, dout.Y, dout.Z);
+        hipLaunchKernelGGL(k_to_float_region4<T>, g, 256, 0, s, (const T*)in, di, off.x, off.y, off.z, out, dout, mn,
+                           mx);
+    } else {
+        const dim3 g((dout.X + 255) / 256, dout.Y, dout.Z);
+        hipLaunchKernelGGL(k_to_float_region<T>, g, 256, 0, s, (const T*)in, di, off.x, off.y, off.z, out, dout, mn,
+                           mx);
+    }
+}
+
 bool valid_type(int t) { return t >= 0 && t <= 4; }
 
 // first index from `from` stepping by `dir` (while `i > stop` going down,
@@ -419,14 +462,12 @@ tsf_status tsf_to_float_region(const void* in, int32_t in_type, tsf_dims in_dims
         return set_error(TSF_ERR_DIMENSIONS, "%s: region outside the volume", W);
     if (!(maximum != minimum)) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: maximum == minimum", W);
     const Dims di = D(in_dims), dout = D(out_dims);
-    const dim3 g((dout.X + 255) / 256, dout.Y, dout.Z);
-    const int ox = offset.x, oy = offset.y, oz = offset.z;
     switch (in_type) {
-        case 0: hipLaunchKernelGGL(k_to_float_region<uint8_t>, g, 256, 0, stream, (const uint8_t*)in, di, ox, oy, oz, out, dout, minimum, maximum); break;
-        case 1: hipLaunchKernelGGL(k_to_float_region<int8_t>, g, 256, 0, stream, (const int8_t*)in, di, ox, oy, oz, out, dout, minimum, maximum); break;
-        case 2: hipLaunchKernelGGL(k_to_float_region<uint16_t>, g, 256, 0, stream, (const uint16_t*)in, di, ox, oy, oz, out, dout, minimum, maximum); break;
-        case 3: hipLaunchKernelGGL(k_to_float_region<int16_t>, g, 256, 0, stream, (const int16_t*)in, di, ox, oy, oz, out, dout, minimum, maximum); break;
-        default: hipLaunchKernelGGL(k_to_float_region<float>, g, 256, 0, stream, (const float*)in, di, ox, oy, oz, out, dout, minimum, maximum); break;
+        case 0: launch_to_float_region<uint8_t>(in, di, offset, out, dout, minimum, maximum, stream); break;
+        case 1: launch_to_float_region<int8_t>(in, di, offset, out, dout, minimum, maximum, stream); break;
+        case 2: launch_to_float_region<uint16_t>(in, di, offset, out, dout, minimum, maximum, stream); break;
+        case 3: launch_to_float_region<int16_t>(in, di, offset, out, dout, minimum, maximum, stream); break;
+        default: launch_to_float_region<float>(in, di, offset, out, dout, minimum, maximum, stream); break;
     }
     return check_launch(W);
 }
