@@ -713,6 +713,54 @@ static void run_tests(const std::string& dsdir) {
                 return std::string();
             });
         }
+    // runRidgeTraversal on an ideal straight tube along x: an inward radial
+    // field plus a constant axial component eps (the host Hessian divides by
+    // |F|), so |F| = eps is minimal on the axis (every axis voxel is a start
+    // point), 1 - |F| is largest there, and nothing varies along x (e1 = x).
+    // The walk must follow the axis from x = 2 to X - 2 (it stops after
+    // stepping past x = 3 or X - 3) and mark nothing else.
+    run("runRidgeTraversal_straight_tube_known_answer", [&] {
+        const int TX = 40, TY = 21, TZ = 21;
+        const size_t tn = (size_t)TX * TY * TZ;
+        std::vector<float> fx(tn), fy(tn), fz(tn), tdf(tn), rad(tn, 2.0f);
+        for (int z = 0; z < TZ; z++)
+            for (int y = 0; y < TY; y++)
+                for (int x = 0; x < TX; x++) {
+                    const size_t i = (size_t)x + (size_t)TX * ((size_t)y + (size_t)TY * z);
+                    const float dy = (float)y - 10.0f, dz = (float)z - 10.0f, r = std::sqrt(dy * dy + dz * dz);
+                    const float m = r <= 6.0f ? r / 6.0f : std::max(0.0f, 2.0f - r / 6.0f);
+                    fx[i] = 0.05f;
+                    fy[i] = r > 0 ? -m * dy / r : 0.0f;
+                    fz[i] = r > 0 ? -m * dz / r : 0.0f;
+                    tdf[i] = std::max(0.0f, 1.0f - r / 3.0f);
+                }
+        TubeSegmentation TS;
+        TS.Fx = fx.data();
+        TS.Fy = fy.data();
+        TS.Fz = fz.data();
+        TS.TDF = tdf.data();
+        TS.radius = rad.data();
+        paramList q = initParameters();
+        setParameter(q, "tdf-high", "0.5");
+        setParameter(q, "min-distance", "5");
+        char* c = runRidgeTraversal(TS, I3(TX, TY, TZ), q, std::stack<CenterlinePoint>());
+        releaseEigenMirror(TS);
+        std::string err;
+        size_t on = 0;
+        for (int z = 0; z < TZ; z++)
+            for (int y = 0; y < TY; y++)
+                for (int x = 0; x < TX; x++) {
+                    const bool want = y == 10 && z == 10 && x >= 2 && x <= TX - 2;
+                    const bool got = c[(size_t)x + (size_t)TX * ((size_t)y + (size_t)TY * z)] == 1;
+                    on += got;
+                    if (want != got && err.empty())
+                        err = "voxel (" + std::to_string(x) + "," + std::to_string(y) + "," + std::to_string(z) +
+                              ") " + (got ? "marked" : "missing");
+                }
+        delete[] c;
+        std::printf("  straight tube: %zu centerline voxels (expected %d)\n", on, TX - 3);
+        return err;
+    });
     // threshold cropping on a box: x [10, 31) -> [10, 30) (the last slice is
     // the exclusive end), size 20; y [5, 27) -> [5, 26) grown alternately to
     // [4, 28), size 24; z [7, 20) from the end -> [7, 19), size 12.  Without
