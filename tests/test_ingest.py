@@ -122,3 +122,26 @@ def test_to_float_region_vs_oracle(tsf, orc, dt):
     out = tsf.DeviceArray((37, 45, 70), np.float32)
     with pytest.raises(tsf.TSFError):
         tsf.to_float_region(tsf.DeviceArray.from_host(v), (1, 0, 0), out, 0.0, 1.0)
+
+
+def test_ingest_argument_errors_on_host():
+    """The ingest entry points reject bad arguments before any device work:
+    NULL buffers, unknown element types and methods, lung cropping of a
+    float volume, a region outside the volume, empty dimensions."""
+    import ctypes
+    import tsf
+    from tsf._capi import Dims, lib
+    L = lib()
+    d = Dims(8, 8, 8)
+    fake = ctypes.c_void_p(256)  # never dereferenced: every call below fails its checks first
+    f = ctypes.c_float()
+    assert L.tsf_volume_limits(None, 0, d, ctypes.byref(f), ctypes.byref(f), None) == 1
+    assert L.tsf_volume_limits(fake, 9, d, ctypes.byref(f), ctypes.byref(f), None) == 1
+    assert L.tsf_volume_limits(fake, 0, Dims(0, 8, 8), ctypes.byref(f), ctypes.byref(f), None) == 2
+    assert L.tsf_crop_scan_lines(fake, 0, d, 7, 0.0, fake, fake, fake, None) == 1
+    assert L.tsf_crop_scan_lines(fake, 4, d, tsf.CROP_LUNG, 0.0, fake, fake, fake, None) == 1
+    assert L.tsf_crop_scan_lines(fake, 0, d, tsf.CROP_THRESHOLD, 0.0, None, fake, fake, None) == 1
+    assert L.tsf_to_float_region(fake, 0, d, Dims(1, 0, 0), fake, d, 0.0, 1.0, None) == 2
+    assert L.tsf_to_float_region(fake, 0, d, Dims(0, 0, 0), fake, d, 1.0, 1.0, None) == 1
+    assert L.tsf_crop_region(None, None, None, d, 0, 0, None, None) == 1
+    assert "NULL" in L.tsf_last_error().decode()
