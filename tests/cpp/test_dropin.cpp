@@ -713,6 +713,39 @@ static void run_tests(const std::string& dsdir) {
                 return std::string();
             });
         }
+    // the output switches of runCircleFittingAndNewCenterlineAlg
+    // (tube-segmentation.cpp:957-976): tdf-only stops after the TDF,
+    // no-segmentation after the centerline, sphere-segmentation draws balls of
+    // the radius around every centerline voxel (so it covers the centerline)
+    run("run_output_switches", [&] {
+        const size_t n = (size_t)X * Y * Z;
+        paramList p = vascusynth();
+        setParameter(p, "tdf-only", "true");
+        TSFOutput* out = run(dsdir + "/noisy.mhd", p, "");
+        const bool ok1 = out->hasTDF() && !out->hasCenterlineVoxels() && !out->hasSegmentation();
+        delete out;
+        if (!ok1) return std::string("tdf-only");
+        p = vascusynth();
+        setParameter(p, "no-segmentation", "true");
+        out = run(dsdir + "/noisy.mhd", p, "");
+        const bool ok2 = out->hasCenterlineVoxels() && !out->hasSegmentation();
+        delete out;
+        if (!ok2) return std::string("no-segmentation");
+        p = vascusynth();
+        setParameter(p, "sphere-segmentation", "true");
+        out = run(dsdir + "/noisy.mhd", p, "");
+        const char* c = out->getCenterlineVoxels();
+        const char* g = out->getSegmentation();
+        size_t cl = 0, seg = 0, uncovered = 0;
+        for (size_t i = 0; i < n; i++) {
+            cl += c[i] == 1;
+            seg += g[i] == 1;
+            uncovered += c[i] == 1 && g[i] != 1;
+        }
+        delete out;
+        std::printf("  sphere segmentation: %zu voxels around %zu centerline voxels\n", seg, cl);
+        return uncovered == 0 && seg > cl ? std::string() : std::string("sphere segmentation misses the centerline");
+    });
     // runRidgeTraversal on an ideal straight tube along x: an inward radial
     // field plus a constant axial component eps (the host Hessian divides by
     // |F|), so |F| = eps is minimal on the axis (every axis voxel is a start

@@ -40,7 +40,7 @@ def test_dropin_run_ingest_and_output(tsf):
     r = _run("run", os.path.join(ROOT, "tests", "golden", "dataset_1"), timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 12
+    assert r.stdout.count("PASS") == 13
 
 
 @pytest.mark.gpu
