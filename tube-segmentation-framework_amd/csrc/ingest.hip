@@ -92,6 +92,39 @@ __global__ void __launch_bounds__(LIM_THREADS) k_limits(const T* __restrict__ in
     block_reduce(mn, mx, part + blockIdx.x);
 }
 
+// 16 bytes per thread and iteration (n a multiple of the vector length,
+// `in` 16-byte aligned): the elements of each vector in index order.
+template <class T>
+__global__ void __launch_bounds__(LIM_THREADS) k_limits16(const uint4* __restrict__ in, i64 nvec,
+                                                          Limits* __restrict__ part) {
+    constexpr int E = 16 / sizeof(T);
+    const i64 stride = (i64)gridDim.x * LIM_THREADS;
+    Cand mn{0.0f, -1}, mx{0.0f, -1};
+    for (i64 c = (i64)blockIdx.x * LIM_THREADS + threadIdx.x; c < nvec; c += stride) {
+        const uint4 u = in[c];
+        T e[E];
+        __builtin_memcpy(e, &u, 16);
+#pragma unroll
+        for (int k = 0; k < E; k++) {
+            const float v = (float)e[k];
+            if (v != v) continue;
+            const i64 i = c * E + k;
+            if (mn.i < 0 || v < mn.v) mn = Cand{v, i};
+            if (mx.i < 0 || v > mx.v) mx = Cand{v, i};
+        }
+    }
+    block_reduce(mn, mx, part + blockIdx.x);
+}
+
+template <class T>
+void launch_limits(const void* in, i64 n, unsigned g, Limits* part, hipStream_t s) {
+    constexpr int E = 16 / sizeof(T);
+    if (n % E == 0 && (uintptr_t)in % 16 == 0)
+        hipLaunchKernelGGL(k_limits16<T>, g, LIM_THREADS, 0, s, (const uint4*)in, n / E, part);
+    else
+        hipLaunchKernelGGL(k_limits<T>, g, LIM_THREADS, 0, s, (const T*)in, n, part);
+}
+
 __global__ void __launch_bounds__(LIM_THREADS) k_limits_final(const Limits* __restrict__ part, int n,
                                                               Limits* __restrict__ out) {
     Cand mn{0.0f, -1}, mx{0.0f, -1};
@@ -355,11 +388,11 @@ tsf_status tsf_volume_limits(const void* in, int32_t in_type, tsf_dims dims, flo
     Limits* part = (Limits*)sc.ptr;
     const unsigned g = (unsigned)blocks;
     switch (in_type) {
-        case 0: hipLaunchKernelGGL(k_limits<uint8_t>, g, LIM_THREADS, 0, stream, (const uint8_t*)in, n, part); break;
-        case 1: hipLaunchKernelGGL(k_limits<int8_t>, g, LIM_THREADS, 0, stream, (const int8_t*)in, n, part); break;
-        case 2: hipLaunchKernelGGL(k_limits<uint16_t>, g, LIM_THREADS, 0, stream, (const uint16_t*)in, n, part); break;
-        case 3: hipLaunchKernelGGL(k_limits<int16_t>, g, LIM_THREADS, 0, stream, (const int16_t*)in, n, part); break;
-        default: hipLaunchKernelGGL(k_limits<float>, g, LIM_THREADS, 0, stream, (const float*)in, n, part); break;
+        case 0: launch_limits<uint8_t>(in, n, g, part, stream); break;
+        case 1: launch_limits<int8_t>(in, n, g, part, stream); break;
+        case 2: launch_limits<uint16_t>(in, n, g, part, stream); break;
+        case 3: launch_limits<int16_t>(in, n, g, part, stream); break;
+        default: launch_limits<float>(in, n, g, part, stream); break;
     }
     TSF_TRY(check_launch(W));
     hipLaunchKernelGGL(k_limits_final, 1, LIM_THREADS, 0, stream, part, (int)blocks, part + blocks);
