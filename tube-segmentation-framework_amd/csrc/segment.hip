@@ -158,12 +158,15 @@ __device__ __forceinline__ unsigned is1(unsigned w) {
     for (int k = 0; k < 32; k += 8) o |= (((w >> k) & 0xffu) == 1u ? 1u : 0u) << k;
     return o;
 }
+constexpr int CLOSE_ZC = 8;  // planes per workgroup of the word passes
 template <int AXIS, bool DILATE>
 __global__ void __launch_bounds__(256) k_seg_close_words(const unsigned* __restrict__ in, unsigned* __restrict__ out,
                                                          Dims d) {
     const int XW = d.X >> 2;
-    const int xw = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y, z = blockIdx.z;
+    const int xw = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
     if (xw >= XW || y >= d.Y) return;
+    const int z1 = min(d.Z, (int)(blockIdx.z + 1) * CLOSE_ZC);
+    for (int z = blockIdx.z * CLOSE_ZC; z < z1; z++) {
     const i64 i = (i64)xw + (i64)XW * ((i64)y + (i64)d.Y * z);
     const unsigned c = is1(in[i]);
     unsigned r;
@@ -180,6 +183,7 @@ __global__ void __launch_bounds__(256) k_seg_close_words(const unsigned* __restr
         r = DILATE ? (c | lo | hi) : (c & lo & hi);
     }
     out[i] = r;
+    }
 }
 
 // sphereSegmentation: a ball of the voxel's radius (|offset| < r) around
@@ -267,7 +271,7 @@ tsf_status tsf_segment_inverse_gradient(const uint8_t* centerlines, const float*
     // closing: A -> acc -> nom -> B (dilation), B -> acc -> nom -> segmentation (erosion)
     if (d.X % 4 == 0 && ((uintptr_t)segmentation & 3u) == 0) {
         typedef unsigned* W;
-        const dim3 gw((d.X / 4 + 63) / 64, (d.Y + 3) / 4, d.Z);
+        const dim3 gw((d.X / 4 + 63) / 64, (d.Y + 3) / 4, (d.Z + CLOSE_ZC - 1) / CLOSE_ZC);
         hipLaunchKernelGGL((k_seg_close_words<0, true>), gw, block, 0, stream, (const unsigned*)A, (W)acc, d);
         hipLaunchKernelGGL((k_seg_close_words<1, true>), gw, block, 0, stream, (const unsigned*)acc, (W)nom, d);
         hipLaunchKernelGGL((k_seg_close_words<2, true>), gw, block, 0, stream, (const unsigned*)nom, (W)B, d);
