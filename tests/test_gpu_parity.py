@@ -328,7 +328,8 @@ def test_combine_candidates(tsf, orc):
 
 # ------------------------------------------------------------------ compaction
 @pytest.mark.parametrize("shape,p", [((64, 64, 64), 0.05), ((20, 12, 28), 0.3), ((5, 9, 7), 0.5), ((100, 100, 100), 0.01),
-                                     ((17, 33, 65), 1.0), ((16, 16, 16), 0.0), ((40, 24, 200), 0.002)])
+                                     ((17, 33, 65), 1.0), ((16, 16, 16), 0.0), ((40, 24, 200), 0.002),
+                                     ((48, 80, 32), 0.1), ((3, 2, 4112), 0.3), ((2, 3, 4200), 0.3)])
 def test_hp_compact_bit_exact(tsf, orc, shape, p):
     mask = (rng(12).uniform(0, 1, shape) < p).astype(np.uint8)
     n = int(mask.sum())
