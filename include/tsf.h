@@ -184,6 +184,28 @@ tsf_status tsf_gvf_euler(const tsf_field* f, const tsf_field* out, tsf_dims dims
                          int32_t fuse, void* workspace, size_t workspace_bytes, hipStream_t stream);
 size_t tsf_gvf_euler_workspace_size(tsf_dims dims, int32_t storage);
 
+/* The precision form of tsf_gvf_euler (GVF3DIteration, kernels_no_3d_write.cl:
+ * 1746-1787, the recurrence of runFastGVF, gradientVectorFlow.cpp:1007-1172):
+ * the iterate streams through device memory in f's 16-bit storage (F16
+ * holding value * scale, or BF16) and every step computes in float, two
+ * Jacobi steps per launch.  Each launch rounds its result to 16 bits
+ * stochastically: the magnitude rounds up with probability equal to the
+ * discarded fraction (to 2^-10), the random bits a hash of the voxel index
+ * and seed ^ (launch * 0x9e3779b9), so results are reproducible.  Round to
+ * nearest would drop every update smaller than half a 16-bit ulp and stall
+ * the iteration short of its fixed point.  The last launch writes `out` as
+ * F32 planes with w = |v| (1 where 0): the final iterate at full precision,
+ * as the Hessian of the TDF needs it (see tsf_tdf_circle_mixed).  An odd
+ * count starts with one step rounded to nearest.  The update is the
+ * reference's v + mu lap(v) - (v - f)|f|^2 arranged as
+ * v (1 - 6 mu - |f|^2) + f |f|^2 + mu sum(neighbours), with the reference's
+ * mirror rule at the faces.  f: F16 or BF16 planes (x, y, z; 4-byte
+ * aligned), out: F32 with w.  Needs X even and >= 64, Y >= 32, Z >= 4.
+ * Workspace: tsf_gvf_euler_sr_workspace_size (two 16-bit iterates). */
+tsf_status tsf_gvf_euler_sr(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
+                            uint32_t seed, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims);
+
 /* runFMGGVF (gradientVectorFlow.cpp:729-1005): per component, `iterations`
  * defect-correction steps r = fmgResidual(f); e = fullMultigrid(r); f += e
  * with red-black Gauss-Seidel smoothing (GVFgaussSeidel/GVFgaussSeidel2,
@@ -256,6 +278,22 @@ tsf_status tsf_tdf_spline(const tsf_field* field, tsf_dims dims, float rmin, flo
                           hipStream_t stream);
 size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage);
 
+/* The circleFittingTDF of tsf_tdf_circle in the precision form's mixed mode:
+ * the Hessian and its eigenvectors from the F32 `field` itself, the 32 x n_r
+ * trilinear samples from a copy of its x, y, z planes rounded to 16-bit
+ * texels (sample_storage F16 or BF16, holding value * sample_scale, a power
+ * of two; 2^14 suits |v| <= 1 fields).  The eigenvectors are the part of the
+ * TDF that amplifies storage rounding (the Hessian is a central difference);
+ * the samples are averages and are not sensitive to it, and the 8-byte
+ * texels keep the sampling at the 16-bit path's speed.  field: F32, with w
+ * when rmax >= 4.  Workspace: tsf_tdf_circle_mixed_workspace_size (8 bytes
+ * per voxel).  Replaces runCircleFittingTDF (tubeDetectionFilters.cpp:57-73)
+ * where the field comes from the precision form's GVF. */
+tsf_status tsf_tdf_circle_mixed(const tsf_field* field, int32_t sample_storage, float sample_scale, tsf_dims dims,
+                                float rmin, float rmax, float rstep, float* tdf, float* radius, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream);
+size_t tsf_tdf_circle_mixed_workspace_size(tsf_dims dims);
+
 /* combine (kernels_no_3d_write.cl:1087-1098): where tdf_large < tdf_small,
  * take the small-scale TDF and radius.  In place on the large buffers. */
 tsf_status tsf_combine(const float* tdf_small, const float* radius_small, float* tdf_large, float* radius_large,
@@ -289,6 +327,7 @@ tsf_status tsf_candidates(const float* tdf, uint8_t* mask, int64_t count, float 
 tsf_status tsf_hp_compact(const uint8_t* mask, tsf_dims dims, int32_t* positions, int64_t capacity,
                           int64_t* count, void* workspace, size_t workspace_bytes, hipStream_t stream);
 size_t tsf_hp_compact_workspace_size(tsf_dims dims);
+
 
 /* findCandidateCenterpoints2 (kernels_no_3d_write.cl:1019-1084; the image
  * path kernels.cl:1309-1390 differs only in the cap of maxD): for each listed

@@ -142,6 +142,65 @@ def tdf_circle(F, rmin, rmax, rstep, prec="f32"):
     return tdf, rad
 
 
+def tdf_circle_split(FH, FS, rmin, rmax, rstep, prec="f32"):
+    """circleFittingTDF with the Hessian from FH and the samples from FS."""
+    FH = np.ascontiguousarray(FH, _dt(prec))
+    FS = np.ascontiguousarray(FS, _dt(prec))
+    assert FH.shape == FS.shape
+    X, Y, Z = _dims(FH)
+    tdf = np.empty(FH.shape[1:], _dt(prec))
+    rad = np.empty(FH.shape[1:], _dt(prec))
+    _fn("tdf_circle_split", prec)(_p(FH), _p(FS), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax),
+                                  ctypes.c_float(rstep), _p(tdf), _p(rad))
+    return tdf, rad
+
+
+def gvf_euler_sr16(f16, storage, scale, mu, iters, seed=0):
+    """The precision form's GVF (tsf_gvf_euler_sr): f16 = (3, Z, Y, X) uint16
+    planes (F16 holding value * scale when storage == 1, BF16 bits when 2);
+    returns the (4, Z, Y, X) float32 output (x, y, z, w)."""
+    f16 = np.ascontiguousarray(f16, np.uint16)
+    X, Y, Z = _dims(f16)
+    out = np.empty((4,) + f16.shape[1:], np.float32)
+    lib().orc_gvf_euler_sr16(_p(f16), int(storage), ctypes.c_float(scale), ctypes.c_float(mu), int(iters),
+                             ctypes.c_uint(seed), X, Y, Z, _p(out))
+    return out
+
+
+def half_from_float(a):
+    a = np.ascontiguousarray(a, np.float32)
+    h = np.empty(a.shape, np.uint16)
+    lib().orc_half_from_float(_p(a), _p(h), ctypes.c_longlong(a.size))
+    return h
+
+
+def half_to_float(h):
+    h = np.ascontiguousarray(h, np.uint16)
+    a = np.empty(h.shape, np.float32)
+    lib().orc_half_to_float(_p(h), _p(a), ctypes.c_longlong(h.size))
+    return a
+
+
+def s16_sr(a, r, storage):
+    """Stochastic rounding of float32 values (storage units) with random words r."""
+    a = np.ascontiguousarray(a, np.float32)
+    r = np.ascontiguousarray(r, np.uint32)
+    h = np.empty(a.shape, np.uint16)
+    lib().orc_s16_sr(_p(a), _p(r), int(storage), _p(h), ctypes.c_longlong(a.size))
+    return h
+
+
+def sr_hash(i, seed):
+    lib().orc_sr_hash.restype = ctypes.c_uint
+    return int(lib().orc_sr_hash(ctypes.c_uint(i), ctypes.c_uint(seed)))
+
+
+def f16_round(a, scale=2.0 ** 14):
+    """Values as 16-bit texels holding value * scale store them (round to nearest even)."""
+    a = np.asarray(a, np.float32)
+    return ((a * np.float32(scale)).astype(np.float16).astype(np.float32) / np.float32(scale)).astype(np.float32)
+
+
 def tdf_spline(F, rmin, rmax, rstep, arms=12, min_average_mag=0.1, prec="f32"):
     """splineTDF: (tdf, radius) of a 4-plane field (w read by the normalised Hessian)."""
     F = np.ascontiguousarray(F, _dt(prec))

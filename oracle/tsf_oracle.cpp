@@ -619,6 +619,200 @@ static void eigen3(const T A[3][3], T V[3][3], T d[3]) {
     ql_implicit3(V, d, e);
 }
 
+// ---------------------------------------------------------------- precision form: 16-bit state, stochastic rounding
+// Restatement of tsf_gvf_euler_sr (csrc/gvf_sr.hip), the precision form of
+// GVF3DIteration (kernels_no_3d_write.cl:1746-1787): the iterate is stored
+// as 16-bit numbers (F16 holding value * scale, or BF16) and every launch of
+// two Jacobi steps rounds its result stochastically with random bits from a
+// hash of the voxel index and the launch; the last launch writes floats and
+// w = |v|.  The arithmetic is in storage units and in the device kernel's
+// order, fma included, so the device result is reproduced bit for bit.
+
+static inline uint32_t f_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static inline float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// IEEE binary16 <-> binary32 (round to nearest even), by bit manipulation.
+static inline float half_to_float(uint16_t h) {
+    const uint32_t s = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0) {
+        const float v = std::ldexp((float)m, -24);  // subnormal (or zero): exact
+        return s ? -v : v;
+    }
+    if (e == 31) return bits_f(s | 0x7f800000u | (m << 13));
+    return bits_f(s | ((e + 112u) << 23) | (m << 13));
+}
+static inline uint16_t float_to_half_rne(float f) {
+    const uint32_t x = f_bits(f), s = (x >> 16) & 0x8000u, a = x & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(s | 0x7e00u);   // NaN
+    if (a >= 0x477ff000u) return (uint16_t)(s | 0x7c00u);  // rounds to infinity (>= 65520)
+    if (a < 0x38800000u) {                                  // below 2^-14: a half subnormal
+        const float q = bits_f(a) * 16777216.0f;             // in units of 2^-24, exact
+        return (uint16_t)(s | (uint32_t)std::nearbyint(q));  // ties to even (default rounding mode)
+    }
+    uint32_t r = a - 0x38000000u;  // rebias the exponent by 127 - 15
+    r = r + 0xfffu + ((r >> 13) & 1u);
+    return (uint16_t)(s | (r >> 13));
+}
+// binary32 -> binary16 rounding toward zero (v_cvt_pkrtz_f16_f32): the
+// magnitude truncated, finite values beyond the range to the largest half.
+static inline uint16_t float_to_half_rtz(float f) {
+    const uint32_t x = f_bits(f), s = (x >> 16) & 0x8000u, a = x & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(s | 0x7e00u);   // NaN
+    if (a == 0x7f800000u) return (uint16_t)(s | 0x7c00u);  // infinity
+    if (a >= 0x47800000u) return (uint16_t)(s | 0x7bffu);  // >= 65536: largest finite
+    if (a < 0x38800000u) {                                  // below 2^-14: a half subnormal
+        const float q = bits_f(a) * 16777216.0f;             // in units of 2^-24, exact
+        return (uint16_t)(s | (uint32_t)q);                  // truncated
+    }
+    return (uint16_t)(s | ((a - 0x38000000u) >> 13));
+}
+static inline uint16_t float_to_bf16_rne(float f) {
+    const uint32_t x = f_bits(f);
+    return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+// storage 1 = F16, 2 = BF16 (tsf_storage)
+static inline float s16_get(uint16_t h, int st) { return st == 1 ? half_to_float(h) : bits_f((uint32_t)h << 16); }
+static inline uint16_t s16_rne(float v, int st) { return st == 1 ? float_to_half_rne(v) : float_to_bf16_rne(v); }
+// r in [0, 1024) scaled to the bits below the 16-bit mantissa (13 for F16,
+// 16 for BF16), added to the float's bits, then rounded toward zero: the
+// magnitude rounds up with probability within 2^-10 of (discarded bits) /
+// 2^bits.
+static inline uint16_t s16_sr(float v, uint32_t r, int st) {
+    const uint32_t b = f_bits(v);
+    if (st == 1) return float_to_half_rtz(bits_f(b + (r << 3)));
+    return (uint16_t)((b + (r << 6)) >> 16);
+}
+// The lowbias32 mixer with 24-bit multiplies (the device's full-rate
+// v_mul_u32_u24), of index ^ seed; bits 0-9, 10-19, 20-29 are the random
+// numbers of components x, y, z.
+static inline uint32_t umul24(uint32_t a, uint32_t b) {
+    return (uint32_t)((uint64_t)(a & 0xffffffu) * (uint64_t)(b & 0xffffffu));
+}
+static inline uint32_t sr_hash(uint32_t i, uint32_t seed) {
+    uint32_t h = i ^ seed;
+    h ^= h >> 16;
+    h = umul24(h, 0x7feb35u);
+    h ^= h >> 15;
+    h = umul24(h, 0x846ca7u);
+    h ^= h >> 16;
+    return h;
+}
+
+// One step of a two-step launch at interior voxel (x,y,z):
+// v' = v (1 - 6 mu - |f|^2) + f |f|^2 + mu (sum of the six neighbours), in
+// the device's order: s = (u(x-1) + u(x+1)) + (u(y-1) + u(y+1)),
+// A = fma(mu, fma(zmul, u(z-1), s), fma(u, c1, g)), out = fma(mz, u(z+1), A)
+// with c1 = (1 - 6 mu) - |f|^2, g = f |f|^2.  The mirror rule in z for the
+// second step: on plane 1 the z-1 term is left out (zmul 0) and the z+1
+// term doubled (mz 2 mu), on plane Z-2 the z-1 term is doubled and the z+1
+// term left out.
+static inline float sr_stage(const float* u, const float* c1, const float* g, Dims d, int x, int y, int z, float mu,
+                             bool zmirror) {
+    const i64 i = d.at(x, y, z), sx = 1, sy = d.X, sz = (i64)d.X * d.Y;
+    const float s = (u[i - sx] + u[i + sx]) + (u[i - sy] + u[i + sy]);
+    float zmul = 1.0f, mz = mu;
+    if (zmirror && z == 1) {
+        zmul = 0.0f;
+        mz = 2.0f * mu;
+    } else if (zmirror && z == d.Z - 2) {
+        zmul = 2.0f;
+        mz = 0.0f;
+    }
+    const float A = std::fma(mu, std::fma(zmul, u[i - sz], s), std::fma(u[i], c1[i], g[i]));
+    return std::fma(mz, u[i + sz], A);
+}
+
+static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, int iters, uint32_t seed, Dims d,
+                           float* out4) {
+    const i64 N = d.n();
+    const float inv = 1.0f / scale, m2s = 1.0f / (scale * scale);
+    const float c0 = 1.0f - 6.0f * mu;
+    std::vector<float> f(3 * N), c1(N), g(3 * N);
+    for (i64 i = 0; i < 3 * N; i++) f[i] = s16_get(f16[i], st);
+    for (i64 i = 0; i < N; i++) {
+        const float m2 = std::fma(f[2 * N + i], f[2 * N + i], std::fma(f[N + i], f[N + i], f[i] * f[i])) * m2s;
+        c1[i] = c0 - m2;
+        for (int c = 0; c < 3; c++) g[c * N + i] = f[c * N + i] * m2;
+    }
+    std::vector<uint16_t> v16(f16, f16 + 3 * N);
+    std::vector<float> u(3 * N), v1(3 * N), v2(3 * N);
+    auto faces = [&](std::vector<float>& a) {
+        for (int c = 0; c < 3; c++)
+            for (int z = 0; z < d.Z; z++)
+                for (int y = 0; y < d.Y; y++)
+                    for (int x = 0; x < d.X; x++)
+                        if (x == 0 || y == 0 || z == 0 || x == d.X - 1 || y == d.Y - 1 || z == d.Z - 1)
+                            a[c * N + d.at(x, y, z)] = a[c * N + d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z))];
+    };
+    int j = 0;
+    if (iters % 2) {
+        // one step in value space, the reference's order, round to nearest
+        std::vector<float> fv(3 * N), vv(3 * N), o(3 * N);
+        for (i64 i = 0; i < 3 * N; i++) {
+            fv[i] = f[i] * inv;
+            vv[i] = s16_get(v16[i], st) * inv;
+        }
+        gvf_step<float>(fv.data(), vv.data(), o.data(), d, mu);
+        for (i64 i = 0; i < 3 * N; i++) v16[i] = s16_rne(o[i] * scale, st);
+        j = 1;
+    }
+    const int pairs = iters / 2;
+    for (int k = 0; k < pairs; k++, j++) {
+        for (i64 i = 0; i < 3 * N; i++) u[i] = s16_get(v16[i], st);
+        for (int c = 0; c < 3; c++)
+            for (int z = 1; z < d.Z - 1; z++)
+                for (int y = 1; y < d.Y - 1; y++)
+                    for (int x = 1; x < d.X - 1; x++)
+                        v1[c * N + d.at(x, y, z)] =
+                            sr_stage(u.data() + c * N, c1.data(), g.data() + c * N, d, x, y, z, mu, false);
+        faces(v1);
+        for (int c = 0; c < 3; c++)
+            for (int z = 1; z < d.Z - 1; z++)
+                for (int y = 1; y < d.Y - 1; y++)
+                    for (int x = 1; x < d.X - 1; x++)
+                        v2[c * N + d.at(x, y, z)] =
+                            sr_stage(v1.data() + c * N, c1.data(), g.data() + c * N, d, x, y, z, mu, true);
+        faces(v2);
+        if (k == pairs - 1) {
+            for (i64 i = 0; i < N; i++) {
+                const float x = v2[i] * inv, y = v2[N + i] * inv, z = v2[2 * N + i] * inv;
+                const float l = std::sqrt(x * x + y * y + z * z);
+                out4[i] = x;
+                out4[N + i] = y;
+                out4[2 * N + i] = z;
+                out4[3 * N + i] = l > 0.0f ? l : 1.0f;
+            }
+            return;
+        }
+        // every voxel rounded with its own random bits (the x faces hold their
+        // mirror's value here); the y and z faces then copy their mirror's
+        // stored value
+        const uint32_t sj = seed ^ ((uint32_t)j * 0x9e3779b9u);
+        for (i64 i = 0; i < N; i++) {
+            const uint32_t h = sr_hash((uint32_t)i, sj);
+            const uint32_t r[3] = {h & 0x3ffu, (h >> 10) & 0x3ffu, (h >> 20) & 0x3ffu};
+            for (int c = 0; c < 3; c++) v16[c * N + i] = s16_sr(v2[c * N + i], r[c], st);
+        }
+        for (int c = 0; c < 3; c++)
+            for (int z = 0; z < d.Z; z++)
+                for (int y = 0; y < d.Y; y++)
+                    for (int x = 0; x < d.X; x++)
+                        if (y == 0 || z == 0 || y == d.Y - 1 || z == d.Z - 1)
+                            v16[c * N + d.at(x, y, z)] = v16[c * N + d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z))];
+    }
+    // no two-step launch: the output is the 16-bit iterate (f, or the single step)
+    for (i64 i = 0; i < N; i++) {
+        const float x = s16_get(v16[i], st) * inv, y = s16_get(v16[N + i], st) * inv;
+        const float z = s16_get(v16[2 * N + i], st) * inv;
+        const float l = std::sqrt(x * x + y * y + z * z);
+        out4[i] = x;
+        out4[N + i] = y;
+        out4[2 * N + i] = z;
+        out4[3 * N + i] = l > 0.0f ? l : 1.0f;
+    }
+}
+
 // ---------------------------------------------------------------- Hessian (A8)
 // gradient / gradientNormalized on the vector field (kernels_no_3d_write.cl:90-200)
 // and the matrix layout of circleFittingTDF (1522-1526).  With normalize, each
@@ -703,9 +897,12 @@ static inline void trilinear3(const T* F, Dims d, T u, T v, T w, T out[3]) {
     }
 }
 
-// circleFittingTDF, kernels_no_3d_write.cl:1503-1572.
+// circleFittingTDF, kernels_no_3d_write.cl:1503-1572, with the Hessian taken
+// from FH and the trilinear samples from FS.  FH == FS is the reference's
+// filter; FS = FH rounded to 16 bits is the precision form's mixed mode
+// (the Hessian at full precision, the samples from 16-bit texels).
 template <typename T>
-static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, T* tdf, T* rad) {
+static void tdf_circle_split(const T* FH, const T* F, Dims d, float rMin, float rMax, float rStep, T* tdf, T* rad) {
     make_tables();
     const bool normalize = !(rMax < 4);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -713,7 +910,7 @@ static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, 
         for (int y = 0; y < d.Y; y++)
             for (int x = 0; x < d.X; x++) {
                 T H[3][3], V[3][3], ev[3];
-                hessian_at(F, d, x, y, z, normalize, H);
+                hessian_at(FH, d, x, y, z, normalize, H);
                 eigen3(H, V, ev);
                 const T e2[3] = {V[0][1], V[1][1], V[2][1]};
                 const T e3[3] = {V[0][2], V[1][2], V[2][2]};
@@ -741,6 +938,11 @@ static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, 
                 tdf[i] = maxSum;
                 rad[i] = maxRadius;
             }
+}
+
+template <typename T>
+static void tdf_circle(const T* F, Dims d, float rMin, float rMax, float rStep, T* tdf, T* rad) {
+    tdf_circle_split(F, F, d, rMin, rMax, rStep, tdf, rad);
 }
 
 // splineTDF, kernels_no_3d_write.cl:1574-1667 (enqueued by runSplineTDF,
@@ -1408,6 +1610,10 @@ static void chain_gradient(const T* I, Dims d, T sigma, float Fmax, int sign, T 
                                        T* tdf, T* rad) {                                                     \
         tdf_circle<T>(F, DIMS, rmin, rmax, rstep, tdf, rad);                                                 \
     }                                                                                                        \
+    extern "C" void orc_tdf_circle_split_##S(const T* FH, const T* FS, int X, int Y, int Z, float rmin,       \
+                                             float rmax, float rstep, T* tdf, T* rad) {                      \
+        tdf_circle_split<T>(FH, FS, DIMS, rmin, rmax, rstep, tdf, rad);                                      \
+    }                                                                                                        \
     extern "C" void orc_tdf_spline_##S(const T* F, int X, int Y, int Z, float rmin, float rmax, float rstep, \
                                        int arms, float min_avg, T* tdf, T* rad) {                            \
         tdf_spline<T>(F, DIMS, rmin, rmax, rstep, arms, min_avg, tdf, rad);                                  \
@@ -1485,6 +1691,20 @@ extern "C" void orc_blur_mask(float sigma, float* mask, int* maskSize) { blur_ma
 extern "C" void orc_sn16_round(float* a, long long n) {
     for (long long i = 0; i < n; i++) a[i] = sn16_round(a[i]);
 }
+extern "C" void orc_gvf_euler_sr16(const uint16_t* f16, int storage, float scale, float mu, int iters,
+                                   unsigned seed, int X, int Y, int Z, float* out4) {
+    gvf_euler_sr16(f16, storage, scale, mu, iters, seed, Dims{X, Y, Z}, out4);
+}
+extern "C" void orc_half_from_float(const float* a, uint16_t* h, long long n) {
+    for (long long i = 0; i < n; i++) h[i] = float_to_half_rne(a[i]);
+}
+extern "C" void orc_half_to_float(const uint16_t* h, float* a, long long n) {
+    for (long long i = 0; i < n; i++) a[i] = half_to_float(h[i]);
+}
+extern "C" void orc_s16_sr(const float* a, const unsigned* r, int storage, uint16_t* h, long long n) {
+    for (long long i = 0; i < n; i++) h[i] = s16_sr(a[i], r[i], storage);
+}
+extern "C" unsigned orc_sr_hash(unsigned i, unsigned seed) { return sr_hash(i, seed); }
 extern "C" void orc_sn16_store(const float* a, int16_t* s, long long n) {
     for (long long i = 0; i < n; i++) s[i] = sn16_store(a[i]);
 }

@@ -285,6 +285,31 @@ def test_tdf_circle(tsf, orc, rmin, rmax, rstep):
     assert np.array_equal(gr, er)
 
 
+def _bf16_round(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("storage,rmin,rmax", [(1, 1.0, 3.0), (1, 2.5, 4.0), (2, 0.5, 3.0), (2, 2.5, 5.0)])
+def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
+    """Mixed mode: Hessian from the F32 field, samples from its 16-bit
+    rounding -- the oracle's split filter on the same two fields."""
+    shape = (32, 32, 32)
+    V = _gvf_field(orc, shape, 11)
+    fld = tsf.Field.from_host(V)
+    t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    tsf.tdf_circle_mixed(fld, rmin, rmax, 0.5, t, r, sample_storage=storage,
+                         sample_scale=2.0 ** 14 if storage == 1 else 1.0)
+    gt, gr = t.download(), r.download()
+    Vs = V.copy()
+    Vs[:3] = orc.f16_round(V[:3]) if storage == 1 else _bf16_round(V[:3])
+    et, er = orc.tdf_circle_split(V, Vs, rmin, rmax, 0.5, "f32")
+    nbad, rel = close(gt, et)
+    print(f"mixed TDF: exact fraction {np.mean(gt == et):.6f}, radius agree {np.mean(gr == er):.6f}")
+    assert nbad == 0, rel
+    assert np.array_equal(gr, er)
+
+
 @pytest.mark.parametrize("storage", [1, 2])
 def test_tdf_reduced_precision_field(tsf, orc, storage):
     shape = (32, 32, 32)

@@ -373,6 +373,50 @@ def test_tdf_uniform_field(orc, sign):
     assert np.all(tdf == tdf[4, 4, 4])
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_tdf_split_reduces_to_tdf(orc, prec):
+    """Hessian and samples from one field: the reference's filter, bit for bit."""
+    F = np.concatenate([rng(30).uniform(-1, 1, (3, 10, 11, 12)), rng(31).uniform(0.5, 1.5, (1, 10, 11, 12))])
+    for rmin, rmax in ((1.0, 3.0), (2.5, 4.0)):
+        t0, r0 = orc.tdf_circle(F, rmin, rmax, 0.5, prec)
+        t1, r1 = orc.tdf_circle_split(F, F, rmin, rmax, 0.5, prec)
+        assert np.array_equal(t0, t1) and np.array_equal(r0, r1)
+
+
+def test_tdf_split_hessian_and_samples_from_their_fields(orc):
+    """Hessian field: linear, so its Jacobian is a known non-axis-aligned
+    symmetric matrix at every interior voxel; sample field: uniform c.  Then
+    S(r) = -c . (e3 sum c_j + e2 sum s_j) / 32 with e2, e3 of the linear
+    field's Hessian at every radius (a uniform field samples to c), so the
+    response is that value at rmin, or 0; taking the Hessian from the sample
+    field (zero Jacobian, identity basis) or the samples from the Hessian
+    field gives other values."""
+    shape = (12, 12, 12)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    A = np.array([[0.30, 0.10, -0.05], [0.10, -0.20, 0.15], [-0.05, 0.15, 0.05]])  # symmetric Jacobian
+    FH = np.zeros((4,) + shape)
+    P = np.stack([x, y, z])  # d F_i / d x_j = A[i, j]
+    for i in range(3):
+        FH[i] = sum(A[i, j] * P[j] for j in range(3))
+    FH[3] = 1
+    c, s = orc.circle_tables()
+    for cvec in (np.array([0.3, -0.4, 0.2]), np.array([-0.3, 0.4, -0.2])):
+        FS = np.zeros((4,) + shape)
+        FS[:3] = cvec[:, None, None, None]
+        FS[3] = 1
+        tdf, rad = orc.tdf_circle_split(FH, FS, 1.0, 3.0, 0.5, "f64")
+        lam, vec = orc.hessian_eigen(FH, 0, prec="f64")
+        e2, e3 = vec[3:6, 6, 6, 6], vec[6:9, 6, 6, 6]
+        assert np.allclose(np.abs(np.sort(np.abs(lam[:, 6, 6, 6]))), np.sort(np.abs(np.linalg.eigvalsh(A))))
+        S = -(cvec @ e3 * c.astype(np.float64).sum() + cvec @ e2 * s.astype(np.float64).sum()) / 32
+        if S > 0:
+            assert np.isclose(tdf[6, 6, 6], S, rtol=1e-12) and rad[6, 6, 6] == 1.0
+        else:
+            assert tdf[6, 6, 6] == 0 and rad[6, 6, 6] == 0
+        ident = -(cvec[2] * c.astype(np.float64).sum() + cvec[1] * s.astype(np.float64).sum()) / 32
+        assert not np.isclose(max(S, 0.0), max(ident, 0.0)) or S <= 0 and ident <= 0
+
+
 def test_tdf_linear_inward_field(orc):
     Z, Y, X = 9, 24, 24
     z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
@@ -879,3 +923,121 @@ def test_spline_tdf_off_axis_and_invalid_cases(orc):
     u[3] = 0.3
     t, r = orc.tdf_spline(u, 1.0, 5.0, 0.5)
     assert (t == 0).all() and (r == 0).all()
+
+
+# ------------------------------------------------------------------ precision form: stochastic rounding
+def test_half_conversion_matches_numpy(orc):
+    """float -> half (round to nearest even) and back against numpy's IEEE
+    conversions: random values over the whole half range (normal,
+    subnormal, ties, overflow) and the special values."""
+    g = rng(40)
+    a = np.concatenate([
+        g.uniform(-1, 1, 20000).astype(np.float32) * np.float32(2.0) ** g.integers(-30, 17, 20000).astype(np.float32),
+        np.array([0.0, -0.0, 65504, 65519.99, 65520, -65520, 1e-8, 2 ** -24, 2 ** -25, 3 * 2 ** -26, 2 ** -14,
+                  np.inf, -np.inf, 1 + 2 ** -11, 1 + 3 * 2 ** -11], np.float32)])
+    h = orc.half_from_float(a)
+    with np.errstate(over="ignore"):
+        ref = a.astype(np.float16).view(np.uint16)
+    assert np.array_equal(h, ref)
+    codes = np.arange(65536, dtype=np.uint32).astype(np.uint16)
+    back = orc.half_to_float(codes)
+    fin = np.isfinite(codes.view(np.float16))
+    assert np.array_equal(back[fin], codes.view(np.float16)[fin].astype(np.float32))
+
+
+@pytest.mark.parametrize("storage", [1, 2])
+def test_stochastic_rounding_is_one_of_the_neighbours_and_unbiased(orc, storage):
+    g = rng(41)
+    v = g.uniform(-3, 3, 4000).astype(np.float32) * np.float32(2 ** 14 if storage == 1 else 1)
+    dec = orc.half_to_float if storage == 1 else (lambda h: (h.astype(np.uint32) << 16).view(np.float32))
+    # r = 0 truncates towards zero; the magnitude rounds up when r makes the
+    # discarded bits carry (r = 1023 for all but the last 2^-10 of an ulp)
+    lo = dec(orc.s16_sr(v, np.zeros(v.size, np.uint32), storage)).astype(np.float64)
+    up = lo
+    if storage == 1:  # the next half away from zero
+        up = np.nextafter(lo.astype(np.float16), (np.sign(v) * np.inf).astype(np.float16))
+    else:  # the next bfloat16 away from zero
+        b = (lo.astype(np.float32).view(np.uint32) + np.uint32(1 << 16)).view(np.float32)
+        up = np.where(lo == v, lo, b)
+    hi = np.where(lo == v, lo, up.astype(np.float64))
+    assert np.all(np.abs(lo) <= np.abs(v)) and np.all(np.abs(hi) >= np.abs(v))
+    # the mean over many random numbers is the value to within 2^-10 ulp
+    # (the resolution of the 10 random bits) plus sampling error
+    K = 2000
+    r = g.integers(0, 1024, (K, v.size), dtype=np.uint32)
+    got = dec(orc.s16_sr(np.broadcast_to(v, (K, v.size)).copy(), r, storage)).astype(np.float64)
+    mean = got.mean(0)
+    ulp = np.abs(hi - lo)
+    assert np.all(np.abs(mean - v) <= 4 * ulp / np.sqrt(K) + ulp / 1024 + 1e-30)
+    # each result is one of the two neighbours
+    assert np.all((got == lo) | (got == hi))
+
+
+def test_sr_hash_bits_are_uniform(orc):
+    h = np.array([orc.sr_hash(i, 12345) for i in range(20000)], np.uint64)
+    for b in range(32):
+        frac = float(((h >> np.uint64(b)) & np.uint64(1)).mean())
+        assert abs(frac - 0.5) < 0.02, (b, frac)
+    assert orc.sr_hash(7, 1) != orc.sr_hash(7, 2) and orc.sr_hash(7, 1) == orc.sr_hash(7, 1)
+
+
+def _f16_field(shape, seed):
+    F = _field_from_phantom(shape, seed)
+    return F, (F[:3] * np.float32(2 ** 14)).astype(np.float16).view(np.uint16)
+
+
+def _field_from_phantom(shape, seed):
+    import oracle as o
+    from tsf import synth
+    vol, _ = synth.tube_volume(shape, 4, 1.0, 3.0, seed, noise=0.05)
+    return o.vector_field(o.blur(vol, 0.5, "f32"), 0.2, 1, "f32")
+
+
+@pytest.mark.parametrize("shape", [(6, 48, 64), (4, 50, 66)])
+def test_gvf_sr_one_launch_is_two_float_steps(orc, shape):
+    """Two iterations are one launch with a float output and no 16-bit
+    rounding of the iterate: equal to two FP64 Euler steps of the reference
+    recurrence (mirror rule included) on the stored f, to float rounding."""
+    F, f16 = _f16_field(shape, 42)
+    out = orc.gvf_euler_sr16(f16, 1, 2.0 ** 14, 0.05, 2)
+    f = np.concatenate([f16.view(np.float16).astype(np.float64) / 2 ** 14, np.ones((1,) + shape)])
+    ref = orc.gvf_euler(f, 0.05, 2, "f64")
+    err = np.abs(out[:3] - ref[:3]).max() / np.abs(ref[:3]).max()
+    assert err < 2e-6, err
+    assert np.allclose(out[3], np.sqrt((out[:3].astype(np.float64) ** 2).sum(0)), rtol=1e-6)
+
+
+def test_gvf_sr_mu_zero_keeps_f(orc):
+    """mu = 0 with v0 = f: every update of an interior voxel is
+    v (1 - |f|^2) + f |f|^2 = f up to float rounding, so after six steps
+    (two stochastic roundings) each voxel is within one 16-bit ulp of f;
+    face voxels take their mirror's (0 -> 2, N-1 -> N-3)."""
+    shape = (5, 48, 64)
+    F, f16 = _f16_field(shape, 43)
+    out = orc.gvf_euler_sr16(f16, 1, 2.0 ** 14, 0.0, 6)
+    f = f16.view(np.float16).astype(np.float32) / np.float32(2 ** 14)
+    dev = np.abs(out[:3, 1:-1, 1:-1, 1:-1] - f[:, 1:-1, 1:-1, 1:-1])
+    assert np.all(dev <= np.abs(f[:, 1:-1, 1:-1, 1:-1]) * 2.0 ** -10 + 2.0 ** -38)
+    rm = lambda n: np.r_[2, np.arange(1, n - 1), n - 3]
+    Z, Y, X = shape
+    assert np.array_equal(out[:3], out[:3][:, rm(Z)][:, :, rm(Y)][:, :, :, rm(X)])
+
+
+def test_gvf_sr_converges_where_round_to_nearest_stalls(orc):
+    """Against the FP64 iterate after 200 steps, stochastic rounding of the
+    stored iterate stays within a few 16-bit ulps in L2 while round to
+    nearest (tsf_gvf_euler's F16 storage, every launch rounded to nearest)
+    stalls further away; both start from the same stored f."""
+    shape = (16, 48, 64)
+    F, f16 = _f16_field(shape, 44)
+    f = np.concatenate([f16.view(np.float16).astype(np.float64) / 2 ** 14, np.ones((1,) + shape)])
+    ref = orc.gvf_euler(f, 0.05, 200, "f64")[:3]
+    sr = orc.gvf_euler_sr16(f16, 1, 2.0 ** 14, 0.05, 200, seed=5)[:3]
+    f32 = f[:3].astype(np.float32)
+    rne = f32.copy()
+    for _ in range(100):  # two float steps, then round to nearest
+        rne = orc.f16_round(orc.gvf_step(f32, orc.gvf_step(f32, rne, 0.05), 0.05))
+    e_sr = np.linalg.norm(sr - ref) / np.linalg.norm(ref)
+    e_rne = np.linalg.norm(rne - ref) / np.linalg.norm(ref)
+    print(f"200 steps: stochastic {e_sr:.2e}, nearest {e_rne:.2e}")
+    assert e_sr < 3e-4 and e_sr < 0.6 * e_rne

@@ -37,11 +37,13 @@ __global__ void k_pack_16(const uint16_t* __restrict__ x, const uint16_t* __rest
 // circleFittingTDF, kernels_no_3d_write.cl:1503-1572, at one voxel: the
 // (normalised) Hessian, its eigenvectors e2, e3, and the radius search with
 // float accumulation and a first-maximum break.
-template <bool NORM, class A>
+template <bool NORM, class A, class AH = A>
 __device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int y, int z, float rmin, float rmax,
-                                          float rstep, const CircleTables& tab, float& maxSum, float& maxRadius) {
+                                          float rstep, const CircleTables& tab, float& maxSum, float& maxRadius,
+                                          const AH* TH = nullptr) {
     float H[3][3], V[3][3], ev[3];
-    hessian_tex<NORM>(T, d, x, y, z, H);
+    if (TH) hessian_tex<NORM>(*TH, d, x, y, z, H);
+    else hessian_tex<NORM>(T, d, x, y, z, H);
     eigen3(H, V, ev);
     const float e2x = V[0][1], e2y = V[1][1], e2z = V[2][1];
     const float e3x = V[0][2], e3y = V[1][2], e3z = V[2][2];
@@ -191,9 +193,11 @@ __device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx,
 // a wave's lanes stay busy and waves with no survivor retire.  The radius of
 // round k is the same float accumulation for every voxel, so each voxel's
 // arithmetic is that of the sequential loop.  T is the staged tile (its
-// fill is the caller's, followed by a barrier).
-template <bool NORM, int BX, int BY, int BZ, class A>
-__device__ __forceinline__ void tdf_brick(const A& T, Dims d, int bx0, int by0, int bz0, float rmin, float rmax,
+// fill is the caller's, followed by a barrier).  The Hessian reads TH, which
+// is T itself except in the mixed mode (F32 planes for the Hessian, 16-bit
+// texels for the samples).
+template <bool NORM, int BX, int BY, int BZ, class A, class AH>
+__device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int bx0, int by0, int bz0, float rmin, float rmax,
                                           float rstep, const CircleTables& tab, float* __restrict__ tdf,
                                           float* __restrict__ rad) {
     constexpr int NT = BX * BY * BZ, NW = NT / 64;
@@ -210,7 +214,7 @@ __device__ __forceinline__ void tdf_brick(const A& T, Dims d, int bx0, int by0, 
         const int x = bx0 + threadIdx.x, y = by0 + threadIdx.y, z = bz0 + threadIdx.z;
         if (x < d.X && y < d.Y && z < d.Z) {
             float H[3][3], V[3][3], ev[3];
-            hessian_tex<NORM>(T, d, x, y, z, H);
+            hessian_tex<NORM>(TH, d, x, y, z, H);
             eigen3(H, V, ev);
             e2[0] = V[0][1]; e2[1] = V[1][1]; e2[2] = V[2][1];
             e3[0] = V[0][2]; e3[1] = V[1][2]; e3[2] = V[2][2];
@@ -316,7 +320,70 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>:
         tile[c] = tex[d.at(gx, gy, gz)];
     }
     __syncthreads();
-    tdf_brick<NORM, BX, BY, BZ>(T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+    tdf_brick<NORM, BX, BY, BZ>(T, T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
+// Mixed mode with texels gathered from device memory (halos too wide for
+// the LDS tile).
+template <class S, bool NORM>
+__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_mixed(Tex<S> T, PlanesF32 H, Dims d, float rmin, float rmax,
+                                                             float rstep, CircleTables tab, float* __restrict__ tdf,
+                                                             float* __restrict__ rad) {
+    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
+    const int z = blockIdx.z * CBZ + threadIdx.z;
+    if (x >= d.X || y >= d.Y || z >= d.Z) return;
+    float t, r;
+    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r, &H);
+    const i64 i = d.at(x, y, z);
+    tdf[i] = t;
+    rad[i] = r;
+}
+
+// Mixed mode: the samples from a tile of 16-bit texels packed from an F32
+// field, the Hessian from the F32 planes themselves (device memory), so the
+// eigenvectors see the field at full precision.
+template <class S, bool NORM, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_mixed(const typename Texel<S>::P* __restrict__ tex, float inv, PlanesF32 H, Dims d, float rmin,
+                    float rmax, float rstep, CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad, int h) {
+    constexpr int NT = BX * BY * BZ;
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    LdsTex<S> T{tile, inv, bx0 - h, by0 - h, bz0 - h, BX + 2 * h, BY + 2 * h, BZ + 2 * h};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    const int cells = T.tx * T.ty * T.tz;
+    for (int c = tid; c < cells; c += NT) {
+        const int lx = c % T.tx, rr = c / T.tx, ly = rr % T.ty, lz = rr / T.ty;
+        const int gx = clampi(T.ox + lx, 0, d.X - 1), gy = clampi(T.oy + ly, 0, d.Y - 1);
+        const int gz = clampi(T.oz + lz, 0, d.Z - 1);
+        tile[c] = tex[d.at(gx, gy, gz)];
+    }
+    __syncthreads();
+    tdf_brick<NORM, BX, BY, BZ>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
+// F32 planes -> 16-bit texels holding value * scale (x, y, z; w = 0, the
+// mixed mode's Hessian reads w from its F32 plane).
+// (round to nearest even)
+template <class S>
+__device__ __forceinline__ unsigned bits16(float v);
+template <>
+__device__ __forceinline__ unsigned bits16<F16>(float v) {
+    return __half_as_ushort(__float2half_rn(v));
+}
+template <>
+__device__ __forceinline__ unsigned bits16<BF16>(float v) {
+    return __bfloat16_as_ushort(__float2bfloat16(v));
+}
+template <class S>
+__global__ void k_pack_f32_to_16(const float* __restrict__ x, const float* __restrict__ y,
+                                 const float* __restrict__ z, float scale, uint2* __restrict__ out, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x)
+        out[i] = make_uint2(bits16<S>(x[i] * scale) | (bits16<S>(y[i] * scale) << 16), bits16<S>(z[i] * scale));
 }
 
 // F32 fields straight from their planes: the tile holds x, y and z as three
@@ -346,7 +413,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         tile[2 * cells + c] = fz[g];
     }
     __syncthreads();
-    tdf_brick<NORM, BX, BY, BZ>(T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+    tdf_brick<NORM, BX, BY, BZ>(T, T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
 static unsigned grid1d(i64 n) {
@@ -407,11 +474,82 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     hipLaunchKernelGGL((k_tdf<S, NORM>), grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf, radius);
 }
 
+template <class S, bool NORM>
+static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float inv, Dims d, float rmin, float rmax,
+                                   float rstep, float* tdf, float* radius, hipStream_t stream, const char* W) {
+    static const CircleTables tab = make_circle_tables();
+    typedef typename Texel<S>::P P;
+    constexpr int BX = 16, BY = 4, BZ = 4;
+    const int h = tdf_halo(rmin, rmax);
+    const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
+    const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
+    const PlanesF32 H{(const float*)f->x, (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr};
+    if (h > 64 || lds + state > 65536) {
+        const dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
+        hipLaunchKernelGGL((k_tdf_mixed<S, NORM>), grid, block, 0, stream, Tex<S>{(const P*)tex, inv}, H, d, rmin,
+                           rmax, rstep, tab, tdf, radius);
+        (void)W;
+        return TSF_OK;
+    }
+    dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
+    hipLaunchKernelGGL((k_tdf_lds_mixed<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, H, d,
+                       rmin, rmax, rstep, tab, tdf, radius, h);
+    return TSF_OK;
+}
+
 }  // namespace tsf
 
 using namespace tsf;
 
 extern "C" {
+
+size_t tsf_tdf_circle_mixed_workspace_size(tsf_dims dims) { return (size_t)D(dims).n() * 8; }
+
+tsf_status tsf_tdf_circle_mixed(const tsf_field* field, int32_t sample_storage, float sample_scale, tsf_dims dims,
+                                float rmin, float rmax, float rstep, float* tdf, float* radius, void* workspace,
+                                size_t workspace_bytes, hipStream_t stream) {
+    const char* W = "tsf_tdf_circle_mixed";
+    const bool norm = !(rmax < 4.0f);
+    TSF_TRY(check_field(field, norm, W));
+    TSF_TRY(check_dims(dims, 1, W));
+    if (field->storage != TSF_STORAGE_F32) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: field must be F32", W);
+    if (sample_storage != TSF_STORAGE_F16 && sample_storage != TSF_STORAGE_BF16)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: sample storage must be F16 or BF16", W);
+    int ex;
+    if (!(sample_scale > 0.0f) || frexpf(sample_scale, &ex) != 0.5f)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: sample scale must be a positive power of two", W);
+    if (!tdf || !radius) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: NULL output", W);
+    if (!(rstep > 0.0f) || !__builtin_isfinite(rmin) || !__builtin_isfinite(rmax))
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: radius range must be finite with rstep > 0", W);
+    if ((double)(rmax - rmin) / rstep > 100000.0)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: more than 1e5 radii", W);
+    Dims d = D(dims);
+    const i64 n = d.n();
+    Scratch sc;
+    TSF_TRY(sc.get(workspace, workspace_bytes, tsf_tdf_circle_mixed_workspace_size(dims), W));
+    const float inv = 1.0f / sample_scale;
+    if (sample_storage == TSF_STORAGE_F16) {
+        hipLaunchKernelGGL(k_pack_f32_to_16<F16>, grid1d(n), 256, 0, stream, (const float*)field->x,
+                           (const float*)field->y, (const float*)field->z, sample_scale, (uint2*)sc.ptr, n);
+        const tsf_status st =
+            norm ? launch_tdf_mixed<F16, true>(field, sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream, W)
+                 : launch_tdf_mixed<F16, false>(field, sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream, W);
+        if (st != TSF_OK) return st;
+    } else {
+        hipLaunchKernelGGL(k_pack_f32_to_16<BF16>, grid1d(n), 256, 0, stream, (const float*)field->x,
+                           (const float*)field->y, (const float*)field->z, sample_scale, (uint2*)sc.ptr, n);
+        const tsf_status st =
+            norm ? launch_tdf_mixed<BF16, true>(field, sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream, W)
+                 : launch_tdf_mixed<BF16, false>(field, sc.ptr, inv, d, rmin, rmax, rstep, tdf, radius, stream, W);
+        if (st != TSF_OK) return st;
+    }
+    TSF_TRY(check_launch(W));
+    if (sc.owned) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+    }
+    return TSF_OK;
+}
 
 size_t tsf_tdf_circle_workspace_size(tsf_dims dims, int32_t storage) {
     if (storage < 0 || storage > 3) return 0;

@@ -1152,6 +1152,22 @@ tsf_status gvf_states_f32(const float* const f[3], const float* v0, float* const
     return TSF_OK;
 }
 
+// One Euler step from f (F16 or BF16 planes) into the 16-bit planes out[0..2]
+// (round to nearest), for the stochastic-rounding entry point's odd counts.
+tsf_status gvf_single_step16(const tsf_field* f, void* const out[3], Dims d, float mu, hipStream_t stream) {
+    const float inv = 1.0f / f->scale, sc = f->scale;
+    if (f->storage == TSF_STORAGE_F16) {
+        Planes3<F16> F{(const __half*)f->x, (const __half*)f->y, (const __half*)f->z};
+        Out4<F16> O{(__half*)out[0], (__half*)out[1], (__half*)out[2], nullptr};
+        launch_step<F16, 1>(false, F, F, O, d, mu, inv, inv, sc, stream);
+    } else {
+        Planes3<BF16> F{(const __hip_bfloat16*)f->x, (const __hip_bfloat16*)f->y, (const __hip_bfloat16*)f->z};
+        Out4<BF16> O{(__hip_bfloat16*)out[0], (__hip_bfloat16*)out[1], (__hip_bfloat16*)out[2], nullptr};
+        launch_step<BF16, 1>(false, F, F, O, d, mu, inv, inv, sc, stream);
+    }
+    return check_launch("gvf_single_step16");
+}
+
 }  // namespace tsf
 
 using namespace tsf;

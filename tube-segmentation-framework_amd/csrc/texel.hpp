@@ -142,6 +142,23 @@ struct LdsSoaTex {
     }
 };
 
+// PlanesF32: an F32 field's x, y, z and w planes in device memory as a texel
+// source (w = 1 when absent): the Hessian's source when the samples come from
+// 16-bit texels of the same field.
+struct PlanesF32 {
+    const float* __restrict__ x;
+    const float* __restrict__ y;
+    const float* __restrict__ z;
+    const float* __restrict__ w;
+    __device__ __forceinline__ i64 row(const Dims& d, int yy, int zz) const {
+        return (i64)d.X * ((i64)yy + (i64)d.Y * zz);
+    }
+    __device__ __forceinline__ float4 at(i64 r, int xx) const {
+        const i64 i = r + xx;
+        return make_float4(x[i], y[i], z[i], w ? w[i] : 1.0f);
+    }
+};
+
 template <bool NORM, class A>
 __device__ __forceinline__ void hessian_tex(const A& T, const Dims& d, int x, int y, int z, float H[3][3]) {
     const int xm = max(x - 1, 0), xp = min(x + 1, d.X - 1);

@@ -67,6 +67,10 @@ _SIGS = {
     "tsf_tdf_round_unorm16": ([_vp, _i64, _vp], _i32),
     "tsf_hp_compact": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp, _sz, _vp], _i32),
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
+    "tsf_gvf_euler_sr": ([_PF, _PF, Dims, _f32, _i32, ctypes.c_uint32, _vp, _sz, _vp], _i32),
+    "tsf_gvf_euler_sr_workspace_size": ([Dims], _sz),
+    "tsf_tdf_circle_mixed": ([_PF, _i32, _f32, Dims, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
+    "tsf_tdf_circle_mixed_workspace_size": ([Dims], _sz),
     "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
     "tsf_cube_max": ([_vp, _vp, Dims, _i32, _vp, _vp], _i32),
     "tsf_link_centerpoints": ([_vp, _i32, _vp, Dims, _f32, _f32, _vp, _i32, ctypes.POINTER(_i32), _vp, _sz, _vp],
@@ -441,6 +445,32 @@ def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=Non
     w, wb = _ws(workspace)
     _check(lib().tsf_tdf_circle(ctypes.byref(d), _dims(field.shape), rmin, rmax, rstep, tdf.ptr, radius.ptr, w, wb,
                                 _sp(stream)))
+
+
+def gvf_euler_sr_workspace_size(shape):
+    return lib().tsf_gvf_euler_sr_workspace_size(_dims(shape))
+
+
+def gvf_euler_sr(f, out, mu, iterations, seed=0, workspace=None, stream=None):
+    """Precision-form GVF: f F16/BF16, iterate streamed in f's storage with
+    stochastic rounding, out (F32 field with w) the final iterate."""
+    df, do = f.desc(), out.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_gvf_euler_sr(ctypes.byref(df), ctypes.byref(do), _dims(f.shape), mu, iterations, seed, w, wb,
+                                  _sp(stream)))
+
+
+def tdf_circle_mixed_workspace_size(shape):
+    return lib().tsf_tdf_circle_mixed_workspace_size(_dims(shape))
+
+
+def tdf_circle_mixed(field, rmin, rmax, rstep, tdf, radius, sample_storage=STORAGE_F16, sample_scale=2.0 ** 14,
+                     workspace=None, stream=None):
+    """Circle TDF of an F32 field: Hessian from the F32 planes, samples from 16-bit texels."""
+    d = field.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_tdf_circle_mixed(ctypes.byref(d), sample_storage, sample_scale, _dims(field.shape), rmin, rmax,
+                                      rstep, tdf.ptr, radius.ptr, w, wb, _sp(stream)))
 
 
 def tdf_spline(field, rmin, rmax, rstep, tdf, radius, min_average_mag=0.1, workspace=None, stream=None):
