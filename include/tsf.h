@@ -328,6 +328,16 @@ tsf_status tsf_hp_compact(const uint8_t* mask, tsf_dims dims, int32_t* positions
                           int64_t* count, void* workspace, size_t workspace_bytes, hipStream_t stream);
 size_t tsf_hp_compact_workspace_size(tsf_dims dims);
 
+/* tsf_hp_compact without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The cube counts and their scan use a static device array of
+ * 3 MiB + 8 B that each GPU holds until the process ends; volumes whose
+ * largest extent exceeds 1024 are rejected with TSF_ERR_DIMENSIONS.  Same
+ * outputs, count read-back and errors as tsf_hp_compact.  Calls that overlap
+ * on one GPU (two streams or two host threads) share the array and race. */
+tsf_status tsf_hp_compact_noalloc(const uint8_t* mask, tsf_dims dims, int32_t* positions, int64_t capacity,
+                                  int64_t* count, hipStream_t stream);
 
 /* findCandidateCenterpoints2 (kernels_no_3d_write.cl:1019-1084; the image
  * path kernels.cl:1309-1390 differs only in the cap of maxD): for each listed

@@ -80,3 +80,16 @@ def test_workspace_sizes():
     assert tsf.gvf_euler_workspace_size((64, 64, 64), 1) == 3 * 64 ** 3 * 2
     assert tsf.hp_compact_workspace_size((64, 64, 64)) > 0
     assert tsf.gvf_fmg_workspace_size((64, 64, 64), 2) > 5 * 64 ** 3 * 4
+
+
+def test_hp_compact_noalloc_rejects_extent_above_scratch():
+    """The no-allocation compaction refuses volumes whose largest extent
+    exceeds its static scratch (1024) on the host, before any device work."""
+    import tsf
+    L = tsf.lib()
+    cnt = ctypes.c_int64(-1)
+    st = L.tsf_hp_compact_noalloc(ctypes.c_void_p(256), tsf.Dims(1025, 8, 8), None, 0, ctypes.byref(cnt), None)
+    assert st == 2  # TSF_ERR_DIMENSIONS
+    assert b"1024" in L.tsf_last_error()
+    st = L.tsf_hp_compact_noalloc(None, tsf.Dims(8, 8, 8), None, 0, ctypes.byref(cnt), None)
+    assert st == 1  # NULL mask

@@ -12,7 +12,7 @@ from ._capi import (  # noqa: F401
     STORAGE_F32, STORAGE_F16, STORAGE_BF16, STORAGE_SN16, storage_dtype, sn16_encode, sn16_decode,
     blur_gaussian, create_blur_mask, to_float, create_vector_field, gvf_euler, gvf_fmg,
     hessian_eigen, do_eigen_points, tdf_circle, combine, tdf_round_unorm16, candidates, hp_compact,
-    candidate_nms, cube_max, gvf_euler_workspace_size, hp_compact_workspace_size,
+    candidate_nms, cube_max, gvf_euler_workspace_size, hp_compact_workspace_size, hp_compact_noalloc,
     gvf_fmg_workspace_size, tdf_circle_workspace_size, device_count, exported_symbols,
     blur_adjoint, vector_field_adjoint, gvf_checkpoint_count, gvf_euler_checkpointed, gvf_euler_adjoint,
     gvf_euler_adjoint_workspace_size, gvf_finish_adjoint, tdf_adjoint, tdf_adjoint_workspace_size, mem_info, mse,

@@ -71,6 +71,7 @@ _SIGS = {
     "tsf_gvf_euler_sr_workspace_size": ([Dims], _sz),
     "tsf_tdf_circle_mixed": ([_PF, _i32, _f32, Dims, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
     "tsf_tdf_circle_mixed_workspace_size": ([Dims], _sz),
+    "tsf_hp_compact_noalloc": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp], _i32),
     "tsf_candidate_nms": ([_vp, _vp, _PF, Dims, _vp, _i64, _i32, _vp, _vp], _i32),
     "tsf_cube_max": ([_vp, _vp, Dims, _i32, _vp, _vp], _i32),
     "tsf_link_centerpoints": ([_vp, _i32, _vp, Dims, _f32, _f32, _vp, _i32, ctypes.POINTER(_i32), _vp, _sz, _vp],
@@ -506,6 +507,16 @@ def hp_compact(mask, positions=None, workspace=None, stream=None):
     w, wb = _ws(workspace)
     _check(lib().tsf_hp_compact(mask.ptr, _dims(mask.shape), positions.ptr if positions is not None else None, cap,
                                 ctypes.byref(cnt), w, wb, _sp(stream)))
+    return cnt.value
+
+
+def hp_compact_noalloc(mask, positions=None, stream=None):
+    """tsf_hp_compact with the library's static scratch (no device allocation);
+    returns the count, raises on overflow."""
+    cnt = _i64(0)
+    cap = positions.shape[0] if positions is not None else 0
+    _check(lib().tsf_hp_compact_noalloc(mask.ptr, _dims(mask.shape), positions.ptr if positions is not None else None,
+                                        cap, ctypes.byref(cnt), _sp(stream)))
     return cnt.value
 
 
