@@ -4,17 +4,31 @@ Workload (one "step"): the circle-fitting forward of runCircleFittingMethod
 (tube-segmentation.cpp:212-939) on a seeded 512^3 thin-vessel MRA-like volume
 (120 tubes, radius 0.5-3, Rician noise 0.08): small-scale branch (blur 0.5 ->
 vector field -> TDF r = 0.5..3.0) and large-scale branch (blur 0.5 -> vector
-field -> GVF Euler, mu = 0.05, 400 iterations, field stored in FP16 with
-float arithmetic -> TDF r = 2.5..4.0, step 0.5) -> combine.
+field -> GVF Euler, mu = 0.05, 400 iterations -> TDF r = 2.5..4.0, step 0.5)
+-> combine.
+
+The default is the precision form: the large branch's vector field stored
+as SNORM16 (the reference's 16-bit vector format), the GVF iterate streamed
+as F16 with stochastic rounding, two steps per launch (tsf_gvf_euler_sr),
+the last --tail iterations in float with the final iterate F32; both TDFs
+take the Hessian from F32 planes and the circle samples from F16 texels
+(tsf_tdf_circle_mixed); the small branch's vector field is F32.  --form
+plain runs the fields in one storage type throughout (tsf_gvf_euler,
+round-to-nearest 16-bit stores).
 
 Headline value: GVF throughput in Gvoxel-iter/s (voxels x iterations / GVF
-time) of the FP16-stored field.  Also reported: forward ms per step, the same
-GVF in plain FP32 storage (speed-up), the GVF kernel's achieved HBM bandwidth
-against the 8 TB/s peak, the oracle's FP32 GVF on the host CPU, and one
-forward + adjoint step of the differentiable FP32 chain at the same size
-(blur 0.5 -> vector field -> GVF 400 iterations checkpointed every 40 ->
-TDF r = 0.5..4.0, backward to dL/dvolume, dL/dmu, dL/dsigma) with its
-device memory at the backward's peak.
+time, float tail included).  Also reported: forward ms per step; the same
+forward in FP32 with four GVF steps per launch (the fastest FP32 form) and
+with one (the plain FP32 path), and the speed-ups; the roofline of the
+dominant kernel (the two-step 16-bit launch: its average duration from the
+difference of two GVF runs that differ only in the number of such launches,
+timed with HIP events on its stream; its algorithmic bytes, 18 per voxel: v
+and f read, v written, 3 x 2 bytes each); the errors against FP64 measured
+by tests/test_gpu_precision.py (recorded in profiles/); the oracle's FP32
+GVF on the host CPU; one forward + adjoint step of the differentiable FP32
+chain at the same size (blur 0.5 -> vector field -> GVF 400 iterations
+checkpointed every 40 -> TDF r = 0.5..4.0, backward to dL/dvolume, dL/dmu,
+dL/dsigma) with its device memory at the backward's peak.
 
 --fit-steps K runs K gradient-descent steps fitting mu and sigma of that
 chain on the noisy volume to the TDF of the noise-free rendering of the
@@ -36,6 +50,7 @@ sys.path.insert(0, os.path.join(ROOT, "tube-segmentation-framework_amd"))
 import numpy as np  # noqa: E402
 
 PEAK_HBM = 8.0e12  # bytes/s, MI355X HBM3E
+SR_SEED = 7
 
 
 def log(msg):
@@ -50,9 +65,14 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--iters", type=int, default=400)
     ap.add_argument("--mu", type=float, default=0.05)
-    ap.add_argument("--storage", choices=["f16", "bf16", "sn16", "f32"], default="f16",
-                    help="vector-field storage; sn16 is the reference's own 16-bit format")
-    ap.add_argument("--fuse", type=int, default=2, choices=[1, 2, 4], help="GVF steps per launch")
+    ap.add_argument("--form", choices=["precision", "plain"], default="precision",
+                    help="precision: 16-bit GVF iterate with stochastic rounding, float tail, mixed TDF; "
+                         "plain: every field in --storage")
+    ap.add_argument("--storage", choices=["sn16", "f16", "bf16", "f32"], default=None,
+                    help="precision form: storage of the GVF's f (sn16, the default: iterate F16; f16; bf16); "
+                         "plain form: storage of every field (default f16; sn16 is the reference's own format)")
+    ap.add_argument("--tail", type=int, default=32, help="precision form: float iterations at the end of the GVF")
+    ap.add_argument("--fuse", type=int, default=2, choices=[1, 2, 4], help="plain form: GVF steps per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-f32-compare", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -69,6 +89,11 @@ def main():
         print("bench.py measures one GPU: the volume-filter path runs on a single device "
               "(multi-GPU z-slab sharding is designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    precision = args.form == "precision"
+    if args.storage is None:
+        args.storage = "sn16" if precision else "f16"
+    if precision and args.storage == "f32":
+        ap.error("the precision form's f is sn16, f16 or bf16")
 
     import tsf
     from tsf import synth
@@ -78,6 +103,8 @@ def main():
     S = {"f32": tsf.STORAGE_F32, "f16": tsf.STORAGE_F16, "bf16": tsf.STORAGE_BF16,
          "sn16": tsf.STORAGE_SN16}[args.storage]
     scale = 2.0 ** 14 if S == tsf.STORAGE_F16 else 1.0
+    # the precision form's 16-bit texels of the TDF samples
+    TS, TSC = (tsf.STORAGE_BF16, 1.0) if args.storage == "bf16" else (tsf.STORAGE_F16, 2.0 ** 14)
     n = args.size
     shape = (n, n, n)
     N = n ** 3
@@ -92,81 +119,116 @@ def main():
         fit(args, vol, d_vol, shape, stream)
         return
     d_blur = tsf.DeviceArray(shape, np.float32)
-    F_small = tsf.Field(shape, S, scale)
-    F = tsf.Field(shape, S, scale)
-    V = tsf.Field(shape, S, scale)
-    ws = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, S),), np.uint8)
+    F = tsf.Field(shape, S, scale)  # the large branch's vector field (the GVF's f)
     t_s, r_s = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
     t_l, r_l = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
-    tws = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, S),), np.uint8)
     ev = [tsf.Event() for _ in range(4)]
+    if precision:
+        F_small = tsf.Field(shape)
+        V = tsf.Field(shape)
+        ws = tsf.DeviceArray((tsf.gvf_euler_sr_workspace_size(shape, args.tail),), np.uint8)
+        tws = tsf.DeviceArray((tsf.tdf_circle_mixed_workspace_size(shape),), np.uint8)
+    else:
+        F_small = tsf.Field(shape, S, scale)
+        V = tsf.Field(shape, S, scale)
+        ws = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, S),), np.uint8)
+        tws = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, S),), np.uint8)
 
     if args.pmc_workload:
-        # Fixed launches for counter passes: k_gvf_copy_finish on F32 planes is
-        # the calibration copy (reads 3 x 4 B, writes 4 x 4 B per voxel), then
-        # GVF steps in the benchmark storage (and FP32 with one, two and four
-        # steps per launch) and both TDF passes.
+        # Fixed launches for counter passes: k_sr_finish (reads 3 x 4 B, writes
+        # 4 B per voxel) calibrates the memory-side byte counters; then two
+        # GVF runs of the precision form (SNORM16 f: a first launch reading
+        # v0 from f, two body launches, a last one writing F32), the plain
+        # FP32 GVF with four steps per launch, and both mixed TDF passes.
+        Fsn = tsf.Field(shape, tsf.STORAGE_SN16)
         F32c, V32c = tsf.Field(shape), tsf.Field(shape)
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, F32c, 0.2, 1, stream)
-        tsf.gvf_euler(F32c, V32c, args.mu, 0, 1, None, stream)
-        tsf.create_vector_field(d_blur, F, 0.2, 1, stream)
-        tsf.gvf_euler(F, V, args.mu, 4, 1, ws, stream)  # single steps
-        tsf.gvf_euler(F, V, args.mu, 8, 2, ws, stream)  # two steps per launch
-        tsf.gvf_euler(F32c, V32c, args.mu, 2, 1, None, stream)
-        tsf.gvf_euler(F32c, V32c, args.mu, 4, 2, None, stream)
-        tsf.gvf_euler(F32c, V32c, args.mu, 8, 4, None, stream)  # four steps per launch
-        tsf.tdf_circle(F, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
-        tsf.tdf_circle(V, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
+        tsf.create_vector_field(d_blur, Fsn, 0.2, 1, stream)
+        sws = tsf.DeviceArray((tsf.gvf_euler_sr_workspace_size(shape, 4),), np.uint8)
+        tsf.gvf_euler_sr(Fsn, V32c, args.mu, 8, SR_SEED, 4, sws, stream)  # body, then 4 float steps (k_sr_finish)
+        tsf.gvf_euler_sr(Fsn, V32c, args.mu, 8, SR_SEED, 0, sws, stream)
+        tsf.gvf_euler(F32c, V32c, args.mu, 8, 4, None, stream)
+        tsf.tdf_circle_mixed(F32c, 0.5, 3.0, 0.5, t_s, r_s, tsf.STORAGE_F16, 2.0 ** 14, None, stream)
+        tsf.tdf_circle_mixed(V32c, 2.5, 4.0, 0.5, t_l, r_l, tsf.STORAGE_F16, 2.0 ** 14, None, stream)
         stream.synchronize()
         log("pmc workload done")
         return
 
-    def forward(Fs, Fl, Vl, gws, tws, fuse):
+    def gvf(Fl, Vl, gws, iters, tail=None, fuse=None):
+        if precision and fuse is None:
+            tsf.gvf_euler_sr(Fl, Vl, args.mu, iters, SR_SEED, args.tail if tail is None else tail, gws, stream)
+        else:
+            tsf.gvf_euler(Fl, Vl, args.mu, iters, fuse or args.fuse, gws, stream)
+
+    def forward(Fs, Fl, Vl, gws, tws, fuse=None):
+        mixed = precision and fuse is None
         ev[0].record(stream)
         # small-radius branch (radius-min < 2.5)
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, Fs, 0.2, 1, stream)
-        tsf.tdf_circle(Fs, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
+        if mixed:
+            tsf.tdf_circle_mixed(Fs, 0.5, 3.0, 0.5, t_s, r_s, TS, TSC, tws, stream)
+        else:
+            tsf.tdf_circle(Fs, 0.5, 3.0, 0.5, t_s, r_s, tws, stream)
         # large-radius branch
         tsf.blur_gaussian(d_vol, d_blur, 0.5, stream)
         tsf.create_vector_field(d_blur, Fl, 0.2, 1, stream)
         ev[1].record(stream)
-        tsf.gvf_euler(Fl, Vl, args.mu, args.iters, fuse, gws, stream)
+        gvf(Fl, Vl, gws, args.iters, fuse=fuse)
         ev[2].record(stream)
-        tsf.tdf_circle(Vl, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
+        if mixed:
+            tsf.tdf_circle_mixed(Vl, 2.5, 4.0, 0.5, t_l, r_l, TS, TSC, tws, stream)
+        else:
+            tsf.tdf_circle(Vl, 2.5, 4.0, 0.5, t_l, r_l, tws, stream)
         tsf.combine(t_s, r_s, t_l, r_l, stream)
         ev[3].record(stream)
 
-    def step():
-        forward(F_small, F, V, ws, tws, args.fuse)
-
     for i in range(args.warmup):
-        step()
+        forward(F_small, F, V, ws, tws)
         stream.synchronize()
         log(f"warmup {i} done")
-    fwd, gvf = [], []
+    fwd, gv = [], []
     stream.synchronize()
     wall0 = time.time()
     for i in range(args.steps):
-        step()
+        forward(F_small, F, V, ws, tws)
         stream.synchronize()
         fwd.append(ev[0].elapsed_ms(ev[3]))
-        gvf.append(ev[1].elapsed_ms(ev[2]))
-        log(f"step {i}: forward {fwd[-1]:.1f} ms, GVF {gvf[-1]:.1f} ms")
+        gv.append(ev[1].elapsed_ms(ev[2]))
+        log(f"step {i}: forward {fwd[-1]:.1f} ms, GVF {gv[-1]:.1f} ms")
     stream.synchronize()
     wall = time.time() - wall0
-
-    gvf_ms = float(np.median(gvf))
+    gvf_ms = float(np.median(gv))
     fwd_ms = float(np.median(fwd))
     gvox = N * args.iters / (gvf_ms * 1e-3) / 1e9
-    # launches: fours, then at most one two-step and one single (as tsf_gvf_euler schedules them)
-    spl = args.fuse
-    quads = args.iters // 4 if spl == 4 else 0
-    pairs = (args.iters - 4 * quads) // 2 if spl >= 2 else 0
-    launches = max(1, quads + pairs + (args.iters - 4 * quads - 2 * pairs))
-    bytes_per_voxel = 9 * (4 if S == tsf.STORAGE_F32 else 2)  # read v, f; write v (3 components each)
-    launch_ms = gvf_ms / launches
+
+    # the dominant kernel's average launch: two GVF runs that differ only in
+    # K launches of it (each followed by its small face-copy kernel)
+    def gvf_time(iters, reps=3):
+        ts = []
+        for _ in range(reps):
+            ev[1].record(stream)
+            gvf(F, V, ws, iters, tail=0)
+            ev[2].record(stream)
+            stream.synchronize()
+            ts.append(ev[1].elapsed_ms(ev[2]))
+        return float(np.median(ts))
+
+    if precision:
+        K = 50
+        spl = 2
+        launch_ms = (gvf_time(2 * K + 4) - gvf_time(4)) / K
+        kname = {"sn16": "k_gvf_sr2<F16, F16, SN16, false>", "f16": "k_gvf_sr2<F16, F16, F16, false>",
+                 "bf16": "k_gvf_sr2<BF16, BF16, BF16, false>"}[args.storage]
+        bytes_per_voxel = 18  # v and f read, v written: 3 components x 2 bytes each
+    else:
+        spl = args.fuse
+        K = 48 // spl
+        launch_ms = (gvf_time(spl * (K + 2)) - gvf_time(2 * spl)) / K
+        gk = {4: "k_gvf_fused4<{}, false>", 2: "k_gvf_fused2<{}, false>"}.get(spl, "k_gvf_march<{}, 2, false>")
+        kname = gk.format(args.storage.upper())
+        bytes_per_voxel = 9 * (4 if S == tsf.STORAGE_F32 else 2)
     achieved = bytes_per_voxel * N / (launch_ms * 1e-3)
 
     # the stages around the filters, on this forward's TDF, radius and field
@@ -175,17 +237,15 @@ def main():
     if not args.no_downstream:
         downstream = downstream_stages(vol, shape, t_s, r_s, V, stream)
 
-    # the same forward with float storage, for the speed-up of the
-    # reduced-precision form: two steps per launch, and the plain FP32 path
-    # (one step per launch)
+    # the same forward in FP32: four GVF steps per launch (the fastest FP32
+    # form) and one (the plain FP32 path)
     f32 = None
-    if not args.no_f32_compare and S != tsf.STORAGE_F32:
+    if not args.no_f32_compare and not (S == tsf.STORAGE_F32 and not precision):
         Fs32, F32, V32 = tsf.Field(shape), tsf.Field(shape), tsf.Field(shape)
         ws32 = tsf.DeviceArray((tsf.gvf_euler_workspace_size(shape, 0),), np.uint8)
         tws32 = tsf.DeviceArray((tsf.tdf_circle_workspace_size(shape, tsf.STORAGE_F32),), np.uint8)
         res = {}
-        f32_fuse = 4  # four steps per launch: the fastest FP32 form
-        for fz in (f32_fuse, 1):
+        for fz in (4, 1):
             forward(Fs32, F32, V32, ws32, tws32, fz)  # warm
             stream.synchronize()
             tf, tg = [], []
@@ -195,17 +255,17 @@ def main():
                 tf.append(ev[0].elapsed_ms(ev[3]))
                 tg.append(ev[1].elapsed_ms(ev[2]))
             res[fz] = (float(np.median(tf)), float(np.median(tg)))
-        (f32_fwd, f32_ms), (f32_fwd1, f32_1) = res[f32_fuse], res[1]
-        f32 = {"steps_per_launch": f32_fuse, "gvf_ms": round(f32_ms, 3),
-               "gvoxel_iter_per_s": round(N * args.iters / (f32_ms * 1e-3) / 1e9, 2),
-               "forward_ms": round(f32_fwd, 3),
-               "speedup_reduced_vs_f32": round(f32_ms / gvf_ms, 3),
-               "hbm_frac_per_launch": round(36 * N / (f32_ms / (args.iters // f32_fuse) * 1e-3) / PEAK_HBM, 4),
-               "single_step_gvf_ms": round(f32_1, 3),
-               "single_step_gvoxel_iter_per_s": round(N * args.iters / (f32_1 * 1e-3) / 1e9, 2),
-               "single_step_forward_ms": round(f32_fwd1, 3),
-               "speedup_reduced_vs_f32_single_step": round(f32_1 / gvf_ms, 3),
-               "forward_speedup_reduced_vs_plain_f32": round(f32_fwd1 / fwd_ms, 3)}
+            log(f"FP32 forward, {fz} GVF steps per launch: {res[fz][0]:.1f} ms, GVF {res[fz][1]:.1f} ms")
+        (f4_fwd, f4_gvf), (f1_fwd, f1_gvf) = res[4], res[1]
+        f32 = {"fastest": {"gvf_steps_per_launch": 4, "gvf_ms": round(f4_gvf, 3), "forward_ms": round(f4_fwd, 3),
+                           "gvoxel_iter_per_s": round(N * args.iters / (f4_gvf * 1e-3) / 1e9, 2),
+                           "hbm_frac_per_launch": round(36 * N / (f4_gvf / (args.iters // 4) * 1e-3) / PEAK_HBM, 4)},
+               "plain": {"gvf_steps_per_launch": 1, "gvf_ms": round(f1_gvf, 3), "forward_ms": round(f1_fwd, 3),
+                         "gvoxel_iter_per_s": round(N * args.iters / (f1_gvf * 1e-3) / 1e9, 2)},
+               "speedup_gvf_vs_fastest_f32": round(f4_gvf / gvf_ms, 3),
+               "speedup_forward_vs_fastest_f32": round(f4_fwd / fwd_ms, 3),
+               "speedup_gvf_vs_plain_f32": round(f1_gvf / gvf_ms, 3),
+               "speedup_forward_vs_plain_f32": round(f1_fwd / fwd_ms, 3)}
         del Fs32, F32, V32, ws32, tws32
 
     adjoint = None
@@ -217,22 +277,29 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(vol)
 
-    # HBM bytes per launch of the dominant kernel, measured by the two
-    # rocprofv3 counter passes recorded in profiles/ (null when not measured
-    # for this storage type)
+    # HBM bytes per launch of the dominant kernel: memory-side read and write
+    # counters of the committed kernels, recorded in profiles/ with the
+    # command that produced them (null when not measured for this kernel)
     traffic = None
-    gk = {4: "k_gvf_fused4<{}, false>", 2: "k_gvf_fused2<{}, false>"}.get(args.fuse, "k_gvf_march<{}, 2, false>")
-    kname = {"f16": gk.format("F16"), "f32": gk.format("F32"), "sn16": gk.format("SN16"),
-             "bf16": gk.format("BF16")}.get(args.storage)
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_circle_forward_512cubed.json")
-    if kname and n == 512 and os.path.exists(tpath):
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_512cubed.json")
+    if n == 512 and os.path.exists(tpath):
         with open(tpath) as fh:
             rec = json.load(fh)["kernels"].get(kname)
         if rec:
-            traffic = round(rec["traffic_bytes"] / 1e9, 4)
+            traffic = round(rec["bytes_per_dispatch"] / 1e9, 4)
+    # errors against FP64 (tests/test_gpu_precision.py, C5 and C3 crops)
+    errors = None
+    epath = os.path.join(ROOT, "profiles", "precision_vs_fp64.json")
+    if os.path.exists(epath):
+        with open(epath) as fh:
+            errors = json.load(fh)["forms"].get(f"precision/{args.storage}/tail{args.tail}" if precision else
+                                                f"plain/{args.storage}")
 
+    form = (f"precision form: f {args.storage.upper()}, iterate {'BF16' if args.storage == 'bf16' else 'F16'} "
+            f"stochastic rounding, last {args.tail} iterations F32, mixed TDF" if precision else
+            f"{args.storage} storage, round to nearest, {args.fuse} steps per launch")
     out = {
-        "metric": f"GVF Gvoxel-iter/s ({args.storage.upper()}-stored field, {n}^3, mu {args.mu}, {args.iters} iterations)",
+        "metric": f"GVF Gvoxel-iter/s ({n}^3, mu {args.mu}, {args.iters} iterations; {form})",
         "value": round(gvox, 3),
         "unit": "Gvoxel-iter/s",
         "n_gpus": 1,
@@ -242,25 +309,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f16": "f16-storage/f32-arith", "bf16": "bf16-storage/f32-arith", "sn16": "snorm16-storage/f32-arith",
-                  "f32": "f32"}[args.storage],
+        "dtype": ("f32-arith; GVF iterate f16 (bf16) stochastic rounding, f " + args.storage + ", TDF samples f16"
+                  if precision else {"f16": "f16-storage/f32-arith", "bf16": "bf16-storage/f32-arith",
+                                     "sn16": "snorm16-storage/f32-arith", "f32": "f32"}[args.storage]),
         "data": "synthetic (seeded 512^3 thin-vessel MRA-like tree, Rician noise)",
         "config": {"workload": f"circle-fitting forward at {n}^3: blur, vector field, GVF Euler "
-                               f"{args.iters} it mu {args.mu} ({args.storage} storage), TDF small 0.5-3 + large "
-                               f"2.5-4, combine", "voxels": N, "gvf_iterations": args.iters,
-                   "fuse": args.fuse, "parallelism": "single-gpu"},
+                               f"{args.iters} it mu {args.mu}, TDF small 0.5-3 + large 2.5-4, combine ({form})",
+                   "voxels": N, "gvf_iterations": args.iters, "form": args.form, "storage": args.storage,
+                   "f32_tail": args.tail if precision else None, "parallelism": "single-gpu"},
         "gvf_ms": round(gvf_ms, 3),
         "forward_ms": round(fwd_ms, 3),
         "wall_s_timed": round(wall, 3),
-        "f32_gvf": f32,
+        "f32": f32,
+        "errors_vs_fp64": errors,
         "adjoint": adjoint,
         "downstream": downstream,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM, 4), "traffic": traffic, "traffic_unit": "GB/launch",
                      "algorithmic_gb_per_launch": round(bytes_per_voxel * N / 1e9, 4),
-                     "kernel": kname or "k_gvf", "steps_per_launch": args.fuse,
-                     "bytes_per_voxel_per_launch": bytes_per_voxel,
-                     "launch_ms": round(launch_ms, 4)},
+                     "kernel": kname, "steps_per_launch": spl, "bytes_per_voxel_per_launch": bytes_per_voxel,
+                     "launch_ms": round(launch_ms, 4),
+                     "launch_ms_method": f"difference of two GVF runs {K} launches apart, HIP events"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -186,25 +186,37 @@ size_t tsf_gvf_euler_workspace_size(tsf_dims dims, int32_t storage);
 
 /* The precision form of tsf_gvf_euler (GVF3DIteration, kernels_no_3d_write.cl:
  * 1746-1787, the recurrence of runFastGVF, gradientVectorFlow.cpp:1007-1172):
- * the iterate streams through device memory in f's 16-bit storage (F16
- * holding value * scale, or BF16) and every step computes in float, two
- * Jacobi steps per launch.  Each launch rounds its result to 16 bits
- * stochastically: the magnitude rounds up with probability equal to the
- * discarded fraction (to 2^-10), the random bits a hash of the voxel index
- * and seed ^ (launch * 0x9e3779b9), so results are reproducible.  Round to
- * nearest would drop every update smaller than half a 16-bit ulp and stall
- * the iteration short of its fixed point.  The last launch writes `out` as
- * F32 planes with w = |v| (1 where 0): the final iterate at full precision,
- * as the Hessian of the TDF needs it (see tsf_tdf_circle_mixed).  An odd
- * count starts with one step rounded to nearest.  The update is the
- * reference's v + mu lap(v) - (v - f)|f|^2 arranged as
- * v (1 - 6 mu - |f|^2) + f |f|^2 + mu sum(neighbours), with the reference's
- * mirror rule at the faces.  f: F16 or BF16 planes (x, y, z; 4-byte
- * aligned), out: F32 with w.  Needs X even and >= 64, Y >= 32, Z >= 4.
- * Workspace: tsf_gvf_euler_sr_workspace_size (two 16-bit iterates). */
+ * the iterate streams through device memory as 16-bit planes and every step
+ * computes in float, two Jacobi steps per launch.  Each launch rounds its
+ * result to 16 bits stochastically: the magnitude rounds up with
+ * probability equal to the discarded fraction (to 2^-10), the random bits a
+ * hash of the voxel index and seed ^ (launch * 0x9e3779b9), so results are
+ * reproducible.  Round to nearest would drop every update smaller than half
+ * a 16-bit ulp and stall the iteration short of its fixed point.  The last
+ * launch writes `out` as F32 planes with w = |v| (1 where 0): the final
+ * iterate at full precision, as the Hessian of the TDF needs it (see
+ * tsf_tdf_circle_mixed).  An odd count starts with one step rounded to
+ * nearest.  The update is the reference's v + mu lap(v) - (v - f)|f|^2
+ * arranged as v (1 - 6 mu - |f|^2) + f |f|^2 + mu sum(neighbours), with the
+ * reference's mirror rule at the faces.  The last f32_tail iterations (if
+ * > 0) run with the iterate in F32 planes, in the reference's operation
+ * order (as tsf_gvf_euler on F32 fields), on f's stored values: they damp
+ * the rounding noise of the last 16-bit stores, which the Hessian's central
+ * differences would otherwise amplify.
+ * f: F16 (value * scale) or BF16 planes -- the iterate then has f's storage
+ * and scale -- or SNORM16 planes, the reference's 16-bit vector format
+ * (the integer s read as max(s, -32767) * (unit / 32767) in units of the
+ * iterate, F16 at unit TSF_SR_SN16_UNIT).  SNORM16 holds a field of
+ * components in [-1, 1] to 1.5e-5 where F16 holds it to 2^-12 relative, and
+ * f's rounding is the larger part of the F16 form's error.  x, y, z planes,
+ * 4-byte aligned; out: F32 with w.  Needs X even and >= 64, Y >= 32, Z >= 4.
+ * Workspace: tsf_gvf_euler_sr_workspace_size (two 16-bit iterates, and F32
+ * planes for the tail). */
+#define TSF_SR_SN16_UNIT 16384.0f
 tsf_status tsf_gvf_euler_sr(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
-                            uint32_t seed, void* workspace, size_t workspace_bytes, hipStream_t stream);
-size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims);
+                            int32_t f32_tail, uint32_t seed, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream);
+size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims, int32_t f32_tail);
 
 /* runFMGGVF (gradientVectorFlow.cpp:729-1005): per component, `iterations`
  * defect-correction steps r = fmgResidual(f); e = fullMultigrid(r); f += e

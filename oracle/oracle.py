@@ -155,15 +155,17 @@ def tdf_circle_split(FH, FS, rmin, rmax, rstep, prec="f32"):
     return tdf, rad
 
 
-def gvf_euler_sr16(f16, storage, scale, mu, iters, seed=0):
+def gvf_euler_sr16(f16, storage, scale, mu, iters, seed=0, f32_tail=0):
     """The precision form's GVF (tsf_gvf_euler_sr): f16 = (3, Z, Y, X) uint16
-    planes (F16 holding value * scale when storage == 1, BF16 bits when 2);
-    returns the (4, Z, Y, X) float32 output (x, y, z, w)."""
+    planes (F16 holding value * scale when storage == 1, BF16 bits when 2,
+    SNORM16 integers when 3 -- the iterate then F16 at unit 2^14, scale
+    unused), the last f32_tail iterations in float; returns the
+    (4, Z, Y, X) float32 output (x, y, z, w)."""
     f16 = np.ascontiguousarray(f16, np.uint16)
     X, Y, Z = _dims(f16)
     out = np.empty((4,) + f16.shape[1:], np.float32)
     lib().orc_gvf_euler_sr16(_p(f16), int(storage), ctypes.c_float(scale), ctypes.c_float(mu), int(iters),
-                             ctypes.c_uint(seed), X, Y, Z, _p(out))
+                             int(f32_tail), ctypes.c_uint(seed), X, Y, Z, _p(out))
     return out
 
 

@@ -723,19 +723,58 @@ static inline float sr_stage(const float* u, const float* c1, const float* g, Di
     return std::fma(mz, u[i + sz], A);
 }
 
-static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, int iters, uint32_t seed, Dims d,
-                           float* out4) {
+// f's stored numbers in units of the iterate: F16 / BF16 as stored (value *
+// scale), SNORM16 (storage 3; the iterate then F16 at unit 2^14) as
+// max(s, -32767) * (unit / 32767).
+static inline float sr_f_units(uint16_t c, int st, float unit) {
+    if (st == 3) return std::max((float)(int16_t)c, -32767.0f) * (unit / 32767.0f);
+    return s16_get(c, st);
+}
+static inline float sr_unit(int st, float scale) { return st == 3 ? 16384.0f : scale; }
+
+static void gvf_euler_sr16_body(const uint16_t* f16, int st, float scale, float mu, int iters, uint32_t seed,
+                                Dims d, float* out4);
+
+// The last `tail` iterations in float (gvf_step, the reference's order) on
+// the stored f's values, from the 16-bit part's float result; then w.
+static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, int iters, int tail, uint32_t seed,
+                           Dims d, float* out4) {
+    tail = std::min(tail, iters);
+    gvf_euler_sr16_body(f16, st, scale, mu, iters - tail, seed, d, out4);
+    if (tail == 0) return;
     const i64 N = d.n();
-    const float inv = 1.0f / scale, m2s = 1.0f / (scale * scale);
+    const float unit = sr_unit(st, scale), inv = 1.0f / unit;
+    std::vector<float> f(3 * N), v(out4, out4 + 3 * N), o(3 * N);
+    for (i64 i = 0; i < 3 * N; i++) f[i] = sr_f_units(f16[i], st, unit) * inv;
+    for (int k = 0; k < tail; k++) {
+        gvf_step<float>(f.data(), v.data(), o.data(), d, mu);
+        v.swap(o);
+    }
+    for (i64 i = 0; i < N; i++) {
+        const float x = v[i], y = v[N + i], z = v[2 * N + i];
+        const float l = std::sqrt(x * x + y * y + z * z);
+        out4[i] = x;
+        out4[N + i] = y;
+        out4[2 * N + i] = z;
+        out4[3 * N + i] = l > 0.0f ? l : 1.0f;
+    }
+}
+
+static void gvf_euler_sr16_body(const uint16_t* f16, int st, float scale, float mu, int iters, uint32_t seed,
+                                Dims d, float* out4) {
+    const i64 N = d.n();
+    const float unit = sr_unit(st, scale), inv = 1.0f / unit, m2s = 1.0f / (unit * unit);
+    const int sv = st == 3 ? 1 : st;  // the iterate's storage
     const float c0 = 1.0f - 6.0f * mu;
     std::vector<float> f(3 * N), c1(N), g(3 * N);
-    for (i64 i = 0; i < 3 * N; i++) f[i] = s16_get(f16[i], st);
+    for (i64 i = 0; i < 3 * N; i++) f[i] = sr_f_units(f16[i], st, unit);
     for (i64 i = 0; i < N; i++) {
         const float m2 = std::fma(f[2 * N + i], f[2 * N + i], std::fma(f[N + i], f[N + i], f[i] * f[i])) * m2s;
         c1[i] = c0 - m2;
         for (int c = 0; c < 3; c++) g[c * N + i] = f[c * N + i] * m2;
     }
-    std::vector<uint16_t> v16(f16, f16 + 3 * N);
+    std::vector<uint16_t> v16(3 * N);
+    bool from_f = true;  // the iterate is still f (the first launch reads v0 from f's planes)
     std::vector<float> u(3 * N), v1(3 * N), v2(3 * N);
     auto faces = [&](std::vector<float>& a) {
         for (int c = 0; c < 3; c++)
@@ -749,17 +788,16 @@ static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, i
     if (iters % 2) {
         // one step in value space, the reference's order, round to nearest
         std::vector<float> fv(3 * N), vv(3 * N), o(3 * N);
-        for (i64 i = 0; i < 3 * N; i++) {
-            fv[i] = f[i] * inv;
-            vv[i] = s16_get(v16[i], st) * inv;
-        }
+        for (i64 i = 0; i < 3 * N; i++) fv[i] = vv[i] = f[i] * inv;
         gvf_step<float>(fv.data(), vv.data(), o.data(), d, mu);
-        for (i64 i = 0; i < 3 * N; i++) v16[i] = s16_rne(o[i] * scale, st);
+        for (i64 i = 0; i < 3 * N; i++) v16[i] = s16_rne(o[i] * unit, sv);
+        from_f = false;
         j = 1;
     }
     const int pairs = iters / 2;
     for (int k = 0; k < pairs; k++, j++) {
-        for (i64 i = 0; i < 3 * N; i++) u[i] = s16_get(v16[i], st);
+        for (i64 i = 0; i < 3 * N; i++) u[i] = from_f ? f[i] : s16_get(v16[i], sv);
+        from_f = false;
         for (int c = 0; c < 3; c++)
             for (int z = 1; z < d.Z - 1; z++)
                 for (int y = 1; y < d.Y - 1; y++)
@@ -792,7 +830,7 @@ static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, i
         for (i64 i = 0; i < N; i++) {
             const uint32_t h = sr_hash((uint32_t)i, sj);
             const uint32_t r[3] = {h & 0x3ffu, (h >> 10) & 0x3ffu, (h >> 20) & 0x3ffu};
-            for (int c = 0; c < 3; c++) v16[c * N + i] = s16_sr(v2[c * N + i], r[c], st);
+            for (int c = 0; c < 3; c++) v16[c * N + i] = s16_sr(v2[c * N + i], r[c], sv);
         }
         for (int c = 0; c < 3; c++)
             for (int z = 0; z < d.Z; z++)
@@ -802,9 +840,9 @@ static void gvf_euler_sr16(const uint16_t* f16, int st, float scale, float mu, i
                             v16[c * N + d.at(x, y, z)] = v16[c * N + d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z))];
     }
     // no two-step launch: the output is the 16-bit iterate (f, or the single step)
+    auto val = [&](i64 i) { return (from_f ? f[i] : s16_get(v16[i], sv)) * inv; };
     for (i64 i = 0; i < N; i++) {
-        const float x = s16_get(v16[i], st) * inv, y = s16_get(v16[N + i], st) * inv;
-        const float z = s16_get(v16[2 * N + i], st) * inv;
+        const float x = val(i), y = val(N + i), z = val(2 * N + i);
         const float l = std::sqrt(x * x + y * y + z * z);
         out4[i] = x;
         out4[N + i] = y;
@@ -1691,9 +1729,9 @@ extern "C" void orc_blur_mask(float sigma, float* mask, int* maskSize) { blur_ma
 extern "C" void orc_sn16_round(float* a, long long n) {
     for (long long i = 0; i < n; i++) a[i] = sn16_round(a[i]);
 }
-extern "C" void orc_gvf_euler_sr16(const uint16_t* f16, int storage, float scale, float mu, int iters,
+extern "C" void orc_gvf_euler_sr16(const uint16_t* f16, int storage, float scale, float mu, int iters, int tail,
                                    unsigned seed, int X, int Y, int Z, float* out4) {
-    gvf_euler_sr16(f16, storage, scale, mu, iters, seed, Dims{X, Y, Z}, out4);
+    gvf_euler_sr16(f16, storage, scale, mu, iters, tail, seed, Dims{X, Y, Z}, out4);
 }
 extern "C" void orc_half_from_float(const float* a, uint16_t* h, long long n) {
     for (long long i = 0; i < n; i++) h[i] = float_to_half_rne(a[i]);

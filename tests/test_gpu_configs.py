@@ -1,7 +1,7 @@
 """The benchmark's configurations as GPU parity cases.
 
-Each synthetic workload (tsf.synth.config1 .. config4; config5 is the bench's
-own 512^3 forward) runs through the path on the device and is checked
+Each synthetic workload (tsf.synth.config1 .. config4) runs through the path
+on the device and is checked
 against the oracle on the same input where the oracle finishes in seconds,
 and through size-independent properties at the full size otherwise.
 
@@ -12,10 +12,9 @@ and through size-independent properties at the full size otherwise.
   C2  airway tree (dark lumen: vector field sign -1), FMG with 3 levels,
       mu 0.15, TDF, candidate compaction:
       oracle parity on a 64^3 airway, properties on the 256^3 one.
-  C3  512x512x400 CT-like volume, GVF x 300 stored as FP16 and BF16: error
-      of the reduced-precision field, eigenvalues and TDF against the FP32
-      path (mu 0.05: mu 0.1 is unstable, see C1), bounded at the measured
-      levels.
+  C3  and C5 (the benchmark's 512^3 forward): tests/test_gpu_precision.py,
+      every storage form against the FP64 oracle (mu 0.05: mu 0.1 is
+      unstable, see C1).
   C4  the differentiation form at 128^3 is tests/test_gpu_adjoint.py
       (test_chain_finite_differences_128).
 """
@@ -142,42 +141,3 @@ def test_config2_airway_256_properties(tsf):
     k = hp_keys(p.astype(np.int64))
     assert np.all(k[1:] > k[:-1])
     print(f"C2 256^3: {cnt} candidates")
-
-
-def test_config3_precision_forms(tsf):
-    from tsf import synth
-    vol, _ = synth.config3()
-    shape = vol.shape
-    d_vol = tsf.DeviceArray.from_host(vol)
-    d_b = tsf.DeviceArray(shape, np.float32)
-    tsf.blur_gaussian(d_vol, d_b, 0.5)
-    del d_vol
-    res = {}
-    for name, S, sc, fuse in (("f32", tsf.STORAGE_F32, 1.0, 4), ("f16", tsf.STORAGE_F16, 2.0 ** 14, 2),
-                              ("bf16", tsf.STORAGE_BF16, 1.0, 2)):
-        F = tsf.Field(shape, S, sc)
-        tsf.create_vector_field(d_b, F, 0.2, 1)
-        V = tsf.Field(shape, S, sc)
-        tsf.gvf_euler(F, V, 0.05, 300, fuse=fuse)
-        del F
-        lam = tsf.DeviceArray((3,) + shape, np.float32)
-        tsf.hessian_eigen(V, 1, lam)
-        t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
-        tsf.tdf_circle(V, 2.5, 5.0, 0.5, t, r)
-        res[name] = (V.download()[:3], lam.download(), t.download(), r.download())
-        del V, lam, t, r
-    v32, l32, t32, r32 = res["f32"]
-    assert np.all(np.isfinite(v32))
-    # measured on this volume: FP16 field 1.5e-3, TDF 1.0e-2, eigenvalues
-    # 1.5e-2; BF16 1.2e-2, 3.4e-2, 4.2e-2 (L2 relative; TDF on responding
-    # voxels T > 0.1 whose radius is unchanged, eigenvalues on responding
-    # voxels): the Hessian's central differences amplify the storage error of
-    # the field about sevenfold in the TDF
-    for name, bf, bt, be in (("f16", 2e-3, 2e-2, 3e-2), ("bf16", 2e-2, 6e-2, 1e-1)):
-        v, lam, t, r = res[name]
-        resp = t32 > 0.1
-        sel = resp & (r == r32)
-        ev, et, el = rel(v, v32), rel(t[sel], t32[sel]), rel(lam[:, resp], l32[:, resp])
-        print(f"C3 {name}: field {ev:.2e}, TDF {et:.2e}, eigenvalues {el:.2e} on {int(resp.sum())} responding "
-              f"voxels; radius changed at {int((resp & (r != r32)).sum())} of them")
-        assert ev < bf and et < bt and el < be

@@ -1007,6 +1007,60 @@ def test_gvf_sr_one_launch_is_two_float_steps(orc, shape):
     assert np.allclose(out[3], np.sqrt((out[:3].astype(np.float64) ** 2).sum(0)), rtol=1e-6)
 
 
+def _sn16_field(shape, seed):
+    import oracle as o
+    F = _field_from_phantom(shape, seed)
+    return F, o.sn16_store(F[:3]).view(np.uint16)
+
+
+def _sn16_values(codes):
+    return np.maximum(codes.view(np.int16).astype(np.float64), -32767.0) / 32767.0
+
+
+@pytest.mark.parametrize("shape", [(6, 48, 64), (4, 50, 66)])
+def test_gvf_sr_snorm16_f_one_launch_is_two_float_steps(orc, shape):
+    """f in SNORM16 (the iterate in F16 at unit 2^14): one launch from v0 = f
+    is two FP64 Euler steps on f's SNORM values max(s, -32767) / 32767, to
+    float rounding (the decode as s * (2^14 / 32767) in float adds an ulp)."""
+    F, codes = _sn16_field(shape, 45)
+    out = orc.gvf_euler_sr16(codes, 3, 0.0, 0.05, 2)
+    f = np.concatenate([_sn16_values(codes), np.ones((1,) + shape)])
+    ref = orc.gvf_euler(f, 0.05, 2, "f64")
+    err = np.abs(out[:3] - ref[:3]).max() / np.abs(ref[:3]).max()
+    assert err < 2e-6, err
+
+
+def test_gvf_sr_snorm16_f_odd_step_rounds_to_nearest_half(orc):
+    """One step (an odd count) from SNORM16 f: the FP64 step rounded to the
+    nearest F16 at unit 2^14, to within half a half-ulp plus float error."""
+    shape = (5, 48, 64)
+    F, codes = _sn16_field(shape, 46)
+    out = orc.gvf_euler_sr16(codes, 3, 0.0, 0.05, 1)[:3].astype(np.float64)
+    f = np.concatenate([_sn16_values(codes), np.ones((1,) + shape)])
+    ref = orc.gvf_euler(f, 0.05, 1, "f64")[:3]
+    half_ulp = 2.0 ** -11 * np.abs(ref) + 2.0 ** -39
+    assert np.all(np.abs(out - ref) <= half_ulp * 1.001 + 1e-7 * np.abs(ref).max())
+    # and it is a half at unit 2^14
+    assert np.array_equal((out * 2 ** 14).astype(np.float16).astype(np.float64) / 2 ** 14, out)
+
+
+def test_gvf_sr_snorm16_f_is_closer_to_the_unrounded_field(orc):
+    """Against FP64 GVF of the float vector field itself (not of a stored
+    copy), 200 steps: the form with f in SNORM16 lands several times closer
+    than the one with f in F16, since f's rounding dominates the error and
+    SNORM16 resolves components in [-1, 1] to 1.5e-5 where F16 has 2^-12
+    relative."""
+    shape = (16, 48, 64)
+    F, codes = _sn16_field(shape, 47)
+    f16 = (F[:3] * np.float32(2 ** 14)).astype(np.float16).view(np.uint16)
+    ref = orc.gvf_euler(F.astype(np.float64), 0.05, 200, "f64")[:3]
+    e_sn = orc.gvf_euler_sr16(codes, 3, 0.0, 0.05, 200, seed=5)[:3]
+    e_h = orc.gvf_euler_sr16(f16, 1, 2.0 ** 14, 0.05, 200, seed=5)[:3]
+    r = lambda a: np.linalg.norm(a - ref) / np.linalg.norm(ref)
+    print(f"200 steps vs FP64 of the float field: f SNORM16 {r(e_sn):.2e}, f F16 {r(e_h):.2e}")
+    assert r(e_sn) < 0.6 * r(e_h)
+
+
 def test_gvf_sr_mu_zero_keeps_f(orc):
     """mu = 0 with v0 = f: every update of an interior voxel is
     v (1 - |f|^2) + f |f|^2 = f up to float rounding, so after six steps

@@ -52,16 +52,20 @@ constexpr int SROX = SRTX - 4, SROY = SRTY - 4;  // output tile
 constexpr int SRZ = 32;                          // output planes per z chunk
 constexpr unsigned SR_OOB = 0x80000000u;         // buffer offset past every resource: the access is dropped
 
-// The two 16-bit numbers of a 32-bit word as floats, and the stochastic
-// rounding of a float (in storage units) to 16 bits with 10 random bits r.
+// The two 16-bit numbers of a 32-bit word as floats in units (the stored
+// numbers: value * scale for F16, the value for BF16; for SNORM16 f, read
+// as the integer times k = unit / 32767, at least -unit), one number alone,
+// and the stochastic rounding of two floats (in units) to 16 bits with 10
+// random bits each.
 template <class S>
 struct H16;
 template <>
 struct H16<F16> {
-    static __device__ __forceinline__ float lo(unsigned u) {
+    static __device__ __forceinline__ float one(unsigned short c, float) { return __half2float(__ushort_as_half(c)); }
+    static __device__ __forceinline__ float lo(unsigned u, float) {
         return __half2float(__ushort_as_half((unsigned short)(u & 0xffffu)));
     }
-    static __device__ __forceinline__ float hi(unsigned u) {
+    static __device__ __forceinline__ float hi(unsigned u, float) {
         return __half2float(__ushort_as_half((unsigned short)(u >> 16)));
     }
     // Two values with random numbers ra, rb in [0, 1024), scaled to the 13
@@ -76,12 +80,23 @@ struct H16<F16> {
 };
 template <>
 struct H16<BF16> {
-    static __device__ __forceinline__ float lo(unsigned u) { return __uint_as_float(u << 16); }
-    static __device__ __forceinline__ float hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+    static __device__ __forceinline__ float one(unsigned short c, float) { return __uint_as_float((unsigned)c << 16); }
+    static __device__ __forceinline__ float lo(unsigned u, float) { return __uint_as_float(u << 16); }
+    static __device__ __forceinline__ float hi(unsigned u, float) { return __uint_as_float(u & 0xffff0000u); }
     // the high halves of the two sums (bf16 truncation), packed by v_perm
     static __device__ __forceinline__ unsigned sr2(float a, unsigned ra, float b, unsigned rb) {
         return __builtin_amdgcn_perm(__float_as_uint(b) + (rb << 6), __float_as_uint(a) + (ra << 6), 0x07060302u);
     }
+};
+template <>
+struct H16<SN16> {  // f only (the reference's 16-bit vector format)
+    static __device__ __forceinline__ float one(unsigned short c, float k) {
+        return fmaxf((float)(int)(short)c, -32767.0f) * k;
+    }
+    static __device__ __forceinline__ float lo(unsigned u, float k) {
+        return fmaxf((float)(int)(short)(u & 0xffffu), -32767.0f) * k;
+    }
+    static __device__ __forceinline__ float hi(unsigned u, float k) { return fmaxf((float)((int)u >> 16), -32767.0f) * k; }
 };
 
 // Hash of the voxel index and the launch seed, after the lowbias32 mixer
@@ -109,11 +124,12 @@ __device__ __forceinline__ float from_right(float v) {
 
 struct SrArgs {
     const void* f[3];  // f planes (16-bit)
-    const void* v[3];  // input iterate planes (16-bit)
+    const void* v[3];  // input iterate planes (16-bit: the iterate's storage, or f's on the first launch)
     void* o[4];        // output planes: 16-bit x, y, z, or F32 x, y, z, w
     Dims d;
     float mu, m2s, inv;  // m2s = 1 / unit^2, inv = 1 / unit (unit = the 16-bit scale)
     float c0;            // 1 - 6 mu
+    float k;             // SNORM16 integer -> units (unit / 32767)
     unsigned seed;
     int zspan;  // planes covered by the load resources of a chunk
 };
@@ -140,10 +156,12 @@ __device__ __forceinline__ void sr_partial(const float (&u)[2], float2 up, float
 // One z chunk of one tile: XT for tiles at an x face of the volume (the
 // mirror columns), a separate instance so the interior tiles' code has no
 // branch inside its stencil.
-template <class S, bool OUT32, bool XT>
+template <class S, class SV, class SF, bool OUT32, bool XT>
 __device__ __forceinline__ void sr_march(const SrArgs& a, float2 (&L0)[2][3][SRTY][32], float2 (&L1)[2][3][SRTY][32],
                                          int x0, int y0, int qs, int qe) {
     typedef H16<S> H;
+    typedef H16<SV> HV;
+    typedef H16<SF> HF;
     const Dims d = a.d;
     // lane li of half hf of wave w: columns x0 + 2 li, +1 of row y0 + 2 w + hf
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, li = lane & 31, hf = lane >> 5;
@@ -203,8 +221,8 @@ __device__ __forceinline__ void sr_march(const SrArgs& a, float2 (&L0)[2][3][SRT
         for (int c = 0; c < 3; c++) t[c] = __builtin_amdgcn_raw_buffer_load_b32(rv[c], (int)voff, plane_off(q0 - 1), 0);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            v0p[c][0] = H::lo(t[c]);
-            v0p[c][1] = H::hi(t[c]);
+            v0p[c][0] = HV::lo(t[c], a.k);
+            v0p[c][1] = HV::hi(t[c], a.k);
 #pragma unroll
             for (int k = 0; k < 2; k++) a1[c][k] = v1p[c][k] = a2[c][k] = gp[c][k] = 0.0f;
         }
@@ -234,10 +252,10 @@ __device__ __forceinline__ void sr_march(const SrArgs& a, float2 (&L0)[2][3][SRT
             float fq[3][2];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                u0[c][0] = H::lo(pv[SL][c]);
-                u0[c][1] = H::hi(pv[SL][c]);
-                fq[c][0] = H::lo(pf[SL][c]);
-                fq[c][1] = H::hi(pf[SL][c]);
+                u0[c][0] = HV::lo(pv[SL][c], a.k);
+                u0[c][1] = HV::hi(pv[SL][c], a.k);
+                fq[c][0] = HF::lo(pf[SL][c], a.k);
+                fq[c][1] = HF::hi(pf[SL][c], a.k);
             }
 #pragma unroll
             for (int k = 0; k < 2; k++) {
@@ -365,7 +383,7 @@ __device__ __forceinline__ void sr_march(const SrArgs& a, float2 (&L0)[2][3][SRT
     }
 }
 
-template <class S, bool OUT32>
+template <class S, class SV, class SF, bool OUT32>
 __global__ void __launch_bounds__(64 * SRW) k_gvf_sr2(SrArgs a) {
     __shared__ float2 L0[2][3][SRTY][32];  // v0 rows of the current plane
     __shared__ float2 L1[2][3][SRTY][32];  // v1 rows of the previous plane
@@ -375,9 +393,9 @@ __global__ void __launch_bounds__(64 * SRW) k_gvf_sr2(SrArgs a) {
     const int x0 = min(tbx * SROX, d.X - SRTX), y0 = min(tby * SROY, d.Y - SRTY);
     const int qs = 1 + tbz * SRZ, qe = min(qs + SRZ, d.Z - 1);  // output planes [qs, qe)
     if (x0 == 0 || x0 + SRTX == d.X)
-        sr_march<S, OUT32, true>(a, L0, L1, x0, y0, qs, qe);
+        sr_march<S, SV, SF, OUT32, true>(a, L0, L1, x0, y0, qs, qe);
     else
-        sr_march<S, OUT32, false>(a, L0, L1, x0, y0, qs, qe);
+        sr_march<S, SV, SF, OUT32, false>(a, L0, L1, x0, y0, qs, qe);
 }
 
 // The y and z faces of an output from their mirrors (written by k_gvf_sr2,
@@ -423,7 +441,9 @@ static bool sr_shape_ok(Dims d) {
            plane * 4 * SRZ < 2147483648.0;
 }
 
-template <class S>
+// One two-step launch: iterate S out, SV in (f's storage on the first
+// launch, which reads v0 = f), f in SF; unit = the iterate's scale.
+template <class S, class SV, class SF>
 static void launch_sr2(bool out32, const void* const f[3], const void* const v[3], void* const o[4], Dims d,
                        float mu, float unit, unsigned seed, hipStream_t stream) {
     SrArgs a;
@@ -437,12 +457,13 @@ static void launch_sr2(bool out32, const void* const f[3], const void* const v[3
     a.m2s = 1.0f / (unit * unit);
     a.inv = 1.0f / unit;
     a.c0 = 1.0f - 6.0f * mu;
+    a.k = unit / 32767.0f;
     a.seed = seed;
     a.zspan = SRZ + 6;
     const dim3 grid((unsigned)std::max(1, (d.X - 4 + SROX - 1) / SROX), (unsigned)std::max(1, (d.Y - 4 + SROY - 1) / SROY),
                     (unsigned)((d.Z - 2 + SRZ - 1) / SRZ));
-    if (out32) hipLaunchKernelGGL((k_gvf_sr2<S, true>), grid, dim3(64 * SRW), 0, stream, a);
-    else hipLaunchKernelGGL((k_gvf_sr2<S, false>), grid, dim3(64 * SRW), 0, stream, a);
+    if (out32) hipLaunchKernelGGL((k_gvf_sr2<S, SV, SF, true>), grid, dim3(64 * SRW), 0, stream, a);
+    else hipLaunchKernelGGL((k_gvf_sr2<S, SV, SF, false>), grid, dim3(64 * SRW), 0, stream, a);
     const i64 nf = 2 * ((i64)d.X * d.Y + (i64)d.X * (d.Z - 2));
     const unsigned g = (unsigned)std::min<i64>((nf + 255) / 256, 4096);
     if (out32)
@@ -453,54 +474,138 @@ static void launch_sr2(bool out32, const void* const f[3], const void* const v[3
                            (unsigned short*)o[1], (unsigned short*)o[2], (unsigned short*)nullptr, d);
 }
 
-// iterations == 0: the F32 output is f itself, w = |f| or 1.
-template <class S>
-__global__ void k_sr_copy(const void* fx, const void* fy, const void* fz, float inv, float* ox, float* oy, float* oz,
-                          float* ow, i64 n) {
-    const typename S::T* P[3] = {(const typename S::T*)fx, (const typename S::T*)fy, (const typename S::T*)fz};
+// 16-bit planes (codec C, units) as F32 values, and optionally w = |v| (1
+// where 0): the output of a run with no two-step launch, and f for the
+// float tail.
+template <class C, bool W>
+__global__ void k_sr_values(const void* px, const void* py, const void* pz, float k, float inv, float* ox, float* oy,
+                            float* oz, float* ow, i64 n) {
+    const unsigned short* P[3] = {(const unsigned short*)px, (const unsigned short*)py, (const unsigned short*)pz};
     for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
-        const float x = S::ld(P[0], i, inv), y = S::ld(P[1], i, inv), z = S::ld(P[2], i, inv);
-        const float l = sqrtf(x * x + y * y + z * z);
+        const float x = H16<C>::one(P[0][i], k) * inv, y = H16<C>::one(P[1][i], k) * inv;
+        const float z = H16<C>::one(P[2][i], k) * inv;
         ox[i] = x;
         oy[i] = y;
         oz[i] = z;
-        ow[i] = l > 0.0f ? l : 1.0f;
+        if (W) {
+            const float l = sqrtf(x * x + y * y + z * z);
+            ow[i] = l > 0.0f ? l : 1.0f;
+        }
     }
 }
 
-template <class S>
-static tsf_status run_sr(const tsf_field* f, const tsf_field* out, Dims d, float mu, int iters, unsigned seed,
+// One Euler step from v0 = f (codec SF) in values, the reference's order
+// (kernels_no_3d_write.cl:1746-1787, the mirror rule at the faces), rounded
+// to nearest into the iterate's 16-bit planes (S, value * unit): the first
+// step of an odd count when f is SNORM16.  Volumes of a few hundred million
+// voxels: one pass, read through the caches.
+template <class SF, class S>
+__global__ void k_sr_step1(const void* fx, const void* fy, const void* fz, Dims d, float mu, float k, float inv,
+                           float unit, void* ox, void* oy, void* oz) {
+    const unsigned short* F[3] = {(const unsigned short*)fx, (const unsigned short*)fy, (const unsigned short*)fz};
+    typename S::T* O[3] = {(typename S::T*)ox, (typename S::T*)oy, (typename S::T*)oz};
+    const i64 n = d.n(), sy = d.X, sz = (i64)d.X * d.Y;
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const int x = (int)(i % d.X), y = (int)((i / d.X) % d.Y), z = (int)(i / sz);
+        const i64 q = d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z));
+        auto val = [&](int c, i64 j) { return H16<SF>::one(F[c][j], k) * inv; };
+        const float ix = val(0, q), iy = val(1, q), iz = val(2, q);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float vc = c == 0 ? ix : (c == 1 ? iy : iz);
+            float lap = -6.0f * vc;
+            lap = lap + val(c, q + 1);
+            lap = lap + val(c, q - 1);
+            lap = lap + val(c, q + sy);
+            lap = lap + val(c, q - sy);
+            lap = lap + val(c, q + sz);
+            lap = lap + val(c, q - sz);
+            S::st(O[c], i, vc + mu * lap, unit);  // v0 = f: the data term (v - f) |f|^2 is zero
+        }
+    }
+}
+
+// The 16-bit part of a run: f in SF (scale `unit` for F16/BF16; SNORM16 read
+// as value * unit), iterate in S at `unit`; `iters` Euler steps from v0 = f,
+// the last launch's output as F32 x, y, z, w in O.
+template <class S, class SF>
+static tsf_status run_sr(const tsf_field* f, float unit, float* const O[4], Dims d, float mu, int iters, unsigned seed,
                          void* ws, hipStream_t stream) {
     const i64 n = d.n();
+    const float k = unit / 32767.0f, inv = 1.0f / unit;
     const void* F[3] = {f->x, f->y, f->z};
-    void* O[4] = {out->x, out->y, out->z, out->w};
+    void* const OV[4] = {O[0], O[1], O[2], O[3]};
+    const unsigned G = 4096;
     if (iters == 0) {
-        hipLaunchKernelGGL(k_sr_copy<S>, 4096, 256, 0, stream, f->x, f->y, f->z, 1.0f / f->scale, (float*)out->x,
-                           (float*)out->y, (float*)out->z, (float*)out->w, n);
+        hipLaunchKernelGGL((k_sr_values<SF, true>), G, 256, 0, stream, f->x, f->y, f->z, k, inv, O[0], O[1], O[2], O[3],
+                           n);
         return check_launch("tsf_gvf_euler_sr");
     }
     char* A = (char*)ws;
     void* WA[4] = {A, A + 2 * n, A + 4 * n, nullptr};
     void* WB[4] = {A + 6 * n, A + 8 * n, A + 10 * n, nullptr};
     const void* src[3] = {F[0], F[1], F[2]};
+    bool from_f = true;  // src holds f (codec SF), not the iterate
     int j = 0;
     if (iters % 2) {
-        TSF_TRY(gvf_single_step16(f, WA, d, mu, stream));
+        if (std::is_same<S, SF>::value) {
+            TSF_TRY(gvf_single_step16(f, WA, d, mu, stream));
+        } else {
+            hipLaunchKernelGGL((k_sr_step1<SF, S>), G, 256, 0, stream, f->x, f->y, f->z, d, mu, k, inv, unit, WA[0],
+                               WA[1], WA[2]);
+            TSF_TRY(check_launch("tsf_gvf_euler_sr"));
+        }
         for (int c = 0; c < 3; c++) src[c] = WA[c];
+        from_f = false;
         j = 1;
     }
     const int pairs = iters / 2;
-    for (int k = 0; k < pairs; k++, j++) {
-        const bool last = k == pairs - 1;
-        void* const* dst = last ? O : ((j % 2) ? WB : WA);
-        const unsigned s = seed ^ ((unsigned)j * 0x9e3779b9u);
-        launch_sr2<S>(last, F, src, dst, d, mu, f->scale, s, stream);
+    for (int p = 0; p < pairs; p++, j++) {
+        const bool last = p == pairs - 1;
+        void* const* dst = last ? OV : ((j % 2) ? WB : WA);
+        const unsigned sj = seed ^ ((unsigned)j * 0x9e3779b9u);
+        if (from_f) launch_sr2<S, SF, SF>(last, F, src, dst, d, mu, unit, sj, stream);
+        else launch_sr2<S, S, SF>(last, F, src, dst, d, mu, unit, sj, stream);
         for (int c = 0; c < 3; c++) src[c] = dst[c];
+        from_f = false;
         TSF_TRY(check_launch("tsf_gvf_euler_sr"));
     }
     if (pairs == 0)  // one step: its 16-bit result as the F32 output
-        hipLaunchKernelGGL(k_sr_copy<S>, 4096, 256, 0, stream, WA[0], WA[1], WA[2], 1.0f / f->scale,
-                           (float*)out->x, (float*)out->y, (float*)out->z, (float*)out->w, n);
+        hipLaunchKernelGGL((k_sr_values<S, true>), G, 256, 0, stream, WA[0], WA[1], WA[2], k, inv, O[0], O[1], O[2],
+                           O[3], n);
+    return check_launch("tsf_gvf_euler_sr");
+}
+
+// w = |v| (1 where 0) of F32 planes.
+__global__ void k_sr_finish(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                            float* __restrict__ w, i64 n) {
+    for (i64 i = blockIdx.x * (i64)blockDim.x + threadIdx.x; i < n; i += (i64)gridDim.x * blockDim.x) {
+        const float a = x[i], b = y[i], c = z[i];
+        const float l = sqrtf(a * a + b * b + c * c);
+        w[i] = l > 0.0f ? l : 1.0f;
+    }
+}
+
+// float `iters` Euler steps from v0 (gvf.hip; the reference's operation order).
+tsf_status gvf_steps_f32(const float* const f[3], const float* const v0[3], float* const out[3], Dims d, float mu,
+                         int iters, float* ws, hipStream_t stream);
+
+template <class S, class SF>
+static tsf_status run_sr_tail(const tsf_field* f, float unit, const tsf_field* out, Dims d, float mu, int body,
+                              int tail, unsigned seed, void* ws, hipStream_t stream) {
+    const i64 n = d.n();
+    float* T = tail > 0 ? (float*)((char*)ws + 12 * n) : nullptr;  // f32 f, body result (4), ping-pong (3)
+    float* const Obody[4] = {tail > 0 ? T + 3 * n : (float*)out->x, tail > 0 ? T + 4 * n : (float*)out->y,
+                             tail > 0 ? T + 5 * n : (float*)out->z, tail > 0 ? T + 6 * n : (float*)out->w};
+    TSF_TRY((run_sr<S, SF>(f, unit, Obody, d, mu, body, seed, ws, stream)));
+    if (tail == 0) return TSF_OK;
+    hipLaunchKernelGGL((k_sr_values<SF, false>), 4096, 256, 0, stream, f->x, f->y, f->z, unit / 32767.0f, 1.0f / unit,
+                       T, T + n, T + 2 * n, nullptr, n);
+    const float* const ff[3] = {T, T + n, T + 2 * n};
+    const float* const v0[3] = {Obody[0], Obody[1], Obody[2]};
+    float* const o[3] = {(float*)out->x, (float*)out->y, (float*)out->z};
+    TSF_TRY(gvf_steps_f32(ff, v0, o, d, mu, tail, T + 7 * n, stream));
+    hipLaunchKernelGGL(k_sr_finish, 4096, 256, 0, stream, o[0], o[1], o[2], (float*)out->w, n);
     return check_launch("tsf_gvf_euler_sr");
 }
 
@@ -510,20 +615,27 @@ using namespace tsf;
 
 extern "C" {
 
-size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims) { return (size_t)12 * (size_t)D(dims).n(); }
+size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims, int32_t f32_tail) {
+    const size_t n = (size_t)D(dims).n();
+    // two 16-bit iterates; with a float tail also f, the 16-bit part's result
+    // (x, y, z, w) and the float steps' ping-pong as F32 planes
+    return 12 * n + (f32_tail > 0 ? 40 * n : 0);
+}
 
 tsf_status tsf_gvf_euler_sr(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
-                            uint32_t seed, void* workspace, size_t workspace_bytes, hipStream_t stream) {
+                            int32_t f32_tail, uint32_t seed, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream) {
     const char* W = "tsf_gvf_euler_sr";
     TSF_TRY(check_field(f, false, W));
     TSF_TRY(check_field(out, true, W));
     TSF_TRY(check_dims(dims, 4, W));
-    if (f->storage != TSF_STORAGE_F16 && f->storage != TSF_STORAGE_BF16)
-        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: f must be F16 or BF16", W);
+    if (f->storage != TSF_STORAGE_F16 && f->storage != TSF_STORAGE_BF16 && f->storage != TSF_STORAGE_SN16)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: f must be F16, BF16 or SNORM16", W);
     if (out->storage != TSF_STORAGE_F32) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: out must be F32", W);
-    if (iterations < 0) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: negative iteration count", W);
+    if (iterations < 0 || f32_tail < 0)
+        return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: negative iteration count", W);
     if (!(mu >= 0.0f) || !__builtin_isfinite(mu)) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: mu must be >= 0", W);
-    if (!(f->scale >= 0x1p-30f && f->scale <= 0x1p30f))
+    if (f->storage != TSF_STORAGE_SN16 && !(f->scale >= 0x1p-30f && f->scale <= 0x1p30f))
         return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: scale out of range", W);
     Dims d = D(dims);
     if (!sr_shape_ok(d))
@@ -531,12 +643,22 @@ tsf_status tsf_gvf_euler_sr(const tsf_field* f, const tsf_field* out, tsf_dims d
                          SRTY, d.X, d.Y, d.Z);
     for (const void* p : {f->x, f->y, f->z})
         if ((uintptr_t)p % 4) return set_error(TSF_ERR_INVALID_ARGUMENT, "%s: f planes must be 4-byte aligned", W);
+    const int tail = f32_tail < iterations ? f32_tail : iterations, body = iterations - tail;
     Scratch sc;
-    if (iterations > 2 || iterations % 2)
-        TSF_TRY(sc.get(workspace, workspace_bytes, tsf_gvf_euler_sr_workspace_size(dims), W));
-    const tsf_status st = f->storage == TSF_STORAGE_F16
-                              ? run_sr<F16>(f, out, d, mu, iterations, seed, sc.ptr, stream)
-                              : run_sr<BF16>(f, out, d, mu, iterations, seed, sc.ptr, stream);
+    if (iterations > 2 || iterations % 2 || tail > 0)
+        TSF_TRY(sc.get(workspace, workspace_bytes, tsf_gvf_euler_sr_workspace_size(dims, tail), W));
+    tsf_status st;
+    switch (f->storage) {
+        case TSF_STORAGE_F16:
+            st = run_sr_tail<F16, F16>(f, f->scale, out, d, mu, body, tail, seed, sc.ptr, stream);
+            break;
+        case TSF_STORAGE_BF16:
+            st = run_sr_tail<BF16, BF16>(f, f->scale, out, d, mu, body, tail, seed, sc.ptr, stream);
+            break;
+        default:  // SNORM16 f, the iterate in F16
+            st = run_sr_tail<F16, SN16>(f, TSF_SR_SN16_UNIT, out, d, mu, body, tail, seed, sc.ptr, stream);
+            break;
+    }
     if (st == TSF_OK && sc.owned) {
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));

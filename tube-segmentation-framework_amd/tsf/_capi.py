@@ -67,8 +67,8 @@ _SIGS = {
     "tsf_tdf_round_unorm16": ([_vp, _i64, _vp], _i32),
     "tsf_hp_compact": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp, _sz, _vp], _i32),
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
-    "tsf_gvf_euler_sr": ([_PF, _PF, Dims, _f32, _i32, ctypes.c_uint32, _vp, _sz, _vp], _i32),
-    "tsf_gvf_euler_sr_workspace_size": ([Dims], _sz),
+    "tsf_gvf_euler_sr": ([_PF, _PF, Dims, _f32, _i32, _i32, ctypes.c_uint32, _vp, _sz, _vp], _i32),
+    "tsf_gvf_euler_sr_workspace_size": ([Dims, _i32], _sz),
     "tsf_tdf_circle_mixed": ([_PF, _i32, _f32, Dims, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
     "tsf_tdf_circle_mixed_workspace_size": ([Dims], _sz),
     "tsf_hp_compact_noalloc": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp], _i32),
@@ -448,17 +448,18 @@ def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=Non
                                 _sp(stream)))
 
 
-def gvf_euler_sr_workspace_size(shape):
-    return lib().tsf_gvf_euler_sr_workspace_size(_dims(shape))
+def gvf_euler_sr_workspace_size(shape, f32_tail=0):
+    return lib().tsf_gvf_euler_sr_workspace_size(_dims(shape), f32_tail)
 
 
-def gvf_euler_sr(f, out, mu, iterations, seed=0, workspace=None, stream=None):
+def gvf_euler_sr(f, out, mu, iterations, seed=0, f32_tail=0, workspace=None, stream=None):
     """Precision-form GVF: f F16/BF16, iterate streamed in f's storage with
-    stochastic rounding, out (F32 field with w) the final iterate."""
+    stochastic rounding, the last f32_tail iterations in float, out (F32
+    field with w) the final iterate."""
     df, do = f.desc(), out.desc()
     w, wb = _ws(workspace)
-    _check(lib().tsf_gvf_euler_sr(ctypes.byref(df), ctypes.byref(do), _dims(f.shape), mu, iterations, seed, w, wb,
-                                  _sp(stream)))
+    _check(lib().tsf_gvf_euler_sr(ctypes.byref(df), ctypes.byref(do), _dims(f.shape), mu, iterations, f32_tail, seed,
+                                  w, wb, _sp(stream)))
 
 
 def tdf_circle_mixed_workspace_size(shape):
