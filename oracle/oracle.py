@@ -155,6 +155,20 @@ def tdf_circle_split(FH, FS, rmin, rmax, rstep, prec="f32"):
     return tdf, rad
 
 
+def tdf_circle_mixed(FH, FS, rmin, rmax, rstep, prec="f32"):
+    """The precision form's TDF (tsf_tdf_circle_mixed): the Hessian from FH,
+    the samples from FS (FH's 16-bit rounding) with fused multiply-adds."""
+    FH = np.ascontiguousarray(FH, _dt(prec))
+    FS = np.ascontiguousarray(FS, _dt(prec))
+    assert FH.shape == FS.shape
+    X, Y, Z = _dims(FH)
+    tdf = np.empty(FH.shape[1:], _dt(prec))
+    rad = np.empty(FH.shape[1:], _dt(prec))
+    _fn("tdf_circle_mixed", prec)(_p(FH), _p(FS), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax),
+                                  ctypes.c_float(rstep), _p(tdf), _p(rad))
+    return tdf, rad
+
+
 def gvf_euler_sr16(f16, storage, scale, mu, iters, seed=0, f32_tail=0):
     """The precision form's GVF (tsf_gvf_euler_sr): f16 = (3, Z, Y, X) uint16
     planes (F16 holding value * scale when storage == 1, BF16 bits when 2,

@@ -935,11 +935,39 @@ static inline void trilinear3(const T* F, Dims d, T u, T v, T w, T out[3]) {
     }
 }
 
+// The precision form's trilinear sample (tsf_tdf_circle_mixed): the cell and
+// weights of trilinear3 at uu = u - 0.5 given directly, the eight taps
+// accumulated by fused multiply-adds in the same order.
+template <typename T>
+static inline void trilinear3_fused(const T* F, Dims d, T uu, T vv, T ww, T out[3]) {
+    const i64 N = d.n();
+    T fu = std::floor(uu), fv = std::floor(vv), fw = std::floor(ww);
+    T a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    int i1 = clampi(i0 + 1, 0, d.X - 1), j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1); j0 = clampi(j0, 0, d.Y - 1); k0 = clampi(k0, 0, d.Z - 1);
+    T wt[8] = {((T)1 - a) * ((T)1 - b) * ((T)1 - c), a * ((T)1 - b) * ((T)1 - c),
+               ((T)1 - a) * b * ((T)1 - c),          a * b * ((T)1 - c),
+               ((T)1 - a) * ((T)1 - b) * c,          a * ((T)1 - b) * c,
+               ((T)1 - a) * b * c,                    a * b * c};
+    i64 idx[8] = {d.at(i0, j0, k0), d.at(i1, j0, k0), d.at(i0, j1, k0), d.at(i1, j1, k0),
+                  d.at(i0, j0, k1), d.at(i1, j0, k1), d.at(i0, j1, k1), d.at(i1, j1, k1)};
+    for (int comp = 0; comp < 3; comp++) {
+        const T* p = F + comp * N;
+        T s = wt[0] * p[idx[0]];
+        for (int k = 1; k < 8; k++) s = std::fma(wt[k], p[idx[k]], s);
+        out[comp] = s;
+    }
+}
+
 // circleFittingTDF, kernels_no_3d_write.cl:1503-1572, with the Hessian taken
 // from FH and the trilinear samples from FS.  FH == FS is the reference's
 // filter; FS = FH rounded to 16 bits is the precision form's mixed mode
-// (the Hessian at full precision, the samples from 16-bit texels).
-template <typename T>
+// (the Hessian at full precision, the samples from 16-bit texels).  FUSED:
+// the precision form's sample arithmetic -- the direction fma(c, e3, s e2),
+// the position fma(r, V, x - 0.5), fused taps, the dot product as
+// fma(s2, v2, fma(s1, v1, s0 v0)) subtracted from the sum.
+template <typename T, bool FUSED = false>
 static void tdf_circle_split(const T* FH, const T* F, Dims d, float rMin, float rMax, float rStep, T* tdf, T* rad) {
     make_tables();
     const bool normalize = !(rMax < 4);
@@ -956,13 +984,20 @@ static void tdf_circle_split(const T* FH, const T* F, Dims d, float rMin, float 
                 for (float radius = rMin; radius <= rMax; radius += rStep) {
                     T sum = 0;
                     for (int j = 0; j < 32; j++) {
-                        T va[3];
-                        for (int k = 0; k < 3; k++) va[k] = (T)g_cos[j] * e3[k] + (T)g_sin[j] * e2[k];
-                        T s[3];
-                        trilinear3(F, d, (T)x + (T)radius * va[0], (T)y + (T)radius * va[1],
-                                   (T)z + (T)radius * va[2], s);
-                        T dp = (-s[0]) * va[0] + (-s[1]) * va[1] + (-s[2]) * va[2];
-                        sum += dp;
+                        T va[3], s[3];
+                        if (FUSED) {
+                            for (int k = 0; k < 3; k++) va[k] = std::fma((T)g_cos[j], e3[k], (T)g_sin[j] * e2[k]);
+                            trilinear3_fused(F, d, std::fma((T)radius, va[0], (T)x - (T)0.5),
+                                             std::fma((T)radius, va[1], (T)y - (T)0.5),
+                                             std::fma((T)radius, va[2], (T)z - (T)0.5), s);
+                            sum = sum - std::fma(s[2], va[2], std::fma(s[1], va[1], s[0] * va[0]));
+                        } else {
+                            for (int k = 0; k < 3; k++) va[k] = (T)g_cos[j] * e3[k] + (T)g_sin[j] * e2[k];
+                            trilinear3(F, d, (T)x + (T)radius * va[0], (T)y + (T)radius * va[1],
+                                       (T)z + (T)radius * va[2], s);
+                            T dp = (-s[0]) * va[0] + (-s[1]) * va[1] + (-s[2]) * va[2];
+                            sum += dp;
+                        }
                     }
                     sum /= (T)32;
                     if (sum > maxSum) {
@@ -1651,6 +1686,10 @@ static void chain_gradient(const T* I, Dims d, T sigma, float Fmax, int sign, T 
     extern "C" void orc_tdf_circle_split_##S(const T* FH, const T* FS, int X, int Y, int Z, float rmin,       \
                                              float rmax, float rstep, T* tdf, T* rad) {                      \
         tdf_circle_split<T>(FH, FS, DIMS, rmin, rmax, rstep, tdf, rad);                                      \
+    }                                                                                                        \
+    extern "C" void orc_tdf_circle_mixed_##S(const T* FH, const T* FS, int X, int Y, int Z, float rmin,       \
+                                             float rmax, float rstep, T* tdf, T* rad) {                      \
+        tdf_circle_split<T, true>(FH, FS, DIMS, rmin, rmax, rstep, tdf, rad);                                \
     }                                                                                                        \
     extern "C" void orc_tdf_spline_##S(const T* F, int X, int Y, int Z, float rmin, float rmax, float rstep, \
                                        int arms, float min_avg, T* tdf, T* rad) {                            \

@@ -293,7 +293,8 @@ def _bf16_round(a):
 @pytest.mark.parametrize("storage,rmin,rmax", [(1, 1.0, 3.0), (1, 2.5, 4.0), (2, 0.5, 3.0), (2, 2.5, 5.0)])
 def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
     """Mixed mode: Hessian from the F32 field, samples from its 16-bit
-    rounding -- the oracle's split filter on the same two fields."""
+    rounding with fused multiply-adds -- the oracle's mixed filter on the
+    same two fields."""
     shape = (32, 32, 32)
     V = _gvf_field(orc, shape, 11)
     fld = tsf.Field.from_host(V)
@@ -303,7 +304,7 @@ def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
     gt, gr = t.download(), r.download()
     Vs = V.copy()
     Vs[:3] = orc.f16_round(V[:3]) if storage == 1 else _bf16_round(V[:3])
-    et, er = orc.tdf_circle_split(V, Vs, rmin, rmax, 0.5, "f32")
+    et, er = orc.tdf_circle_mixed(V, Vs, rmin, rmax, 0.5, "f32")
     nbad, rel = close(gt, et)
     print(f"mixed TDF: exact fraction {np.mean(gt == et):.6f}, radius agree {np.mean(gr == er):.6f}")
     assert nbad == 0, rel

@@ -417,6 +417,61 @@ def test_tdf_split_hessian_and_samples_from_their_fields(orc):
         assert not np.isclose(max(S, 0.0), max(ident, 0.0)) or S <= 0 and ident <= 0
 
 
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_tdf_mixed_known_answer(orc, prec, rtol):
+    """The precision form's fused sampling on the linear-Hessian / uniform
+    sample field pair: a uniform field samples to c exactly (the weights sum
+    to one in any rounding order only to within an ulp, hence rtol), so the
+    response is S(rmin) = -c . (e3 sum c_j + e2 sum s_j) / 32 or 0."""
+    shape = (12, 12, 12)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    A = np.array([[0.30, 0.10, -0.05], [0.10, -0.20, 0.15], [-0.05, 0.15, 0.05]])
+    FH = np.zeros((4,) + shape)
+    P = np.stack([x, y, z])
+    for i in range(3):
+        FH[i] = sum(A[i, j] * P[j] for j in range(3))
+    FH[3] = 1
+    c, s = orc.circle_tables()
+    lam, vec = orc.hessian_eigen(FH, 0, prec="f64")
+    e2, e3 = vec[3:6, 6, 6, 6], vec[6:9, 6, 6, 6]
+    for cvec in (np.array([0.3, -0.4, 0.2]), np.array([-0.3, 0.4, -0.2])):
+        FS = np.zeros((4,) + shape)
+        FS[:3] = cvec[:, None, None, None]
+        FS[3] = 1
+        tdf, rad = orc.tdf_circle_mixed(FH, FS, 1.0, 3.0, 0.5, prec)
+        S = -(cvec @ e3 * c.astype(np.float64).sum() + cvec @ e2 * s.astype(np.float64).sum()) / 32
+        if S > 0:
+            assert np.isclose(tdf[6, 6, 6], S, rtol=rtol) and rad[6, 6, 6] == 1.0
+        else:
+            assert tdf[6, 6, 6] == 0 and rad[6, 6, 6] == 0
+
+
+def test_tdf_mixed_fused_is_as_close_to_fp64(orc):
+    """On a GVF field (FH = FS, no 16-bit rounding): the fused float
+    sampling agrees with the FP64 filter to float rounding on the voxels
+    whose radius matches, at least as closely as the reference-order float
+    filter, and changes no more radii; and the two float forms differ (the
+    fused arithmetic is in use)."""
+    from tsf import synth
+    vol, _ = synth.tube_volume((24, 28, 32), 5, 1.0, 3.0, 60, noise=0.05)
+    F = orc.vector_field(orc.blur(vol, 0.5, "f64"), 0.2, 1, "f64")
+    V = orc.gvf_euler(F, 0.05, 40, "f64")
+    for rmin, rmax in ((0.5, 3.0), (2.5, 4.0)):
+        t64, r64 = orc.tdf_circle(V, rmin, rmax, 0.5, "f64")
+        tf, rf = orc.tdf_circle_mixed(V.astype(np.float32), V.astype(np.float32), rmin, rmax, 0.5, "f32")
+        tu, ru = orc.tdf_circle(V.astype(np.float32), rmin, rmax, 0.5, "f32")
+        resp = t64 > 0.05
+        for t, r in ((tf, rf), (tu, ru)):
+            same = resp & (r == r64)
+            assert np.abs(t[same] - t64[same]).max() < 1e-5 * np.abs(t64[same]).max()
+        same_f, same_u = resp & (rf == r64), resp & (ru == r64)
+        ef = np.linalg.norm(tf[same_f] - t64[same_f]) / np.linalg.norm(t64[same_f])
+        eu = np.linalg.norm(tu[same_u] - t64[same_u]) / np.linalg.norm(t64[same_u])
+        print(f"r {rmin}-{rmax}: fused {ef:.2e}, reference order {eu:.2e}")
+        assert ef <= 1.5 * eu and (resp & (rf != r64)).sum() <= (resp & (ru != r64)).sum() + 2
+        assert not np.array_equal(tf, tu)
+
+
 def test_tdf_linear_inward_field(orc):
     Z, Y, X = 9, 24, 24
     z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")

@@ -34,10 +34,44 @@ __global__ void k_pack_16(const uint16_t* __restrict__ x, const uint16_t* __rest
 
 
 
+// The mean of dot(-F(p + r V_j), V_j) over the 32 circle directions, in the
+// reference's operation order (kernels_no_3d_write.cl:1539-1556); FUSED: the
+// precision form's, with the direction fma(c_j, e3, s_j e2), the position
+// fma(r, V_j, p - 0.5), fused taps (trilinear_fused) and the dot product as
+// a chain of fused multiply-adds subtracted from the sum with one rounding.
+template <bool FUSED, class A>
+__device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx, float fy, float fz, float radius,
+                                            const float (&e2)[3], const float (&e3)[3], const CircleTables& tab) {
+    float sum = 0.0f;
+    if constexpr (FUSED) {
+        const float xm = fx - 0.5f, ym = fy - 0.5f, zm = fz - 0.5f;
+#pragma unroll 8
+        for (int j = 0; j < 32; j++) {
+            const float cj = tab.c[j], sj = tab.s[j];
+            const float vx = __builtin_fmaf(cj, e3[0], sj * e2[0]), vy = __builtin_fmaf(cj, e3[1], sj * e2[1]);
+            const float vz = __builtin_fmaf(cj, e3[2], sj * e2[2]);
+            float s[3];
+            trilinear_fused(T, d, __builtin_fmaf(radius, vx, xm), __builtin_fmaf(radius, vy, ym),
+                            __builtin_fmaf(radius, vz, zm), s);
+            const float dp = __builtin_fmaf(s[2], vz, __builtin_fmaf(s[1], vy, s[0] * vx));
+            sum = __builtin_fmaf(-T.inv, dp, sum);
+        }
+    } else {
+        for (int j = 0; j < 32; j++) {
+            const float cj = tab.c[j], sj = tab.s[j];
+            const float vx = cj * e3[0] + sj * e2[0], vy = cj * e3[1] + sj * e2[1], vz = cj * e3[2] + sj * e2[2];
+            float s[3];
+            trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
+            sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
+        }
+    }
+    return sum / 32.0f;
+}
+
 // circleFittingTDF, kernels_no_3d_write.cl:1503-1572, at one voxel: the
 // (normalised) Hessian, its eigenvectors e2, e3, and the radius search with
 // float accumulation and a first-maximum break.
-template <bool NORM, class A, class AH = A>
+template <bool NORM, bool FUSED = false, class A, class AH = A>
 __device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int y, int z, float rmin, float rmax,
                                           float rstep, const CircleTables& tab, float& maxSum, float& maxRadius,
                                           const AH* TH = nullptr) {
@@ -45,21 +79,12 @@ __device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int 
     if (TH) hessian_tex<NORM>(*TH, d, x, y, z, H);
     else hessian_tex<NORM>(T, d, x, y, z, H);
     eigen3(H, V, ev);
-    const float e2x = V[0][1], e2y = V[1][1], e2z = V[2][1];
-    const float e3x = V[0][2], e3y = V[1][2], e3z = V[2][2];
+    const float e2[3] = {V[0][1], V[1][1], V[2][1]};
+    const float e3[3] = {V[0][2], V[1][2], V[2][2]};
     maxSum = 0.0f;
     maxRadius = 0.0f;
-    const float fx = (float)x, fy = (float)y, fz = (float)z;
     for (float radius = rmin; radius <= rmax; radius += rstep) {
-        float sum = 0.0f;
-        for (int j = 0; j < 32; j++) {
-            const float cj = tab.c[j], sj = tab.s[j];
-            const float vx = cj * e3x + sj * e2x, vy = cj * e3y + sj * e2y, vz = cj * e3z + sj * e2z;
-            float s[3];
-            trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
-            sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
-        }
-        sum /= 32.0f;
+        const float sum = circle_sum<FUSED>(T, d, (float)x, (float)y, (float)z, radius, e2, e3, tab);
         if (sum > maxSum) {
             maxSum = sum;
             maxRadius = radius;
@@ -168,21 +193,6 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_spline(Tex<S> T, Dims d, 
     tdf[i] = fminf(1.0f, (sum / (float)ARMS) * sym + 0.2f);
 }
 
-// The mean of dot(-F(p + r V_j), V_j) over the 32 circle directions.
-template <class A>
-__device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx, float fy, float fz, float radius,
-                                            const float (&e2)[3], const float (&e3)[3], const CircleTables& tab) {
-    float sum = 0.0f;
-    for (int j = 0; j < 32; j++) {
-        const float cj = tab.c[j], sj = tab.s[j];
-        const float vx = cj * e3[0] + sj * e2[0], vy = cj * e3[1] + sj * e2[1], vz = cj * e3[2] + sj * e2[2];
-        float s[3];
-        trilinear_tex(T, d, fx + radius * vx, fy + radius * vy, fz + radius * vz, s);
-        sum += (-s[0]) * vx + (-s[1]) * vy + (-s[2]) * vz;
-    }
-    return sum / 32.0f;
-}
-
 // The brick's footprint -- the brick plus a halo of h voxels, clamp-to-edge
 // -- staged in LDS once; the Hessian and every trilinear tap read LDS.  h
 // covers the Hessian's +-1 and the farthest sample corner,
@@ -196,7 +206,7 @@ __device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx,
 // fill is the caller's, followed by a barrier).  The Hessian reads TH, which
 // is T itself except in the mixed mode (F32 planes for the Hessian, 16-bit
 // texels for the samples).
-template <bool NORM, int BX, int BY, int BZ, class A, class AH>
+template <bool NORM, int BX, int BY, int BZ, bool FUSED = false, class A, class AH>
 __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int bx0, int by0, int bz0, float rmin, float rmax,
                                           float rstep, const CircleTables& tab, float* __restrict__ tdf,
                                           float* __restrict__ rad) {
@@ -220,7 +230,7 @@ __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int 
             e3[0] = V[0][2]; e3[1] = V[1][2]; e3[2] = V[2][2];
             float r0 = 0.0f;
             if (rmin <= rmax) {
-                const float sum = circle_sum(T, d, (float)x, (float)y, (float)z, rmin, e2, e3, tab);
+                const float sum = circle_sum<FUSED>(T, d, (float)x, (float)y, (float)z, rmin, e2, e3, tab);
                 if (sum > 0.0f) {
                     best = sum;
                     r0 = rmin;
@@ -284,7 +294,7 @@ __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int 
         if (act) {
             int x, y, z;
             coords(v, x, y, z);
-            const float sum = circle_sum(T, d, (float)x, (float)y, (float)z, radius, e2, e3, tab);
+            const float sum = circle_sum<FUSED>(T, d, (float)x, (float)y, (float)z, radius, e2, e3, tab);
             if (sum > best) {
                 best = sum;
             } else {
@@ -333,7 +343,7 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_mixed(Tex<S> T, PlanesF32
     const int z = blockIdx.z * CBZ + threadIdx.z;
     if (x >= d.X || y >= d.Y || z >= d.Z) return;
     float t, r;
-    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r, &H);
+    tdf_voxel<NORM, true>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r, &H);
     const i64 i = d.at(x, y, z);
     tdf[i] = t;
     rad[i] = r;
@@ -363,7 +373,7 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         tile[c] = tex[d.at(gx, gy, gz)];
     }
     __syncthreads();
-    tdf_brick<NORM, BX, BY, BZ>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+    tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
 // F32 planes -> 16-bit texels holding value * scale (x, y, z; w = 0, the

@@ -18,6 +18,9 @@ struct Texel<F32> {
     static __device__ __forceinline__ float mulx(float w, P p) { return w * p.x; }
     static __device__ __forceinline__ float muly(float w, P p) { return w * p.y; }
     static __device__ __forceinline__ float mulz(float w, P p) { return w * p.z; }
+    static __device__ __forceinline__ float fmx(float w, P p, float acc) { return __builtin_fmaf(w, p.x, acc); }
+    static __device__ __forceinline__ float fmy(float w, P p, float acc) { return __builtin_fmaf(w, p.y, acc); }
+    static __device__ __forceinline__ float fmz(float w, P p, float acc) { return __builtin_fmaf(w, p.z, acc); }
 };
 template <>
 struct Texel<F16> {
@@ -47,6 +50,14 @@ struct Texel<F16> {
         asm("v_fma_mix_f32 %0, %1, %2, neg(0) op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(p.y));
         return r;
     }
+    // fma(w, h, acc), one rounding (v_fma_mix_f32 reads the half directly)
+    static __device__ __forceinline__ float fmx(float w, P p, float acc) {
+        return __builtin_fmaf(w, h(p.x & 0xffffu), acc);
+    }
+    static __device__ __forceinline__ float fmy(float w, P p, float acc) { return __builtin_fmaf(w, h(p.x >> 16), acc); }
+    static __device__ __forceinline__ float fmz(float w, P p, float acc) {
+        return __builtin_fmaf(w, h(p.y & 0xffffu), acc);
+    }
 };
 template <>
 struct Texel<BF16> {
@@ -61,6 +72,13 @@ struct Texel<BF16> {
     static __device__ __forceinline__ float mulx(float w, P p) { return w * b(p.x & 0xffffu); }
     static __device__ __forceinline__ float muly(float w, P p) { return w * b(p.x >> 16); }
     static __device__ __forceinline__ float mulz(float w, P p) { return w * b(p.y & 0xffffu); }
+    static __device__ __forceinline__ float fmx(float w, P p, float acc) {
+        return __builtin_fmaf(w, b(p.x & 0xffffu), acc);
+    }
+    static __device__ __forceinline__ float fmy(float w, P p, float acc) { return __builtin_fmaf(w, b(p.x >> 16), acc); }
+    static __device__ __forceinline__ float fmz(float w, P p, float acc) {
+        return __builtin_fmaf(w, b(p.y & 0xffffu), acc);
+    }
 };
 
 // SN16 texels read in units of 2^-15 (32768 * value, exactly); the samplers
@@ -254,6 +272,75 @@ __device__ __forceinline__ void trilinear_tex(const LdsTex<S>& T, const Dims&, f
     out[0] = sx * T.inv;
     out[1] = sy_ * T.inv;
     out[2] = sz_ * T.inv;
+}
+
+// The precision form's sampling (tsf_tdf_circle_mixed): the position
+// arrives as uu = u - 0.5 (formed by one fused multiply-add), and the eight
+// taps accumulate by fused multiply-adds, one rounding each, in the same
+// order; the sums stay in the texels' raw units (the caller applies the
+// scale once).  The cell and weight arithmetic is trilinear_tex's.
+template <class S>
+__device__ __forceinline__ void trilinear_fused(const LdsTex<S>& T, const Dims&, float uu, float vv, float ww,
+                                                float out[3]) {
+    typedef Texel<S> X;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const int sy = T.tx, sz = __mul24(T.tx, T.ty);
+    // the cell offset in the tile: in range for every finite sample; an
+    // unsigned min keeps any other in bounds (its weights are NaN, so the
+    // cell read does not matter)
+    const int li = (int)min((unsigned)((int)fu - T.ox), (unsigned)(T.tx - 2));
+    const int lj = (int)min((unsigned)((int)fv - T.oy), (unsigned)(T.ty - 2));
+    const int lk = (int)min((unsigned)((int)fw - T.oz), (unsigned)(T.tz - 2));
+    const typename X::P* q = T.p + (li + __mul24(lj, sy) + __mul24(lk, sz));
+    const typename X::P p0 = q[0], p1 = q[1], p2 = q[sy], p3 = q[sy + 1];
+    const typename X::P p4 = q[sz], p5 = q[sz + 1], p6 = q[sz + sy], p7 = q[sz + sy + 1];
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    float sx = X::mulx(wt0, p0), sy_ = X::muly(wt0, p0), sz_ = X::mulz(wt0, p0);
+    sx = X::fmx(wt1, p1, sx); sy_ = X::fmy(wt1, p1, sy_); sz_ = X::fmz(wt1, p1, sz_);
+    sx = X::fmx(wt2, p2, sx); sy_ = X::fmy(wt2, p2, sy_); sz_ = X::fmz(wt2, p2, sz_);
+    sx = X::fmx(wt3, p3, sx); sy_ = X::fmy(wt3, p3, sy_); sz_ = X::fmz(wt3, p3, sz_);
+    sx = X::fmx(wt4, p4, sx); sy_ = X::fmy(wt4, p4, sy_); sz_ = X::fmz(wt4, p4, sz_);
+    sx = X::fmx(wt5, p5, sx); sy_ = X::fmy(wt5, p5, sy_); sz_ = X::fmz(wt5, p5, sz_);
+    sx = X::fmx(wt6, p6, sx); sy_ = X::fmy(wt6, p6, sy_); sz_ = X::fmz(wt6, p6, sz_);
+    sx = X::fmx(wt7, p7, sx); sy_ = X::fmy(wt7, p7, sy_); sz_ = X::fmz(wt7, p7, sz_);
+    out[0] = sx;
+    out[1] = sy_;
+    out[2] = sz_;
+}
+
+// The same from texels in device memory (clamp-to-edge at the volume).
+template <class S>
+__device__ __forceinline__ void trilinear_fused(const Tex<S>& T, const Dims& d, float uu, float vv, float ww,
+                                                float out[3]) {
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    int i0 = (int)fu, j0 = (int)fv, k0 = (int)fw;
+    int i1 = clampi(i0 + 1, 0, d.X - 1);
+    const int j1 = clampi(j0 + 1, 0, d.Y - 1), k1 = clampi(k0 + 1, 0, d.Z - 1);
+    i0 = clampi(i0, 0, d.X - 1);
+    j0 = clampi(j0, 0, d.Y - 1);
+    k0 = clampi(k0, 0, d.Z - 1);
+    const auto r00 = T.row(d, j0, k0), r10 = T.row(d, j1, k0), r01 = T.row(d, j0, k1), r11 = T.row(d, j1, k1);
+    const float4 t[8] = {T.at_raw(r00, i0), T.at_raw(r00, i1), T.at_raw(r10, i0), T.at_raw(r10, i1),
+                         T.at_raw(r01, i0), T.at_raw(r01, i1), T.at_raw(r11, i0), T.at_raw(r11, i1)};
+    const float wt[8] = {(1.0f - a) * (1.0f - b) * (1.0f - c), a * (1.0f - b) * (1.0f - c),
+                         (1.0f - a) * b * (1.0f - c),          a * b * (1.0f - c),
+                         (1.0f - a) * (1.0f - b) * c,          a * (1.0f - b) * c,
+                         (1.0f - a) * b * c,                    a * b * c};
+    float sx = wt[0] * t[0].x, sy = wt[0] * t[0].y, sz = wt[0] * t[0].z;
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        sx = __builtin_fmaf(wt[k], t[k].x, sx);
+        sy = __builtin_fmaf(wt[k], t[k].y, sy);
+        sz = __builtin_fmaf(wt[k], t[k].z, sz);
+    }
+    out[0] = sx;
+    out[1] = sy;
+    out[2] = sz;
 }
 
 // The same sampling from the three-plane F32 tile (same taps, terms and
