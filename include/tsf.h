@@ -184,6 +184,22 @@ tsf_status tsf_gvf_euler(const tsf_field* f, const tsf_field* out, tsf_dims dims
                          int32_t fuse, void* workspace, size_t workspace_bytes, hipStream_t stream);
 size_t tsf_gvf_euler_workspace_size(tsf_dims dims, int32_t storage);
 
+/* runLowMemoryGVF (gradientVectorFlow.cpp:1173-1445; GVF3DInit_one_component,
+ * GVF3DIteration_one_component, GVF3DFinish_one_component,
+ * kernels_no_3d_write.cl:1669-1744), which the reference selects when a whole
+ * field exceeds one allocation (tube-segmentation.cpp:743-762): each
+ * component iterates alone with f's component and |f|^2, so besides f and
+ * out the device holds one |f|^2 plane and one ping-pong plane.  |f|^2 is
+ * stored in the field's format, as in the reference: F32 results equal
+ * tsf_gvf_euler's bit for bit; SNORM16 ones differ from them where |f|^2
+ * rounds.  One step per launch.  f, out: F32 or SNORM16, the same storage;
+ * returns iterate `iterations` (the C++ runGVF adapter rounds to the
+ * reference's 2*floor(N/2)).  Workspace:
+ * tsf_gvf_euler_low_memory_workspace_size (two planes). */
+tsf_status tsf_gvf_euler_low_memory(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu,
+                                    int32_t iterations, void* workspace, size_t workspace_bytes, hipStream_t stream);
+size_t tsf_gvf_euler_low_memory_workspace_size(tsf_dims dims, int32_t storage);
+
 /* The precision form of tsf_gvf_euler (GVF3DIteration, kernels_no_3d_write.cl:
  * 1746-1787, the recurrence of runFastGVF, gradientVectorFlow.cpp:1007-1172):
  * the iterate streams through device memory as 16-bit planes and every step

@@ -494,6 +494,17 @@ def vector_field_sn16(vol, fmax, sign=1):
     return F
 
 
+def gvf_euler_low_memory(F, mu, iters, sn16=False):
+    """runLowMemoryGVF: each component alone with (f_c, |f|^2), |f|^2 and every
+    iterate in the field's format (SNORM16 when sn16; F then holds SNORM16
+    values)."""
+    F = np.ascontiguousarray(F, np.float32)
+    X, Y, Z = _dims(F)
+    V = np.empty((4,) + F.shape[1:], np.float32)
+    lib().orc_gvf_euler_low_memory(_p(F), _p(V), X, Y, Z, ctypes.c_float(mu), int(iters), int(bool(sn16)))
+    return V
+
+
 def gvf_euler_sn16(F, mu, iters):
     """F: (4 or 3, Z, Y, X) float32 holding SNORM16 values."""
     F = np.ascontiguousarray(F, np.float32)

@@ -238,6 +238,52 @@ static void gvf_euler_sn16(const float* f4, float* v4, Dims d, float mu, int ite
     }
 }
 
+// runLowMemoryGVF, gradientVectorFlow.cpp:1173-1445 (GVF3DInit_one_component,
+// GVF3DIteration_one_component, GVF3DFinish_one_component,
+// kernels_no_3d_write.cl:1669-1744): each component iterates alone with the
+// pair (f_c, |f|^2) of the init kernel, |f|^2 = x x + y y + z z stored in the
+// field's format (rounded to SNORM16 in the 16-bit mode), the update
+// v + (mu lap - (v - f_c) |f|^2) at the mirror voxel, every iterate stored
+// in the field's format; then w = |v| (1 where 0).  f4 holds the field's
+// values (SNORM16 values when sn16).
+static void gvf_euler_low_memory(const float* f4, float* v4, Dims d, float mu, int iters, bool sn16) {
+    const i64 N = d.n(), sy = d.X, sz = (i64)d.X * d.Y;
+    auto st = [sn16](float v) { return sn16 ? sn16_round(v) : v; };
+    std::vector<float> m2(N), a(N), b(N);
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < N; i++) {
+        const float x = f4[i], y = f4[N + i], z = f4[2 * N + i];
+        m2[i] = st(x * x + y * y + z * z);
+    }
+    for (int c = 0; c < 3; c++) {
+        const float* f = f4 + c * N;
+        for (i64 i = 0; i < N; i++) a[i] = st(f[i]);
+        for (int it = 0; it < iters; it++) {
+#pragma omp parallel for schedule(static)
+            for (int z = 0; z < d.Z; z++)
+                for (int y = 0; y < d.Y; y++)
+                    for (int x = 0; x < d.X; x++) {
+                        const i64 q = d.at(remap(x, d.X), remap(y, d.Y), remap(z, d.Z));
+                        const float vc = a[q];
+                        float lap = -6.0f * vc;
+                        lap = lap + a[q + 1];
+                        lap = lap + a[q - 1];
+                        lap = lap + a[q + sy];
+                        lap = lap + a[q - sy];
+                        lap = lap + a[q + sz];
+                        lap = lap + a[q - sz];
+                        b[d.at(x, y, z)] = st(vc + (mu * lap - (vc - f[q]) * m2[q]));
+                    }
+            a.swap(b);
+        }
+        std::memcpy(v4 + c * N, a.data(), sizeof(float) * N);
+    }
+    for (i64 i = 0; i < N; i++) {
+        const float l = std::sqrt(v4[i] * v4[i] + v4[N + i] * v4[N + i] + v4[2 * N + i] * v4[2 * N + i]);
+        v4[3 * N + i] = st(l > 0.0f ? l : 1.0f);
+    }
+}
+
 // ---------------------------------------------------------------- FMG GVF (A7)
 // Multigrid solver of gradientVectorFlow.cpp:5-441, 631-1005 with the kernels
 // of kernels_no_3d_write.cl:2071-2354 (FP32 path: no SNORM16 quantisation).
@@ -1800,6 +1846,10 @@ extern "C" void orc_unorm16_round(float* a, long long n) {
 extern "C" void orc_vector_field_sn16(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign) {
     vector_field<float>(I, F, DIMS, Fmax, sign);
     orc_sn16_round(F, 4LL * X * Y * Z);
+}
+extern "C" void orc_gvf_euler_low_memory(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters,
+                                         int sn16) {
+    gvf_euler_low_memory(f4, v4, DIMS, mu, iters, sn16 != 0);
 }
 extern "C" void orc_gvf_euler_sn16(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters) {
     gvf_euler_sn16(f4, v4, DIMS, mu, iters);

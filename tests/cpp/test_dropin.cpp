@@ -54,6 +54,14 @@ void orc_do_eigen_f32(const float* Fx, const float* Fy, const float* Fz, int X, 
                       float* lam, float* e1, float* e2, float* e3);
 void orc_vector_field_sn16(const float* I, float* F, int X, int Y, int Z, float Fmax, int sign);
 void orc_gvf_euler_sn16(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters);
+void orc_gvf_euler_low_memory(const float* f4, float* v4, int X, int Y, int Z, float mu, int iters, int sn16);
+void orc_gvf_euler_sr16(const uint16_t* f16, int storage, float scale, float mu, int iters, int tail, unsigned seed,
+                        int X, int Y, int Z, float* out4);
+void orc_tdf_circle_mixed_f32(const float* FH, const float* FS, int X, int Y, int Z, float rmin, float rmax,
+                              float rstep, float* tdf, float* rad);
+void orc_half_from_float(const float* a, uint16_t* h, long long n);
+void orc_half_to_float(const uint16_t* h, float* a, long long n);
+void orc_sn16_store(const float* a, int16_t* s, long long n);
 void orc_unorm16_round(float* a, long long n);
 int orc_link_centerpoints(const int32_t* v, int sum, const float* T, int X, int Y, int Z, float maxDistance,
                           float minAvgTDF, int32_t* edgesOut, int cap);
@@ -496,6 +504,82 @@ static void gpu_tests(const std::string& dsdir) {
         std::vector<float> o(4 * n);
         orc_gvf_euler_f32(F.data(), o.data(), X, Y, Z, 0.05f, 6);
         return compare(out.downloadPlane(ocl, 0), std::vector<float>(o.begin(), o.begin() + n), 1e-5f, 1e-8f, "x");
+    });
+    // runGVF(..., useLessMemory = true): the reference's one-component GVF
+    // (runLowMemoryGVF).  In F32 its arithmetic is the full-field path's.
+    run("runGVF_low_memory_equals_full_field_f32", [&] {
+        paramList q = p;
+        setParameter(q, "gvf-iterations", "7");
+        Image3D a(size, 4), b(size, 4);
+        for (int c = 0; c < 4; c++) {
+            a.uploadPlane(ocl, c, F.data() + c * n);
+            b.uploadPlane(ocl, c, F.data() + c * n);
+        }
+        I3 s = size;
+        Image3D lo = runGVF(ocl, &a, q, s, true), full = runGVF(ocl, &b, q, s, false);
+        if (a.valid()) return std::string("input field not released");
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++)
+            r = compare(lo.downloadPlane(ocl, c), full.downloadPlane(ocl, c), 0.0f, 0.0f, "one-component field");
+        return r;
+    });
+    // In the 16-bit mode the reference's one-component GVF stores |f|^2 as
+    // SNORM16 (GVF3DInit_one_component): the oracle's restatement, bit for bit.
+    run("runGVF_low_memory_16bit_vectors", [&] {
+        paramList q = p;
+        setParameter(q, "16bit-vectors", "true");
+        setParameter(q, "gvf-iterations", "9");  // iterate 8
+        std::vector<float> S(4 * n), W(4 * n), Wf(4 * n);
+        orc_vector_field_sn16(blur.data(), S.data(), X, Y, Z, 0.1f, 1);
+        Image3D a(size, 4, TSF_STORAGE_SN16, 1.0f);
+        for (int c = 0; c < 4; c++) a.uploadPlane(ocl, c, S.data() + c * n);
+        I3 s = size;
+        Image3D lo = runGVF(ocl, &a, q, s, true);
+        if (lo.storage() != TSF_STORAGE_SN16) return std::string("field not stored as SNORM16");
+        orc_gvf_euler_low_memory(S.data(), W.data(), X, Y, Z, 0.05f, 8, 1);
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++)
+            r = compare(lo.downloadPlane(ocl, c), std::vector<float>(W.begin() + c * n, W.begin() + (c + 1) * n), 0.0f,
+                        0.0f, "one-component 16-bit field");
+        // and it is not the full-field 16-bit result (|f|^2 rounds there per step)
+        orc_gvf_euler_sn16(S.data(), Wf.data(), X, Y, Z, 0.05f, 8);
+        if (r.empty() && std::equal(W.begin(), W.begin() + 3 * n, Wf.begin()))
+            r = "the one-component 16-bit GVF equals the full-field one: |f|^2 not stored as SNORM16";
+        return r;
+    });
+    // vector-storage precision: SNORM16 f, F16 iterate with stochastic
+    // rounding and a 32-step F32 tail, both TDFs in the mixed mode.
+    run("runCircleFittingMethod_precision_form", [&] {
+        paramList q = p;
+        setParameter(q, "vector-storage", "precision");
+        Image3D vfp, tdfp, radp;
+        Image3D ds3 = dataset;
+        runCircleFittingMethod(ocl, &ds3, size, q, vfp, tdfp, radp);
+        if (vfp.storage() != TSF_STORAGE_F32) return std::string("the precision form's GVF result is not F32");
+        auto f16 = [&](const std::vector<float>& a) {  // x, y, z as F16 texels at scale 2^14
+            std::vector<float> b(a), t(3 * n);
+            std::vector<uint16_t> h(3 * n);
+            for (size_t i = 0; i < 3 * n; i++) t[i] = a[i] * 16384.0f;
+            orc_half_from_float(t.data(), h.data(), (long long)(3 * n));
+            orc_half_to_float(h.data(), b.data(), (long long)(3 * n));
+            for (size_t i = 0; i < 3 * n; i++) b[i] = b[i] / 16384.0f;
+            return b;
+        };
+        std::vector<float> S(4 * n), W(4 * n), t1(n), r1(n), t2(n), r2(n);
+        orc_tdf_circle_mixed_f32(Fs.data(), f16(Fs).data(), X, Y, Z, 0.5f, 3.0f, 0.5f, t1.data(), r1.data());
+        orc_vector_field_sn16(blur.data(), S.data(), X, Y, Z, 0.1f, 1);
+        std::vector<int16_t> codes(3 * n);
+        orc_sn16_store(S.data(), codes.data(), (long long)(3 * n));
+        orc_gvf_euler_sr16((const uint16_t*)codes.data(), 3, 0.0f, 0.05f, 250, 32, 0u, X, Y, Z, W.data());
+        orc_tdf_circle_mixed_f32(W.data(), f16(W).data(), X, Y, Z, 2.5f, 4.0f, 1.0f, t2.data(), r2.data());
+        orc_combine_f32(t1.data(), r1.data(), t2.data(), r2.data(), (long long)n);
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++)
+            r = compare(vfp.downloadPlane(ocl, c), std::vector<float>(W.begin() + c * n, W.begin() + (c + 1) * n),
+                        0.0f, 0.0f, "precision-form GVF field");
+        if (r.empty()) r = compare(tdfp.downloadPlane(ocl, 0), t2, 0.0f, 0.0f, "precision-form TDF");
+        if (r.empty()) r = compare(radp.downloadPlane(ocl, 0), r2, 0.0f, 0.0f, "precision-form radius");
+        return r;
     });
 }
 

@@ -1062,6 +1062,40 @@ def test_gvf_sr_one_launch_is_two_float_steps(orc, shape):
     assert np.allclose(out[3], np.sqrt((out[:3].astype(np.float64) ** 2).sum(0)), rtol=1e-6)
 
 
+def test_gvf_low_memory_f32_equals_full_field(orc):
+    """runLowMemoryGVF in float: each component alone with the stored |f|^2
+    is the full-field recurrence exactly (the components couple only through
+    |f|^2, computed in the same order)."""
+    F = _field_from_phantom((9, 14, 17), 48)
+    for iters in (0, 1, 6):
+        lo = orc.gvf_euler_low_memory(F, 0.05, iters)
+        full = orc.gvf_euler(F, 0.05, iters, "f32")
+        assert np.array_equal(lo, full), iters
+
+
+def test_gvf_low_memory_sn16_rounds_only_the_magnitude(orc):
+    """In the 16-bit mode |f|^2 is stored as SNORM16 once: on a field whose
+    |f|^2 is exactly 0 or 1 (one component +-1 per voxel, or none) that
+    rounding is exact and the one-component GVF equals the full-field 16-bit
+    GVF bit for bit; on a general field it differs, by less than a few
+    codes."""
+    g = rng(49)
+    shape = (8, 11, 13)
+    F = np.zeros((4,) + shape, np.float32)
+    comp = g.integers(0, 4, shape)
+    sign = np.where(g.random(shape) < 0.5, -1.0, 1.0).astype(np.float32)
+    for c in range(3):
+        F[c][comp == c] = sign[comp == c]
+    F[3] = 1.0
+    for iters in (1, 5):
+        assert np.array_equal(orc.gvf_euler_low_memory(F, 0.05, iters, sn16=True), orc.gvf_euler_sn16(F, 0.05, iters))
+    Fs = orc.sn16_round(_field_from_phantom(shape, 50))
+    lo = orc.gvf_euler_low_memory(Fs, 0.05, 6, sn16=True)
+    full = orc.gvf_euler_sn16(Fs, 0.05, 6)
+    assert not np.array_equal(lo[:3], full[:3])
+    assert np.abs(lo[:3] - full[:3]).max() <= 8 / 32767
+
+
 def _sn16_field(shape, seed):
     import oracle as o
     F = _field_from_phantom(shape, seed)

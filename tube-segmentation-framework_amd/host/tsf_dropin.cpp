@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <locale>
@@ -101,8 +102,9 @@ paramList initParameters() {
     B("use-spline-tdf", false, "tube-detection-filter");
     B("use-fmg-gvf", false, "gradient-vector-flow");
     B("3d_write", true, "advanced");
-    S("vector-storage", "auto", {"auto", "f32", "f16", "bf16", "sn16"}, "advanced");
+    S("vector-storage", "auto", {"auto", "f32", "f16", "bf16", "sn16", "precision"}, "advanced");
     N("gvf-fuse", 2, 1, 16, 1, "gradient-vector-flow");
+    N("gvf-f32-tail", 32, 0, 10000, 1, "gradient-vector-flow");
     return p;
 }
 
@@ -312,8 +314,11 @@ std::vector<float> Image3D::downloadPlane(OpenCL& ocl, int c) const {
     return out;
 }
 
+bool precisionForm(const paramList& p) { return getParamStr(p, "vector-storage") == "precision"; }
+
 tsf_storage vectorStorage(const paramList& p) {
     const std::string s = getParamStr(p, "vector-storage");
+    if (s == "precision") return TSF_STORAGE_SN16;  // the large branch's f; the small branch's field is F32
     if (s == "f32") return TSF_STORAGE_F32;
     if (s == "f16") return TSF_STORAGE_F16;
     if (s == "bf16") return TSF_STORAGE_BF16;
@@ -338,19 +343,52 @@ float* createBlurMask(float sigma, int* maskSizePointer) {
 }
 
 Image3D runGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size, bool useLessMemory) {
-    (void)useLessMemory;
     if (!vectorField || !vectorField->valid()) fail(TSF_ERR_INVALID_ARGUMENT, "runGVF: no input field");
     const int iters = (int)getParam(parameters, "gvf-iterations");
     const float mu = getParam(parameters, "gvf-mu");
-    const int fuse = (int)getParam(parameters, "gvf-fuse");
     Image3D in = *vectorField;
-    Image3D out(size, 4, in.storage(), in.scale());
-    tsf_field f = in.field(), o = out.field();
-    Buffer ws(tsf_gvf_euler_workspace_size(out.dims(), out.storage()));
-    check(tsf_gvf_euler(&f, &o, out.dims(), mu, 2 * (iters / 2), fuse, ws.get(), ws.size(), ocl.queue));
+    tsf_field f = in.field();
+    Image3D out;
+    if (precisionForm(parameters) && in.storage() == TSF_STORAGE_SN16) {
+        // the precision form: 16-bit iterate with stochastic rounding, the
+        // last gvf-f32-tail iterations in float, the result F32
+        out = Image3D(size, 4);
+        tsf_field o = out.field();
+        const int tail = (int)getParam(parameters, "gvf-f32-tail");
+        Buffer ws(tsf_gvf_euler_sr_workspace_size(out.dims(), tail));
+        check(tsf_gvf_euler_sr(&f, &o, out.dims(), mu, 2 * (iters / 2), tail, 0u, ws.get(), ws.size(), ocl.queue));
+    } else if (useLessMemory && (in.storage() == TSF_STORAGE_F32 || in.storage() == TSF_STORAGE_SN16)) {
+        // runLowMemoryGVF: one component at a time (gradientVectorFlow.cpp:1173-1445)
+        out = Image3D(size, 4, in.storage(), in.scale());
+        tsf_field o = out.field();
+        Buffer ws(tsf_gvf_euler_low_memory_workspace_size(out.dims(), out.storage()));
+        check(tsf_gvf_euler_low_memory(&f, &o, out.dims(), mu, 2 * (iters / 2), ws.get(), ws.size(), ocl.queue));
+    } else {
+        if (useLessMemory)
+            std::fprintf(stderr, "NOTE: the one-component GVF takes F32 or SNORM16 fields; running the full-field "
+                                 "GVF on this %s field\n", in.storage() == TSF_STORAGE_F16 ? "F16" : "BF16");
+        out = Image3D(size, 4, in.storage(), in.scale());
+        tsf_field o = out.field();
+        Buffer ws(tsf_gvf_euler_workspace_size(out.dims(), out.storage()));
+        check(tsf_gvf_euler(&f, &o, out.dims(), mu, 2 * (iters / 2), (int)getParam(parameters, "gvf-fuse"), ws.get(),
+                            ws.size(), ocl.queue));
+    }
     ocl.finish();
     *vectorField = Image3D();  // the reference deletes the input field
     return out;
+}
+
+// The reference runs the one-component GVF when a whole 4-channel field
+// exceeds the device's largest allocation (tube-segmentation.cpp:743-762).
+// HIP allocations are bounded by free memory instead: the one-component
+// path is taken when the full-field one (the output field and three
+// workspace planes) does not fit in it.
+static bool gvfNeedsLowMemory(const Image3D& F, const paramList& p) {
+    if (precisionForm(p)) return false;
+    size_t freeB = 0, total = 0;
+    check(tsf_mem_info(&freeB, &total));
+    const size_t e = F.storage() == TSF_STORAGE_F32 ? 4 : 2;
+    return 7 * e * voxels(F.size()) > freeB;
 }
 
 Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size) {
@@ -378,8 +416,17 @@ Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3
 // are 16-bit too (TDF_TYPE ushort, kernels_no_3d_write.cl:19-21): an SN16
 // field gets its TDF rounded to that format.  Synchronises (the workspace is
 // released on return).
-static void circleTDF(OpenCL& ocl, const Image3D& field, float rmin, float rmax, float rstep, float* T, float* R) {
+static void circleTDF(OpenCL& ocl, const Image3D& field, float rmin, float rmax, float rstep, float* T, float* R,
+                      bool mixed = false) {
     tsf_field f = field.field();
+    if (mixed && field.storage() == TSF_STORAGE_F32) {
+        // the precision form's TDF: Hessian from the F32 planes, samples from F16 texels
+        Buffer ws(tsf_tdf_circle_mixed_workspace_size(field.dims()));
+        check(tsf_tdf_circle_mixed(&f, TSF_STORAGE_F16, 16384.0f, field.dims(), rmin, rmax, rstep, T, R, ws.get(),
+                                   ws.size(), ocl.queue));
+        ocl.finish();
+        return;
+    }
     Buffer ws(tsf_tdf_circle_workspace_size(field.dims(), field.storage()));
     check(tsf_tdf_circle(&f, field.dims(), rmin, rmax, rstep, T, R, ws.get(), ws.size(), ocl.queue));
     if (field.storage() == TSF_STORAGE_SN16) check(tsf_tdf_round_unorm16(T, (int64_t)voxels(field.size()), ocl.queue));
@@ -422,8 +469,10 @@ static Image3D blurOrSame(OpenCL& ocl, Image3D& data, float sigma) {
     return out;
 }
 
-static Image3D vectorFieldOf(OpenCL& ocl, Image3D& vol, paramList& p) {
-    const tsf_storage s = vectorStorage(p);
+static Image3D vectorFieldOf(OpenCL& ocl, Image3D& vol, paramList& p, bool large) {
+    // the precision form keeps the small branch's field in F32 (its TDF takes
+    // the Hessian from it) and stores the large branch's, the GVF's f, as SNORM16
+    const tsf_storage s = precisionForm(p) && !large ? TSF_STORAGE_F32 : vectorStorage(p);
     Image3D F(vol.size(), 4, s, vectorScale(s));
     tsf_field f = F.field();
     const int sign = getParamStr(p, "mode") == "black" ? -1 : 1;
@@ -440,10 +489,10 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
     Image3D tdfSmall, radSmall;
     if (rMin < 2.5f) {
         Image3D blurred = blurOrSame(ocl, *dataset, getParam(p, "small-blur"));
-        Image3D Fs = vectorFieldOf(ocl, blurred, p);
+        Image3D Fs = vectorFieldOf(ocl, blurred, p, false);
         tdfSmall = Image3D(size, 1);
         radSmall = Image3D(size, 1);
-        circleTDF(ocl, Fs, rMin, 3.0f, 0.5f, (float*)tdfSmall.plane(0), (float*)radSmall.plane(0));
+        circleTDF(ocl, Fs, rMin, 3.0f, 0.5f, (float*)tdfSmall.plane(0), (float*)radSmall.plane(0), precisionForm(p));
         if (rMax < 2.5f) {
             TDF = tdfSmall;
             radiusImage = radSmall;
@@ -453,12 +502,12 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
         }
     }
     Image3D blurred = blurOrSame(ocl, *dataset, getParam(p, "large-blur"));
-    Image3D F = vectorFieldOf(ocl, blurred, p);
+    Image3D F = vectorFieldOf(ocl, blurred, p, true);
     ocl.finish();
     *dataset = Image3D();
     blurred = Image3D();
     if (getParamBool(p, "use-fmg-gvf")) vectorField = runFMGGVF(ocl, &F, p, size);
-    else vectorField = runGVF(ocl, &F, p, size, false);
+    else vectorField = runGVF(ocl, &F, p, size, gvfNeedsLowMemory(F, p));
     TDF = Image3D(size, 1);
     radiusImage = Image3D(size, 1);
     // the large-radius TDF: spline (from 1.5) or circle fitting (from 2.5),
@@ -468,7 +517,7 @@ void runCircleFittingMethod(OpenCL& ocl, Image3D* dataset, int3 size, paramList&
                   (float*)radiusImage.plane(0));
     else
         circleTDF(ocl, vectorField, std::max(2.5f, rMin), rMax, rStep, (float*)TDF.plane(0),
-                  (float*)radiusImage.plane(0));
+                  (float*)radiusImage.plane(0), precisionForm(p));
     if (rMin < 2.5f)
         check(tsf_combine((const float*)tdfSmall.plane(0), (const float*)radSmall.plane(0), (float*)TDF.plane(0),
                           (float*)radiusImage.plane(0), (int64_t)n, ocl.queue));

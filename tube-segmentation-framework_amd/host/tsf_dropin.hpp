@@ -178,8 +178,13 @@ class Image3D {
     std::vector<Buffer> planes;
 };
 
-// Storage the path uses for the vector field under these parameters.
+// Storage the path uses for the vector field under these parameters
+// (vector-storage precision: SNORM16 for the GVF's f).
 tsf_storage vectorStorage(const paramList& parameters);
+// vector-storage precision: the GVF iterate in F16 with stochastic rounding
+// (tsf_gvf_euler_sr, the last gvf-f32-tail iterations in F32, the result F32)
+// and both circle TDFs in the mixed mode (tsf_tdf_circle_mixed).
+bool precisionForm(const paramList& parameters);
 float vectorScale(tsf_storage s);
 
 // ----------------------------------------------------------------- the volume-filter path
@@ -190,8 +195,11 @@ float* createBlurMask(float sigma, int* maskSizePointer);
 // reference deletes it) and return the new field (channels 4, w = |v|).
 // runGVF returns iterate 2*floor(gvf-iterations/2): the reference's finish
 // step reads the first ping-pong buffer (gradientVectorFlow.cpp:1071, 1161).
-// useLessMemory selects the reference's per-component variant, which computes
-// the same values; here both run the same kernels.
+// useLessMemory selects the reference's one-component variant
+// (runLowMemoryGVF, tsf_gvf_euler_low_memory) for F32 and SNORM16 fields:
+// equal to the full-field result in F32; in the 16-bit mode |f|^2 is stored
+// as SNORM16 as the reference does.  F16 / BF16 fields run the full-field
+// path (a note on stderr).  vector-storage precision runs tsf_gvf_euler_sr.
 Image3D runGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size, bool useLessMemory);
 Image3D runFMGGVF(OpenCL& ocl, Image3D* vectorField, paramList& parameters, int3& size);
 

@@ -56,6 +56,8 @@ _SIGS = {
     "tsf_create_vector_field": ([_vp, _PF, Dims, _f32, _i32, _vp], _i32),
     "tsf_gvf_euler": ([_PF, _PF, Dims, _f32, _i32, _i32, _vp, _sz, _vp], _i32),
     "tsf_gvf_euler_workspace_size": ([Dims, _i32], _sz),
+    "tsf_gvf_euler_low_memory": ([_PF, _PF, Dims, _f32, _i32, _vp, _sz, _vp], _i32),
+    "tsf_gvf_euler_low_memory_workspace_size": ([Dims, _i32], _sz),
     "tsf_gvf_fmg": ([_PF, _PF, Dims, _f32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _sz, _vp], _i32),
     "tsf_gvf_fmg_workspace_size": ([Dims, _i32], _sz),
     "tsf_hessian_eigen": ([_PF, Dims, _i32, _vp, _vp, _vp], _i32),
@@ -410,6 +412,18 @@ def gvf_euler(f, out, mu, iterations, fuse=1, workspace=None, stream=None):
     w, wb = _ws(workspace)
     _check(lib().tsf_gvf_euler(ctypes.byref(fd), ctypes.byref(od), _dims(f.shape), mu, iterations, fuse, w, wb,
                                _sp(stream)))
+
+
+def gvf_euler_low_memory_workspace_size(shape, storage):
+    return lib().tsf_gvf_euler_low_memory_workspace_size(_dims(shape), storage)
+
+
+def gvf_euler_low_memory(f, out, mu, iterations, workspace=None, stream=None):
+    """runLowMemoryGVF: one component at a time (F32 or SNORM16 fields)."""
+    fd, od = f.desc(), out.desc()
+    w, wb = _ws(workspace)
+    _check(lib().tsf_gvf_euler_low_memory(ctypes.byref(fd), ctypes.byref(od), _dims(f.shape), mu, iterations, w, wb,
+                                          _sp(stream)))
 
 
 def gvf_fmg_workspace_size(shape, levels):
