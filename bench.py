@@ -507,7 +507,7 @@ def fit(args, vol, d_vol, shape, stream):
                                f"{CHAIN['rmin']}-{CHAIN['rmax']}", "start": {"mu": 0.03, "sigma": 0.8}},
         "loss_first": hist[0]["loss"], "loss_last": hist[-1]["loss"], "mu": mu, "sigma": sigma,
         "hbm_peak_gb": round(max(h["hbm_bytes"] or 0 for h in hist) / 1e9, 2),
-        "history": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in h.items() if k != "hbm_bytes"}
+        "history": [{k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in h.items() if k != "hbm_bytes"}
                     for h in hist]}), flush=True)
 
 

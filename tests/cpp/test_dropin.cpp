@@ -951,14 +951,49 @@ static void run_tests(const std::string& dsdir) {
     });
 }
 
+// The ridge method of the benchmark's C1 workload on a volume given as a
+// .mhd file (intensities 0..1): mode white, blur 0.5 for both branches, GVF
+// mu 0.1 x 250 in the reference's default 16-bit mode, radii 1..5 step 0.5;
+// the centerline mask is written to `out` (one byte per voxel) for the
+// caller to score against the known tube axes.
+static void ridge_volume(const std::string& mhd, const std::string& outRaw) {
+    run("ridge_on_volume", [&] {
+        paramList p = initParameters();
+        setParameter(p, "centerline-method", "ridge");
+        for (auto kv : std::vector<std::pair<const char*, const char*>>{
+                 {"mode", "white"}, {"minimum", "0"}, {"maximum", "1"}, {"small-blur", "0.5"}, {"large-blur", "0.5"},
+                 {"gvf-mu", "0.1"}, {"gvf-iterations", "250"}, {"radius-min", "1"}, {"radius-max", "5"},
+                 {"radius-step", "0.5"}})
+            setParameter(p, kv.first, kv.second);
+        TSFOutput* out = run(mhd, p, "");
+        const tsf::int3* sz = out->getSize();
+        const size_t n = (size_t)sz->x * sz->y * sz->z;
+        const char* cl = out->getCenterlineVoxels();
+        std::string r;
+        if (!cl) r = "no centerline";
+        else {
+            std::ofstream f(outRaw, std::ios::binary);
+            f.write(cl, (std::streamsize)n);
+            long cnt = 0;
+            for (size_t i = 0; i < n; i++) cnt += cl[i] != 0;
+            std::printf("  ridge: %ld centerline voxels of %zu\n", cnt, n);
+            if (!f) r = "cannot write " + outRaw;
+        }
+        delete out;
+        return r;
+    });
+}
+
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "params";
     if (mode == "params") param_tests();
     else if (mode == "gpu" && argc > 2) gpu_tests(argv[2]);
     else if (mode == "autograd") autograd_tests();
     else if (mode == "run" && argc > 2) run_tests(argv[2]);
+    else if (mode == "ridge_volume" && argc > 3) ridge_volume(argv[2], argv[3]);
     else {
-        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir> | run <dataset_dir> | autograd\n");
+        std::fprintf(stderr, "usage: test_dropin params | gpu <dataset_dir> | run <dataset_dir> | autograd | "
+                             "ridge_volume <volume.mhd> <centerline.raw>\n");
         return 2;
     }
     std::printf("%d failure(s)\n", failures);

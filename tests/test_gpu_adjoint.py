@@ -240,66 +240,70 @@ def test_chain_gradient_vs_fp64_oracle(tsf, orc):
     assert abs(dsig - odsig) <= 1e-3 * abs(odsig)
 
 
-def test_chain_finite_differences_128(tsf, orc):
-    """Differentiation form at 128^3 (tube tree r ~ U(1, 5), noise 0.1): 500
-    GVF iterations checkpointed every 50, TDF radii 1..5.  mu is 0.05: mu = 0.1
-    breaks the explicit scheme's stability bound 12 mu + |f|^2 <= 2 where
-    |f| = 1, and the field diverges (in FP64 as well).
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("seed", [401, 402, 403, 404])
+def test_chain_finite_differences_128(tsf, orc, seed):
+    """Differentiation form at 128^3 on the four seeded volumes of the
+    benchmark's C4 (tube tree r ~ U(1, 5), noise 0.1): 500 GVF iterations
+    checkpointed every 50, TDF radii 1..5.  mu is 0.05: mu = 0.1 breaks the
+    explicit scheme's stability bound 12 mu + |f|^2 <= 2 where |f| = 1, and
+    the field diverges (in FP64 as well).
 
-    Central differences, h = 1e-3: in mu and sigma on the float GPU forward;
-    along a signed random subset of 64 voxels on the FP64 forward, because the
-    float forward's rounding noise (~5e-5 in L over this volume) is of the
-    order of 1e-3 of that directional difference.  Voxels whose radius
-    switches, whose response has a kink inside [-h, h], or (for the float /
-    double comparison) whose Hessian eigenvalues lie within 1e-2 of each
-    other are left out of the loss."""
+    The gradient of the float GPU chain (dL/dmu, dL/dsigma, and dL/dvolume
+    along a signed random subset of 64 voxels) against central differences of
+    the FP64 forward with h = 1e-4.  Voxels whose radius switches, or whose
+    response has a kink inside [-h, h] (a second difference above
+    (h / 1e-3)^2 * 1e-4, i.e. a curvature beyond 100), are left out of the
+    loss, and so are voxels whose Hessian eigenvalues lie within 1e-2 of each
+    other (the eigenvector derivative ~ 1/gap amplifies the float / double
+    difference of the field there).  With h = 1e-3 a kink inside the
+    interval can pass the detector: FP64 analytic against FP64 differences
+    then disagree by up to 8e-3 on seed 403, and agree to 3e-7 at 1e-4."""
     from tsf import synth
-    vol, _ = synth.config4(seeds=(401,))[0]
-    mu, sigma, h = 0.05, 0.8, 1e-3
+    vol, _ = synth.config4(seeds=(seed,))[0]
+    mu, sigma, h = 0.05, 0.8, 1e-4
     kw = dict(iterations=500, rmin=1.0, rmax=5.0, rstep=1.0, checkpoint_every=50)
     ctx, t0, r0 = _forward(tsf, vol, mu=mu, sigma=sigma, **kw)
     g = rng(6)
     gT = np.where(t0 > 0.05, g.uniform(0.5, 1.5, t0.shape), 0.0)
-    smooth = gT != 0
+    V64 = vol.astype(np.float64)
 
-    def kinks(name, tp, tm, rp, rm, tc, rc):
-        flip = (rp != rc) | (rm != rc)
-        kink = np.abs(tp - 2.0 * tc + tm) > 1e-4
+    def fwd64(I, m, sg):
+        return orc.chain_forward(I, sg, 0.2, 1, m, 500, 1.0, 5.0, 1.0, "f64")
+
+    t64, r64 = fwd64(V64, float(f32(mu)), float(f32(sigma)))
+    smooth = (gT != 0) & (r64 == r0)
+    kthr = 1e-4 * (h / 1e-3) ** 2
+
+    def kinks(name, tp, tm, rp, rm):
+        flip = (rp != r64) | (rm != r64)
+        kink = np.abs(tp - 2.0 * t64 + tm) > kthr
         print(f"{name}: radius switches {int((flip & (gT != 0)).sum())}, "
               f"other kinks {int((kink & ~flip & (gT != 0)).sum())}")
         return ~flip & ~kink
 
-    diffs = {}
-    for name, a, b, dh in (("mu", dict(mu=mu + h, sigma=sigma), dict(mu=mu - h, sigma=sigma), f32(mu + h) - f32(mu - h)),
-                           ("sigma", dict(mu=mu, sigma=sigma + h), dict(mu=mu, sigma=sigma - h),
-                            f32(sigma + h) - f32(sigma - h))):
-        _, tp, rp = _forward(tsf, vol, **a, **kw)
-        _, tm, rm = _forward(tsf, vol, **b, **kw)
-        smooth &= kinks(name, tp, tm, rp, rm, t0, r0)
-        diffs[name] = (tp.astype(np.float64) - tm.astype(np.float64), dh)
-    # volume direction on the FP64 forward
+    # volume direction: 64 voxels of large |dL/dI|, signed as the gradient
     dI0, _, _ = _backward(tsf, ctx, gT)
     nz = np.flatnonzero(np.abs(dI0) >= np.median(np.abs(dI0[dI0 != 0])))
     sub = g.choice(nz, 64, replace=False)
     D = np.zeros(vol.size)
     D[sub] = np.sign(dI0.ravel()[sub])
     D = D.reshape(vol.shape)
-    V64 = vol.astype(np.float64)
-    args = (f32(sigma), 0.2, 1, f32(mu), 500, 1.0, 5.0, 1.0, "f64")
-    t64, r64 = orc.chain_forward(V64, *args)
-    tp, rp = orc.chain_forward(V64 + h * D, *args)
-    tm, rm = orc.chain_forward(V64 - h * D, *args)
-    smooth &= (r64 == r0) & kinks("volume (FP64)", tp, tm, rp, rm, t64, r64)
-    # float against double: leave out voxels whose Hessian eigenvalues nearly
-    # coincide (the eigenvector derivative ~ 1/gap amplifies the float/double
-    # difference of the field there)
+    diffs = {}
+    m64, s64 = float(f32(mu)), float(f32(sigma))
+    for name, a, b, dh in (("mu", (V64, m64 + h, s64), (V64, m64 - h, s64), 2 * h),
+                           ("sigma", (V64, m64, s64 + h), (V64, m64, s64 - h), 2 * h),
+                           ("volume", (V64 + h * D, m64, s64), (V64 - h * D, m64, s64), None)):
+        tp, rp = fwd64(*a)
+        tm, rm = fwd64(*b)
+        smooth &= kinks(name, tp, tm, rp, rm)
+        diffs[name] = (tp - tm, dh)
     lam = tsf.DeviceArray((3,) + vol.shape, np.float32)
     tsf.hessian_eigen(ctx.saved["V"], True, lam)
     L = lam.download()
     gap = np.minimum(np.abs(L[1] - L[0]), np.minimum(np.abs(L[1] - L[2]), np.abs(L[2] - L[0])))
     print(f"eigenvalue gap < 1e-2: {int(((gap < 1e-2) & (gT != 0)).sum())} voxels")
     smooth &= gap >= 1e-2
-    diffs["volume"] = (tp - tm, None)
     print(f"voxels in loss: {int(smooth.sum())} of {int((gT != 0).sum())}")
     assert smooth.sum() > 0.8 * (gT != 0).sum()
     gTm = np.where(smooth, gT, 0.0).astype(np.float32).astype(np.float64)

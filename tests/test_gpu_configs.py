@@ -10,7 +10,8 @@ and through size-independent properties at the full size otherwise.
       TDF), which is where mu = 0.1 is meaningful: the float scheme is
       unstable there (12 mu + |f|^2 > 2) and the 16-bit stores saturate.
   C2  airway tree (dark lumen: vector field sign -1), FMG with 3 levels,
-      mu 0.15, TDF, candidate compaction:
+      mu 0.15, 10 iterations (the AAA-Vessels-CT preset), TDF, candidate
+      compaction:
       oracle parity on a 64^3 airway, properties on the 256^3 one.
   C3  and C5 (the benchmark's 512^3 forward): tests/test_gpu_precision.py,
       every storage form against the FP64 oracle (mu 0.05: mu 0.1 is
@@ -81,7 +82,8 @@ def test_config1_reference_16bit_mode(tsf, orc):
     assert np.array_equal(gr, ro)
 
 
-def _c2_pipeline(tsf, vol, levels=3, mu=0.15, cycles=3, threshold=0.5):
+def _c2_pipeline(tsf, vol, levels=3, mu=0.15, cycles=10, threshold=0.5):
+    # gvf-iterations 10: the AAA-Vessels-CT preset (parameters/centerline-gpu/AAA-Vessels-CT:7)
     # dark lumen in bright walls: the reference's "black" mode (sign -1)
     shape = vol.shape
     F = tsf.Field(shape)
@@ -104,7 +106,7 @@ def test_config2_airway_fmg_64(tsf, orc):
     F, V, norms, t, r, m, cnt, pos = _c2_pipeline(tsf, vol)
     Fh = F.download()
     assert np.array_equal(Fh, orc.vector_field(vol, 0.2, -1, "f32"))
-    Vo = orc.gvf_fmg(Fh, 0.15, 3, 1, 2, 2, 3, "f32")
+    Vo = orc.gvf_fmg(Fh, 0.15, 10, 1, 2, 2, 3, "f32")
     nbad, e = close(V.download(), Vo, 1e-4, 1e-6)
     print(f"C2 64^3 FMG: L2 rel {e:.2e}; residual norms {norms[:, -1]}")
     assert nbad == 0, e
