@@ -434,6 +434,19 @@ def tdf_adjoint(F, rmin, rmax, rstep, gT, prec="f64"):
     return dV, dW
 
 
+def tdf_adjoint_magnitude(F, rmin, rmax, rstep, gT, prec="f64"):
+    """The TDF adjoint's scatter of |contribution|: per voxel, the sum of the
+    magnitudes that dV (and dW) add up -- the scale of their rounding."""
+    F = np.ascontiguousarray(F, _dt(prec))
+    gT = np.ascontiguousarray(gT, _dt(prec))
+    X, Y, Z = _dims(F)
+    dV = np.empty((3,) + F.shape[1:], _dt(prec))
+    dW = np.empty(F.shape[1:], _dt(prec))
+    _fn("tdf_adjoint_abs", prec)(_p(F), X, Y, Z, ctypes.c_float(rmin), ctypes.c_float(rmax), ctypes.c_float(rstep),
+                                 _p(gT), _p(dV), _p(dW))
+    return dV, dW
+
+
 def chain_forward(I, sigma, fmax, sign, mu, iters, rmin, rmax, rstep, prec="f64"):
     I = np.ascontiguousarray(I, _dt(prec))
     X, Y, Z = _dims(I)

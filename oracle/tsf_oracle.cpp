@@ -1566,8 +1566,11 @@ static void trilinear_grad(const T* F, Dims d, T u, T v, T w, T s[3], T J[3][3],
 }
 
 // TDF adjoint: dV (3 planes) and dW (w plane) from gT, radius r* held fixed.
-template <typename T>
+// ABS: the same scatter of |contribution|, a per-voxel scale of what the
+// sums add up (for tolerances where contributions cancel).
+template <typename T, bool ABS = false>
 static void tdf_adjoint(const T* F, Dims d, float rMin, float rMax, float rStep, const T* gT, T* dV, T* dW) {
+    auto m = [](T v) { return ABS ? std::fabs(v) : v; };
     make_tables();
     const i64 N = d.n();
     const bool norm = !(rMax < 4);
@@ -1595,7 +1598,7 @@ static void tdf_adjoint(const T* F, Dims d, float rMin, float rMax, float rStep,
                     trilinear_grad(F, d, (T)x + r * dj[0], (T)y + r * dj[1], (T)z + r * dj[2], s, J, idx, wt);
                     const T cf = -g / (T)32;
                     for (int k = 0; k < 8; k++)
-                        for (int c = 0; c < 3; c++) dV[c * N + idx[k]] += cf * dj[c] * wt[k];
+                        for (int c = 0; c < 3; c++) dV[c * N + idx[k]] += m(cf * dj[c] * wt[k]);
                     T u[3];
                     for (int mm = 0; mm < 3; mm++)
                         u[mm] = s[mm] + r * (dj[0] * J[0][mm] + dj[1] * J[1][mm] + dj[2] * J[2][mm]);
@@ -1624,10 +1627,10 @@ static void tdf_adjoint(const T* F, Dims d, float rMin, float rMax, float rStep,
                     const i64 q = d.at(clampi(xx, 0, d.X - 1), clampi(yy, 0, d.Y - 1), clampi(zz, 0, d.Z - 1));
                     if (norm) {
                         const T wq = W[q];
-                        dV[comp * N + q] += coefv / wq;
-                        dW[q] -= coefv * F[comp * N + q] / (wq * wq);
+                        dV[comp * N + q] += m(coefv / wq);
+                        dW[q] += m(-(coefv * F[comp * N + q] / (wq * wq)));
                     } else {
-                        dV[comp * N + q] += coefv;
+                        dV[comp * N + q] += m(coefv);
                     }
                 };
                 put(0, x + 1, y, z, (T)0.5 * gxx);
@@ -1790,6 +1793,10 @@ static void chain_gradient(const T* I, Dims d, T sigma, float Fmax, int sign, T 
     extern "C" void orc_tdf_adjoint_##S(const T* F, int X, int Y, int Z, float rmin, float rmax,            \
                                         float rstep, const T* gT, T* dV, T* dW) {                            \
         tdf_adjoint<T>(F, DIMS, rmin, rmax, rstep, gT, dV, dW);                                              \
+    }                                                                                                        \
+    extern "C" void orc_tdf_adjoint_abs_##S(const T* F, int X, int Y, int Z, float rmin, float rmax,        \
+                                            float rstep, const T* gT, T* dV, T* dW) {                        \
+        tdf_adjoint<T, true>(F, DIMS, rmin, rmax, rstep, gT, dV, dW);                                        \
     }                                                                                                        \
     extern "C" void orc_chain_forward_##S(const T* I, int X, int Y, int Z, double sigma, float Fmax,        \
                                           int sign, double mu, int iters, float rmin, float rmax,           \

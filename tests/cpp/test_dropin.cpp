@@ -464,7 +464,58 @@ static void gpu_tests(const std::string& dsdir) {
             F3 t = getTubeDirection(T, q, size);
             if (t.x != a.x || t.y != a.y || t.z != a.z) return std::string("getTubeDirection != e1");
         }
-        releaseEigenMirror(T);
+        return std::string();
+    });
+    // doEigen reads the host arrays at every call, as the reference's does: a
+    // change made in place between two calls shows in the second.
+    run("doEigen_sees_arrays_changed_in_place", [&] {
+        TubeSegmentation T;
+        std::vector<float> fx = vf.downloadPlane(ocl, 0), fy = vf.downloadPlane(ocl, 1), fz = vf.downloadPlane(ocl, 2);
+        T.Fx = fx.data();
+        T.Fy = fy.data();
+        T.Fz = fz.data();
+        const I3 q(40, 45, 50);
+        F3 l0, a0, b0, c0, l1, a1, b1, c1;
+        doEigen(T, q, size, &l0, &a0, &b0, &c0);
+        // a linear ramp added to Fx around q changes the Jacobian there
+        for (int z = q.z - 1; z <= q.z + 1; z++)
+            for (int y = q.y - 1; y <= q.y + 1; y++)
+                for (int x = q.x - 1; x <= q.x + 1; x++)
+                    fx[(size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z)] += 0.3f * (float)(x + 2 * y - z);
+        doEigen(T, q, size, &l1, &a1, &b1, &c1);
+        float ol[3], o1[3], o2[3], o3[3];
+        orc_do_eigen_f32(fx.data(), fy.data(), fz.data(), X, Y, Z, q.x, q.y, q.z, ol, o1, o2, o3);
+        const float g[6] = {l1.x, l1.y, l1.z, a1.x, a1.y, a1.z}, o[6] = {ol[0], ol[1], ol[2], o1[0], o1[1], o1[2]};
+        for (int i = 0; i < 6; i++)
+            if (!(std::fabs(g[i] - o[i]) <= 1e-4f * std::fabs(o[i]) + 1e-5f)) return std::string("not the new arrays");
+        if (l0.x == l1.x && l0.y == l1.y && l0.z == l1.z) return std::string("eigenvalues unchanged");
+        return std::string();
+    });
+    // the caller-owned device copy for batches: equal to the host doEigen
+    // (the same operations and order), and stale until refresh()
+    run("EigenField_batch_equals_doEigen", [&] {
+        TubeSegmentation T;
+        std::vector<float> fx = vf.downloadPlane(ocl, 0), fy = vf.downloadPlane(ocl, 1), fz = vf.downloadPlane(ocl, 2);
+        T.Fx = fx.data();
+        T.Fy = fy.data();
+        T.Fz = fz.data();
+        std::vector<I3> pts;
+        for (int i = 0; i < 500; i++) pts.push_back(I3(1 + (i * 37) % 98, 1 + (i * 53) % 98, 1 + (i * 71) % 98));
+        pts.push_back(I3(0, 5, 5));  // out of range: NaN on both
+        EigenField E(ocl, T, size);
+        std::vector<float> dev, host;
+        E.batch(pts, dev);
+        doEigenBatch(T, size, pts, host);
+        for (size_t i = 0; i < host.size(); i++)
+            if (!(dev[i] == host[i] || (std::isnan(dev[i]) && std::isnan(host[i]))))
+                return std::string("device batch differs from the host doEigen");
+        for (float& v : fx) v = 2.0f * v + 0.1f;
+        E.refresh(T);
+        E.batch(pts, dev);
+        doEigenBatch(T, size, pts, host);
+        for (size_t i = 0; i < host.size(); i++)
+            if (!(dev[i] == host[i] || (std::isnan(dev[i]) && std::isnan(host[i]))))
+                return std::string("device batch after refresh differs from the host doEigen");
         return std::string();
     });
     // The reference's default parameters: 16bit-vectors on.  Vector fields and
@@ -861,7 +912,6 @@ static void run_tests(const std::string& dsdir) {
         setParameter(q, "tdf-high", "0.5");
         setParameter(q, "min-distance", "5");
         char* c = runRidgeTraversal(TS, I3(TX, TY, TZ), q, std::stack<CenterlinePoint>());
-        releaseEigenMirror(TS);
         std::string err;
         size_t on = 0;
         for (int z = 0; z < TZ; z++)

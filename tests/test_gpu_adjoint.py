@@ -175,6 +175,47 @@ def test_tdf_adjoint(tsf, orc, rmin, rmax, rstep):
         assert er < 1e-6
 
 
+def test_tdf_adjoint_gradient_spanning_six_decades(tsf, orc):
+    """dL/dT spread over six decades inside every brick (10^-6u, u uniform).
+    The scatter tile's fixed point is scaled by the brick's largest |dL/dT|;
+    its quantum must stay far below the float rounding of the values it sums,
+    also in cells fed only by the brick's small gradients.  Per voxel, the
+    error against FP64 is bounded by 1e-3 of the sum of the magnitudes of
+    the contributions it adds up (the oracle's |contribution| scatter; sums
+    that cancel are judged against what they summed), in every decade of
+    that magnitude.  Measured: 2e-5 to 4e-4 across ten decades, the float
+    contributions' own level; with a quantum of 2^-31 of the brick's largest
+    gradient the error grew to the whole value in the smallest decades."""
+    from tsf import synth
+    vol, _ = synth.tube_volume((24, 26, 28), 5, 1.5, 3.5, 3, noise=0.05)
+    V = orc.gvf_euler(orc.vector_field(orc.blur(vol, 0.8, "f32"), 0.2, 1, "f32"), 0.05, 30, "f32")
+    fld = tsf.Field.from_host(V)
+    rmin, rmax, rstep = 1.0, 3.5, 0.5
+    T = tsf.DeviceArray(vol.shape, np.float32)
+    R = tsf.DeviceArray(vol.shape, np.float32)
+    tsf.tdf_circle(fld, rmin, rmax, rstep, T, R)
+    t, r = T.download(), R.download()
+    _, r64 = orc.tdf_circle(V.astype(np.float64), rmin, rmax, rstep, "f64")
+    g = rng(12)
+    gT = np.where((t > 0.02) & (r64 == r), 10.0 ** (-6.0 * g.random(t.shape)), 0.0).astype(np.float32)
+    d = tsf.Field(V.shape[1:])
+    for p in d.planes:
+        p.zero()
+    tsf.tdf_adjoint(fld, rmin, rmax, rstep, tsf.DeviceArray.from_host(gT), d, radius=R)
+    got = d.download()[:3].astype(np.float64)
+    V64, g64 = V.astype(np.float64), gT.astype(np.float64)
+    odV, _ = orc.tdf_adjoint(V64, rmin, rmax, rstep, g64)
+    A, _ = orc.tdf_adjoint_magnitude(V64, rmin, rmax, rstep, g64)
+    q = np.abs(got - odV)[A > 0] / A[A > 0]
+    a = A[A > 0]
+    for k in range(0, 10):
+        sel = (a >= 10.0 ** (-k - 1)) & (a < 10.0 ** -k)
+        if sel.sum():
+            print(f"scale in [1e-{k + 1}, 1e-{k}): {int(sel.sum())} values, error / scale: max {q[sel].max():.2e}, "
+                  f"99.9 % {np.quantile(q[sel], 0.999):.2e}")
+    assert q.max() < 1e-3
+
+
 def test_tdf_adjoint_linear_in_gradient_scale(tsf, orc):
     # the scatter tile accumulates in fixed point scaled by each brick's
     # largest |dL/dT|: the gradient must still be linear in dL/dT, for a

@@ -129,3 +129,19 @@ def test_chain_gradient_fd(orc):
         assert np.isclose(fd(lambda s: L(I_=I + s * E), 0.0), dI[tuple(p)], rtol=1e-4, atol=1e-9)
     assert np.isclose(fd(lambda s: L(m_=s), args["mu"]), dmu, rtol=1e-4)
     assert np.isclose(fd(lambda s: L(s_=s), sigma), dsig, rtol=1e-4)
+
+
+def test_tdf_adjoint_magnitude_bounds_the_adjoint(orc):
+    """The |contribution| scatter bounds the adjoint voxel by voxel (triangle
+    inequality), is linear in |dL/dT|, and exceeds |dV| where contributions
+    of both signs meet."""
+    from tsf import synth
+    vol, _ = synth.tube_volume((14, 15, 16), 3, 1.5, 3.0, 5, noise=0.05)
+    V = orc.gvf_euler(orc.vector_field(orc.blur(vol, 0.8, "f64"), 0.2, 1, "f64"), 0.05, 20, "f64")
+    gT = np.random.default_rng(13).uniform(-1.5, 1.5, vol.shape)
+    dV, dW = orc.tdf_adjoint(V, 1.0, 4.0, 0.5, gT)
+    A, AW = orc.tdf_adjoint_magnitude(V, 1.0, 4.0, 0.5, gT)
+    assert np.all(A >= np.abs(dV) * (1 - 1e-12)) and np.all(AW >= np.abs(dW) * (1 - 1e-12))
+    A2, _ = orc.tdf_adjoint_magnitude(V, 1.0, 4.0, 0.5, 2.0 * gT)
+    assert np.allclose(A2, 2.0 * A, rtol=1e-12, atol=0)
+    assert (A > 1.01 * np.abs(dV)).mean() > 0.5

@@ -772,13 +772,15 @@ __device__ __forceinline__ void tri_eval(const TriTaps& q, float s[3], float J[3
 // voxels, in LDS; contributions outside it go to global atomics.
 // The sample scatter accumulates in an LDS tile over one brick plus a halo
 // of `halo` voxels, as 64-bit fixed point: a contribution v adds
-// rint(v * S) to its cell, S = 2^(30 - e) for the brick's largest |dL/dT|
+// rint(v * S) to its cell, S = 2^(50 - e) for the brick's largest |dL/dT|
 // < 2^e.  Every sample contribution is at most |dL/dT| / 32 in magnitude
 // (times a unit direction component and a trilinear weight), so a rounded
-// contribution fits 32 bits (larger ones, never expected, take the global
-// path) and a cell's sum (at most 8 * 32 contributions from each of the
-// brick's voxels) fits 64 with room to spare; the quantum 2^(e-31) is below a float's rounding of the values
-// added.  Integer adds are exact and order-free, so the tile's sums are
+// contribution is below 2^45 (larger ones, never expected, take the global
+// path) and a cell's sum -- at most 8 * 32 contributions from each of the
+// brick's 512 voxels, 2^17 in all -- stays below 2^62.  The quantum 2^(e-51)
+// is 2^-46 of the largest contribution, far below a float's rounding, so a
+// cell fed only by a brick's small gradients (1e-6 beside 1) and sums that
+// cancel keep float precision.  Integer adds are exact and order-free, so the tile's sums are
 // deterministic, and on this chip an integer LDS atomic costs a third of a
 // float one.  Contributions that are not finite, or fall outside the tile,
 // or arrive when the tile is off, go to global float atomics.  The tile's
@@ -795,12 +797,12 @@ struct AdjTile {
     int ox, oy, oz, tx, ty, tz, cells;
     __device__ __forceinline__ void add(float* const dV[3], const Dims& d, int c, int x, int y, int z, float v) const {
         const int lx = x - ox, ly = y - oy, lz = z - oz;
-        const float q = v * S;  // |q| < 2^25 for a sample's contribution; NaN fails the test
-        if (on && fabsf(q) < 1073741824.0f && (unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty &&
+        const float q = v * S;  // |q| < 2^45 for a sample's contribution; NaN fails the test
+        if (on && fabsf(q) < 70368744177664.0f && (unsigned)lx < (unsigned)tx && (unsigned)ly < (unsigned)ty &&
             (unsigned)lz < (unsigned)tz)
             __hip_atomic_fetch_add((lds_i64*)(a + (c * cells + lx + tx * (ly + ty * lz))),
-                                   (long long)(int)__builtin_rintf(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        else
+                                   (long long)__builtin_rintf(q), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else if (v != 0.0f)
             unsafeAtomicAdd(dV[c] + d.at(x, y, z), v);
     }
 };
@@ -854,10 +856,10 @@ __global__ void __launch_bounds__(TAX* TAY* TAZ) k_tdf_adjoint(Tex<F32> T, Dims 
     tile.on = lds_on && gmax > 0.0f && __builtin_isfinite(gmax);
     int e = 0;
     if (tile.on) frexpf(gmax, &e);  // gmax < 2^e
-    // S = 2^(30 - e) must be a finite float: bricks whose gradients are all
-    // below 2^-97 use the global float atomics
-    tile.on = tile.on && e >= -96;
-    tile.S = tile.on ? ldexpf(1.0f, 30 - e) : 0.0f;
+    // S = 2^(50 - e) must be a finite float: bricks whose gradients are all
+    // below 2^-77 use the global float atomics
+    tile.on = tile.on && e >= -76;
+    tile.S = tile.on ? ldexpf(1.0f, 50 - e) : 0.0f;
     float* const dV[3] = {dVx, dVy, dVz};
     if (g != 0.0f) {
         float H[3][3], E[3][3], lam[3];

@@ -7,8 +7,9 @@
 // (kernels_no_3d_write.cl:1824-2068).  The operation order is the one those
 // routines define, so float results round as the reference's float code does.
 //
-// Every loop has a compile-time trip count and every array index is a
-// constant after unrolling, so V, d and e live in VGPRs (no scratch).  The
+// Host code shares it (the single-point doEigen).  Every loop has a
+// compile-time trip count and every array index is a constant after
+// unrolling, so V, d and e live in VGPRs (no scratch).  The
 // data-dependent QL window [l, m] is expressed with predicated constant-index
 // branches.  Two fixed rules where the reference is unsafe: the small
 // sub-diagonal search stops at index 2, and the QL iteration is capped at
@@ -21,18 +22,20 @@
 
 namespace tsf {
 
-__device__ __forceinline__ float tsqrt(float v) { return sqrtf(v); }
-__device__ __forceinline__ double tsqrt(double v) { return sqrt(v); }
+__host__ __device__ __forceinline__ float tsqrt(float v) { return sqrtf(v); }
+__host__ __device__ __forceinline__ double tsqrt(double v) { return sqrt(v); }
+__host__ __device__ __forceinline__ float tabs(float v) { return fabsf(v); }
+__host__ __device__ __forceinline__ double tabs(double v) { return fabs(v); }
 
 template <typename T>
-__device__ __forceinline__ void tridiag3(T V[3][3], T d[3], T e[3]) {
+__host__ __device__ __forceinline__ void tridiag3(T V[3][3], T d[3], T e[3]) {
 #pragma unroll
     for (int j = 0; j < 3; j++) d[j] = V[2][j];
 #pragma unroll
     for (int i = 2; i > 0; i--) {
         T scale = 0, h = 0;
 #pragma unroll
-        for (int k = 0; k < i; k++) scale = scale + fabs(d[k]);
+        for (int k = 0; k < i; k++) scale = scale + tabs(d[k]);
         if (scale == (T)0) {
             e[i] = d[i - 1];
 #pragma unroll
@@ -118,7 +121,7 @@ __device__ __forceinline__ void tridiag3(T V[3][3], T d[3], T e[3]) {
 }
 
 template <typename T>
-__device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
+__host__ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
     e[0] = e[1];
     e[1] = e[2];
     e[2] = 0;
@@ -126,12 +129,12 @@ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
     const T eps = (T)2.220446049250313080847263336181640625e-16;  // 2^-52
 #pragma unroll
     for (int l = 0; l < 3; l++) {
-        T t = fabs(d[l]) + fabs(e[l]);
+        T t = tabs(d[l]) + tabs(e[l]);
         tst1 = tst1 > t ? tst1 : t;
         int m = l;
 #pragma unroll
         for (int mm = l; mm < 2; mm++)
-            if (m == mm && !(fabs(e[mm]) <= eps * tst1)) m = mm + 1;
+            if (m == mm && !(tabs(e[mm]) <= eps * tst1)) m = mm + 1;
         if (l < 2 && m > l) {
             int iter = 0;
             do {
@@ -176,7 +179,7 @@ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
                 p = -s * s2 * c3 * el1 * e[l] / dl1;
                 e[l] = s * p;
                 d[l] = c * p;
-            } while (fabs(e[l]) > eps * tst1 && iter < TSF_QL_MAX_ITER);
+            } while (tabs(e[l]) > eps * tst1 && iter < TSF_QL_MAX_ITER);
         }
         d[l] = d[l] + f;
         e[l] = 0;
@@ -188,7 +191,7 @@ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
         T p = d[i];
 #pragma unroll
         for (int j = i + 1; j < 3; j++)
-            if (fabs(d[j]) < fabs(p)) {
+            if (tabs(d[j]) < tabs(p)) {
                 k = j;
                 p = d[j];
             }
@@ -209,7 +212,7 @@ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
 }
 
 template <typename T>
-__device__ __forceinline__ void eigen3(const T A[3][3], T V[3][3], T d[3]) {
+__host__ __device__ __forceinline__ void eigen3(const T A[3][3], T V[3][3], T d[3]) {
     T e[3];
 #pragma unroll
     for (int i = 0; i < 3; i++)

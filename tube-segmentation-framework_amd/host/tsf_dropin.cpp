@@ -1,6 +1,8 @@
 // C++ drop-in layer over the C ABI: see tsf_dropin.hpp.
 #include "tsf_dropin.hpp"
 
+#include "../csrc/eigen3.hpp"
+
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -568,59 +570,54 @@ CenterpointVertices runNewCenterlineAlgCandidates(OpenCL& ocl, int3 size, paramL
 }
 
 // ----------------------------------------------------------------- doEigen
-namespace {
-struct EigenMirror {
-    int3 size;
-    Buffer fx, fy, fz;
-    std::unique_ptr<OpenCL> ocl;
-};
-std::map<const float*, EigenMirror>& mirrors() {
-    static std::map<const float*, EigenMirror> m;
-    return m;
+// doEigen (eigenanalysisOfHessian.cpp:264-389) is host code in the reference:
+// it reads T's host arrays at every call.  So is this one -- the 6 samples
+// divided by sqrt(Fx^2 + Fy^2 + Fz^2) of their voxel, central differences,
+// the register eigensolver of the device kernels (eigen3.hpp, the same
+// operation order, no contraction) -- so a caller that changes the arrays in
+// place sees the change at the next call.  Positions whose neighbours leave
+// the volume (the reference reads outside its arrays there) give NaN.
+static void eigen_at(const float* Fx, const float* Fy, const float* Fz, int3 size, int3 p, float o[12]) {
+    if (p.x < 1 || p.y < 1 || p.z < 1 || p.x > size.x - 2 || p.y > size.y - 2 || p.z > size.z - 2) {
+        for (int k = 0; k < 12; k++) o[k] = std::nanf("");
+        return;
+    }
+    auto nv = [&](const float* c, int x, int y, int z) -> float {
+        const size_t i = (size_t)x + (size_t)size.x * ((size_t)y + (size_t)size.y * (size_t)z);
+        return c[i] / std::sqrt(Fx[i] * Fx[i] + Fy[i] * Fy[i] + Fz[i] * Fz[i]);
+    };
+    const int x = p.x, y = p.y, z = p.z;
+    const float fxx = 0.5f * (nv(Fx, x + 1, y, z) - nv(Fx, x - 1, y, z));
+    const float fyx = 0.5f * (nv(Fy, x + 1, y, z) - nv(Fy, x - 1, y, z));
+    const float fyy = 0.5f * (nv(Fy, x, y + 1, z) - nv(Fy, x, y - 1, z));
+    const float fzx = 0.5f * (nv(Fz, x + 1, y, z) - nv(Fz, x - 1, y, z));
+    const float fzy = 0.5f * (nv(Fz, x, y + 1, z) - nv(Fz, x, y - 1, z));
+    const float fzz = 0.5f * (nv(Fz, x, y, z + 1) - nv(Fz, x, y, z - 1));
+    const float H[3][3] = {{fxx, fyx, fzx}, {fyx, fyy, fzy}, {fzx, fzy, fzz}};
+    float V[3][3], ev[3];
+    eigen3<float>(H, V, ev);
+    for (int k = 0; k < 3; k++) {
+        o[k] = ev[k];
+        o[3 + k] = V[k][0];
+        o[6 + k] = V[k][1];
+        o[9 + k] = V[k][2];
+    }
 }
-EigenMirror& mirror(TubeSegmentation& T, int3 size) {
+
+static void requireArrays(const TubeSegmentation& T) {
     if (!T.Fx || !T.Fy || !T.Fz) fail(TSF_ERR_INVALID_ARGUMENT, "doEigen: TubeSegmentation has no field arrays");
-    auto& m = mirrors();
-    auto it = m.find(T.Fx);
-    if (it != m.end() && it->second.size.x == size.x && it->second.size.y == size.y && it->second.size.z == size.z)
-        return it->second;
-    const size_t n = voxels(size);
-    EigenMirror e;
-    e.size = size;
-    e.ocl.reset(new OpenCL());
-    e.fx = Buffer(n * 4);
-    e.fy = Buffer(n * 4);
-    e.fz = Buffer(n * 4);
-    e.fx.upload(*e.ocl, T.Fx, n * 4);
-    e.fy.upload(*e.ocl, T.Fy, n * 4);
-    e.fz.upload(*e.ocl, T.Fz, n * 4);
-    m.erase(T.Fx);
-    return m.emplace(T.Fx, std::move(e)).first->second;
 }
-}  // namespace
 
 void doEigenBatch(TubeSegmentation& T, int3 size, const std::vector<int3>& pos, std::vector<float>& out12) {
-    EigenMirror& e = mirror(T, size);
-    const size_t k = pos.size();
-    out12.assign(12 * k, 0.0f);
-    if (!k) return;
-    std::vector<int32_t> p(3 * k);
-    for (size_t i = 0; i < k; i++) {
-        p[3 * i] = pos[i].x;
-        p[3 * i + 1] = pos[i].y;
-        p[3 * i + 2] = pos[i].z;
-    }
-    Buffer dp(p.size() * 4), dout(out12.size() * 4);
-    dp.upload(*e.ocl, p.data(), p.size() * 4);
-    check(tsf_do_eigen_points((const float*)e.fx.get(), (const float*)e.fy.get(), (const float*)e.fz.get(),
-                              tsf_dims{size.x, size.y, size.z}, (const int32_t*)dp.get(), (int64_t)k,
-                              (float*)dout.get(), e.ocl->queue));
-    dout.download(*e.ocl, out12.data(), out12.size() * 4);
+    requireArrays(T);
+    out12.assign(12 * pos.size(), 0.0f);
+    for (size_t i = 0; i < pos.size(); i++) eigen_at(T.Fx, T.Fy, T.Fz, size, pos[i], out12.data() + 12 * i);
 }
 
 void doEigen(TubeSegmentation& T, int3 pos, int3 size, float3* lambda, float3* e1, float3* e2, float3* e3) {
-    std::vector<float> o;
-    doEigenBatch(T, size, {pos}, o);
+    requireArrays(T);
+    float o[12];
+    eigen_at(T.Fx, T.Fy, T.Fz, size, pos, o);
     if (lambda) *lambda = float3(o[0], o[1], o[2]);
     if (e1) *e1 = float3(o[3], o[4], o[5]);
     if (e2) *e2 = float3(o[6], o[7], o[8]);
@@ -633,6 +630,43 @@ float3 getTubeDirection(TubeSegmentation& T, int3 pos, int3 size) {
     return e1;
 }
 
-void releaseEigenMirror(TubeSegmentation& T) { mirrors().erase(T.Fx); }
+EigenField::EigenField(OpenCL& ocl, const TubeSegmentation& T, int3 size) : ocl_(ocl), size_(size) {
+    requireArrays(T);
+    const size_t n = voxels(size);
+    fx_ = Buffer(n * 4);
+    fy_ = Buffer(n * 4);
+    fz_ = Buffer(n * 4);
+    refresh(T);
+}
+
+void EigenField::refresh(const TubeSegmentation& T) {
+    requireArrays(T);
+    const size_t n = voxels(size_);
+    fx_.upload(ocl_, T.Fx, n * 4);
+    fy_.upload(ocl_, T.Fy, n * 4);
+    fz_.upload(ocl_, T.Fz, n * 4);
+}
+
+void EigenField::batch(const std::vector<int3>& pos, std::vector<float>& out12) {
+    const size_t k = pos.size();
+    out12.assign(12 * k, 0.0f);
+    if (!k) return;
+    if (k > cap_) {  // grow the point and result buffers, never shrink them
+        cap_ = std::max(k, 2 * cap_);
+        points_ = Buffer(cap_ * 12);
+        results_ = Buffer(cap_ * 48);
+    }
+    std::vector<int32_t> p(3 * k);
+    for (size_t i = 0; i < k; i++) {
+        p[3 * i] = pos[i].x;
+        p[3 * i + 1] = pos[i].y;
+        p[3 * i + 2] = pos[i].z;
+    }
+    points_.upload(ocl_, p.data(), p.size() * 4);
+    check(tsf_do_eigen_points((const float*)fx_.get(), (const float*)fy_.get(), (const float*)fz_.get(),
+                              tsf_dims{size_.x, size_.y, size_.z}, (const int32_t*)points_.get(), (int64_t)k,
+                              (float*)results_.get(), ocl_.queue));
+    results_.download(ocl_, out12.data(), out12.size() * 4);
+}
 
 }  // namespace tsf

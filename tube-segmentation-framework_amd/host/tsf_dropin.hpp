@@ -354,12 +354,28 @@ struct TubeSegmentation {
     char* segmentation = nullptr;
     float* intensity = nullptr;
 };
+// eigenanalysisOfHessian.hpp:7-9.  Host code, as in the reference: each call
+// reads T's current host arrays (no device copy to go stale).
 void doEigen(TubeSegmentation& T, int3 pos, int3 size, float3* lambda, float3* e1, float3* e2, float3* e3);
 float3 getTubeDirection(TubeSegmentation& T, int3 pos, int3 size);
-// Batched form: one launch for many points.
+// Many points on the host (12 floats each: lambda, e1, e2, e3).
 void doEigenBatch(TubeSegmentation& T, int3 size, const std::vector<int3>& pos, std::vector<float>& out12);
-// Drops the device mirror of T's arrays (call when the host arrays change).
-void releaseEigenMirror(TubeSegmentation& T);
+// A device copy of T's arrays, owned by the caller, for large batches
+// (tsf_do_eigen_points): the copy is taken at construction and by
+// refresh(), which a caller that changes the host arrays calls; batch()
+// reuses its point buffers, grown as needed.  Not thread-safe.
+class EigenField {
+   public:
+    EigenField(OpenCL& ocl, const TubeSegmentation& T, int3 size);
+    void refresh(const TubeSegmentation& T);
+    void batch(const std::vector<int3>& pos, std::vector<float>& out12);
+
+   private:
+    OpenCL& ocl_;
+    int3 size_;
+    Buffer fx_, fy_, fz_, points_, results_;
+    size_t cap_ = 0;
+};
 
 // ridgeTraversalCenterlineExtraction.hpp:9-15: the ridge-traversal method.
 // Returns the centerline mask (new[]'d, one byte per voxel): the longest

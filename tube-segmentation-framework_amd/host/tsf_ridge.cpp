@@ -6,8 +6,8 @@
 // start points taken in decreasing TDF, it steps to the neighbour ahead (a
 // direction within ~84 degrees of the current one) with the largest
 // 1 - |F| (or TDF * (1 - |F|) where the radius is below 1.5), re-aiming
-// along the Hessian's tube direction at every step (doEigen, on the
-// device), until it leaves the thresholds, meets an existing centerline or
+// along the Hessian's tube direction at every step (doEigen, host code as
+// in the reference), until it leaves the thresholds, meets an existing centerline or
 // loops.  Two choices fix what the reference leaves open: the start points
 // enter the priority queue in index order (the reference pushes them from an
 // OpenMP loop), and among trees of equal length the smallest id is the
@@ -83,7 +83,6 @@ char* runRidgeTraversal(TubeSegmentation& T, int3 size, paramList& parameters,
     T.Fx[0] = 1;
     T.Fy[0] = 0;
     T.Fz[0] = 0;
-    releaseEigenMirror(T);
 
     std::vector<int> centerlines(total, 0);
     int counter = 1;
@@ -243,14 +242,7 @@ void runCircleFittingAndRidgeTraversal(OpenCL& ocl, Image3D* dataset, int3* size
     TS.TDF = new float[total];
     std::copy(tdf.begin(), tdf.end(), TS.TDF);
     output->setTDF(TS.TDF);
-    char* centerline = nullptr;
-    try {
-        centerline = runRidgeTraversal(TS, *size, parameters, std::stack<CenterlinePoint>());
-    } catch (...) {
-        releaseEigenMirror(TS);
-        throw;
-    }
-    releaseEigenMirror(TS);
+    char* centerline = runRidgeTraversal(TS, *size, parameters, std::stack<CenterlinePoint>());
     output->setCenterlineVoxels(centerline);
     if (!getParamBool(parameters, "no-segmentation")) {
         Buffer cl(total);
