@@ -122,6 +122,16 @@ def eigen3(A, prec="f32"):
     return d, V.reshape(3, 3)
 
 
+def eigen3_fp64_fallback(A):
+    """The precision form's float eigensolve with its FP64 fallback for
+    near-degenerate matrices (tsf_tdf_circle_mixed)."""
+    A = np.ascontiguousarray(A, np.float32).reshape(9)
+    V = np.empty(9, np.float32)
+    d = np.empty(3, np.float32)
+    lib().orc_eigen3_fp64_fallback_f32(_p(A), _p(V), _p(d))
+    return d, V.reshape(3, 3)
+
+
 def hessian_eigen(F, normalize, vectors=True, prec="f32"):
     F = np.ascontiguousarray(F, _dt(prec))
     X, Y, Z = _dims(F)

@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #include <algorithm>
 #include <limits>
@@ -581,7 +582,7 @@ template <typename T>
 static inline T hyp(T a, T b) { return std::sqrt(a * a + b * b); }
 
 template <typename T>
-static void ql_implicit3(T V[3][3], T d[3], T e[3]) {
+static void ql_implicit3(T V[3][3], T d[3], T e[3], bool* converged = nullptr) {
     const int n = 3;
     for (int i = 1; i < n; i++) e[i - 1] = e[i];
     e[n - 1] = 0;
@@ -631,6 +632,7 @@ static void ql_implicit3(T V[3][3], T d[3], T e[3]) {
                 e[l] = s * p;
                 d[l] = c * p;
             } while (std::fabs(e[l]) > eps * tst1 && iter < TSF_QL_MAX_ITER);
+            if (converged && std::fabs(e[l]) > eps * tst1) *converged = false;
         }
         d[l] = d[l] + f;
         e[l] = 0;
@@ -657,12 +659,35 @@ static void ql_implicit3(T V[3][3], T d[3], T e[3]) {
 }
 
 template <typename T>
-static void eigen3(const T A[3][3], T V[3][3], T d[3]) {
+static void eigen3(const T A[3][3], T V[3][3], T d[3], bool* converged = nullptr) {
     T e[3];
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) V[i][j] = A[i][j];
     householder3(V, d, e);
-    ql_implicit3(V, d, e);
+    ql_implicit3(V, d, e, converged);
+}
+
+// The precision form's eigensolve (tsf_tdf_circle_mixed): the float
+// decomposition, redone in double from the same float matrix and rounded to
+// float when two eigenvalues lie within 2^-10 of the largest |eigenvalue| of
+// each other or the float QL stopped at its sweep cap.  For T = double the
+// plain decomposition.
+template <typename T>
+static void eigen3_fp64_fallback(const T A[3][3], T V[3][3], T d[3]) {
+    bool conv = true;
+    eigen3(A, V, d, &conv);
+    if (!std::is_same<T, float>::value) return;
+    const T m = std::max(std::fabs(d[0]), std::max(std::fabs(d[1]), std::fabs(d[2])));
+    const T g = std::min(std::fabs(d[1] - d[0]), std::min(std::fabs(d[2] - d[1]), std::fabs(d[2] - d[0])));
+    if (conv && !(g < (T)std::ldexp(1.0, -10) * m)) return;
+    double Ad[3][3], Vd[3][3], dd[3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Ad[i][j] = (double)A[i][j];
+    eigen3(Ad, Vd, dd);
+    for (int i = 0; i < 3; i++) {
+        d[i] = (T)dd[i];
+        for (int j = 0; j < 3; j++) V[i][j] = (T)Vd[i][j];
+    }
 }
 
 // ---------------------------------------------------------------- precision form: 16-bit state, stochastic rounding
@@ -1023,7 +1048,8 @@ static void tdf_circle_split(const T* FH, const T* F, Dims d, float rMin, float 
             for (int x = 0; x < d.X; x++) {
                 T H[3][3], V[3][3], ev[3];
                 hessian_at(FH, d, x, y, z, normalize, H);
-                eigen3(H, V, ev);
+                if (FUSED) eigen3_fp64_fallback(H, V, ev);
+                else eigen3(H, V, ev);
                 const T e2[3] = {V[0][1], V[1][1], V[2][1]};
                 const T e3[3] = {V[0][2], V[1][2], V[2][2]};
                 T maxSum = 0, maxRadius = 0;
@@ -1824,6 +1850,12 @@ extern "C" void orc_sn16_round(float* a, long long n) {
 extern "C" void orc_gvf_euler_sr16(const uint16_t* f16, int storage, float scale, float mu, int iters, int tail,
                                    unsigned seed, int X, int Y, int Z, float* out4) {
     gvf_euler_sr16(f16, storage, scale, mu, iters, tail, seed, Dims{X, Y, Z}, out4);
+}
+extern "C" void orc_eigen3_fp64_fallback_f32(const float* A, float* V, float* d) {
+    float a[3][3], v[3][3];
+    for (int i = 0; i < 9; i++) a[i / 3][i % 3] = A[i];
+    eigen3_fp64_fallback(a, v, d);
+    for (int i = 0; i < 9; i++) V[i] = v[i / 3][i % 3];
 }
 extern "C" void orc_half_from_float(const float* a, uint16_t* h, long long n) {
     for (long long i = 0; i < n; i++) h[i] = float_to_half_rne(a[i]);

@@ -472,6 +472,39 @@ def test_tdf_mixed_fused_is_as_close_to_fp64(orc):
         assert not np.array_equal(tf, tu)
 
 
+def test_eigen_fp64_fallback_resolves_near_degenerate_pairs(orc):
+    """Symmetric matrices with two eigenvalues 1e-5 apart (relative): the
+    float QL usually resolves the pair, but in its worst cases leaves the
+    eigenvectors wrong by up to eps / gap; the fallback (FP64 from the same
+    float matrix, rounded) matches numpy's FP64 eigh to float rounding in
+    every case.  Well-separated matrices take the float result unchanged."""
+    g = rng(51)
+    err_f, err_fb = [], []
+    for _ in range(200):
+        Q, _ = np.linalg.qr(g.standard_normal((3, 3)))
+        lam = np.array([0.3, 1.0, 1.0 + 1e-5]) * g.choice([-1.0, 1.0], 3)
+        A = ((Q * lam) @ Q.T).astype(np.float32)
+        A = (A + A.T) / 2
+        w, U = np.linalg.eigh(A.astype(np.float64))
+        o = np.argsort(np.abs(w))
+        w, U = w[o], U[:, o]
+        for fn, errs in ((lambda M: orc.eigen3(M, "f32"), err_f), (orc.eigen3_fp64_fallback, err_fb)):
+            d, V = fn(A)
+            errs.append(max(min(np.abs(V[:, k] - U[:, k]).max(), np.abs(V[:, k] + U[:, k]).max()) for k in range(3)))
+            assert np.allclose(d, w, rtol=1e-5, atol=1e-6)
+    print(f"eigenvector error: float {np.median(err_f):.1e} (max {max(err_f):.1e}), fallback {max(err_fb):.1e}")
+    assert max(err_fb) < 2e-6 and max(err_f) > 100 * max(err_fb)
+    for _ in range(50):
+        A = g.standard_normal((3, 3)).astype(np.float32)
+        A = (A + A.T) / 2
+        d, V = orc.eigen3(A, "f32")
+        w = np.sort(np.abs(d))
+        if (min(w[1] - w[0], w[2] - w[1]) < 2 ** -9 * w[2]):
+            continue
+        d2, V2 = orc.eigen3_fp64_fallback(A)
+        assert np.array_equal(d, d2) and np.array_equal(V, V2)
+
+
 def test_tdf_linear_inward_field(orc):
     Z, Y, X = 9, 24, 24
     z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")

@@ -78,7 +78,8 @@ __device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int 
     float H[3][3], V[3][3], ev[3];
     if (TH) hessian_tex<NORM>(*TH, d, x, y, z, H);
     else hessian_tex<NORM>(T, d, x, y, z, H);
-    eigen3(H, V, ev);
+    if constexpr (FUSED) eigen3_fp64_fallback(H, V, ev);
+    else eigen3(H, V, ev);
     const float e2[3] = {V[0][1], V[1][1], V[2][1]};
     const float e3[3] = {V[0][2], V[1][2], V[2][2]};
     maxSum = 0.0f;
@@ -225,7 +226,8 @@ __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int 
         if (x < d.X && y < d.Y && z < d.Z) {
             float H[3][3], V[3][3], ev[3];
             hessian_tex<NORM>(TH, d, x, y, z, H);
-            eigen3(H, V, ev);
+            if constexpr (FUSED) eigen3_fp64_fallback(H, V, ev);
+            else eigen3(H, V, ev);
             e2[0] = V[0][1]; e2[1] = V[1][1]; e2[2] = V[2][1];
             e3[0] = V[0][2]; e3[1] = V[1][2]; e3[2] = V[2][2];
             float r0 = 0.0f;

@@ -121,7 +121,7 @@ __host__ __device__ __forceinline__ void tridiag3(T V[3][3], T d[3], T e[3]) {
 }
 
 template <typename T>
-__host__ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
+__host__ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3], bool* converged = nullptr) {
     e[0] = e[1];
     e[1] = e[2];
     e[2] = 0;
@@ -180,6 +180,7 @@ __host__ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
                 e[l] = s * p;
                 d[l] = c * p;
             } while (tabs(e[l]) > eps * tst1 && iter < TSF_QL_MAX_ITER);
+            if (converged && tabs(e[l]) > eps * tst1) *converged = false;
         }
         d[l] = d[l] + f;
         e[l] = 0;
@@ -212,14 +213,42 @@ __host__ __device__ __forceinline__ void ql3(T V[3][3], T d[3], T e[3]) {
 }
 
 template <typename T>
-__host__ __device__ __forceinline__ void eigen3(const T A[3][3], T V[3][3], T d[3]) {
+__host__ __device__ __forceinline__ void eigen3(const T A[3][3], T V[3][3], T d[3], bool* converged = nullptr) {
     T e[3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) V[i][j] = A[i][j];
     tridiag3(V, d, e);
-    ql3(V, d, e);
+    ql3(V, d, e, converged);
+}
+
+// The precision form's eigensolve: eigen3 in float, redone in FP64 from
+// the same float matrix -- and rounded to float -- when two eigenvalues lie
+// within 2^-10 of the largest |eigenvalue| of each other (a float
+// eigenvector's error grows as eps |H| / gap: about 1e-4 there) or the
+// float QL sweeps stopped at their cap.  Same routine and ordering rules in
+// both precisions.
+__host__ __device__ __forceinline__ void eigen3_fp64_fallback(const float A[3][3], float V[3][3], float d[3]) {
+    bool conv = true;
+    eigen3<float>(A, V, d, &conv);
+    auto mx = [](float a, float b) { return a < b ? b : a; };  // as std::max / std::min
+    auto mn = [](float a, float b) { return b < a ? b : a; };
+    const float m = mx(tabs(d[0]), mx(tabs(d[1]), tabs(d[2])));
+    const float g = mn(tabs(d[1] - d[0]), mn(tabs(d[2] - d[1]), tabs(d[2] - d[0])));
+    if (conv && !(g < 0x1p-10f * m)) return;
+    double Ad[3][3], Vd[3][3], dd[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) Ad[i][j] = (double)A[i][j];
+    eigen3<double>(Ad, Vd, dd);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        d[i] = (float)dd[i];
+#pragma unroll
+        for (int j = 0; j < 3; j++) V[i][j] = (float)Vd[i][j];
+    }
 }
 
 }  // namespace tsf
