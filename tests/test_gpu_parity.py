@@ -38,9 +38,10 @@ def dev(tsf, a, dtype=None):
 
 
 # ------------------------------------------------------------------ prep
-@pytest.mark.parametrize("sigma", [0.5, 1.0, 2.6])
-def test_blur(tsf, orc, sigma):
-    v = rng(1).standard_normal((20, 23, 37)).astype(np.float32)
+@pytest.mark.parametrize("shape", [(20, 23, 37), (3, 2, 1), (17, 9, 70)])
+@pytest.mark.parametrize("sigma", [0.5, 1.0, 1.5, 2.0, 2.6])  # mask half-widths 1..5
+def test_blur(tsf, orc, sigma, shape):
+    v = rng(1).standard_normal(shape).astype(np.float32)
     a, b = dev(tsf, v), tsf.DeviceArray(v.shape, np.float32)
     tsf.blur_gaussian(a, b, sigma)
     got = b.download()
@@ -48,6 +49,26 @@ def test_blur(tsf, orc, sigma):
     nbad, rel = close(got, exp, 1e-6, 1e-7)
     assert nbad == 0, rel
     assert np.array_equal(got, exp)  # same summation order, no contraction
+
+
+@pytest.mark.parametrize("sigma,n", [(0.5, 9500), (2.6, 5500)])
+def test_blur_wide_plane(tsf, orc, sigma, n):
+    """Planes so large that a brick's planes span 2 GiB or more: the kernel
+    addresses them with 64-bit pointers instead of buffer offsets.  One
+    plane, so every z offset clamps to it; crops of the oracle's blur agree
+    with the volume's away from the crop's cut edges (and on the volume's
+    own edges, where both clamp)."""
+    v = rng(7).standard_normal((1, n, n)).astype(np.float32)
+    a, b = dev(tsf, v), tsf.DeviceArray(v.shape, np.float32)
+    tsf.blur_gaussian(a, b, sigma)
+    got = b.download()
+    m = int(np.ceil(sigma / 0.5))
+    for y, x in [(0, 0), (n // 2 - 17, n // 3 + 5), (n - 40, n - 40)]:
+        crop = np.ascontiguousarray(v[:, y:y + 40, x:x + 40])
+        exp = orc.blur(crop, sigma, "f32")
+        ylo, xlo = (0 if y == 0 else m), (0 if x == 0 else m)
+        yhi, xhi = (40 if y + 40 == n else 40 - m), (40 if x + 40 == n else 40 - m)
+        assert np.array_equal(got[:, y + ylo:y + yhi, x + xlo:x + xhi], exp[:, ylo:yhi, xlo:xhi])
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int8, np.uint16, np.int16, np.float32])

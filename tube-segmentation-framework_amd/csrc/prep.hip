@@ -8,41 +8,154 @@
 namespace tsf {
 
 // ---------------------------------------------------------------- blur
-// Output tile per workgroup: BX x BY x BZ voxels; the input tile with an
-// m-voxel apron is staged in LDS once (clamp-to-edge reads), then every
-// thread sums the (2m+1)^3 products in the reference's order (x offset
-// outermost, z offset innermost) so the float result is the reference's.
-constexpr int BX = 16, BY = 8, BZ = 4;
-constexpr int MMAX = 5;
-constexpr int TILE_MAX = (BX + 2 * MMAX) * (BY + 2 * MMAX) * (BZ + 2 * MMAX);
+// A workgroup of 32 x 8 threads owns a 32 x 8 x BZV voxel brick: each thread
+// one (x, y) column of BZV consecutive voxels in z.  The brick's input with
+// an m-voxel apron (clamp-to-edge reads) is staged in LDS; every load is
+// issued before the first LDS store, so one memory latency covers them all.
+// For every (a, b) offset a thread reads the BZV + 2m values of its shifted
+// column once and feeds them to its BZV sums; each voxel's sum still takes
+// its (2m+1)^3 products in the reference's order (x offset outermost, z
+// offset innermost), so the float result is the reference's.  m is a
+// template parameter (1..5): the loops unroll and the mask weights are
+// uniform scalars, passed in the kernel arguments (m <= 4, up to 729
+// weights) or in device memory (m = 5).
+constexpr int BX = 32, BY = 8;
+// voxels per thread in z: small bricks for the 27-point mask keep the
+// registers (and so the waves per SIMD that hide the loads) in bounds
+__host__ __device__ constexpr int blur_zv(int m) { return m == 1 ? 4 : 8; }
 
-__global__ void __launch_bounds__(BX* BY* BZ) k_blur(const float* __restrict__ in, float* __restrict__ out, Dims d,
-                                                     int m, const float* __restrict__ mask) {
-    __shared__ float tile[TILE_MAX];
-    const int TX = BX + 2 * m, TY = BY + 2 * m, TZ = BZ + 2 * m;
+template <int M>
+struct BlurMask {
+    float w[M * M * M];
+};
+
+template <int m, bool ARG, bool BUF>
+__global__ void __launch_bounds__(BX* BY) k_blur(const float* __restrict__ in, float* __restrict__ out, Dims d,
+                                                 BlurMask<ARG ? 2 * m + 1 : 1> W, const float* __restrict__ gw) {
+    constexpr int BZV = blur_zv(m), M = 2 * m + 1, TX = BX + 2 * m, TY = BY + 2 * m, TZ = BZV + 2 * m;
+    constexpr int NY = (TY + BY - 1) / BY;  // tile rows per thread
+    __shared__ float tile[TX * TY * TZ];
     int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz);  // tiles sharing aprons meet in one L2
-    const int x0 = tbx * BX - m, y0 = tby * BY - m, z0 = tbz * BZ - m;
-    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
-    const int total = TX * TY * TZ;
-    for (int i = tid; i < total; i += BX * BY * BZ) {
-        int tx = i % TX, ty = (i / TX) % TY, tz = i / (TX * TY);
-        int gx = clampi(x0 + tx, 0, d.X - 1), gy = clampi(y0 + ty, 0, d.Y - 1), gz = clampi(z0 + tz, 0, d.Z - 1);
-        tile[i] = in[d.at(gx, gy, gz)];
-    }
-    __syncthreads();
-    const int x = tbx * BX + threadIdx.x, y = tby * BY + threadIdx.y, z = tbz * BZ + threadIdx.z;
-    if (x >= d.X || y >= d.Y || z >= d.Z) return;
-    const int M = 2 * m + 1;
-    float sum = 0.0f;
-    for (int a = 0; a < M; a++)
-        for (int b = 0; b < M; b++)
-            for (int c = 0; c < M; c++) {
-                float w = mask[a + b * M + c * M * M];
-                float v = tile[(threadIdx.x + a) + TX * ((threadIdx.y + b) + TY * (threadIdx.z + c))];
-                sum += w * v;
+    xcd_tile(tbx, tby, tbz);  // bricks sharing aprons meet in one L2
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = tbx * BX - m, y0 = tby * BY - m, zb = tbz * BZV, z0 = zb - m;
+    // Thread row ly = ty (+ BY, ...) takes every plane of its row: the
+    // lane's x and the row's y are clamped once, each plane's offset is
+    // uniform.  Lanes tx < 2m also fetch the row's apron columns.
+    const int gx = min(tbx * BX + tx, d.X - 1);
+    const int alx = tx < m ? tx : BX + tx;  // apron column of lanes tx < 2m
+    const int ax = clampi(x0 + alx, 0, d.X - 1);
+    const bool apron = tx < 2 * m;
+    const i64 plane = (i64)d.X * d.Y;
+    // BUF: buffer loads from a base at the brick's first clamped plane, with
+    // a per-lane 32-bit row offset and a uniform plane offset (the host
+    // picks this form when TZ planes span less than 2 GiB)
+    const int zlo = clampi(z0, 0, d.Z - 1);
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(in + plane * zlo), 0, (int)(BUF ? TZ * plane * 4 : 0), BUFFER_WORD3);
+    auto rowok = [&](int j) { return j == 0 || ty + BY * j < TY; };
+    auto ld = [&](int ry, int lz, int col) -> float {
+        const int gz = clampi(z0 + lz, 0, d.Z - 1);
+        if constexpr (BUF)
+            return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, (int)(4u * (unsigned)(ry * d.X + col)),
+                                                                        (int)(4 * plane * (gz - zlo)), 0));
+        else
+            return in[plane * gz + (i64)ry * d.X + col];
+    };
+    float v[NY][TZ], va[NY][TZ];
+#pragma unroll
+    for (int j = 0; j < NY; j++)
+        if (rowok(j)) {
+            const int ry = clampi(y0 + ty + BY * j, 0, d.Y - 1);
+#pragma unroll
+            for (int lz = 0; lz < TZ; lz++) v[j][lz] = ld(ry, lz, gx);
+        }
+    if (apron)
+#pragma unroll
+        for (int j = 0; j < NY; j++)
+            if (rowok(j)) {
+                const int ry = clampi(y0 + ty + BY * j, 0, d.Y - 1);
+#pragma unroll
+                for (int lz = 0; lz < TZ; lz++) va[j][lz] = ld(ry, lz, ax);
             }
-    out[d.at(x, y, z)] = sum;
+#pragma unroll
+    for (int j = 0; j < NY; j++)
+        if (rowok(j))
+#pragma unroll
+            for (int lz = 0; lz < TZ; lz++) tile[TX * (ty + BY * j + TY * lz) + m + tx] = v[j][lz];
+    if (apron)
+#pragma unroll
+        for (int j = 0; j < NY; j++)
+            if (rowok(j))
+#pragma unroll
+                for (int lz = 0; lz < TZ; lz++) tile[TX * (ty + BY * j + TY * lz) + alx] = va[j][lz];
+    __syncthreads();
+
+    const int x = tbx * BX + tx, y = tby * BY + ty;
+    if (x >= d.X || y >= d.Y) return;
+    float sum[BZV];
+#pragma unroll
+    for (int q = 0; q < BZV; q++) sum[q] = 0.0f;
+    // m = 1 unrolls completely (the weights stay in scalar registers); for
+    // larger masks the x offset loop stays rolled so that only the weights
+    // of one x offset are live at once
+    constexpr int UA = m == 1 ? M : 1;
+#pragma unroll UA
+    for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = 0; b < M; b++) {
+            const float* t = tile + (tx + a) + TX * (ty + b);
+            float col[TZ];
+#pragma unroll
+            for (int j = 0; j < TZ; j++) col[j] = t[TX * TY * j];
+#pragma unroll
+            for (int c = 0; c < M; c++) {
+                // The mask is a Gaussian of the squared offset, so z offsets
+                // c and 2m - c carry the same weight: reading it from one
+                // index lets w * col[j] be computed once for both (the same
+                // product, so the same float)
+                const int i = a + b * M + (c <= m ? c : 2 * m - c) * M * M;
+                const float w = ARG ? W.w[i] : gw[i];
+#pragma unroll
+                for (int q = 0; q < BZV; q++) sum[q] += w * col[q + c];
+            }
+        }
+    if constexpr (BUF) {
+        const __amdgpu_buffer_rsrc_t ro =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(out + plane * zb), 0, (int)(BZV * plane * 4), BUFFER_WORD3);
+        const unsigned vo = 4u * (unsigned)(y * d.X + x);
+#pragma unroll
+        for (int q = 0; q < BZV; q++)
+            if (zb + q < d.Z) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(sum[q]), ro, (int)vo, (int)(4 * plane * q), 0);
+    } else {
+#pragma unroll
+        for (int q = 0; q < BZV; q++)
+            if (zb + q < d.Z) out[d.at(x, y, zb + q)] = sum[q];
+    }
+}
+
+template <int m, bool ARG, bool BUF, class W>
+static void launch_blur(const float* in, float* out, Dims d, const W& w, const float* dmask, hipStream_t stream) {
+    constexpr int BZV = blur_zv(m);
+    const dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZV - 1) / BZV), block(BX, BY);
+    hipLaunchKernelGGL((k_blur<m, ARG, BUF>), grid, block, 0, stream, in, out, d, w, dmask);
+}
+
+template <int m>
+static void launch_blur(const float* in, float* out, Dims d, const float* hmask, const float* dmask,
+                        hipStream_t stream) {
+    constexpr int M = 2 * m + 1, TZ = blur_zv(m) + 2 * m;
+    const bool buf = (i64)TZ * d.X * d.Y * 4 < ((i64)1 << 31);
+    if constexpr (m == 5) {
+        BlurMask<1> none{};
+        if (buf) launch_blur<m, false, true>(in, out, d, none, dmask, stream);
+        else launch_blur<m, false, false>(in, out, d, none, dmask, stream);
+    } else {
+        BlurMask<M> w;
+        for (int i = 0; i < M * M * M; i++) w.w[i] = hmask[i];
+        if (buf) launch_blur<m, true, true>(in, out, d, w, (const float*)nullptr, stream);
+        else launch_blur<m, true, false>(in, out, d, w, (const float*)nullptr, stream);
+    }
 }
 
 // ---------------------------------------------------------------- toFloat
@@ -164,19 +277,28 @@ tsf_status tsf_blur_gaussian(const float* in, float* out, tsf_dims dims, float s
     float hmask[11 * 11 * 11];
     int m;
     TSF_TRY(tsf_create_blur_mask(sigma, hmask, &m));
-    const int M = 2 * m + 1;
-    Scratch sc;
-    // the mask is small; stage it with a blocking copy into scratch
-    TSF_TRY(sc.get(nullptr, 0, sizeof(float) * M * M * M, W));
-    hipError_t e = hipMemcpyAsync(sc.ptr, hmask, sizeof(float) * M * M * M, hipMemcpyHostToDevice, stream);
-    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
-    Dims d = D(dims);
-    dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ);
-    hipLaunchKernelGGL(k_blur, grid, dim3(BX, BY, BZ), 0, stream, in, out, d, m, (const float*)sc.ptr);
-    TSF_TRY(check_launch(W));
-    e = hipStreamSynchronize(stream);  // scratch is freed on return
-    if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
-    return TSF_OK;
+    const Dims d = D(dims);
+    switch (m) {
+        case 1: launch_blur<1>(in, out, d, hmask, nullptr, stream); break;
+        case 2: launch_blur<2>(in, out, d, hmask, nullptr, stream); break;
+        case 3: launch_blur<3>(in, out, d, hmask, nullptr, stream); break;
+        case 4: launch_blur<4>(in, out, d, hmask, nullptr, stream); break;
+        default: {
+            // 1331 weights: staged in device memory with a blocking copy (the
+            // scratch is freed on return)
+            const size_t bytes = sizeof(float) * 11 * 11 * 11;
+            Scratch sc;
+            TSF_TRY(sc.get(nullptr, 0, bytes, W));
+            hipError_t e = hipMemcpyAsync(sc.ptr, hmask, bytes, hipMemcpyHostToDevice, stream);
+            if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+            launch_blur<5>(in, out, d, nullptr, (const float*)sc.ptr, stream);
+            TSF_TRY(check_launch(W));
+            e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) return set_error(TSF_ERR_HIP, "%s: %s", W, hipGetErrorString(e));
+            return TSF_OK;
+        }
+    }
+    return check_launch(W);
 }
 
 tsf_status tsf_to_float(const void* in, int32_t in_type, float* out, tsf_dims dims, float minimum, float maximum,
