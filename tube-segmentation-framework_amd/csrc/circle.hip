@@ -378,6 +378,32 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
+// The same with the halo HC a compile-time constant: the tile's strides are
+// immediates, so a sample's cell costs a few float operations and its eight
+// taps are reads at constant offsets from one address.
+template <class S, bool NORM, int BX, int BY, int BZ, int HC>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_mixed_h(const typename Texel<S>::P* __restrict__ tex, float inv, PlanesF32 H, Dims d, float rmin,
+                      float rmax, float rstep, CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad) {
+    constexpr int NT = BX * BY * BZ, TX = BX + 2 * HC, TY = BY + 2 * HC, TZ = BZ + 2 * HC, CELLS = TX * TY * TZ;
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
+    const LdsTexC<S, TX, TY, TZ> T{tile, inv, -(float)(((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P))};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    for (int c = tid; c < CELLS; c += NT) {
+        const int lx = c % TX, rr = c / TX, ly = rr % TY, lz = rr / TY;
+        const int gx = clampi(ox + lx, 0, d.X - 1), gy = clampi(oy + ly, 0, d.Y - 1), gz = clampi(oz + lz, 0, d.Z - 1);
+        tile[c] = tex[d.at(gx, gy, gz)];
+    }
+    __syncthreads();
+    tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
 // F32 planes -> 16-bit texels holding value * scale (x, y, z; w = 0, the
 // mixed mode's Hessian reads w from its F32 plane).
 // (round to nearest even)
@@ -504,6 +530,18 @@ static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float in
         return TSF_OK;
     }
     dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
+    // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides, when
+    // the float cell offsets stay exact: 8 (TX Y + X) well below 2^23
+    const bool exact = 8.0 * ((double)(BX + 2 * h) * (d.Y + 2 * h + 2) + (d.X + 2 * h + 2)) < 4194304.0;
+    if (exact && (h == 4 || h == 5)) {
+        if (h == 4)
+            hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, 4>), grid, block, lds, stream, (const P*)tex,
+                               inv, H, d, rmin, rmax, rstep, tab, tdf, radius);
+        else
+            hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, 5>), grid, block, lds, stream, (const P*)tex,
+                               inv, H, d, rmin, rmax, rstep, tab, tdf, radius);
+        return TSF_OK;
+    }
     hipLaunchKernelGGL((k_tdf_lds_mixed<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, H, d,
                        rmin, rmax, rstep, tab, tdf, radius, h);
     return TSF_OK;

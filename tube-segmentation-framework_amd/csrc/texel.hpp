@@ -312,6 +312,54 @@ __device__ __forceinline__ void trilinear_fused(const LdsTex<S>& T, const Dims&,
     out[2] = sz_;
 }
 
+// LdsTexC: an LdsTex whose tile dimensions are compile-time constants, for
+// the fused sampler below.  o8 is minus the byte offset of the tile's
+// origin (ox, oy, oz) as an integer-valued float.
+template <class S, int TX, int TY, int TZ>
+struct LdsTexC {
+    const typename Texel<S>::P* p;
+    float inv;
+    float o8;
+};
+
+// trilinear_fused on an LdsTexC: the same cell, weights, taps and order.
+// The cell's byte offset is formed in float, fma(fw, 8 TX TY, o8) then y
+// then x, every term an integer: exact while the partial sums stay below
+// 2^24 in magnitude (8 (TX Y + X) < 2^23 is checked by the launcher).  A
+// NaN position clamps to offset 0 and any other out-of-tile one to the
+// last cell (their weights are NaN, so the cell read does not matter).
+template <class S, int TX, int TY, int TZ>
+__device__ __forceinline__ void trilinear_fused(const LdsTexC<S, TX, TY, TZ>& T, const Dims&, float uu, float vv,
+                                                float ww, float out[3]) {
+    typedef Texel<S> X;
+    typedef typename X::P P;
+    constexpr int SY = TX, SZ = TX * TY, B = (int)sizeof(P);
+    constexpr float LAST = (float)((TX * TY * TZ - 1 - SZ - SY - 1) * B);
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const float fo =
+        __builtin_fmaf(fu, (float)B, __builtin_fmaf(fv, (float)(SY * B), __builtin_fmaf(fw, (float)(SZ * B), T.o8)));
+    const unsigned off = (unsigned)fminf(fmaxf(fo, 0.0f), LAST);
+    const P* q = (const P*)((const char*)T.p + off);
+    const P p0 = q[0], p1 = q[1], p2 = q[SY], p3 = q[SY + 1];
+    const P p4 = q[SZ], p5 = q[SZ + 1], p6 = q[SZ + SY], p7 = q[SZ + SY + 1];
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    float sx = X::mulx(wt0, p0), sy_ = X::muly(wt0, p0), sz_ = X::mulz(wt0, p0);
+    sx = X::fmx(wt1, p1, sx); sy_ = X::fmy(wt1, p1, sy_); sz_ = X::fmz(wt1, p1, sz_);
+    sx = X::fmx(wt2, p2, sx); sy_ = X::fmy(wt2, p2, sy_); sz_ = X::fmz(wt2, p2, sz_);
+    sx = X::fmx(wt3, p3, sx); sy_ = X::fmy(wt3, p3, sy_); sz_ = X::fmz(wt3, p3, sz_);
+    sx = X::fmx(wt4, p4, sx); sy_ = X::fmy(wt4, p4, sy_); sz_ = X::fmz(wt4, p4, sz_);
+    sx = X::fmx(wt5, p5, sx); sy_ = X::fmy(wt5, p5, sy_); sz_ = X::fmz(wt5, p5, sz_);
+    sx = X::fmx(wt6, p6, sx); sy_ = X::fmy(wt6, p6, sy_); sz_ = X::fmz(wt6, p6, sz_);
+    sx = X::fmx(wt7, p7, sx); sy_ = X::fmy(wt7, p7, sy_); sz_ = X::fmz(wt7, p7, sz_);
+    out[0] = sx;
+    out[1] = sy_;
+    out[2] = sz_;
+}
+
 // The same from texels in device memory (clamp-to-edge at the volume).
 template <class S>
 __device__ __forceinline__ void trilinear_fused(const Tex<S>& T, const Dims& d, float uu, float vv, float ww,
