@@ -459,7 +459,9 @@ static void launch_sr2(bool out32, const void* const f[3], const void* const v[3
     a.c0 = 1.0f - 6.0f * mu;
     a.k = unit / 32767.0f;
     a.seed = seed;
-    a.zspan = SRZ + 6;
+    // a chunk uses planes qs-2 .. qe+1: SRZ + 4 of them; the look-ahead loads
+    // past those fall outside the resources and fetch nothing
+    a.zspan = SRZ + 4;
     const dim3 grid((unsigned)std::max(1, (d.X - 4 + SROX - 1) / SROX), (unsigned)std::max(1, (d.Y - 4 + SROY - 1) / SROY),
                     (unsigned)((d.Z - 2 + SRZ - 1) / SRZ));
     if (out32) hipLaunchKernelGGL((k_gvf_sr2<S, SV, SF, true>), grid, dim3(64 * SRW), 0, stream, a);
