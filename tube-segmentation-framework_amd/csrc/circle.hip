@@ -393,7 +393,8 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     xcd_tile(tbx, tby, tbz);
     const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
     const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
-    const LdsTexC<S, TX, TY, TZ> T{tile, inv, -(float)(((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P))};
+    const int lo = (int)(size_t)(__attribute__((address_space(3))) P*)tile;  // the tile's LDS address
+    const LdsTexC<S, TX, TY, TZ> T{inv, (float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P)), (float)lo};
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
     for (int c = tid; c < CELLS; c += NT) {
         const int lx = c % TX, rr = c / TX, ly = rr % TY, lz = rr / TY;
@@ -512,6 +513,24 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     hipLaunchKernelGGL((k_tdf<S, NORM>), grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf, radius);
 }
 
+// The compile-time-halo kernel on BX x BY x BZ bricks, when the float cell
+// addresses stay exact (8 (TX Y + X) + 64 KiB below 2^23) and the tile and
+// the brick's state fit in 64 KiB; false when it does not run.
+template <class S, bool NORM, int BX, int BY, int BZ, int HC>
+static bool launch_tdf_mixed_h(const void* tex, float inv, const PlanesF32& H, Dims d, float rmin, float rmax,
+                               float rstep, const CircleTables& tab, float* tdf, float* radius, hipStream_t stream) {
+    typedef typename Texel<S>::P P;
+    constexpr int TX = BX + 2 * HC;
+    const size_t lds = (size_t)TX * (BY + 2 * HC) * (BZ + 2 * HC) * sizeof(P);
+    const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
+    const bool exact = 8.0 * ((double)TX * (d.Y + 2 * HC + 2) + (d.X + 2 * HC + 2)) + 65536.0 < 4194304.0;
+    if (!exact || lds + state > 65536) return false;
+    const dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
+    hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, HC>), grid, block, lds, stream, (const P*)tex, inv, H,
+                       d, rmin, rmax, rstep, tab, tdf, radius);
+    return true;
+}
+
 template <class S, bool NORM>
 static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float inv, Dims d, float rmin, float rmax,
                                    float rstep, float* tdf, float* radius, hipStream_t stream, const char* W) {
@@ -529,19 +548,16 @@ static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float in
         (void)W;
         return TSF_OK;
     }
-    dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
-    // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides, when
-    // the float cell offsets stay exact: 8 (TX Y + X) well below 2^23
-    const bool exact = 8.0 * ((double)(BX + 2 * h) * (d.Y + 2 * h + 2) + (d.X + 2 * h + 2)) < 4194304.0;
-    if (exact && (h == 4 || h == 5)) {
-        if (h == 4)
-            hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, 4>), grid, block, lds, stream, (const P*)tex,
-                               inv, H, d, rmin, rmax, rstep, tab, tdf, radius);
-        else
-            hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, 5>), grid, block, lds, stream, (const P*)tex,
-                               inv, H, d, rmin, rmax, rstep, tab, tdf, radius);
+    // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides.
+    // Halo 5 on 8 x 2 x 16 bricks: tile strides 18 and 216 texels, so all
+    // eight taps are LDS reads at 8-bit offsets from one address (measured
+    // 28.7 -> 28.4 ms at 512^3); halo 4 keeps 16 x 4 x 4 (its 8 x 4 x 8
+    // equivalent measured 27.2 -> 28.5 ms).
+    if (h == 4 && launch_tdf_mixed_h<S, NORM, 16, 4, 4, 4>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
         return TSF_OK;
-    }
+    if (h == 5 && launch_tdf_mixed_h<S, NORM, 8, 2, 16, 5>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
+        return TSF_OK;
+    dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
     hipLaunchKernelGGL((k_tdf_lds_mixed<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, H, d,
                        rmin, rmax, rstep, tab, tdf, radius, h);
     return TSF_OK;

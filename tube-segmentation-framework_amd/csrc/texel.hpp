@@ -313,21 +313,22 @@ __device__ __forceinline__ void trilinear_fused(const LdsTex<S>& T, const Dims&,
 }
 
 // LdsTexC: an LdsTex whose tile dimensions are compile-time constants, for
-// the fused sampler below.  o8 is minus the byte offset of the tile's
-// origin (ox, oy, oz) as an integer-valued float.
+// the fused sampler below, addressed by LDS byte addresses held as
+// integer-valued floats.
 template <class S, int TX, int TY, int TZ>
 struct LdsTexC {
-    const typename Texel<S>::P* p;
     float inv;
-    float o8;
+    float o8;  // LDS address of the tile (lo) minus the byte offset of its origin
+    float lo;  // LDS address of the tile
 };
 
 // trilinear_fused on an LdsTexC: the same cell, weights, taps and order.
-// The cell's byte offset is formed in float, fma(fw, 8 TX TY, o8) then y
+// The cell's LDS address is formed in float, fma(fw, 8 TX TY, o8) then y
 // then x, every term an integer: exact while the partial sums stay below
-// 2^24 in magnitude (8 (TX Y + X) < 2^23 is checked by the launcher).  A
-// NaN position clamps to offset 0 and any other out-of-tile one to the
-// last cell (their weights are NaN, so the cell read does not matter).
+// 2^24 in magnitude (the launcher checks 8 (TX Y + X) + 64 KiB < 2^23).  A
+// median with the tile's first and last cell keeps every address inside
+// the workgroup's LDS, whatever it makes of a NaN (the weights of a
+// non-finite position are NaN, so the cell read does not matter).
 template <class S, int TX, int TY, int TZ>
 __device__ __forceinline__ void trilinear_fused(const LdsTexC<S, TX, TY, TZ>& T, const Dims&, float uu, float vv,
                                                 float ww, float out[3]) {
@@ -339,10 +340,13 @@ __device__ __forceinline__ void trilinear_fused(const LdsTexC<S, TX, TY, TZ>& T,
     const float a = uu - fu, b = vv - fv, c = ww - fw;
     const float fo =
         __builtin_fmaf(fu, (float)B, __builtin_fmaf(fv, (float)(SY * B), __builtin_fmaf(fw, (float)(SZ * B), T.o8)));
-    const unsigned off = (unsigned)fminf(fmaxf(fo, 0.0f), LAST);
-    const P* q = (const P*)((const char*)T.p + off);
-    const P p0 = q[0], p1 = q[1], p2 = q[SY], p3 = q[SY + 1];
-    const P p4 = q[SZ], p5 = q[SZ + 1], p6 = q[SZ + SY], p7 = q[SZ + SY + 1];
+    const unsigned off = (unsigned)__builtin_amdgcn_fmed3f(fo, T.lo, T.lo + LAST);
+    static_assert(sizeof(P) == 8, "8-byte texels");
+    typedef __attribute__((address_space(3))) const unsigned long long LW;
+    LW* q = (LW*)(size_t)off;
+    auto tx = [](unsigned long long w) { P t; __builtin_memcpy(&t, &w, 8); return t; };
+    const P p0 = tx(q[0]), p1 = tx(q[1]), p2 = tx(q[SY]), p3 = tx(q[SY + 1]);
+    const P p4 = tx(q[SZ]), p5 = tx(q[SZ + 1]), p6 = tx(q[SZ + SY]), p7 = tx(q[SZ + SY + 1]);
     const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
     const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
     const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
