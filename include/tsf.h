@@ -104,7 +104,11 @@ tsf_status tsf_create_blur_mask(float sigma, float* mask, int32_t* mask_size);
 /* blurVolumeWithGaussian (kernels_no_3d_write.cl:1442-1465) with the mask of
  * tsf_create_blur_mask(sigma): clamp-to-edge (2m+1)^3 correlation, summed in
  * the reference's loop order.  Replaces the two blur call sites of
- * runCircleFittingMethod (tube-segmentation.cpp:247-291, 536-582). */
+ * runCircleFittingMethod (tube-segmentation.cpp:247-291, 536-582).
+ * m <= 4 (sigma <= 2): the weights travel in the kernel arguments, so the
+ * call is asynchronous and allocates nothing.  m = 5: the 1331 weights are
+ * copied to a device buffer the call allocates, and it synchronises the
+ * stream before freeing it. */
 tsf_status tsf_blur_gaussian(const float* in, float* out, tsf_dims dims, float sigma, hipStream_t stream);
 
 /* toFloat (kernels_no_3d_write.cl:1414-1440): clamp to [minimum, maximum],
