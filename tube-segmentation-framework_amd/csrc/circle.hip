@@ -217,7 +217,7 @@ __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int 
     __shared__ int st_v[NT];
     __shared__ int wcount[NW];
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
-    // round 0 at the thread's own voxel
+    // the first radius (rmin) at the thread's own voxel
     int v = tid;  // voxel of the brick this thread works on
     float e2[3], e3[3], best = 0.0f;
     bool act = false;
