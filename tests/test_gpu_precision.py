@@ -17,6 +17,9 @@ that block; both sides see the same input).  Errors are L2-relative:
                direction e1 is ill-conditioned: the FP64 and FP32 results
                already sit on either side of the switch for any
                perturbation of the field; their share is reported).
+Also reported, on the combined TDF: the largest |TDF - TDF64| over the voxels
+with TDF64 >= tdf-high (0.5, the parameter's default) and the number of
+voxels whose candidate mask (TDF >= tdf-high) differs from FP64's.
 Forms: F32 (tsf_gvf_euler, four steps per launch, FP32 TDF); the precision
 form (tsf_gvf_euler_sr + tsf_tdf_circle_mixed: the 16-bit iterate with
 stochastic rounding, a float tail of 32 iterations, the final iterate F32;
@@ -46,6 +49,7 @@ pytestmark = pytest.mark.gpu
 MU = 0.05
 TAIL = 32
 F16_SCALE = 2.0 ** 14
+TDF_HIGH = 0.5  # tdf-high's default (parameters/parameters:14)
 
 
 def densest_block(vol, n, thr=0.7, step=32):
@@ -159,6 +163,9 @@ def precision_table(tsf, orc, vol, iters, label):
         es = tdf_errors(s[0], s[1], s64[0], s64[1], cond)
         elg = tdf_errors(l[0], l[1], l64[0], l64[1], cond)
         ec = tdf_errors(c[0], c[1], c64[0], c64[1], cond)
+        high = c64[0] >= TDF_HIGH
+        mx = float(np.abs(c[0][high].astype(np.float64) - c64[0][high]).max()) if high.any() else 0.0
+        flips = int(((c[0] >= TDF_HIGH) != high).sum())
         res[form] = (ev, el, es, elg, ec)
         print("PRECISION " + json.dumps({"volume": label, "form": form,
                                          "tail": TAIL if form in ("sn16", "f16", "bf16") else None,
@@ -167,11 +174,13 @@ def precision_table(tsf, orc, vol, iters, label):
                                          "tdf_combined_well_conditioned": ec[1], "radius_changed": ec[2],
                                          "responding": ec[3], "jumps_over_1pct": ec[4],
                                          "tdf_combined_without_jumps": ec[5],
-                                         "basis_ambiguous": int((resp & ~cond).sum())}))
+                                         "basis_ambiguous": int((resp & ~cond).sum()),
+                                         "tdf_max_abs_at_tdf_high": mx, "candidate_mask_flips": flips}))
         print(f"{label} {form:4s}: field {ev:.2e} eigenvalues {el:.2e} | TDF small {es[0]:.2e} ({es[1]:.2e} "
               f"well-cond.) large {elg[0]:.2e} ({elg[1]:.2e}) combined {ec[0]:.2e} ({ec[1]:.2e}); "
               f"radius changed {ec[2]} of {ec[3]} responding; basis-ambiguous {int((resp & ~cond).sum())}; "
-              f"jumps > 1 %: {ec[4]}, L2 without them {ec[5]:.2e}")
+              f"jumps > 1 %: {ec[4]}, L2 without them {ec[5]:.2e}; max |dT| at T >= {TDF_HIGH} {mx:.2e}, "
+              f"candidate-mask flips {flips}")
     return res
 
 
