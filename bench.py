@@ -511,10 +511,52 @@ def fit(args, vol, d_vol, shape, stream):
                     for h in hist]}), flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_stages(oracle, vol):
+    """The forward's stages in the oracle's FP32 restatement (all threads) on
+    the central 128^3 block: median of 3 after one warm-up pass, ms."""
+    c = min(128, vol.shape[0])
+    o = (vol.shape[0] - c) // 2
+    sub = np.ascontiguousarray(vol[o:o + c, o:o + c, o:o + c])
+    names = ["blur", "create vector field", "TDF small (r 0.5-3)", "GVF (mu 0.05, 400 iterations)",
+             "TDF large (r 2.5-4)", "combine", "candidates + HP compaction"]
+    runs = []
+    for rep in range(4):
+        t = []
+        def timed(f):
+            t0 = time.perf_counter()
+            r = f()
+            t.append((time.perf_counter() - t0) * 1e3)
+            return r
+        b = timed(lambda: oracle.blur(sub, 0.5, "f32"))
+        F = timed(lambda: oracle.vector_field(b, 0.2, 1, "f32"))
+        ts, rs = timed(lambda: oracle.tdf_circle(F, 0.5, 3.0, 0.5, "f32"))
+        V = timed(lambda: oracle.gvf_euler(F, 0.05, 400, "f32"))
+        tl, rl = timed(lambda: oracle.tdf_circle(V, 2.5, 4.0, 0.5, "f32"))
+        tc, _ = timed(lambda: oracle.combine(ts, rs, tl, rl, "f32"))
+        timed(lambda: oracle.hp_compact(oracle.candidates(tc, 0.5, "f32"), "pyramid"))
+        if rep:
+            runs.append(t)
+    return {"sample": f"{c}^3 centre block, median of 3 after a warm-up",
+            "ms": {n: round(float(np.median([r[i] for r in runs])), 1) for i, n in enumerate(names)}}
+
+
 def cpu_baseline(vol):
     """Oracle FP32 GVF (OpenMP, all host threads) on a bounded sample: the
     central 256^3 block of the same volume after the same blur and vector
-    field, timed over enough iterations for ~12 s."""
+    field, timed over enough iterations for ~12 s.  Also: the same GVF on
+    one thread (the 128^3 block, ~3 s), the forward's stages on the 128^3
+    block, and the CPU model."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     c = min(256, vol.shape[0])
@@ -533,9 +575,30 @@ def cpu_baseline(vol):
     t0 = time.time()
     oracle.gvf_euler(F, 0.05, iters, "f32")
     dt = time.time() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(sub.size * iters / dt / 1e9, 4), "unit": "Gvoxel-iter/s", "cores": threads,
-            "kind": "port", "sample": f"oracle FP32 GVF Euler, {c}^3 centre block, {iters} iterations, {dt:.1f}s"}
+    threads = oracle.set_threads()
+    out = {"value": round(sub.size * iters / dt / 1e9, 4), "unit": "Gvoxel-iter/s", "cores": threads,
+           "kind": "port", "sample": f"oracle FP32 GVF Euler, {c}^3 centre block, {iters} iterations, {dt:.1f}s",
+           "cpu_model": cpu_model()}
+    out["stages"] = cpu_stages(oracle, vol)
+    # one thread, on the 128^3 block
+    c1 = min(128, vol.shape[0])
+    o1 = (vol.shape[0] - c1) // 2
+    F1 = np.ascontiguousarray(F[:, o1 - o:o1 - o + c1, o1 - o:o1 - o + c1, o1 - o:o1 - o + c1]) if c1 < c else F
+    oracle.set_threads(1)
+    try:
+        oracle.gvf_euler(F1, 0.05, 2, "f32")
+        t0 = time.time()
+        n1 = 8
+        oracle.gvf_euler(F1, 0.05, n1, "f32")
+        n1 = max(8, int(3.0 / max((time.time() - t0) / n1, 1e-4)))
+        t0 = time.time()
+        oracle.gvf_euler(F1, 0.05, n1, "f32")
+        d1 = time.time() - t0
+    finally:
+        oracle.set_threads(threads)
+    out["one_thread"] = {"value": round(c1 ** 3 * n1 / d1 / 1e9, 4), "unit": "Gvoxel-iter/s",
+                         "sample": f"oracle FP32 GVF Euler, {c1}^3 centre block, {n1} iterations, {d1:.1f}s"}
+    return out
 
 
 if __name__ == "__main__":

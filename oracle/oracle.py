@@ -49,6 +49,14 @@ def _dims(vol):
     return int(X), int(Y), int(Z)
 
 
+def set_threads(n=0):
+    """Worker threads of the oracle's OpenMP loops (n <= 0 leaves them);
+    returns the count in effect."""
+    f = lib().orc_set_threads
+    f.restype = ctypes.c_int
+    return int(f(ctypes.c_int(int(n))))
+
+
 def blur_mask(sigma):
     mask = np.zeros(11 ** 3, np.float32)
     m = ctypes.c_int(0)

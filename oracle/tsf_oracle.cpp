@@ -38,6 +38,9 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define TSF_QL_MAX_ITER 100
 
@@ -2375,4 +2378,16 @@ extern "C" int orc_crop_region(const int32_t* lx, const int32_t* ly, const int32
     size[1] = SY;
     size[2] = SZ;
     return 0;
+}
+
+// Worker threads of the OpenMP loops (the bench's CPU baseline times one
+// thread and all of them); the count in effect.
+extern "C" int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
