@@ -396,10 +396,26 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     const int lo = (int)(size_t)(__attribute__((address_space(3))) P*)tile;  // the tile's LDS address
     const LdsTexC<S, TX, TY, TZ> T{inv, (float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P)), (float)lo};
     const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
-    for (int c = tid; c < CELLS; c += NT) {
-        const int lx = c % TX, rr = c / TX, ly = rr % TY, lz = rr / TY;
-        const int gx = clampi(ox + lx, 0, d.X - 1), gy = clampi(oy + ly, 0, d.Y - 1), gz = clampi(oz + lz, 0, d.Z - 1);
-        tile[c] = tex[d.at(gx, gy, gz)];
+    // every load of the tile is issued before the first LDS store, so one
+    // memory latency covers them all (the cell's coordinates come from
+    // divisions by constants)
+    constexpr int NIT = (CELLS + NT - 1) / NT;
+    const i64 plane = (i64)d.X * d.Y;
+    P v[NIT];
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int c = tid + NT * k;
+        if (CELLS % NT == 0 || c < CELLS) {
+            const int lx = c % TX, rr = c / TX, ly = rr % TY, lz = rr / TY;
+            const int gx = clampi(ox + lx, 0, d.X - 1), gy = clampi(oy + ly, 0, d.Y - 1);
+            const int gz = clampi(oz + lz, 0, d.Z - 1);
+            v[k] = tex[plane * gz + ((i64)gy * d.X + gx)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int c = tid + NT * k;
+        if (CELLS % NT == 0 || c < CELLS) tile[c] = v[k];
     }
     __syncthreads();
     tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
