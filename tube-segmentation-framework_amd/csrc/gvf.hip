@@ -562,12 +562,13 @@ __global__ void __launch_bounds__(FBX* FBY, FINISH ? 2 : 4)
     m.qs = 1 + tbz * FZC;
     m.qe = min(m.qs + FZC, d.Z - 1);
     const int t0 = m.qs - 1, te = m.qe;
-    // buffer resources over the planes this chunk reads, zb .. zb + FZC + 8
-    // at most (the loop runs up to three steps past te and loads three
-    // planes ahead)
+    // buffer resources over the planes the chunk's outputs depend on, zb ..
+    // qe + 1 (two steps from planes below qe); the loop runs up to three
+    // steps past te and loads three planes ahead, and those look-ahead loads
+    // past the resource fetch nothing (they feed no stored plane)
     m.pb = (unsigned)(m.P * (i64)sizeof(typename S::T));
     m.zb = max(t0 - 1, 0);
-    const int zspan = min(d.Z, m.zb + FZC + 9) - m.zb;
+    const int zspan = min(d.Z, m.qe + 2) - m.zb;
     const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
     const typename S::T* FP[3] = {f.x, f.y, f.z};
 #pragma unroll
@@ -867,10 +868,12 @@ __global__ void __launch_bounds__(FBX* F4Y, 4)
     m.qs = 1 + tbz * FZC;
     m.qe = min(m.qs + FZC, d.Z - 1);
     const int t0 = m.qs - 3, te = m.qe + 2;
-    // planes read: max(t0-1, 0) .. te+6 at most
+    // resources over the planes the chunk's outputs depend on, max(t0-1, 0)
+    // .. qe + 3 (four steps from planes below qe); the look-ahead loads past
+    // them (the loop reads up to te+6) fetch nothing and feed no stored plane
     m.pb = (unsigned)(m.P * (i64)sizeof(typename S::T));
     m.zb = max(t0 - 1, 0);
-    const int zspan = min(d.Z, m.zb + FZC + 13) - m.zb;
+    const int zspan = min(d.Z, m.qe + 4) - m.zb;
     const typename S::T* VP[3] = {vin.x, vin.y, vin.z};
     const typename S::T* FP[3] = {f.x, f.y, f.z};
 #pragma unroll
