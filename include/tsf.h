@@ -204,6 +204,13 @@ tsf_status tsf_gvf_euler_low_memory(const tsf_field* f, const tsf_field* out, ts
                                     int32_t iterations, void* workspace, size_t workspace_bytes, hipStream_t stream);
 size_t tsf_gvf_euler_low_memory_workspace_size(tsf_dims dims, int32_t storage);
 
+/* 1 when tsf_gvf_euler_sr runs on volumes of these dimensions (X even and
+ * >= 64, Y >= 32, Z >= 4, and a 38-plane chunk of 16-bit values under
+ * 2 GiB), else 0: its precondition, for callers that pick another path
+ * instead.  No counterpart in the reference (the precision form is this
+ * library's). */
+int32_t tsf_gvf_euler_sr_supported(tsf_dims dims);
+
 /* The precision form of tsf_gvf_euler (GVF3DIteration, kernels_no_3d_write.cl:
  * 1746-1787, the recurrence of runFastGVF, gradientVectorFlow.cpp:1007-1172):
  * the iterate streams through device memory as 16-bit planes and every step

@@ -632,6 +632,52 @@ static void gpu_tests(const std::string& dsdir) {
         if (r.empty()) r = compare(radp.downloadPlane(ocl, 0), r2, 0.0f, 0.0f, "precision-form radius");
         return r;
     });
+    // the precision form on a volume whose X is odd (99): the 16-bit GVF does
+    // not take it, so the large branch keeps an F32 field and runs the FP32
+    // GVF; both TDFs stay in the mixed mode
+    run("runCircleFittingMethod_precision_form_odd_x", [&] {
+        const int Xo = X - 1;
+        const size_t no = (size_t)Xo * Y * Z;
+        if (tsf_gvf_euler_sr_supported(tsf_dims{Xo, Y, Z}) || !tsf_gvf_euler_sr_supported(tsf_dims{X, Y, Z}))
+            return std::string("tsf_gvf_euler_sr_supported: wrong verdict on the odd and the even volume");
+        std::vector<uint8_t> rawo(no);
+        for (size_t r = 0; r < (size_t)Y * Z; r++)
+            std::copy(raw.begin() + r * X, raw.begin() + r * X + Xo, rawo.begin() + r * Xo);
+        const I3 so(Xo, Y, Z);
+        Buffer dro(no);
+        dro.upload(ocl, rawo.data(), no);
+        Image3D dso(so, 1);
+        check(tsf_to_float(dro.get(), 0, (float*)dso.plane(0), dso.dims(), 0.0f, 75.0f, ocl.queue));
+        paramList q = p;
+        setParameter(q, "vector-storage", "precision");
+        Image3D vfo, tdfo, rado;
+        runCircleFittingMethod(ocl, &dso, so, q, vfo, tdfo, rado);
+        if (vfo.storage() != TSF_STORAGE_F32) return std::string("the fallback's GVF result is not F32");
+        auto f16 = [&](const std::vector<float>& a) {
+            std::vector<float> b(a), t(3 * no);
+            std::vector<uint16_t> h(3 * no);
+            for (size_t i = 0; i < 3 * no; i++) t[i] = a[i] * 16384.0f;
+            orc_half_from_float(t.data(), h.data(), (long long)(3 * no));
+            orc_half_to_float(h.data(), b.data(), (long long)(3 * no));
+            for (size_t i = 0; i < 3 * no; i++) b[i] = b[i] / 16384.0f;
+            return b;
+        };
+        std::vector<double> rdo(rawo.begin(), rawo.end());
+        std::vector<float> vo(no), bo(no), Fo(4 * no), Vo(4 * no), t1(no), r1(no), t2(no), r2(no);
+        orc_to_float_f32(rdo.data(), vo.data(), (long long)no, 0.0f, 75.0f);
+        orc_blur_f32(vo.data(), bo.data(), Xo, Y, Z, m, mask);
+        orc_vector_field_f32(bo.data(), Fo.data(), Xo, Y, Z, 0.1f, 1);
+        orc_tdf_circle_mixed_f32(Fo.data(), f16(Fo).data(), Xo, Y, Z, 0.5f, 3.0f, 0.5f, t1.data(), r1.data());
+        orc_gvf_euler_f32(Fo.data(), Vo.data(), Xo, Y, Z, 0.05f, 250);
+        orc_tdf_circle_mixed_f32(Vo.data(), f16(Vo).data(), Xo, Y, Z, 2.5f, 4.0f, 1.0f, t2.data(), r2.data());
+        orc_combine_f32(t1.data(), r1.data(), t2.data(), r2.data(), (long long)no);
+        std::string r;
+        for (int c = 0; c < 4 && r.empty(); c++)
+            r = compare(vfo.downloadPlane(ocl, c), std::vector<float>(Vo.begin() + c * no, Vo.begin() + (c + 1) * no),
+                        1e-4f, 1e-7f, "fallback GVF field");
+        if (r.empty()) r = compare(tdfo.downloadPlane(ocl, 0), t2, 1e-4f, 1e-7f, "fallback TDF");
+        return r;
+    });
 }
 
 // ----------------------------------------------------------------- autograd

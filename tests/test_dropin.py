@@ -32,7 +32,7 @@ def test_dropin_pipeline_on_dataset_1(tsf):
     r = _run("gpu", os.path.join(ROOT, "tests", "golden", "dataset_1"), timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 16
+    assert r.stdout.count("PASS") == 17
 
 
 @pytest.mark.gpu

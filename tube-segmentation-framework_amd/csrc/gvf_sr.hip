@@ -624,6 +624,8 @@ size_t tsf_gvf_euler_sr_workspace_size(tsf_dims dims, int32_t f32_tail) {
     return 12 * n + (f32_tail > 0 ? 40 * n : 0);
 }
 
+int32_t tsf_gvf_euler_sr_supported(tsf_dims dims) { return sr_shape_ok(D(dims)) ? 1 : 0; }
+
 tsf_status tsf_gvf_euler_sr(const tsf_field* f, const tsf_field* out, tsf_dims dims, float mu, int32_t iterations,
                             int32_t f32_tail, uint32_t seed, void* workspace, size_t workspace_bytes,
                             hipStream_t stream) {

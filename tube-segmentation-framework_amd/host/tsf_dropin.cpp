@@ -473,8 +473,16 @@ static Image3D blurOrSame(OpenCL& ocl, Image3D& data, float sigma) {
 
 static Image3D vectorFieldOf(OpenCL& ocl, Image3D& vol, paramList& p, bool large) {
     // the precision form keeps the small branch's field in F32 (its TDF takes
-    // the Hessian from it) and stores the large branch's, the GVF's f, as SNORM16
-    const tsf_storage s = precisionForm(p) && !large ? TSF_STORAGE_F32 : vectorStorage(p);
+    // the Hessian from it) and stores the large branch's, the GVF's f, as
+    // SNORM16; on volumes the 16-bit GVF does not take (odd or small X, ...)
+    // the large branch stays F32 and runs the FP32 GVF
+    tsf_storage s = precisionForm(p) && !large ? TSF_STORAGE_F32 : vectorStorage(p);
+    if (precisionForm(p) && large && !tsf_gvf_euler_sr_supported(vol.dims())) {
+        std::fprintf(stderr, "NOTE: the precision form's 16-bit GVF needs X even and >= 64, Y >= 32, Z >= 4; "
+                             "running the FP32 GVF on this %dx%dx%d volume\n",
+                     vol.size().x, vol.size().y, vol.size().z);
+        s = TSF_STORAGE_F32;
+    }
     Image3D F(vol.size(), 4, s, vectorScale(s));
     tsf_field f = F.field();
     const int sign = getParamStr(p, "mode") == "black" ? -1 : 1;

@@ -19,5 +19,5 @@ from ._capi import (  # noqa: F401
     link_centerpoints, link_centerpoints_workspace_size, graph_components, rasterize_tree_edges,
     segment_inverse_gradient, segment_inverse_gradient_workspace_size, segment_spheres, volume_limits, crop_scan_lines, crop_region, to_float_region,
     CROP_THRESHOLD, CROP_LUNG, tdf_spline, tdf_circle_mixed, tdf_circle_mixed_workspace_size,
-    gvf_euler_sr, gvf_euler_sr_workspace_size, gvf_euler_low_memory, gvf_euler_low_memory_workspace_size,
+    gvf_euler_sr, gvf_euler_sr_workspace_size, gvf_euler_sr_supported, gvf_euler_low_memory, gvf_euler_low_memory_workspace_size,
 )

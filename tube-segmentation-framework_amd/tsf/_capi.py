@@ -71,6 +71,7 @@ _SIGS = {
     "tsf_hp_compact_workspace_size": ([Dims], _sz),
     "tsf_gvf_euler_sr": ([_PF, _PF, Dims, _f32, _i32, _i32, ctypes.c_uint32, _vp, _sz, _vp], _i32),
     "tsf_gvf_euler_sr_workspace_size": ([Dims, _i32], _sz),
+    "tsf_gvf_euler_sr_supported": ([Dims], _i32),
     "tsf_tdf_circle_mixed": ([_PF, _i32, _f32, Dims, _f32, _f32, _f32, _vp, _vp, _vp, _sz, _vp], _i32),
     "tsf_tdf_circle_mixed_workspace_size": ([Dims], _sz),
     "tsf_hp_compact_noalloc": ([_vp, Dims, _vp, _i64, ctypes.POINTER(_i64), _vp], _i32),
@@ -464,6 +465,11 @@ def tdf_circle(field, rmin, rmax, rstep, tdf, radius, workspace=None, stream=Non
 
 def gvf_euler_sr_workspace_size(shape, f32_tail=0):
     return lib().tsf_gvf_euler_sr_workspace_size(_dims(shape), f32_tail)
+
+
+def gvf_euler_sr_supported(shape):
+    """Whether gvf_euler_sr runs on a volume of this (Z, Y, X) shape."""
+    return bool(lib().tsf_gvf_euler_sr_supported(_dims(shape)))
 
 
 def gvf_euler_sr(f, out, mu, iterations, seed=0, f32_tail=0, workspace=None, stream=None):
