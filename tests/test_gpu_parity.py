@@ -332,19 +332,22 @@ def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
     assert np.array_equal(gr, er)
 
 
+@pytest.mark.parametrize("storage", [1, 2])
 @pytest.mark.parametrize("rmin,rmax", [(0.5, 3.0), (2.5, 4.0)])  # halos 4 and 5 (compile-time strides)
 @pytest.mark.parametrize("shape", [(23, 29, 37), (9, 5, 70)])
-def test_tdf_circle_mixed_ragged(tsf, orc, shape, rmin, rmax):
+def test_tdf_circle_mixed_ragged(tsf, orc, shape, rmin, rmax, storage):
     """Mixed mode on volumes that are no multiple of the bricks (16 x 4 x 4
     at halo 4, 8 x 2 x 16 at halo 5) and thinner than the halo in y: the
-    partial bricks and the clamped tile against the oracle."""
+    partial bricks and the clamped tile against the oracle, F16 and BF16
+    samples."""
     V = _gvf_field(orc, shape, 12)
     fld = tsf.Field.from_host(V)
     t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
-    tsf.tdf_circle_mixed(fld, rmin, rmax, 0.5, t, r, sample_storage=1, sample_scale=2.0 ** 14)
+    tsf.tdf_circle_mixed(fld, rmin, rmax, 0.5, t, r, sample_storage=storage,
+                         sample_scale=2.0 ** 14 if storage == 1 else 1.0)
     gt, gr = t.download(), r.download()
     Vs = V.copy()
-    Vs[:3] = orc.f16_round(V[:3])
+    Vs[:3] = orc.f16_round(V[:3]) if storage == 1 else _bf16_round(V[:3])
     et, er = orc.tdf_circle_mixed(V, Vs, rmin, rmax, 0.5, "f32")
     assert np.array_equal(gt, et) and np.array_equal(gr, er)
 
