@@ -422,7 +422,9 @@ def test_tdf_mixed_known_answer(orc, prec, rtol):
     """The precision form's fused sampling on the linear-Hessian / uniform
     sample field pair: a uniform field samples to c exactly (the weights sum
     to one in any rounding order only to within an ulp, hence rtol), so the
-    response is S(rmin) = -c . (e3 sum c_j + e2 sum s_j) / 32 or 0."""
+    response is S(rmin) = -c . (e3 sum c_j + e2 sum s_j) / 32 or 0.  The sum
+    is the same at every radius, so in float a rounding-level rise may carry
+    the search past rmin: the f32 radius is any on the grid."""
     shape = (12, 12, 12)
     z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
     A = np.array([[0.30, 0.10, -0.05], [0.10, -0.20, 0.15], [-0.05, 0.15, 0.05]])
@@ -441,9 +443,54 @@ def test_tdf_mixed_known_answer(orc, prec, rtol):
         tdf, rad = orc.tdf_circle_mixed(FH, FS, 1.0, 3.0, 0.5, prec)
         S = -(cvec @ e3 * c.astype(np.float64).sum() + cvec @ e2 * s.astype(np.float64).sum()) / 32
         if S > 0:
-            assert np.isclose(tdf[6, 6, 6], S, rtol=rtol) and rad[6, 6, 6] == 1.0
+            assert np.isclose(tdf[6, 6, 6], S, rtol=rtol)
+            assert rad[6, 6, 6] == 1.0 if prec == "f64" else rad[6, 6, 6] in (1.0, 1.5, 2.0, 2.5, 3.0)
         else:
             assert tdf[6, 6, 6] == 0 and rad[6, 6, 6] == 0
+
+
+@pytest.mark.parametrize("prec,rtol", [("f64", 1e-11), ("f32", 1e-4)])
+def test_tdf_mixed_linear_field_known_answer(orc, prec, rtol):
+    """The fused sampling on a linear sample field FS(u) = B u + c0, which
+    trilinear interpolation reproduces whatever its weights (so a wrong
+    weight, a swapped plane or a dropped tap shows): the sample at
+    u = p - 0.5 + r V_j is B u + c0, the response at radius r is
+    S(r) = -(1/32) sum_j V_j . (B (p - 0.5 + r V_j) + c0), and the search
+    keeps the first maximum over r = 1, 1.5, .., 3 (0 if S(1) <= 0).  The
+    cases cover a response climbing to rmax, one falling from rmin and none."""
+    shape = (12, 12, 12)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in shape), indexing="ij")
+    A = np.array([[0.30, 0.10, -0.05], [0.10, -0.20, 0.15], [-0.05, 0.15, 0.05]])
+    FH = np.zeros((4,) + shape)
+    P = np.stack([x, y, z])
+    for i in range(3):
+        FH[i] = sum(A[i, j] * P[j] for j in range(3))
+    FH[3] = 1
+    c, s = orc.circle_tables()
+    lam, vec = orc.hessian_eigen(FH, 0, prec="f64")
+    e2, e3 = vec[3:6, 6, 6, 6], vec[6:9, 6, 6, 6]
+    V = np.outer(c.astype(np.float64), e3) + np.outer(s.astype(np.float64), e2)
+    q = np.full(3, 6.0) - 0.5
+    B0 = np.array([[-0.12, 0.03, 0.01], [0.02, -0.07, -0.04], [0.05, 0.01, -0.10]])
+    seen = set()
+    for B, c0 in ((B0, np.array([0.2, -0.1, 0.05])), (-B0, np.array([6.0, 0.0, 0.0])),
+                  (-B0, np.array([1.5, -0.9, -0.7]))):
+        FS = np.zeros((4,) + shape)
+        for i in range(3):
+            FS[i] = sum(B[i, j] * P[j] for j in range(3)) + c0[i]
+        FS[3] = 1
+        tdf, rad = orc.tdf_circle_mixed(FH, FS, 1.0, 3.0, 0.5, prec)
+        best, rbest = 0.0, 0.0
+        for r in np.arange(1.0, 3.01, 0.5):
+            S = -np.mean(np.einsum("ji,ji->j", V, (q + r * V) @ B.T + c0))
+            if S > best:
+                best, rbest = S, r
+            else:
+                break
+        seen.add(rbest)
+        assert rad[6, 6, 6] == rbest
+        assert np.isclose(tdf[6, 6, 6], best, rtol=rtol, atol=0)
+    assert seen == {0.0, 1.0, 3.0}
 
 
 def test_tdf_mixed_fused_is_as_close_to_fp64(orc):
