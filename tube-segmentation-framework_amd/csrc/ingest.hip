@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_scan_z(const T* __restrict__ in, Dims d
         for (int j = 0; j < ZB; j++) v[j] = on && z0 + j < d.Z ? (float)p[(i64)(z0 + j) * plane] : 0.0f;
 #pragma unroll
         for (int j = 0; j < ZB; j++) {
-            if (z0 + j >= d.Z) break;
+            if (z0 + j >= d.Z) continue;  // uniform: the planes past the volume's end
             if (on) line.step(v[j], thr, hu_limit);
             if (ROWS) {
                 const unsigned long long hit = __ballot(on && v[j] > thr);
