@@ -336,8 +336,8 @@ def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
 @pytest.mark.parametrize("rmin,rmax", [(0.5, 3.0), (2.5, 4.0)])  # halos 4 and 5 (compile-time strides)
 @pytest.mark.parametrize("shape", [(23, 29, 37), (9, 5, 70)])
 def test_tdf_circle_mixed_ragged(tsf, orc, shape, rmin, rmax, storage):
-    """Mixed mode on volumes that are no multiple of the bricks (16 x 4 x 4
-    at halo 4, 8 x 2 x 16 at halo 5) and thinner than the halo in y: the
+    """Mixed mode on volumes that are no multiple of the bricks (16 x 8 x 4
+    at halo 4, 8 x 8 x 8 at halo 5) and thinner than the halo in y: the
     partial bricks and the clamped tile against the oracle, F16 and BF16
     samples."""
     V = _gvf_field(orc, shape, 12)

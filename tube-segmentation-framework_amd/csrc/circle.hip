@@ -9,6 +9,8 @@
 // filter then fetches one 16-byte (or 8-byte) texel per tap.  Workgroups
 // cover compact 16 x 4 x 4 voxel bricks so neighbouring threads sample
 // neighbouring circles out of the same L1/L2 lines.
+#include <atomic>
+
 #include "eigen3.hpp"
 #include "field.hpp"
 #include "texel.hpp"
@@ -476,6 +478,32 @@ static unsigned grid1d(i64 n) {
     return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
 }
 
+// LDS of the current device, queried once per device: the most a workgroup
+// may allocate (static + dynamic) and the size of a CU's LDS (both 160 KiB
+// on gfx950).  64 KiB when the query fails.
+static size_t device_lds(bool per_cu) {
+    static std::atomic<int> cache[2][64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+    int v = cache[per_cu][dev].load(std::memory_order_relaxed);
+    if (v <= 0) {
+        const hipDeviceAttribute_t a =
+            per_cu ? hipDeviceAttributeMaxSharedMemoryPerMultiprocessor : hipDeviceAttributeMaxSharedMemoryPerBlock;
+        if (hipDeviceGetAttribute(&v, a, dev) != hipSuccess || v <= 0) v = 65536;
+        cache[per_cu][dev].store(v, std::memory_order_relaxed);
+    }
+    return (size_t)v;
+}
+static size_t lds_per_block() { return device_lds(false); }
+
+// Waves a CU holds of a TDF kernel whose workgroups of nt threads take
+// `bytes` of LDS each (the TDF kernels' registers allow six or more waves
+// per SIMD, so their LDS sets the count); 0 when a workgroup does not fit.
+static int tdf_waves(size_t bytes, int nt) {
+    if (bytes > lds_per_block()) return 0;
+    return (int)(device_lds(true) / bytes) * (nt / 64);
+}
+
 void pack_texels_f32(const tsf_field* field, float4* out, i64 n, hipStream_t stream) {
     hipLaunchKernelGGL(k_pack_f32, grid1d(n), 256, 0, stream, (const float*)field->x, (const float*)field->y,
                        (const float*)field->z, (const float*)field->w, out, n);
@@ -489,24 +517,48 @@ static int tdf_halo(float rmin, float rmax) {
     return (int)floorf(r + 0.5f) + 1;
 }
 
-// F32 through the three-plane tile (bricks of 8 x 8 x 4 voxels): whether
-// it fits LDS, and its size.
-constexpr int F3X = 8, F3Y = 8, F3Z = 4;
-static bool tdf_f32_lds(float rmin, float rmax, size_t& lds) {
+// F32 through the three-plane tile: bricks of 16 x F3Y x 4 voxels, F3Y 8 or
+// 16, whichever holds more waves per CU (the pass wants every wave it can
+// get; 16 x 16 x 4 spends less LDS on a wide halo per voxel: measured at
+// 512^3, 43.3 ms at radii 0.5-3 on 16 x 8 x 4 and 45.9 ms at 2.5-4 on
+// 16 x 16 x 4, against 51.6 and 58.0 ms on 8 x 8 x 4 bricks).  F3Y and
+// the tile's size; F3Y = 0 when neither fits LDS.
+static int tdf_f32_brick(float rmin, float rmax, size_t& lds) {
     const int h = tdf_halo(rmin, rmax);
-    lds = (size_t)(F3X + 2 * h) * (F3Y + 2 * h) * (F3Z + 2 * h) * 3 * sizeof(float);
-    const size_t state = (size_t)F3X * F3Y * F3Z * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
-    return h <= 64 && lds + state <= 65536;
+    int best = 0, waves = 0;
+    for (int by : {8, 16}) {
+        const size_t tile = (size_t)(16 + 2 * h) * (by + 2 * h) * (4 + 2 * h) * 3 * sizeof(float);
+        const size_t state = (size_t)16 * by * 4 * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
+        const int w = h <= 64 ? tdf_waves(tile + state, 16 * by * 4) : 0;
+        if (w > waves) {
+            waves = w;
+            best = by;
+            lds = tile;
+        }
+    }
+    return best;
 }
+static bool tdf_f32_lds(float rmin, float rmax, size_t& lds) { return tdf_f32_brick(rmin, rmax, lds) != 0; }
 
-template <bool NORM>
-static void launch_tdf_f32(const tsf_field* f, Dims d, float rmin, float rmax, float rstep, float* tdf,
-                           float* radius, size_t lds, hipStream_t stream) {
+template <bool NORM, int F3Y>
+static void launch_tdf_f32_brick(const tsf_field* f, Dims d, float rmin, float rmax, float rstep, float* tdf,
+                                 float* radius, size_t lds, hipStream_t stream) {
     static const CircleTables tab = make_circle_tables();
+    constexpr int F3X = 16, F3Z = 4;
     dim3 grid((d.X + F3X - 1) / F3X, (d.Y + F3Y - 1) / F3Y, (d.Z + F3Z - 1) / F3Z), block(F3X, F3Y, F3Z);
     hipLaunchKernelGGL((k_tdf_lds_f32<NORM, F3X, F3Y, F3Z>), grid, block, lds, stream, (const float*)f->x,
                        (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr, d, rmin, rmax,
                        rstep, tab, tdf, radius, tdf_halo(rmin, rmax));
+}
+// lds: the tile's size from tdf_f32_brick for the same radii
+template <bool NORM>
+static void launch_tdf_f32(const tsf_field* f, Dims d, float rmin, float rmax, float rstep, float* tdf,
+                           float* radius, size_t lds, hipStream_t stream) {
+    size_t t = 0;
+    if (tdf_f32_brick(rmin, rmax, t) == 16)
+        launch_tdf_f32_brick<NORM, 16>(f, d, rmin, rmax, rstep, tdf, radius, lds, stream);
+    else
+        launch_tdf_f32_brick<NORM, 8>(f, d, rmin, rmax, rstep, tdf, radius, lds, stream);
 }
 
 template <class S, bool NORM>
@@ -514,11 +566,14 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
                        float* radius, hipStream_t stream) {
     static const CircleTables tab = make_circle_tables();
     typedef typename Texel<S>::P P;
-    constexpr int BX = sizeof(P) == 16 ? 8 : 16, BY = 4, BZ = 4;
+    // 16 x 8 x 4 bricks (measured at 512^3 with SNORM16 texels: 43.0 and
+    // 47.0 ms at radii 0.5-3 and 2.5-4, against 53.5 and 51.4 ms on
+    // 16 x 4 x 4 and 59.8 and 48.1 ms on 16 x 16 x 4)
+    constexpr int BX = sizeof(P) == 16 ? 8 : 16, BY = 8, BZ = 4;
     const int h = tdf_halo(rmin, rmax);
     const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
-    if (sizeof(P) == 8 && h <= 64 && lds + state <= 65536) {
+    if (sizeof(P) == 8 && h <= 64 && lds + state <= lds_per_block()) {
         dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
         hipLaunchKernelGGL((k_tdf_lds<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, d, rmin,
                            rmax, rstep, tab, tdf, radius, h);
@@ -539,8 +594,9 @@ static bool launch_tdf_mixed_h(const void* tex, float inv, const PlanesF32& H, D
     constexpr int TX = BX + 2 * HC;
     const size_t lds = (size_t)TX * (BY + 2 * HC) * (BZ + 2 * HC) * sizeof(P);
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
-    const bool exact = 8.0 * ((double)TX * (d.Y + 2 * HC + 2) + (d.X + 2 * HC + 2)) + 65536.0 < 4194304.0;
-    if (!exact || lds + state > 65536) return false;
+    const size_t cap = lds_per_block();
+    const bool exact = 8.0 * ((double)TX * (d.Y + 2 * HC + 2) + (d.X + 2 * HC + 2)) + (double)cap < 4194304.0;
+    if (!exact || lds + state > cap) return false;
     const dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
     hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, HC>), grid, block, lds, stream, (const P*)tex, inv, H,
                        d, rmin, rmax, rstep, tab, tdf, radius);
@@ -552,26 +608,30 @@ static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float in
                                    float rstep, float* tdf, float* radius, hipStream_t stream, const char* W) {
     static const CircleTables tab = make_circle_tables();
     typedef typename Texel<S>::P P;
-    constexpr int BX = 16, BY = 4, BZ = 4;
+    constexpr int BX = 16, BY = 8, BZ = 4;  // other halos: the 8-byte-texel kernel's bricks
     const int h = tdf_halo(rmin, rmax);
     const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
     const PlanesF32 H{(const float*)f->x, (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr};
-    if (h > 64 || lds + state > 65536) {
+    if (h > 64 || lds + state > lds_per_block()) {
         const dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
         hipLaunchKernelGGL((k_tdf_mixed<S, NORM>), grid, block, 0, stream, Tex<S>{(const P*)tex, inv}, H, d, rmin,
                            rmax, rstep, tab, tdf, radius);
         (void)W;
         return TSF_OK;
     }
-    // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides.
-    // Halo 5 on 8 x 2 x 16 bricks: tile strides 18 and 216 texels, so all
-    // eight taps are LDS reads at 8-bit offsets from one address (measured
-    // 28.7 -> 28.4 ms at 512^3); halo 4 keeps 16 x 4 x 4 (its 8 x 4 x 8
-    // equivalent measured 27.2 -> 28.5 ms).
-    if (h == 4 && launch_tdf_mixed_h<S, NORM, 16, 4, 4, 4>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
+    // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides, on
+    // 512-voxel bricks: the pass is bound by instruction issue and wants
+    // every wave a CU can hold, and a larger brick spends less LDS on its
+    // halo per wave.  Halo 4 on 16 x 8 x 4 (tile 24 x 16 x 12, three
+    // workgroups, 24 waves per CU); halo 5 on 8 x 8 x 8 (tile 18^3, two
+    // workgroups, 16 waves).  Measured at 512^3: 25.9 -> 20.9 ms and
+    // 26.4 -> 24.3 ms against 256-voxel bricks (16 x 4 x 4 and 8 x 2 x 16,
+    // 16 and 12 waves); 16-column bricks read the Hessian's F32 rows in
+    // longer runs (8 x 8 x 8 at halo 4: 25.2 ms).
+    if (h == 4 && launch_tdf_mixed_h<S, NORM, 16, 8, 4, 4>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
         return TSF_OK;
-    if (h == 5 && launch_tdf_mixed_h<S, NORM, 8, 2, 16, 5>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
+    if (h == 5 && launch_tdf_mixed_h<S, NORM, 8, 8, 8, 5>(tex, inv, H, d, rmin, rmax, rstep, tab, tdf, radius, stream))
         return TSF_OK;
     dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
     hipLaunchKernelGGL((k_tdf_lds_mixed<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, H, d,
