@@ -621,9 +621,9 @@ static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float in
         return TSF_OK;
     }
     // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides, on
-    // 512-voxel bricks: the pass is bound by instruction issue and wants
-    // every wave a CU can hold, and a larger brick spends less LDS on its
-    // halo per wave.  Halo 4 on 16 x 8 x 4 (tile 24 x 16 x 12, three
+    // 512-voxel bricks: the pass's speed follows the waves a CU holds (its
+    // registers allow six per SIMD, its LDS sets the count), and a larger
+    // brick spends less LDS on its halo per wave.  Halo 4 on 16 x 8 x 4 (tile 24 x 16 x 12, three
     // workgroups, 24 waves per CU); halo 5 on 8 x 8 x 8 (tile 18^3, two
     // workgroups, 16 waves).  Measured at 512^3: 25.9 -> 20.9 ms and
     // 26.4 -> 24.3 ms against 256-voxel bricks (16 x 4 x 4 and 8 x 2 x 16,
