@@ -291,7 +291,8 @@ def test_hessian_eigen(tsf, orc, normalize):
     assert nbad == 0, rel
 
 
-@pytest.mark.parametrize("rmin,rmax,rstep", [(1.0, 3.0, 0.5), (2.5, 5.0, 0.5), (0.5, 4.0, 1.0)])
+# rmax 6: halo 7, the FP32 planes tile at its largest (159 KB of LDS)
+@pytest.mark.parametrize("rmin,rmax,rstep", [(1.0, 3.0, 0.5), (2.5, 5.0, 0.5), (0.5, 4.0, 1.0), (3.0, 6.0, 0.5)])
 def test_tdf_circle(tsf, orc, rmin, rmax, rstep):
     shape = (32, 32, 32)
     V = _gvf_field(orc, shape, 9)
@@ -311,7 +312,7 @@ def _bf16_round(a):
     return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("storage,rmin,rmax", [(1, 1.0, 3.0), (1, 2.5, 4.0), (2, 0.5, 3.0), (2, 2.5, 5.0)])
+@pytest.mark.parametrize("storage,rmin,rmax", [(1, 1.0, 3.0), (1, 2.5, 4.0), (2, 0.5, 3.0), (2, 2.5, 5.0), (1, 3.0, 6.0)])
 def test_tdf_circle_mixed(tsf, orc, storage, rmin, rmax):
     """Mixed mode: Hessian from the F32 field, samples from its 16-bit
     rounding with fused multiply-adds -- the oracle's mixed filter on the
@@ -352,32 +353,35 @@ def test_tdf_circle_mixed_ragged(tsf, orc, shape, rmin, rmax, storage):
     assert np.array_equal(gt, et) and np.array_equal(gr, er)
 
 
+@pytest.mark.parametrize("rmin,rmax", [(1.0, 3.0), (3.0, 6.0)])  # halos 4 and 7
 @pytest.mark.parametrize("storage", [1, 2])
-def test_tdf_reduced_precision_field(tsf, orc, storage):
+def test_tdf_reduced_precision_field(tsf, orc, storage, rmin, rmax):
     shape = (32, 32, 32)
     V = _gvf_field(orc, shape, 10)
     scale = 2.0 ** 14 if storage == 1 else 1.0
     fld = tsf.Field.from_host(V, storage, scale)
     t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
-    tsf.tdf_circle(fld, 1.0, 3.0, 0.5, t, r)
+    tsf.tdf_circle(fld, rmin, rmax, 0.5, t, r)
     gt, gr = t.download(), r.download()
     # kernel parity given the same (quantised) field: FP32 oracle on fld's
     # values (exactly representable in float)
     Vq = fld.download()
-    qt, qr = orc.tdf_circle(Vq, 1.0, 3.0, 0.5, "f32")
+    qt, qr = orc.tdf_circle(Vq, rmin, rmax, 0.5, "f32")
     nbad, rel = close(gt, qt)
     print(f"TDF on the stored field: exact fraction {np.mean(gt == qt):.6f}")
     assert nbad == 0, rel
     assert np.array_equal(gr, qr)
     # quantisation effect (reported): the first-maximum radius search is
     # discontinuous, so a few voxels change radius and dominate the L2 error
-    et, er = orc.tdf_circle(V.astype(np.float64), 1.0, 3.0, 0.5, "f64")
+    et, er = orc.tdf_circle(V.astype(np.float64), rmin, rmax, 0.5, "f64")
     same = gr == er
     rel_all = np.linalg.norm(gt - et) / np.linalg.norm(et)
     rel_same = np.linalg.norm((gt - et)[same]) / np.linalg.norm(et[same])
     print(f"TDF storage {storage}: L2 rel vs FP64 {rel_all:.3e} (all), {rel_same:.3e} "
           f"(voxels whose radius did not change); radius changed at {int((~same).sum())} voxels")
-    assert rel_same < (2e-3 if storage == 1 else 2e-2)
+    # radii from 4 up take the normalised Hessian, whose division by |v|
+    # amplifies the storage rounding (oracle, radii 3-6: 2.2e-3 F16, 1.8e-2 BF16)
+    assert rel_same < (2e-3 if storage == 1 else 2e-2) * (1 if rmax < 4 else 2)
 
 
 def test_combine_candidates(tsf, orc):
