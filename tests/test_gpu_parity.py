@@ -307,6 +307,24 @@ def test_tdf_circle(tsf, orc, rmin, rmax, rstep):
     assert np.array_equal(gr, er)
 
 
+@pytest.mark.parametrize("storage", [0, 1])  # F32 planes, F16 texels
+@pytest.mark.parametrize("rmin,rmax", [(0.5, 3.0), (2.5, 4.0)])  # halos 4 and 5
+@pytest.mark.parametrize("shape", [(23, 29, 37), (9, 5, 70)])
+def test_tdf_circle_ragged(tsf, orc, shape, rmin, rmax, storage):
+    """The reference-order TDF on volumes that are no multiple of its bricks
+    (16 x 8 x 4; 16 x 16 x 4 for F32 planes at halo 5) and thinner than the
+    halo in y: the partial bricks and the clamped tile against the oracle on
+    the field's stored values."""
+    V = _gvf_field(orc, shape, 13)
+    fld = tsf.Field.from_host(V, storage, 2.0 ** 14) if storage else tsf.Field.from_host(V)
+    t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    tsf.tdf_circle(fld, rmin, rmax, 0.5, t, r)
+    et, er = orc.tdf_circle(fld.download(), rmin, rmax, 0.5, "f32")
+    nbad, rel = close(t.download(), et)
+    assert nbad == 0, rel
+    assert np.array_equal(r.download(), er)
+
+
 def _bf16_round(a):
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
     return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
