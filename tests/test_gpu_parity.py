@@ -316,13 +316,20 @@ def test_tdf_circle_ragged(tsf, orc, shape, rmin, rmax, storage):
     halo in y: the partial bricks and the clamped tile against the oracle on
     the field's stored values."""
     V = _gvf_field(orc, shape, 13)
-    fld = tsf.Field.from_host(V, storage, 2.0 ** 14) if storage else tsf.Field.from_host(V)
+    fld = tsf.Field.from_host(V, storage, _f16_scale(V)) if storage else tsf.Field.from_host(V)
     t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
     tsf.tdf_circle(fld, rmin, rmax, 0.5, t, r)
     et, er = orc.tdf_circle(fld.download(), rmin, rmax, 0.5, "f32")
+    assert np.isfinite(et).all()
     nbad, rel = close(t.download(), et)
     assert nbad == 0, rel
     assert np.array_equal(r.download(), er)
+
+
+def _f16_scale(V):
+    """The power of two that keeps every value of V * scale inside F16's
+    range (2^14 for fields within +-2)."""
+    return float(2.0 ** min(14, np.floor(np.log2(32768.0 / max(float(np.abs(V).max()), 1e-30)))))
 
 
 def _bf16_round(a):
@@ -362,12 +369,14 @@ def test_tdf_circle_mixed_ragged(tsf, orc, shape, rmin, rmax, storage):
     V = _gvf_field(orc, shape, 12)
     fld = tsf.Field.from_host(V)
     t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    scale = _f16_scale(V[:3])
     tsf.tdf_circle_mixed(fld, rmin, rmax, 0.5, t, r, sample_storage=storage,
-                         sample_scale=2.0 ** 14 if storage == 1 else 1.0)
+                         sample_scale=scale if storage == 1 else 1.0)
     gt, gr = t.download(), r.download()
     Vs = V.copy()
-    Vs[:3] = orc.f16_round(V[:3]) if storage == 1 else _bf16_round(V[:3])
+    Vs[:3] = orc.f16_round(V[:3], scale) if storage == 1 else _bf16_round(V[:3])
     et, er = orc.tdf_circle_mixed(V, Vs, rmin, rmax, 0.5, "f32")
+    assert np.isfinite(et).all()
     assert np.array_equal(gt, et) and np.array_equal(gr, er)
 
 
