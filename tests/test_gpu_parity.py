@@ -17,10 +17,16 @@ RTOL, ATOL = 1e-4, 1e-7
 
 
 def close(a, b, rtol=RTOL, atol=ATOL):
+    """(count of elements outside the tolerance, L2-relative difference);
+    equal values (infinities included) and NaN on both sides agree, a NaN
+    on one side does not."""
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
-    bad = np.abs(a - b) > rtol * np.abs(b) + atol
-    return int(bad.sum()), float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+    with np.errstate(invalid="ignore"):
+        ok = (a == b) | (np.isnan(a) & np.isnan(b)) | (np.abs(a - b) <= rtol * np.abs(b) + atol)
+        fin = np.isfinite(a) & np.isfinite(b)
+        rel = float(np.linalg.norm((a - b)[fin]) / max(np.linalg.norm(b[fin]), 1e-30))
+    return int((~ok).sum()), rel
 
 
 def rng(s):
