@@ -509,9 +509,9 @@ void pack_texels_f32(const tsf_field* field, float4* out, i64 n, hipStream_t str
                        (const float*)field->z, (const float*)field->w, out, n);
 }
 
-// LDS tile of the brick plus halo h: bricks of 16x4x4 voxels for 8-byte
-// texels, 8x4x4 for 16-byte ones (F32), up to 64 KiB; wider halos gather
-// from device memory.
+// The halo of a brick's LDS tile: floor(r_max + 0.5) + 1 cells keep both
+// corners of every sample inside it.  Tiles that do not fit a workgroup's
+// LDS gather from device memory instead.
 static int tdf_halo(float rmin, float rmax) {
     const float r = fmaxf(fabsf(rmin), fabsf(rmax));
     return (int)floorf(r + 0.5f) + 1;
