@@ -380,28 +380,14 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
-// The same with the halo HC a compile-time constant: the tile's strides are
-// immediates, so a sample's cell costs a few float operations and its eight
-// taps are reads at constant offsets from one address.
-template <class S, bool NORM, int BX, int BY, int BZ, int HC>
-__global__ void __launch_bounds__(BX* BY* BZ)
-    k_tdf_lds_mixed_h(const typename Texel<S>::P* __restrict__ tex, float inv, PlanesF32 H, Dims d, float rmin,
-                      float rmax, float rstep, CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad) {
-    constexpr int NT = BX * BY * BZ, TX = BX + 2 * HC, TY = BY + 2 * HC, TZ = BZ + 2 * HC, CELLS = TX * TY * TZ;
-    typedef typename Texel<S>::P P;
-    extern __shared__ float4 lds_f4[];
-    P* tile = reinterpret_cast<P*>(lds_f4);
-    int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz);
-    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
-    const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
-    const int lo = (int)(size_t)(__attribute__((address_space(3))) P*)tile;  // the tile's LDS address
-    const LdsTexC<S, TX, TY, TZ> T{inv, (float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P)), (float)lo};
-    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
-    // every load of the tile is issued before the first LDS store, so one
-    // memory latency covers them all (the cell's coordinates come from
-    // divisions by constants)
-    constexpr int NIT = (CELLS + NT - 1) / NT;
+// Stage the clamp-to-edge texels of a TX x TY x TZ tile at (ox, oy, oz) in
+// LDS, then a barrier: every load is issued before the first LDS store, so
+// one memory latency covers them all (the cell's coordinates come from
+// divisions by constants).
+template <int NT, int TX, int TY, int TZ, class P>
+__device__ __forceinline__ void stage_tile(P* tile, const P* __restrict__ tex, const Dims& d, int ox, int oy, int oz,
+                                           int tid) {
+    constexpr int CELLS = TX * TY * TZ, NIT = (CELLS + NT - 1) / NT;
     const i64 plane = (i64)d.X * d.Y;
     P v[NIT];
 #pragma unroll
@@ -420,7 +406,52 @@ __global__ void __launch_bounds__(BX* BY* BZ)
         if (CELLS % NT == 0 || c < CELLS) tile[c] = v[k];
     }
     __syncthreads();
+}
+
+// The same with the halo HC a compile-time constant: the tile's strides are
+// immediates, so a sample's cell costs a few float operations and its eight
+// taps are reads at constant offsets from one address.
+template <class S, bool NORM, int BX, int BY, int BZ, int HC>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_mixed_h(const typename Texel<S>::P* __restrict__ tex, float inv, PlanesF32 H, Dims d, float rmin,
+                      float rmax, float rstep, CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad) {
+    constexpr int NT = BX * BY * BZ, TX = BX + 2 * HC, TY = BY + 2 * HC, TZ = BZ + 2 * HC;
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
+    const int lo = (int)(size_t)(__attribute__((address_space(3))) P*)tile;  // the tile's LDS address
+    const LdsTexC<S, TX, TY, TZ> T{inv, (float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P)), (float)lo};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    stage_tile<NT, TX, TY, TZ>(tile, tex, d, ox, oy, oz, tid);
     tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
+// The reference-order TDF on 8-byte texels with the halo HC a compile-time
+// constant: the samples address the tile as k_tdf_lds_mixed_h does
+// (trilinear_tex on an LdsTexC, the same arithmetic as on an LdsTex), the
+// Hessian reads the same tile through an LdsTex.
+template <class S, bool NORM, int BX, int BY, int BZ, int HC>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_h(const typename Texel<S>::P* __restrict__ tex, float inv, Dims d, float rmin, float rmax, float rstep,
+                CircleTables tab, float* __restrict__ tdf, float* __restrict__ rad) {
+    constexpr int NT = BX * BY * BZ, TX = BX + 2 * HC, TY = BY + 2 * HC, TZ = BZ + 2 * HC;
+    typedef typename Texel<S>::P P;
+    extern __shared__ float4 lds_f4[];
+    P* tile = reinterpret_cast<P*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
+    const int lo = (int)(size_t)(__attribute__((address_space(3))) P*)tile;  // the tile's LDS address
+    const LdsTexC<S, TX, TY, TZ> T{inv, (float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(P)), (float)lo};
+    const LdsTex<S> TH{tile, inv, ox, oy, oz, TX, TY, TZ};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    stage_tile<NT, TX, TY, TZ>(tile, tex, d, ox, oy, oz, tid);
+    tdf_brick<NORM, BX, BY, BZ>(T, TH, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
 // F32 planes -> 16-bit texels holding value * scale (x, y, z; w = 0, the
@@ -561,6 +592,16 @@ static void launch_tdf_f32(const tsf_field* f, Dims d, float rmin, float rmax, f
         launch_tdf_f32_brick<NORM, 8>(f, d, rmin, rmax, rstep, tdf, radius, lds, stream);
 }
 
+// Whether the compile-time-stride kernels' float cell addresses are exact
+// on this volume: the tile's origin term o8 and every partial sum of a
+// sample's address are integers bounded by the workgroup's LDS plus 8 bytes
+// times the cell offset of the volume's far corner in the tile's strides
+// (tx, tx ty), and a float holds every integer below 2^24.
+static bool tdf_addresses_exact(Dims d, int tx, int ty, int hc, size_t cap) {
+    const double x = d.X + 2.0 * hc + 2.0, y = d.Y + 2.0 * hc + 2.0, z = d.Z + 2.0 * hc + 2.0;
+    return 8.0 * (x + (double)tx * y + (double)tx * ty * z) + (double)cap < 16777216.0;
+}
+
 template <class S, bool NORM>
 static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rmax, float rstep, float* tdf,
                        float* radius, hipStream_t stream) {
@@ -575,6 +616,18 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
     if (sizeof(P) == 8 && h <= 64 && lds + state <= lds_per_block()) {
         dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
+        // halos 4 and 5 with compile-time strides (the same arithmetic)
+        if constexpr (sizeof(P) == 8) {
+            if ((h == 4 || h == 5) && tdf_addresses_exact(d, BX + 2 * h, BY + 2 * h, h, lds_per_block())) {
+                if (h == 4)
+                    hipLaunchKernelGGL((k_tdf_lds_h<S, NORM, 16, 8, 4, 4>), grid, block, lds, stream, (const P*)tex,
+                                       inv, d, rmin, rmax, rstep, tab, tdf, radius);
+                else
+                    hipLaunchKernelGGL((k_tdf_lds_h<S, NORM, 16, 8, 4, 5>), grid, block, lds, stream, (const P*)tex,
+                                       inv, d, rmin, rmax, rstep, tab, tdf, radius);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_tdf_lds<S, NORM, BX, BY, BZ>), grid, block, lds, stream, (const P*)tex, inv, d, rmin,
                            rmax, rstep, tab, tdf, radius, h);
         return;
@@ -584,19 +637,18 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     hipLaunchKernelGGL((k_tdf<S, NORM>), grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf, radius);
 }
 
-// The compile-time-halo kernel on BX x BY x BZ bricks, when the float cell
-// addresses stay exact (8 (TX Y + X) + 64 KiB below 2^23) and the tile and
-// the brick's state fit in 64 KiB; false when it does not run.
+// The compile-time-halo kernels on BX x BY x BZ bricks, when the float cell
+// addresses are exact and the tile and the brick's state fit the
+// workgroup's LDS; false when they do not run.
 template <class S, bool NORM, int BX, int BY, int BZ, int HC>
 static bool launch_tdf_mixed_h(const void* tex, float inv, const PlanesF32& H, Dims d, float rmin, float rmax,
                                float rstep, const CircleTables& tab, float* tdf, float* radius, hipStream_t stream) {
     typedef typename Texel<S>::P P;
-    constexpr int TX = BX + 2 * HC;
-    const size_t lds = (size_t)TX * (BY + 2 * HC) * (BZ + 2 * HC) * sizeof(P);
+    constexpr int TX = BX + 2 * HC, TY = BY + 2 * HC;
+    const size_t lds = (size_t)TX * TY * (BZ + 2 * HC) * sizeof(P);
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
     const size_t cap = lds_per_block();
-    const bool exact = 8.0 * ((double)TX * (d.Y + 2 * HC + 2) + (d.X + 2 * HC + 2)) + (double)cap < 4194304.0;
-    if (!exact || lds + state > cap) return false;
+    if (!tdf_addresses_exact(d, TX, TY, HC, cap) || lds + state > cap) return false;
     const dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
     hipLaunchKernelGGL((k_tdf_lds_mixed_h<S, NORM, BX, BY, BZ, HC>), grid, block, lds, stream, (const P*)tex, inv, H,
                        d, rmin, rmax, rstep, tab, tdf, radius);

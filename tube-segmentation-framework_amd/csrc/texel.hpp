@@ -364,6 +364,45 @@ __device__ __forceinline__ void trilinear_fused(const LdsTexC<S, TX, TY, TZ>& T,
     out[2] = sz_;
 }
 
+// trilinear_tex on an LdsTexC: the reference-order sample of the LdsTex
+// version above (u - 0.5, the same cell, weights, terms and sums), with the
+// cell's LDS address formed as trilinear_fused forms it.
+template <class S, int TX, int TY, int TZ>
+__device__ __forceinline__ void trilinear_tex(const LdsTexC<S, TX, TY, TZ>& T, const Dims&, float u, float v, float w,
+                                              float out[3]) {
+    typedef Texel<S> X;
+    typedef typename X::P P;
+    constexpr int SY = TX, SZ = TX * TY, B = (int)sizeof(P);
+    constexpr float LAST = (float)((TX * TY * TZ - 1 - SZ - SY - 1) * B);
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const float fo =
+        __builtin_fmaf(fu, (float)B, __builtin_fmaf(fv, (float)(SY * B), __builtin_fmaf(fw, (float)(SZ * B), T.o8)));
+    const unsigned off = (unsigned)__builtin_amdgcn_fmed3f(fo, T.lo, T.lo + LAST);
+    static_assert(sizeof(P) == 8, "8-byte texels");
+    typedef __attribute__((address_space(3))) const unsigned long long LW;
+    LW* q = (LW*)(size_t)off;
+    auto tx = [](unsigned long long x) { P t; __builtin_memcpy(&t, &x, 8); return t; };
+    const P p0 = tx(q[0]), p1 = tx(q[1]), p2 = tx(q[SY]), p3 = tx(q[SY + 1]);
+    const P p4 = tx(q[SZ]), p5 = tx(q[SZ + 1]), p6 = tx(q[SZ + SY]), p7 = tx(q[SZ + SY + 1]);
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    float sx = X::mulx(wt0, p0), sy_ = X::muly(wt0, p0), sz_ = X::mulz(wt0, p0);
+    sx = sx + X::mulx(wt1, p1); sy_ = sy_ + X::muly(wt1, p1); sz_ = sz_ + X::mulz(wt1, p1);
+    sx = sx + X::mulx(wt2, p2); sy_ = sy_ + X::muly(wt2, p2); sz_ = sz_ + X::mulz(wt2, p2);
+    sx = sx + X::mulx(wt3, p3); sy_ = sy_ + X::muly(wt3, p3); sz_ = sz_ + X::mulz(wt3, p3);
+    sx = sx + X::mulx(wt4, p4); sy_ = sy_ + X::muly(wt4, p4); sz_ = sz_ + X::mulz(wt4, p4);
+    sx = sx + X::mulx(wt5, p5); sy_ = sy_ + X::muly(wt5, p5); sz_ = sz_ + X::mulz(wt5, p5);
+    sx = sx + X::mulx(wt6, p6); sy_ = sy_ + X::muly(wt6, p6); sz_ = sz_ + X::mulz(wt6, p6);
+    sx = sx + X::mulx(wt7, p7); sy_ = sy_ + X::muly(wt7, p7); sz_ = sz_ + X::mulz(wt7, p7);
+    out[0] = sx * T.inv;
+    out[1] = sy_ * T.inv;
+    out[2] = sz_ * T.inv;
+}
+
 // The same from texels in device memory (clamp-to-edge at the volume).
 template <class S>
 __device__ __forceinline__ void trilinear_fused(const Tex<S>& T, const Dims& d, float uu, float vv, float ww,
