@@ -504,6 +504,54 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     tdf_brick<NORM, BX, BY, BZ>(T, T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
+// The same with the halo HC a compile-time constant: the samples address
+// the tile through an LdsSoaTexC, the Hessian through an LdsSoaTex (the same
+// arithmetic as k_tdf_lds_f32); every load of the tile is issued before the
+// first LDS store.
+template <bool NORM, int BX, int BY, int BZ, int HC>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_lds_f32_h(const float* __restrict__ fx, const float* __restrict__ fy, const float* __restrict__ fz,
+                    const float* __restrict__ fw, Dims d, float rmin, float rmax, float rstep, CircleTables tab,
+                    float* __restrict__ tdf, float* __restrict__ rad) {
+    constexpr int NT = BX * BY * BZ, TX = BX + 2 * HC, TY = BY + 2 * HC, TZ = BZ + 2 * HC, CELLS = TX * TY * TZ;
+    constexpr int NIT = (CELLS + NT - 1) / NT;
+    extern __shared__ float4 lds_f4[];
+    float* tile = reinterpret_cast<float*>(lds_f4);
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    const int bx0 = tbx * BX, by0 = tby * BY, bz0 = tbz * BZ;
+    const int ox = bx0 - HC, oy = by0 - HC, oz = bz0 - HC;
+    const int lo = (int)(size_t)(__attribute__((address_space(3))) float*)tile;  // the tile's LDS address
+    const LdsSoaTexC<TX, TY, TZ> T{(float)(lo - ((ox + oy * TX) + oz * (TX * TY)) * (int)sizeof(float)), (float)lo};
+    const LdsSoaTex TH{tile, fw, CELLS, ox, oy, oz, TX, TY, TZ};
+    const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+    float vx[NIT], vy[NIT], vz[NIT];
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int c = tid + NT * k;
+        if (CELLS % NT == 0 || c < CELLS) {
+            const int lx = c % TX, rr = c / TX, ly = rr % TY, lz = rr / TY;
+            const int gx = clampi(ox + lx, 0, d.X - 1), gy = clampi(oy + ly, 0, d.Y - 1);
+            const int gz = clampi(oz + lz, 0, d.Z - 1);
+            const i64 g = d.at(gx, gy, gz);
+            vx[k] = fx[g];
+            vy[k] = fy[g];
+            vz[k] = fz[g];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NIT; k++) {
+        const int c = tid + NT * k;
+        if (CELLS % NT == 0 || c < CELLS) {
+            tile[c] = vx[k];
+            tile[CELLS + c] = vy[k];
+            tile[2 * CELLS + c] = vz[k];
+        }
+    }
+    __syncthreads();
+    tdf_brick<NORM, BX, BY, BZ>(T, TH, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
+}
+
 static unsigned grid1d(i64 n) {
     i64 g = (n + 255) / 256;
     return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -533,6 +581,16 @@ static size_t lds_per_block() { return device_lds(false); }
 static int tdf_waves(size_t bytes, int nt) {
     if (bytes > lds_per_block()) return 0;
     return (int)(device_lds(true) / bytes) * (nt / 64);
+}
+
+// Whether the compile-time-stride kernels' float cell addresses are exact
+// on this volume: the tile's origin term o8 and every partial sum of a
+// sample's address are integers bounded by the workgroup's LDS plus 8 bytes
+// times the cell offset of the volume's far corner in the tile's strides
+// (tx, tx ty), and a float holds every integer below 2^24.
+static bool tdf_addresses_exact(Dims d, int tx, int ty, int hc, size_t cap) {
+    const double x = d.X + 2.0 * hc + 2.0, y = d.Y + 2.0 * hc + 2.0, z = d.Z + 2.0 * hc + 2.0;
+    return 8.0 * (x + (double)tx * y + (double)tx * ty * z) + (double)cap < 16777216.0;
 }
 
 void pack_texels_f32(const tsf_field* field, float4* out, i64 n, hipStream_t stream) {
@@ -576,10 +634,22 @@ static void launch_tdf_f32_brick(const tsf_field* f, Dims d, float rmin, float r
                                  float* radius, size_t lds, hipStream_t stream) {
     static const CircleTables tab = make_circle_tables();
     constexpr int F3X = 16, F3Z = 4;
+    const int h = tdf_halo(rmin, rmax);
     dim3 grid((d.X + F3X - 1) / F3X, (d.Y + F3Y - 1) / F3Y, (d.Z + F3Z - 1) / F3Z), block(F3X, F3Y, F3Z);
-    hipLaunchKernelGGL((k_tdf_lds_f32<NORM, F3X, F3Y, F3Z>), grid, block, lds, stream, (const float*)f->x,
-                       (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr, d, rmin, rmax,
-                       rstep, tab, tdf, radius, tdf_halo(rmin, rmax));
+    const float *x = (const float*)f->x, *y = (const float*)f->y, *z = (const float*)f->z;
+    const float* w = NORM ? (const float*)f->w : nullptr;
+    // halos 4 and 5 with compile-time strides (the same arithmetic)
+    if ((h == 4 || h == 5) && tdf_addresses_exact(d, F3X + 2 * h, F3Y + 2 * h, h, lds_per_block())) {
+        if (h == 4)
+            hipLaunchKernelGGL((k_tdf_lds_f32_h<NORM, F3X, F3Y, F3Z, 4>), grid, block, lds, stream, x, y, z, w, d,
+                               rmin, rmax, rstep, tab, tdf, radius);
+        else
+            hipLaunchKernelGGL((k_tdf_lds_f32_h<NORM, F3X, F3Y, F3Z, 5>), grid, block, lds, stream, x, y, z, w, d,
+                               rmin, rmax, rstep, tab, tdf, radius);
+        return;
+    }
+    hipLaunchKernelGGL((k_tdf_lds_f32<NORM, F3X, F3Y, F3Z>), grid, block, lds, stream, x, y, z, w, d, rmin, rmax,
+                       rstep, tab, tdf, radius, h);
 }
 // lds: the tile's size from tdf_f32_brick for the same radii
 template <bool NORM>
@@ -592,15 +662,6 @@ static void launch_tdf_f32(const tsf_field* f, Dims d, float rmin, float rmax, f
         launch_tdf_f32_brick<NORM, 8>(f, d, rmin, rmax, rstep, tdf, radius, lds, stream);
 }
 
-// Whether the compile-time-stride kernels' float cell addresses are exact
-// on this volume: the tile's origin term o8 and every partial sum of a
-// sample's address are integers bounded by the workgroup's LDS plus 8 bytes
-// times the cell offset of the volume's far corner in the tile's strides
-// (tx, tx ty), and a float holds every integer below 2^24.
-static bool tdf_addresses_exact(Dims d, int tx, int ty, int hc, size_t cap) {
-    const double x = d.X + 2.0 * hc + 2.0, y = d.Y + 2.0 * hc + 2.0, z = d.Z + 2.0 * hc + 2.0;
-    return 8.0 * (x + (double)tx * y + (double)tx * ty * z) + (double)cap < 16777216.0;
-}
 
 template <class S, bool NORM>
 static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rmax, float rstep, float* tdf,

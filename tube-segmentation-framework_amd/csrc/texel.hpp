@@ -466,6 +466,51 @@ __device__ __forceinline__ void trilinear_tex(const LdsSoaTex& T, const Dims&, f
     out[2] = sz_;
 }
 
+// LdsSoaTexC: a three-plane tile whose dimensions are compile-time
+// constants, addressed for the samples by LDS byte addresses held as
+// integer-valued floats (as LdsTexC; the Hessian keeps an LdsSoaTex).
+template <int TX, int TY, int TZ>
+struct LdsSoaTexC {
+    float o4;  // LDS address of the tile (lo) minus the byte offset of its origin in the x plane
+    float lo;  // LDS address of the tile
+};
+
+// trilinear_tex on an LdsSoaTexC: the LdsSoaTex version's cell, weights,
+// taps and sums, the cell's LDS address formed as trilinear_fused forms it;
+// the y and z planes follow the x plane at constant offsets.
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ void trilinear_tex(const LdsSoaTexC<TX, TY, TZ>& T, const Dims&, float u, float v, float w,
+                                              float out[3]) {
+    constexpr int SY = TX, SZ = TX * TY, CELLS = TX * TY * TZ, B = (int)sizeof(float);
+    constexpr float LAST = (float)((CELLS - 1 - SZ - SY - 1) * B);
+    const float uu = u - 0.5f, vv = v - 0.5f, ww = w - 0.5f;
+    const float fu = floorf(uu), fv = floorf(vv), fw = floorf(ww);
+    const float a = uu - fu, b = vv - fv, c = ww - fw;
+    const float fo =
+        __builtin_fmaf(fu, (float)B, __builtin_fmaf(fv, (float)(SY * B), __builtin_fmaf(fw, (float)(SZ * B), T.o4)));
+    const unsigned off = (unsigned)__builtin_amdgcn_fmed3f(fo, T.lo, T.lo + LAST);
+    typedef __attribute__((address_space(3))) const float LF;
+    LF* px = (LF*)(size_t)off;
+    LF* py = px + CELLS;
+    LF* pz = px + 2 * CELLS;
+    constexpr int o[8] = {0, 1, SY, SY + 1, SZ, SZ + 1, SZ + SY, SZ + SY + 1};
+    const float wt0 = (1.0f - a) * (1.0f - b) * (1.0f - c), wt1 = a * (1.0f - b) * (1.0f - c);
+    const float wt2 = (1.0f - a) * b * (1.0f - c), wt3 = a * b * (1.0f - c);
+    const float wt4 = (1.0f - a) * (1.0f - b) * c, wt5 = a * (1.0f - b) * c;
+    const float wt6 = (1.0f - a) * b * c, wt7 = a * b * c;
+    const float wt[8] = {wt0, wt1, wt2, wt3, wt4, wt5, wt6, wt7};
+    float sx = wt[0] * px[o[0]], sy_ = wt[0] * py[o[0]], sz_ = wt[0] * pz[o[0]];
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        sx = sx + wt[k] * px[o[k]];
+        sy_ = sy_ + wt[k] * py[o[k]];
+        sz_ = sz_ + wt[k] * pz[o[k]];
+    }
+    out[0] = sx;
+    out[1] = sy_;
+    out[2] = sz_;
+}
+
 constexpr int CBX = 16, CBY = 4, CBZ = 4;  // TDF brick
 
 // Packs the field's planes into float4 texels (x, y, z, w; w = 1 when the
