@@ -332,6 +332,32 @@ def test_tdf_circle_ragged(tsf, orc, shape, rmin, rmax, storage):
     assert np.array_equal(r.download(), er)
 
 
+@pytest.mark.parametrize("mode", ["f32", "f16", "mixed"])
+def test_tdf_deep_volume(tsf, orc, mode):
+    """A volume 5600 planes deep, where the compile-time-stride kernels'
+    float cell addresses would pass 2^24 bytes: the launchers take the
+    run-time-stride kernels, and the results still match the oracle."""
+    shape = (5600, 6, 12)
+    V = _gvf_field(orc, shape, 14, iters=10)
+    t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    if mode == "mixed":
+        fld = tsf.Field.from_host(V)
+        scale = _f16_scale(V[:3])
+        tsf.tdf_circle_mixed(fld, 0.5, 3.0, 0.5, t, r, sample_storage=1, sample_scale=scale)
+        Vs = V.copy()
+        Vs[:3] = orc.f16_round(V[:3], scale)
+        et, er = orc.tdf_circle_mixed(V, Vs, 0.5, 3.0, 0.5, "f32")
+        assert np.array_equal(t.download(), et) and np.array_equal(r.download(), er)
+        return
+    fld = tsf.Field.from_host(V, 1, _f16_scale(V)) if mode == "f16" else tsf.Field.from_host(V)
+    tsf.tdf_circle(fld, 0.5, 3.0, 0.5, t, r)
+    et, er = orc.tdf_circle(fld.download(), 0.5, 3.0, 0.5, "f32")
+    assert np.isfinite(et).all()
+    nbad, rel = close(t.download(), et)
+    assert nbad == 0, rel
+    assert np.array_equal(r.download(), er)
+
+
 def _f16_scale(V):
     """The power of two that keeps every value of V * scale inside F16's
     range (2^14 for fields within +-2)."""
