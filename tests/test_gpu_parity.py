@@ -333,6 +333,31 @@ def test_tdf_circle_ragged(tsf, orc, shape, rmin, rmax, storage):
 
 
 @pytest.mark.parametrize("mode", ["f32", "f16", "mixed"])
+def test_tdf_wide_halo(tsf, orc, mode):
+    """Radii 2.5-9 (halo 10): tiles too large for a workgroup's LDS, so the
+    samples gather texels from device memory, against the oracle."""
+    shape = (30, 34, 40)
+    V = _gvf_field(orc, shape, 15)
+    t, r = tsf.DeviceArray(shape, np.float32), tsf.DeviceArray(shape, np.float32)
+    if mode == "mixed":
+        fld = tsf.Field.from_host(V)
+        scale = _f16_scale(V[:3])
+        tsf.tdf_circle_mixed(fld, 2.5, 9.0, 0.5, t, r, sample_storage=1, sample_scale=scale)
+        Vs = V.copy()
+        Vs[:3] = orc.f16_round(V[:3], scale)
+        et, er = orc.tdf_circle_mixed(V, Vs, 2.5, 9.0, 0.5, "f32")
+        assert np.array_equal(t.download(), et) and np.array_equal(r.download(), er)
+        return
+    fld = tsf.Field.from_host(V, 1, _f16_scale(V)) if mode == "f16" else tsf.Field.from_host(V)
+    tsf.tdf_circle(fld, 2.5, 9.0, 0.5, t, r)
+    et, er = orc.tdf_circle(fld.download(), 2.5, 9.0, 0.5, "f32")
+    assert np.isfinite(et).all() and (et > 0).sum() > 100
+    nbad, rel = close(t.download(), et)
+    assert nbad == 0, rel
+    assert np.array_equal(r.download(), er)
+
+
+@pytest.mark.parametrize("mode", ["f32", "f16", "mixed"])
 def test_tdf_deep_volume(tsf, orc, mode):
     """A volume 5600 planes deep, where the compile-time-stride kernels'
     float cell addresses would pass 2^24 bytes: the launchers take the

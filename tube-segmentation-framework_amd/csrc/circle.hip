@@ -70,48 +70,6 @@ __device__ __forceinline__ float circle_sum(const A& T, const Dims& d, float fx,
     return sum / 32.0f;
 }
 
-// circleFittingTDF, kernels_no_3d_write.cl:1503-1572, at one voxel: the
-// (normalised) Hessian, its eigenvectors e2, e3, and the radius search with
-// float accumulation and a first-maximum break.
-template <bool NORM, bool FUSED = false, class A, class AH = A>
-__device__ __forceinline__ void tdf_voxel(const A& T, const Dims& d, int x, int y, int z, float rmin, float rmax,
-                                          float rstep, const CircleTables& tab, float& maxSum, float& maxRadius,
-                                          const AH* TH = nullptr) {
-    float H[3][3], V[3][3], ev[3];
-    if (TH) hessian_tex<NORM>(*TH, d, x, y, z, H);
-    else hessian_tex<NORM>(T, d, x, y, z, H);
-    if constexpr (FUSED) eigen3_fp64_fallback(H, V, ev);
-    else eigen3(H, V, ev);
-    const float e2[3] = {V[0][1], V[1][1], V[2][1]};
-    const float e3[3] = {V[0][2], V[1][2], V[2][2]};
-    maxSum = 0.0f;
-    maxRadius = 0.0f;
-    for (float radius = rmin; radius <= rmax; radius += rstep) {
-        const float sum = circle_sum<FUSED>(T, d, (float)x, (float)y, (float)z, radius, e2, e3, tab);
-        if (sum > maxSum) {
-            maxSum = sum;
-            maxRadius = radius;
-        } else {
-            break;
-        }
-    }
-}
-
-// Texels gathered from device memory (halos too wide for the LDS tile).
-template <class S, bool NORM>
-__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf(Tex<S> T, Dims d, float rmin, float rmax, float rstep,
-                                                       CircleTables tab, float* __restrict__ tdf,
-                                                       float* __restrict__ rad) {
-    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
-    const int z = blockIdx.z * CBZ + threadIdx.z;
-    if (x >= d.X || y >= d.Y || z >= d.Z) return;
-    float t, r;
-    tdf_voxel<NORM>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r);
-    const i64 i = d.at(x, y, z);
-    tdf[i] = t;
-    rad[i] = r;
-}
-
 // splineTDF (kernels_no_3d_write.cl:1574-1667) at one voxel, with the 12
 // arms runSplineTDF enqueues (tubeDetectionFilters.cpp:46): normalised
 // Hessian, eigenvectors e1..e3, and along each arm cos(a_j) e3 + sin(a_j) e2
@@ -205,10 +163,12 @@ __global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_spline(Tex<S> T, Dims d, 
 // e3, best sum) are compacted to the front of the workgroup through LDS, so
 // a wave's lanes stay busy and waves with no survivor retire.  The radius of
 // round k is the same float accumulation for every voxel, so each voxel's
-// arithmetic is that of the sequential loop.  T is the staged tile (its
-// fill is the caller's, followed by a barrier).  The Hessian reads TH, which
-// is T itself except in the mixed mode (F32 planes for the Hessian, 16-bit
-// texels for the samples).
+// arithmetic is that of the reference's sequential loop over radii
+// (kernels_no_3d_write.cl:1547-1567: radius += step from rmin, break at the
+// first sum that does not exceed the best).  T is the sample source (a
+// staged tile, whose fill is the caller's, followed by a barrier, or the
+// texels in device memory).  The Hessian reads TH: the same tile through a
+// run-time-stride view, or the F32 planes in the mixed mode.
 template <bool NORM, int BX, int BY, int BZ, bool FUSED = false, class A, class AH>
 __device__ __forceinline__ void tdf_brick(const A& T, const AH& TH, Dims d, int bx0, int by0, int bz0, float rmin, float rmax,
                                           float rstep, const CircleTables& tab, float* __restrict__ tdf,
@@ -337,22 +297,6 @@ __global__ void __launch_bounds__(BX* BY* BZ) k_tdf_lds(const typename Texel<S>:
     tdf_brick<NORM, BX, BY, BZ>(T, T, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
 
-// Mixed mode with texels gathered from device memory (halos too wide for
-// the LDS tile).
-template <class S, bool NORM>
-__global__ void __launch_bounds__(CBX* CBY* CBZ) k_tdf_mixed(Tex<S> T, PlanesF32 H, Dims d, float rmin, float rmax,
-                                                             float rstep, CircleTables tab, float* __restrict__ tdf,
-                                                             float* __restrict__ rad) {
-    const int x = blockIdx.x * CBX + threadIdx.x, y = blockIdx.y * CBY + threadIdx.y;
-    const int z = blockIdx.z * CBZ + threadIdx.z;
-    if (x >= d.X || y >= d.Y || z >= d.Z) return;
-    float t, r;
-    tdf_voxel<NORM, true>(T, d, x, y, z, rmin, rmax, rstep, tab, t, r, &H);
-    const i64 i = d.at(x, y, z);
-    tdf[i] = t;
-    rad[i] = r;
-}
-
 // Mixed mode: the samples from a tile of 16-bit texels packed from an F32
 // field, the Hessian from the F32 planes themselves (device memory), so the
 // eigenvectors see the field at full precision.
@@ -379,6 +323,24 @@ __global__ void __launch_bounds__(BX* BY* BZ)
     __syncthreads();
     tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, bx0, by0, bz0, rmin, rmax, rstep, tab, tdf, rad);
 }
+
+// Texels gathered from device memory (halos too wide for a workgroup's LDS
+// tile), with the radius search compacted round by round as in the tiled
+// kernels (tdf_brick), so a wave runs as many radii as its voxels need and
+// not as many as its slowest voxel: only the compaction state is in LDS.
+// MIXED: the precision form's (Hessian from the F32 planes, fused samples).
+template <class S, bool NORM, bool MIXED, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX* BY* BZ)
+    k_tdf_gather(Tex<S> T, PlanesF32 H, Dims d, float rmin, float rmax, float rstep, CircleTables tab,
+                 float* __restrict__ tdf, float* __restrict__ rad) {
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz);
+    if constexpr (MIXED)
+        tdf_brick<NORM, BX, BY, BZ, true>(T, H, d, tbx * BX, tby * BY, tbz * BZ, rmin, rmax, rstep, tab, tdf, rad);
+    else
+        tdf_brick<NORM, BX, BY, BZ>(T, T, d, tbx * BX, tby * BY, tbz * BZ, rmin, rmax, rstep, tab, tdf, rad);
+}
+constexpr int GBX = 16, GBY = 8, GBZ = 4;  // k_tdf_gather's bricks
 
 // Stage the clamp-to-edge texels of a TX x TY x TZ tile at (ox, oy, oz) in
 // LDS, then a barrier: every load is issued before the first LDS store, so
@@ -693,9 +655,13 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
                            rmax, rstep, tab, tdf, radius, h);
         return;
     }
-    Tex<S> T{(const P*)tex, inv};
-    dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
-    hipLaunchKernelGGL((k_tdf<S, NORM>), grid, block, 0, stream, T, d, rmin, rmax, rstep, tab, tdf, radius);
+    // halos too wide for LDS: texels gathered from device memory (measured at
+    // 256^3, radii 2.5-9 and 2.5-15: 15.2 -> 11.4 ms for F32 texels and
+    // 14.7 -> 10.3 ms for SNORM16 against a per-voxel kernel without the
+    // compaction)
+    const dim3 grid((d.X + GBX - 1) / GBX, (d.Y + GBY - 1) / GBY, (d.Z + GBZ - 1) / GBZ), block(GBX, GBY, GBZ);
+    hipLaunchKernelGGL((k_tdf_gather<S, NORM, false, GBX, GBY, GBZ>), grid, block, 0, stream,
+                       Tex<S>{(const P*)tex, inv}, PlanesF32{}, d, rmin, rmax, rstep, tab, tdf, radius);
 }
 
 // The compile-time-halo kernels on BX x BY x BZ bricks, when the float cell
@@ -727,10 +693,10 @@ static tsf_status launch_tdf_mixed(const tsf_field* f, const void* tex, float in
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
     const PlanesF32 H{(const float*)f->x, (const float*)f->y, (const float*)f->z, NORM ? (const float*)f->w : nullptr};
     if (h > 64 || lds + state > lds_per_block()) {
-        const dim3 grid((d.X + CBX - 1) / CBX, (d.Y + CBY - 1) / CBY, (d.Z + CBZ - 1) / CBZ), block(CBX, CBY, CBZ);
-        hipLaunchKernelGGL((k_tdf_mixed<S, NORM>), grid, block, 0, stream, Tex<S>{(const P*)tex, inv}, H, d, rmin,
-                           rmax, rstep, tab, tdf, radius);
         (void)W;
+        const dim3 grid((d.X + GBX - 1) / GBX, (d.Y + GBY - 1) / GBY, (d.Z + GBZ - 1) / GBZ), block(GBX, GBY, GBZ);
+        hipLaunchKernelGGL((k_tdf_gather<S, NORM, true, GBX, GBY, GBZ>), grid, block, 0, stream,
+                           Tex<S>{(const P*)tex, inv}, H, d, rmin, rmax, rstep, tab, tdf, radius);
         return TSF_OK;
     }
     // halos 4 and 5 (radii up to 3.5 and 4.5) with compile-time strides, on
