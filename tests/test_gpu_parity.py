@@ -297,7 +297,7 @@ def test_hessian_eigen(tsf, orc, normalize):
     assert nbad == 0, rel
 
 
-# rmax 6: halo 7, the FP32 planes tile at its largest (159 KB of LDS)
+# rmax 6: halo 7, past the FP32 planes tile (texels gathered from device memory)
 @pytest.mark.parametrize("rmin,rmax,rstep", [(1.0, 3.0, 0.5), (2.5, 5.0, 0.5), (0.5, 4.0, 1.0), (3.0, 6.0, 0.5)])
 def test_tdf_circle(tsf, orc, rmin, rmax, rstep):
     shape = (32, 32, 32)

@@ -580,7 +580,9 @@ static int tdf_f32_brick(float rmin, float rmax, size_t& lds) {
     for (int by : {8, 16}) {
         const size_t tile = (size_t)(16 + 2 * h) * (by + 2 * h) * (4 + 2 * h) * 3 * sizeof(float);
         const size_t state = (size_t)16 * by * 4 * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
-        const int w = h <= 64 ? tdf_waves(tile + state, 16 * by * 4) : 0;
+        // from halo 7 on, k_tdf_gather on F32 texels is faster than the tile
+        // (256^3, radii 2.5-6: 11.2 against 12.5 ms; halo 6: 10.8 against 10.4)
+        const int w = h <= 6 ? tdf_waves(tile + state, 16 * by * 4) : 0;
         if (w > waves) {
             waves = w;
             best = by;
@@ -637,7 +639,10 @@ static void launch_tdf(const void* tex, float inv, Dims d, float rmin, float rma
     const int h = tdf_halo(rmin, rmax);
     const size_t lds = (size_t)(BX + 2 * h) * (BY + 2 * h) * (BZ + 2 * h) * sizeof(P);
     const size_t state = (size_t)BX * BY * BZ * 8 * sizeof(float) + 64;  // tdf_brick's static LDS
-    if (sizeof(P) == 8 && h <= 64 && lds + state <= lds_per_block()) {
+    // the tile up to halo 5; wider, the tile holds a CU to 8-16 waves and
+    // k_tdf_gather is faster (256^3, SNORM16, halos 6, 7 and 8: 9.2, 10.0 and
+    // 10.2 ms against 11.8, 14.4 and 15.3 ms)
+    if (sizeof(P) == 8 && h <= 5 && lds + state <= lds_per_block()) {
         dim3 grid((d.X + BX - 1) / BX, (d.Y + BY - 1) / BY, (d.Z + BZ - 1) / BZ), block(BX, BY, BZ);
         // halos 4 and 5 with compile-time strides (the same arithmetic)
         if constexpr (sizeof(P) == 8) {
