@@ -294,8 +294,9 @@ tsf_status tsf_do_eigen_points(const float* fx, const float* fy, const float* fz
  * increase it.  tdf = max(0, best), radius = its r (0 if none).  The field
  * needs plane w when rmax >= 4.  Workspace: the field repacked as one texel
  * (x, y, z, w) per voxel for the trilinear gathers; an F32 field with
- * max(|rmin|, |rmax|) < 4.5 is read from its planes and needs none (the
- * workspace, if given, is unused). */
+ * max(|rmin|, |rmax|) < 5.5 is read from its planes and needs none (the
+ * workspace, if given, is unused; wider radii gather the packed texels from
+ * device memory). */
 tsf_status tsf_tdf_circle(const tsf_field* field, tsf_dims dims, float rmin, float rmax, float rstep, float* tdf,
                           float* radius, void* workspace, size_t workspace_bytes, hipStream_t stream);
 
